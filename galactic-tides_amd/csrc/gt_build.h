@@ -1,0 +1,38 @@
+// GPU octree builder (see gt_build.hip).
+#pragma once
+
+#include <algorithm>
+
+#include "gt_internal.h"
+
+namespace gth {
+
+// Builds the reference's octree for n particles on the GPU and keeps it in
+// device memory: nodes in level order as host-compatible geometry rows
+// (kGeomStride doubles: centre, half, com, mass, quadrupole, delta) and links
+// (first_child, n_children, begin, end), the tree order and tree-ordered
+// positions and masses.
+class TreeBuilder {
+ public:
+  void build(const double *pos, const double *mass, uint32_t n, int leaf_size, hipStream_t s);
+  uint32_t n() const { return n_; }
+  uint32_t n_nodes() const { return n_nodes_; }
+  int depth() const { return depth_; }
+  const double *geom() const { return geom_.get(); }
+  const int4 *links() const { return lnk_.get(); }
+  const uint32_t *order() const { return vals_.get(); }
+  const double *tree_pos() const { return tpos_.get(); }
+  const double *tree_mass() const { return tmass_.get(); }
+
+ private:
+  uint32_t n_ = 0, n_nodes_ = 0;
+  int depth_ = 0;
+  DBuf<double> part_, root_, geom_, tpos_, tmass_;
+  DBuf<uint64_t> keys_, keys_alt_, ndep_, ndep_alt_;
+  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, nchild_;
+  DBuf<uint8_t> dg_, lcp_, ldep_;
+  DBuf<unsigned long long> sort_tmp_, scan_tmp_, misc_;
+  DBuf<int4> lnk_;
+};
+
+}  // namespace gth

@@ -1,0 +1,508 @@
+// Octree construction on the GPU (reference: Octree::build, src/octree.cpp:298-402).
+//
+// The reference splits each node by a stable counting sort into octants,
+// recursively, until a node has at most leaf_size bodies, is 60 levels deep,
+// or is too small to split (:169-173).  The same tree falls out of a sort:
+//
+//  1. root cube from the bounding box (:313-334);
+//  2. for every particle, its octant path: 21 levels of 3-bit octants found
+//     by comparing against node centres that are updated with exactly the
+//     reference's FP64 operations (centre +- half/2, :258-266), so every
+//     comparison matches the reference's; also the first depth at which the
+//     path's node is degenerate (:172);
+//  3. a stable radix sort of the 63-bit paths (equal paths keep particle
+//     order, as the counting sorts do);
+//  4. lcp[i] = levels shared by sorted neighbours i, i+1.  The depth-d node
+//     containing p holds more than leaf_size bodies iff some window of
+//     leaf_size + 1 consecutive particles around p shares >= d levels, so p's
+//     leaf depth is 1 + max over those windows of their min lcp (capped by
+//     the degenerate depth and 60);
+//  5. the nodes that begin at sorted position p are depths lcp[p-1]+1 ..
+//     leaf_depth(p): a scan numbers them in preorder, a stable sort by depth
+//     lays them out level by level (children of a node are contiguous and in
+//     octant order, as the reference requires);
+//  6. parents by binary search in the level above, node ends, and bodies in
+//     each leaf put back into particle order (the reference's order inside a
+//     leaf is the input order, as no split has reordered them);
+//  7. FP64 moments level by level from the deepest: leaves sum their bodies
+//     in tree order (:175-201), inner nodes combine their children in order
+//     (:203-231), with no FMA contraction, so masses, centres of mass,
+//     quadrupoles and deltas are bit-identical to the reference's.
+#include "gt_build.h"
+
+namespace gth {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLevels = 21;  // octant digits per 64-bit path
+constexpr int kDepthLimit = 60;
+constexpr uint8_t kNoDegenerate = 0xff;
+
+inline uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+#pragma clang fp contract(off)
+
+// ---- 1. root cube ---------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void bbox_partial(const double *__restrict__ pos, uint32_t n,
+                                                       double *__restrict__ part) {
+  __shared__ double s[6][kBlock];
+  double lo[3], hi[3];
+  const uint32_t first = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i0 = first < n ? first : 0;
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = pos[3 * static_cast<size_t>(i0) + a];
+  for (uint32_t i = first; i < n; i += gridDim.x * kBlock)
+    for (int a = 0; a < 3; ++a) {
+      const double v = pos[3 * static_cast<size_t>(i) + a];
+      lo[a] = v < lo[a] ? v : lo[a];
+      hi[a] = hi[a] < v ? v : hi[a];
+    }
+  for (int a = 0; a < 3; ++a) {
+    s[a][threadIdx.x] = lo[a];
+    s[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(w))
+      for (int a = 0; a < 3; ++a) {
+        const double l = s[a][threadIdx.x + w], h = s[3 + a][threadIdx.x + w];
+        if (l < s[a][threadIdx.x]) s[a][threadIdx.x] = l;
+        if (s[3 + a][threadIdx.x] < h) s[3 + a][threadIdx.x] = h;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// root[0..2] centre, root[3] half (src/octree.cpp:324-332).
+__global__ void bbox_final(const double *__restrict__ part, uint32_t nparts,
+                           double *__restrict__ root) {
+  if (threadIdx.x != 0) return;
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = part[a];
+    hi[a] = part[3 + a];
+  }
+  for (uint32_t b = 1; b < nparts; ++b)
+    for (int a = 0; a < 3; ++a) {
+      const double l = part[6 * b + a], h = part[6 * b + 3 + a];
+      lo[a] = l < lo[a] ? l : lo[a];
+      hi[a] = hi[a] < h ? h : hi[a];
+    }
+  double half = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double e = 0.5 * (hi[a] - lo[a]);
+    half = half < e ? e : half;
+  }
+  half = half * (1.0 + 1e-9) + 1e-300;
+  for (int a = 0; a < 3; ++a) root[a] = 0.5 * (lo[a] + hi[a]);
+  root[3] = half;
+}
+
+// ---- 2. octant paths ---------------------------------------------------------------
+
+__device__ inline bool degenerate(double cx, double cy, double cz, double half) {
+  return half < 1e-12 * (fabs(cx) + fabs(cy) + fabs(cz) + 1e-300);
+}
+
+__global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict__ pos, uint32_t n,
+                                                       const double *__restrict__ root,
+                                                       uint64_t *__restrict__ keys,
+                                                       uint32_t *__restrict__ vals,
+                                                       uint8_t *__restrict__ dg) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
+               z = pos[3 * static_cast<size_t>(i) + 2];
+  double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+  uint64_t key = 0;
+  uint8_t first_dg = kNoDegenerate;
+  for (int l = 0; l < kLevels; ++l) {
+    if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = static_cast<uint8_t>(l);
+    const uint32_t o = (x >= cx ? 1u : 0u) | (y >= cy ? 2u : 0u) | (z >= cz ? 4u : 0u);
+    key = (key << 3) | o;
+    const double h2 = 0.5 * h;
+    cx = cx + ((o & 1u) ? h2 : -h2);
+    cy = cy + ((o & 2u) ? h2 : -h2);
+    cz = cz + ((o & 4u) ? h2 : -h2);
+    h = h2;
+  }
+  if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = kLevels;
+  keys[i] = key;
+  vals[i] = i;
+  dg[i] = first_dg;
+}
+
+// ---- 4. shared prefixes and leaf depths ---------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restrict__ keys,
+                                                        uint32_t n, uint8_t *__restrict__ lcp) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint64_t d = keys[i] ^ keys[i + 1];
+  lcp[i] = d ? static_cast<uint8_t>((__clzll(d) - 1) / 3) : static_cast<uint8_t>(kLevels);
+}
+
+// Leaf depth of sorted position p and the number of nodes beginning at p.
+__global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict__ lcp,
+                                                      const uint32_t *__restrict__ vals,
+                                                      const uint8_t *__restrict__ dg, uint32_t n,
+                                                      uint32_t leaf, uint8_t *__restrict__ ldep,
+                                                      uint32_t *__restrict__ starts,
+                                                      uint32_t *__restrict__ unresolved) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  // Windows j .. j+leaf of leaf+1 particles that contain p and fit in [0, n).
+  int best = -1;
+  if (n > leaf) {
+    const uint32_t jlo = p >= leaf ? p - leaf : 0u;
+    const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
+    for (uint32_t j = jlo; j <= jhi; ++j) {
+      int w = kLevels;
+      for (uint32_t k = j; k < j + leaf; ++k) w = min(w, static_cast<int>(lcp[k]));
+      best = max(best, w);
+    }
+  }
+  int d = best + 1;
+  d = min(d, static_cast<int>(dg[vals[p]]));
+  d = min(d, kDepthLimit);
+  if (d > kLevels) atomicOr(unresolved, 1u);
+  ldep[p] = static_cast<uint8_t>(min(d, kLevels));
+  const int first = p == 0 ? 0 : static_cast<int>(lcp[p - 1]) + 1;
+  starts[p] = d >= first ? static_cast<uint32_t>(d - first + 1) : 0u;
+}
+
+// ---- 5. preorder enumeration ---------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__ lcp,
+                                                     const uint32_t *__restrict__ starts,
+                                                     const uint32_t *__restrict__ offs, uint32_t n,
+                                                     uint64_t *__restrict__ node_depth,
+                                                     uint32_t *__restrict__ node_begin) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t c = starts[p];
+  const uint32_t first = p == 0 ? 0u : static_cast<uint32_t>(lcp[p - 1]) + 1u;
+  for (uint32_t t = 0; t < c; ++t) {
+    node_depth[offs[p] + t] = first + t;
+    node_begin[offs[p] + t] = p;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restrict__ node_depth,
+                                                       uint32_t n_nodes,
+                                                       uint32_t *__restrict__ lstart) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= n_nodes) return;
+  const uint32_t d = static_cast<uint32_t>(node_depth[x]);
+  if (x == 0 || static_cast<uint32_t>(node_depth[x - 1]) != d) lstart[d] = x;
+  if (x == n_nodes - 1) lstart[d + 1] = n_nodes;
+}
+
+// ---- 6. links, geometry, leaf order ------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void link_nodes(
+    const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
+    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ keys,
+    const double *__restrict__ root, uint32_t n_nodes, double *__restrict__ geom,
+    int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= n_nodes) return;
+  const uint32_t d = static_cast<uint32_t>(node_depth[x]);
+  const uint32_t b = node_begin[x];
+  // Centre and half size: replay the octant path of the node's first body.
+  const uint64_t key = keys[b];
+  double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+  for (uint32_t l = 0; l < d; ++l) {
+    const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l))) & 7u;
+    const double h2 = 0.5 * h;
+    cx = cx + ((o & 1u) ? h2 : -h2);
+    cy = cy + ((o & 2u) ? h2 : -h2);
+    cz = cz + ((o & 4u) ? h2 : -h2);
+    h = h2;
+  }
+  double *g = geom + static_cast<size_t>(kGeomStride) * x;
+  g[0] = cx;
+  g[1] = cy;
+  g[2] = cz;
+  g[3] = h;
+  g[15] = 0.0;
+  lnk[x] = make_int4(-1, 0, static_cast<int>(b), 0);
+  if (d == 0) {
+    parent[x] = kNone;
+    return;
+  }
+  // Parent: the last node of the level above beginning at or before b.
+  uint32_t lo = lstart[d - 1], hi = lstart[d] - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (node_begin[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  parent[x] = lo;
+  atomicAdd(&nchild[lo], 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restrict__ parent,
+                                                       const uint32_t *__restrict__ nchild,
+                                                       const uint32_t *__restrict__ node_begin,
+                                                       uint32_t n_nodes, int4 *__restrict__ lnk) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= n_nodes) return;
+  const uint32_t p = parent[x];
+  if (p != kNone && node_begin[p] == node_begin[x]) lnk[p].x = static_cast<int>(x);
+  lnk[x].y = static_cast<int>(nchild[x]);
+}
+
+// end of node x: the begin of the node after it in its parent's children, or
+// (for a last child) its parent's end; the root ends at n.
+__global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__ parent,
+                                                    const uint32_t *__restrict__ node_begin,
+                                                    uint32_t n_nodes, uint32_t n,
+                                                    int4 *__restrict__ lnk) {
+  const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
+  if (x0 >= n_nodes) return;
+  uint32_t x = x0;
+  for (;;) {
+    const uint32_t p = parent[x];
+    if (p == kNone) {
+      lnk[x0].w = static_cast<int>(n);
+      return;
+    }
+    const int4 P = lnk[p];
+    if (static_cast<int>(x) != P.x + P.y - 1) {
+      lnk[x0].w = static_cast<int>(x + 1 < n_nodes ? node_begin[x + 1] : n);
+      return;
+    }
+    x = p;
+  }
+}
+
+// Bodies of a leaf back into particle order (insertion sort; leaves are small).
+__global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ lnk, uint32_t n_nodes,
+                                                     uint32_t *__restrict__ vals) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= n_nodes) return;
+  const int4 L = lnk[x];
+  if (L.y != 0) return;
+  const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
+  for (uint32_t k = b + 1; k < e; ++k) {
+    const uint32_t v = vals[k];
+    uint32_t j = k;
+    while (j > b && vals[j - 1] > v) {
+      vals[j] = vals[j - 1];
+      --j;
+    }
+    vals[j] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__restrict__ vals,
+                                                            const double *__restrict__ pos,
+                                                            const double *__restrict__ mass,
+                                                            uint32_t n, double *__restrict__ tpos,
+                                                            double *__restrict__ tmass) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = vals[k];
+  for (int a = 0; a < 3; ++a) tpos[3 * static_cast<size_t>(k) + a] = pos[3 * static_cast<size_t>(i) + a];
+  tmass[k] = mass[i];
+}
+
+// ---- 7. moments, one level at a time -------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void level_moments(uint32_t first, uint32_t last,
+                                                        uint32_t n_nodes,
+                                                        const int4 *__restrict__ lnk,
+                                                        const double *__restrict__ tpos,
+                                                        const double *__restrict__ tmass,
+                                                        double *__restrict__ geom) {
+  const uint32_t x = first + blockIdx.x * kBlock + threadIdx.x;
+  if (x >= last) return;
+  const int4 L = lnk[x];
+  double *g = geom + static_cast<size_t>(kGeomStride) * x;
+  const double ccx = g[0], ccy = g[1], ccz = g[2];
+  double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (L.y == 0) {
+    const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
+    for (uint32_t k = b; k < e; ++k) {
+      const double mi = tmass[k];
+      m += mi;
+      cx += mi * tpos[3 * static_cast<size_t>(k)];
+      cy += mi * tpos[3 * static_cast<size_t>(k) + 1];
+      cz += mi * tpos[3 * static_cast<size_t>(k) + 2];
+    }
+    if (m > 0) {
+      cx = __ddiv_rn(cx, m);
+      cy = __ddiv_rn(cy, m);
+      cz = __ddiv_rn(cz, m);
+    } else {
+      cx = ccx;
+      cy = ccy;
+      cz = ccz;
+    }
+    for (uint32_t k = b; k < e; ++k) {
+      const double mi = tmass[k];
+      const double dx = tpos[3 * static_cast<size_t>(k)] - cx,
+                   dy = tpos[3 * static_cast<size_t>(k) + 1] - cy,
+                   dz = tpos[3 * static_cast<size_t>(k) + 2] - cz;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      q[0] += mi * (3 * dx * dx - r2);
+      q[1] += mi * (3 * dy * dy - r2);
+      q[2] += mi * (3 * dz * dz - r2);
+      q[3] += mi * 3 * dx * dy;
+      q[4] += mi * 3 * dx * dz;
+      q[5] += mi * 3 * dy * dz;
+    }
+  } else {
+    const uint32_t c0 = static_cast<uint32_t>(L.x), c1 = c0 + static_cast<uint32_t>(L.y);
+    if (L.x < 0 || c1 > n_nodes) return;  // malformed links: leave the row unset
+    for (uint32_t c = c0; c < c1; ++c) {
+      const double *h = geom + static_cast<size_t>(kGeomStride) * c;
+      m += h[7];
+      cx += h[7] * h[4];
+      cy += h[7] * h[5];
+      cz += h[7] * h[6];
+    }
+    if (m > 0) {
+      cx = __ddiv_rn(cx, m);
+      cy = __ddiv_rn(cy, m);
+      cz = __ddiv_rn(cz, m);
+    } else {
+      cx = ccx;
+      cy = ccy;
+      cz = ccz;
+    }
+    for (uint32_t c = c0; c < c1; ++c) {
+      const double *h = geom + static_cast<size_t>(kGeomStride) * c;
+      const double mc = h[7];
+      const double dx = h[4] - cx, dy = h[5] - cy, dz = h[6] - cz;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      q[0] += h[8] + mc * (3 * dx * dx - r2);
+      q[1] += h[9] + mc * (3 * dy * dy - r2);
+      q[2] += h[10] + mc * (3 * dz * dz - r2);
+      q[3] += h[11] + mc * 3 * dx * dy;
+      q[4] += h[12] + mc * 3 * dx * dz;
+      q[5] += h[13] + mc * 3 * dy * dz;
+    }
+  }
+  g[4] = cx;
+  g[5] = cy;
+  g[6] = cz;
+  g[7] = m;
+  for (int k = 0; k < 6; ++k) g[8 + k] = q[k];
+  const double ex = cx - ccx, ey = cy - ccy, ez = cz - ccz;
+  g[14] = __dsqrt_rn(ex * ex + ey * ey + ez * ez);
+}
+
+#pragma clang fp contract(fast)
+
+}  // namespace
+
+void TreeBuilder::build(const double *pos, const double *mass, uint32_t n, int leaf_size,
+                        hipStream_t s) {
+  n_ = n;
+  n_nodes_ = 0;
+  depth_ = 0;
+  if (n == 0) return;
+  const uint32_t leaf = static_cast<uint32_t>(leaf_size < 1 ? 1 : leaf_size);
+
+  // 1. root cube
+  const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
+  double *part = part_.ensure(6 * static_cast<size_t>(nparts));
+  root_.ensure(4);
+  hipLaunchKernelGGL(bbox_partial, dim3(nparts), dim3(kBlock), 0, s, pos, n, part);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, part, nparts, root_.get());
+  GTH_LAUNCH_CHECK();
+
+  // 2-3. octant paths, sorted
+  uint64_t *keys = keys_.ensure(n);
+  uint32_t *vals = vals_.ensure(n);
+  uint8_t *dg = dg_.ensure(n);
+  hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
+                     keys, vals, dg);
+  GTH_LAUNCH_CHECK();
+  radix_sort_u64(keys, vals, keys_alt_.ensure(n), vals_alt_.ensure(n), n, 3 * kLevels,
+                 sort_tmp_.ensure(radix_tmp_words(n)), s);
+
+  // 4. shared prefixes, leaf depths, node counts per start position
+  uint8_t *lcp = lcp_.ensure(n);
+  uint8_t *ldep = ldep_.ensure(n);
+  uint32_t *starts = starts_.ensure(n);
+  uint32_t *offs = offs_.ensure(n);
+  unsigned long long *misc = misc_.ensure(4);  // [0] node total, [1] unresolved flag
+  launch_fill_u64(misc, 4, 0ull, s);
+  hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, keys, n, lcp);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
+                     ldep, starts, reinterpret_cast<uint32_t *>(misc + 1));
+  GTH_LAUNCH_CHECK();
+  exclusive_scan_u32(starts, offs, n, misc, scan_tmp_.ensure(scan_tmp_words(n)), s);
+  unsigned long long host_misc[2];
+  GTH_CHECK(hipMemcpyAsync(host_misc, misc, sizeof(host_misc), hipMemcpyDeviceToHost, s));
+  GTH_CHECK(hipStreamSynchronize(s));
+  if (host_misc[1] & 1ull)
+    fail(GT_HIP_ERR_UNSUPPORTED,
+         "octree build: a cluster of more than leaf_size bodies shares 21 octant levels; "
+         "deeper trees are not supported by the GPU builder yet");
+  if (host_misc[0] > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "octree build: too many nodes");
+  const uint32_t nn = static_cast<uint32_t>(host_misc[0]);
+  n_nodes_ = nn;
+
+  // 5. nodes in preorder, then stably by depth (level order)
+  uint64_t *ndep = ndep_.ensure(nn);
+  uint32_t *nbeg = nbeg_.ensure(nn);
+  hipLaunchKernelGGL(emit_nodes, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, starts, offs, n,
+                     ndep, nbeg);
+  GTH_LAUNCH_CHECK();
+  radix_sort_u64(ndep, nbeg, ndep_alt_.ensure(nn), nbeg_alt_.ensure(nn), nn, 6,
+                 sort_tmp_.ensure(radix_tmp_words(nn)), s);
+  uint32_t *lstart = lstart_.ensure(kDepthLimit + 2);
+  hipLaunchKernelGGL(level_starts, dim3(blocks_for(nn)), dim3(kBlock), 0, s, ndep, nn, lstart);
+  GTH_LAUNCH_CHECK();
+
+  // 6. links and geometry
+  double *geom = geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
+  int4 *lnk = lnk_.ensure(nn);
+  uint32_t *parent = parent_.ensure(nn);
+  uint32_t *nchild = nchild_.ensure(nn);
+  launch_fill_u32(nchild, nn, 0u, s);
+  hipLaunchKernelGGL(link_nodes, dim3(blocks_for(nn)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
+                     keys, root_.get(), nn, geom, lnk, parent, nchild);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(finish_links, dim3(blocks_for(nn)), dim3(kBlock), 0, s, parent, nchild, nbeg,
+                     nn, lnk);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(node_ends, dim3(blocks_for(nn)), dim3(kBlock), 0, s, parent, nbeg, nn, n,
+                     lnk);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(nn)), dim3(kBlock), 0, s, lnk, nn, vals);
+  GTH_LAUNCH_CHECK();
+  double *tpos = tpos_.ensure(3 * static_cast<size_t>(n));
+  double *tmass = tmass_.ensure(n);
+  hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
+                     n, tpos, tmass);
+  GTH_LAUNCH_CHECK();
+
+  // 7. moments, deepest level first
+  uint32_t hl[kDepthLimit + 2];
+  GTH_CHECK(hipMemcpyAsync(hl, lstart, sizeof(hl), hipMemcpyDeviceToHost, s));
+  uint64_t last_depth = 0;
+  GTH_CHECK(hipMemcpyAsync(&last_depth, ndep + (nn - 1), sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           s));
+  GTH_CHECK(hipStreamSynchronize(s));
+  depth_ = static_cast<int>(last_depth);
+  for (int d = depth_; d >= 0; --d) {
+    const uint32_t a = hl[d], b = hl[d + 1];
+    if (b <= a || b > nn) fail(GT_HIP_ERR_DEVICE, "octree build: inconsistent levels");
+    hipLaunchKernelGGL(level_moments, dim3(blocks_for(b - a)), dim3(kBlock), 0, s, a, b, nn, lnk,
+                       tpos, tmass, geom);
+    GTH_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace gth

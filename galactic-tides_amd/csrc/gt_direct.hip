@@ -1,0 +1,96 @@
+// O(N^2) kernels.
+//
+//  * direct: FP32 softened body-body sum for every target (reference:
+//    gt_direct, src/metal/gravity_kernels.metal:18-43, and gt_direct_accel,
+//    gravity_kernels.h:173-182).  Lane = target; the source loop is wave-
+//    uniform, so every source body arrives through the scalar cache straight
+//    into SGPRs and is a scalar operand of the VALU ops: no LDS staging, no
+//    barriers, no broadcast instructions.  v_rsq_f32 on the VALU; pairwise
+//    gravity is not a dense contraction, so there is no MFMA here.
+//  * direct_f64_sample: FP64 reference forces for a sample of targets
+//    (apps/galactic_tides_bench.cpp:80-95), used to measure force errors.
+#include "gt_internal.h"
+
+namespace gth {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void direct_fast(const float4 *__restrict__ bodies,
+                                                      uint32_t n, float G, float eps2,
+                                                      float4 *__restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const float4 bi = bodies[i < n ? i : n - 1];
+  const float inf = __builtin_inff();
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, ap = 0.0f;
+#pragma unroll 8
+  for (uint32_t j = 0; j < n; ++j) {
+    const float4 b = bodies[j];
+    const float dx = bi.x - b.x, dy = bi.y - b.y, dz = bi.z - b.z;
+    float r2 = dx * dx + dy * dy + dz * dz + eps2;
+    r2 = (j == i) ? inf : r2;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float gm = G * b.w * inv;
+    const float gm3 = gm * inv * inv;
+    ax -= gm3 * dx;
+    ay -= gm3 * dy;
+    az -= gm3 * dz;
+    ap -= gm;
+  }
+  if (i < n) out[i] = make_float4(ax, ay, az, ap);
+}
+
+#pragma clang fp contract(off)
+// One workgroup per sampled target; threads stride over sources and the
+// partial sums are combined in a fixed tree order (deterministic).
+__global__ __launch_bounds__(kBlock) void direct_f64_sample(const double *__restrict__ pos,
+                                                            const double *__restrict__ mass,
+                                                            uint64_t n, double G, double eps2,
+                                                            const uint64_t *__restrict__ sample,
+                                                            double *__restrict__ acc) {
+  __shared__ double red[3][kBlock];
+  const uint64_t i = sample[blockIdx.x];
+  const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (uint64_t j = threadIdx.x; j < n; j += kBlock) {
+    if (j == i) continue;
+    const double dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
+    const double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+    const double f = G * mass[j] * inv * inv * inv;
+    ax -= f * dx;
+    ay -= f * dy;
+    az -= f * dz;
+  }
+  red[0][threadIdx.x] = ax;
+  red[1][threadIdx.x] = ay;
+  red[2][threadIdx.x] = az;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(w))
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) acc[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+#pragma clang fp contract(fast)
+
+}  // namespace
+
+void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
+                   hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(direct_fast, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, bodies, n,
+                     G, eps2, out);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_direct_f64_sample(const double *pos, const double *mass, uint64_t n, double G,
+                              double eps, const uint64_t *sample, uint64_t ns, double *acc,
+                              hipStream_t s) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(direct_f64_sample, dim3(static_cast<uint32_t>(ns)), dim3(kBlock), 0, s, pos,
+                     mass, n, G, eps * eps, sample, acc);
+  GTH_LAUNCH_CHECK();
+}
+
+}  // namespace gth

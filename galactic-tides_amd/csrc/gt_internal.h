@@ -1,0 +1,148 @@
+// Internal declarations shared by the HIP sources of libgt_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "gt_hip.h"
+
+namespace gth {
+
+// ---- errors -------------------------------------------------------------------
+
+// Carries a gt_hip status code to the C-ABI layer, which turns it into the
+// return value and the context's last-error message.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define GTH_CHECK(call)                                                                  \
+  do {                                                                                   \
+    hipError_t gth_e_ = (call);                                                          \
+    if (gth_e_ != hipSuccess)                                                            \
+      ::gth::fail(gth_e_ == hipErrorOutOfMemory ? GT_HIP_ERR_MEMORY : GT_HIP_ERR_DEVICE, \
+                  std::string(#call) + ": " + hipGetErrorString(gth_e_));                \
+  } while (0)
+
+#define GTH_LAUNCH_CHECK() GTH_CHECK(hipGetLastError())
+
+// ---- device buffers -----------------------------------------------------------
+
+// Device array that keeps its storage between calls and only grows
+// (geometrically), so steady-state steps allocate nothing.
+template <class T>
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() {
+    if (p_) (void)hipFree(p_);
+  }
+  T *ensure(size_t n) {
+    if (n > cap_) {
+      size_t c = cap_ + cap_ / 2;
+      if (c < n) c = n;
+      if (c < 64) c = 64;
+      if (p_) GTH_CHECK(hipFree(p_));
+      p_ = nullptr;
+      cap_ = 0;
+      GTH_CHECK(hipMalloc(reinterpret_cast<void **>(&p_), c * sizeof(T)));
+      cap_ = c;
+    }
+    return p_;
+  }
+  T *get() const { return p_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  T *p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+// ---- device-side layouts -------------------------------------------------------
+
+// kern::GTCell / GTGroup / GTParams with HIP vector types.
+struct Cell {
+  float4 cm;  // com - origin, mass
+  float4 q0;  // xx yy zz xy
+  float4 q1;  // xz yz 0 0
+};
+static_assert(sizeof(Cell) == 48, "GTCell layout");
+static_assert(sizeof(gt_cell) == 48 && sizeof(gt_group) == 32 && sizeof(gt_params) == 32 &&
+                  sizeof(gt_float4) == 16,
+              "reference struct layouts");
+
+// Node as the group walk reads it: geometry for the FP64 opening test.
+struct WalkNode {
+  double cx, cy, cz, half;
+  double mx, my, mz, crit;
+};
+static_assert(sizeof(WalkNode) == 64, "WalkNode layout");
+
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kGeomStride = 16;  // doubles per node in the host-compatible geometry rows
+constexpr int kMaxGroup = 64;    // targets per group: one wave64
+constexpr uint32_t kBodiesIfAtMost = 2;  // accepted cells this small are listed as bodies
+
+// ---- launchers (defined in the .hip sources) -------------------------------------
+
+// gt_scan.hip: exclusive prefix sum of n uint32 values into out (may alias in),
+// total written to *total (device).  tmp must hold scan_tmp_words(n) words.
+size_t scan_tmp_words(size_t n);
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned long long *total,
+                        unsigned long long *tmp, hipStream_t s);
+
+// gt_sort.hip: stable LSD radix sort of (key, value) pairs on key bits [0, bits).
+// The sorted pairs end in (keys, vals); keys_alt/vals_alt are scratch of n
+// entries; tmp must hold radix_tmp_words(n) 64-bit words.
+size_t radix_tmp_words(size_t n);
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                    size_t n, int bits, unsigned long long *tmp, hipStream_t s);
+
+// gt_eval.hip
+void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
+                 const uint32_t *body_list, const gt_group *groups, uint32_t n_groups, float G,
+                 float eps2, uint32_t quad, bool exact, float4 *out, hipStream_t s);
+
+// gt_direct.hip
+void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
+                   hipStream_t s);
+void launch_direct_f64_sample(const double *pos, const double *mass, uint64_t n, double G,
+                              double eps, const uint64_t *sample, uint64_t ns, double *acc,
+                              hipStream_t s);
+
+// gt_walk.hip
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
+void launch_prep_lists(const double *geom, const double *tpos, const double *tmass, uint32_t n,
+                       uint32_t n_nodes, double theta, WalkNode *wn, Cell *cells, float4 *bodies,
+                       hipStream_t s);
+void launch_link_up(const int4 *lnk, uint32_t n_nodes, uint32_t *parent, uint32_t *skip,
+                    hipStream_t s);
+void launch_group_starts(const int4 *lnk, uint32_t n_nodes, uint32_t n, uint32_t gs, int pack,
+                         uint32_t *flags, hipStream_t s);
+void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n, uint32_t ng,
+                        uint32_t *starts, gt_group *groups, hipStream_t s);
+void launch_walk(bool fill, const WalkNode *wn, const int4 *lnk, const uint32_t *skip,
+                 const double *tpos, gt_group *groups, uint32_t ng, const uint32_t *cell_off,
+                 const uint32_t *body_off, uint32_t *cell_count, uint32_t *body_count,
+                 uint32_t *cell_list, uint32_t *body_list, uint32_t nn, uint32_t *error,
+                 hipStream_t s);
+void launch_list_stats(const gt_group *groups, uint32_t ng, unsigned long long *stats,
+                       hipStream_t s);
+
+// gt_step.hip
+void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+                    double *pot, hipStream_t s);
+void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,
+                       double dt, hipStream_t s);
+void launch_kick(double *vel, const double *acc, size_t n, double h, hipStream_t s);
+
+}  // namespace gth

@@ -1,0 +1,108 @@
+// Exclusive prefix sums, three passes: per-tile totals, a scan of the tile
+// totals in one workgroup, then each tile scanned with its offset.  Sums are
+// formed in 64 bits so callers can detect results that no longer fit 32 bits.
+#include "gt_internal.h"
+
+namespace gth {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;
+constexpr int kWave = 64;
+
+// Inclusive scan of one value per thread across the workgroup.
+__device__ inline unsigned long long block_inclusive_scan(unsigned long long v,
+                                                          unsigned long long *wave_tot) {
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const unsigned long long u = __shfl_up(v, off, kWave);
+    if (lane >= off) v += u;
+  }
+  if (lane == kWave - 1) wave_tot[wave] = v;
+  __syncthreads();
+  unsigned long long before = 0;
+  for (int w = 0; w < wave; ++w) before += wave_tot[w];
+  __syncthreads();
+  return v + before;
+}
+
+__global__ __launch_bounds__(kBlock) void tile_totals(const uint32_t *__restrict__ in, size_t n,
+                                                      unsigned long long *__restrict__ tot) {
+  __shared__ unsigned long long wt[kBlock / kWave];
+  const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k)
+    if (base + k < n) s += in[base + k];
+  s = block_inclusive_scan(s, wt);
+  if (threadIdx.x == kBlock - 1) tot[blockIdx.x] = s;
+}
+
+// One workgroup: exclusive scan of the tile totals in place, grand total to *total.
+__global__ __launch_bounds__(kBlock) void scan_totals(unsigned long long *__restrict__ tot,
+                                                      size_t ntiles,
+                                                      unsigned long long *__restrict__ total) {
+  __shared__ unsigned long long wt[kBlock / kWave];
+  __shared__ unsigned long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (size_t base = 0; base < ntiles; base += kBlock) {
+    const size_t k = base + threadIdx.x;
+    const unsigned long long v = k < ntiles ? tot[k] : 0ull;
+    const unsigned long long inc = block_inclusive_scan(v, wt);
+    const unsigned long long c = carry;
+    if (k < ntiles) tot[k] = c + inc - v;
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) carry = c + inc;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void tile_scan(const uint32_t *in, uint32_t *out, size_t n,
+                                                    const unsigned long long *__restrict__ tot) {
+  __shared__ unsigned long long wt[kBlock / kWave];
+  const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+  uint32_t v[kItems];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  const unsigned long long inc = block_inclusive_scan(s, wt);
+  unsigned long long run = tot[blockIdx.x] + inc - s;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (base + k < n) out[base + k] = static_cast<uint32_t>(run);
+    run += v[k];
+  }
+}
+
+__global__ void zero_total(unsigned long long *total) { *total = 0ull; }
+
+}  // namespace
+
+size_t scan_tmp_words(size_t n) { return (n + kTile - 1) / kTile + 1; }
+
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned long long *total,
+                        unsigned long long *tmp, hipStream_t s) {
+  const size_t ntiles = (n + kTile - 1) / kTile;
+  if (ntiles == 0) {
+    hipLaunchKernelGGL(zero_total, dim3(1), dim3(1), 0, s, total);
+    GTH_LAUNCH_CHECK();
+    return;
+  }
+  hipLaunchKernelGGL(tile_totals, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in, n,
+                     tmp);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scan_totals, dim3(1), dim3(kBlock), 0, s, tmp, ntiles, total);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tile_scan, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in, out,
+                     n, tmp);
+  GTH_LAUNCH_CHECK();
+}
+
+}  // namespace gth

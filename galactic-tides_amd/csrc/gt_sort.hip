@@ -1,0 +1,126 @@
+// Stable LSD radix sort of (uint64 key, uint32 value) pairs, 8 bits per pass.
+//
+// Per pass: each workgroup histograms a tile of 2048 keys; an exclusive scan
+// over the digit-major histogram gives every (digit, tile) its output offset;
+// each workgroup then re-reads its tile in eight rounds of 256 consecutive
+// keys and ranks each key among equal digits with wave ballots (match masks),
+// which keeps equal keys in input order.  Used for the octree's octant-path
+// keys, where stability is what reproduces the reference's stable counting
+// sort (src/octree.cpp:233-250).
+#include <utility>
+
+#include "gt_internal.h"
+
+namespace gth {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;
+constexpr int kWave = 64;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kDigits = 256;
+
+__global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restrict__ keys,
+                                                         size_t n, int shift, uint32_t ntiles,
+                                                         uint32_t *__restrict__ counts) {
+  __shared__ uint32_t h[kDigits];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const size_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xff], 1u);
+  }
+  __syncthreads();
+  counts[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void tile_scatter(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, size_t n, int shift,
+    uint32_t ntiles, const uint32_t *__restrict__ offsets, uint64_t *__restrict__ keys_out,
+    uint32_t *__restrict__ vals_out) {
+  __shared__ uint32_t run[kDigits];             // keys of each digit placed so far in this tile
+  __shared__ uint32_t wave_cnt[kWaves][kDigits];  // per-round, per-wave digit counts
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  run[threadIdx.x] = offsets[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x];
+  const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  for (int round = 0; round < kItems; ++round) {
+    for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const size_t i = base + round * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t key = 0;
+    uint32_t val = 0;
+    uint32_t d = 0;
+    if (valid) {
+      key = keys[i];
+      val = vals[i];
+      d = static_cast<uint32_t>((key >> shift) & 0xff);
+    }
+    // Lanes of this wave holding the same digit.
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t set = __ballot(valid && ((d >> b) & 1u));
+      same &= ((d >> b) & 1u) ? set : ~set;
+    }
+    const uint32_t rank_in_wave = __popcll(same & below);
+    const bool leader = valid && rank_in_wave == 0;
+    if (leader) wave_cnt[wave][d] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank_in_wave;
+      for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < kWaves; ++w) add += wave_cnt[w][threadIdx.x];
+    run[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t radix_tmp_words(size_t n) {
+  const size_t ntiles = (n + kTile - 1) / kTile;
+  const size_t counts = static_cast<size_t>(kDigits) * ntiles;
+  // counts (uint32, rounded up to 64-bit words), scan scratch, scan total
+  return (counts + 1) / 2 + scan_tmp_words(counts) + 1;
+}
+
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                    size_t n, int bits, unsigned long long *tmp, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return;
+  const size_t ntiles = (n + kTile - 1) / kTile;
+  const size_t ncounts = static_cast<size_t>(kDigits) * ntiles;
+  uint32_t *counts = reinterpret_cast<uint32_t *>(tmp);
+  unsigned long long *scan_tmp = tmp + (ncounts + 1) / 2;
+  unsigned long long *total = scan_tmp + scan_tmp_words(ncounts);
+  const int passes = (bits + 7) / 8;
+  uint64_t *kin = keys, *kout = keys_alt;
+  uint32_t *vin = vals, *vout = vals_alt;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(tile_histogram, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
+                       kin, n, shift, static_cast<uint32_t>(ntiles), counts);
+    GTH_LAUNCH_CHECK();
+    exclusive_scan_u32(counts, counts, ncounts, total, scan_tmp, s);
+    hipLaunchKernelGGL(tile_scatter, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, kin,
+                       vin, n, shift, static_cast<uint32_t>(ntiles), counts, kout, vout);
+    GTH_LAUNCH_CHECK();
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (kin != keys) {
+    GTH_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    GTH_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  }
+}
+
+}  // namespace gth

@@ -1,0 +1,261 @@
+"""Python binding of libgt_hip.so (the C ABI in include/gt_hip.h).
+
+The library is loaded from this directory; if it has not been built this
+module raises at import.  There is no CPU fallback: every call runs the HIP
+kernels.  Errors from the library become exceptions, following the
+reference's conventions: bad arguments raise ValueError (the reference's
+std::invalid_argument), device failures raise RuntimeError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgt_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run python galactic-tides_amd/build.py")
+_lib = C.CDLL(LIB_PATH)
+
+OK, ERR_ARGUMENT, ERR_DEVICE, ERR_MEMORY, ERR_LIMIT, ERR_UNSUPPORTED = range(6)
+
+
+class GroupConfig(C.Structure):
+    _fields_ = [("G", C.c_double), ("theta", C.c_double), ("eps", C.c_double),
+                ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
+                ("pack", C.c_int), ("exact", C.c_int), ("reserved", C.c_int * 7)]
+
+
+class Params(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("n_groups", C.c_uint32), ("G", C.c_float),
+                ("eps2", C.c_float), ("quadrupole", C.c_uint32), ("pad0", C.c_uint32),
+                ("pad1", C.c_uint32), ("pad2", C.c_uint32)]
+
+
+class ListsInfo(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("n", "n_nodes", "depth", "n_groups", "n_cell_list",
+                                           "n_body_list", "cell_interactions",
+                                           "body_interactions", "max_list", "max_group")]
+
+
+class Timings(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("tree", "lists", "evaluate", "scatter", "total")]
+
+
+_vp = C.c_void_p
+_P = C.POINTER
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_create = _sig("gt_hip_create", C.c_int, [_P(_vp), C.c_int])
+_destroy = _sig("gt_hip_destroy", None, [_vp])
+_last_error = _sig("gt_hip_last_error", C.c_char_p, [_vp])
+_device_name = _sig("gt_hip_device_name", C.c_int, [_vp, C.c_char_p, C.c_size_t])
+_last_gpu_seconds = _sig("gt_hip_last_gpu_seconds", C.c_double, [_vp])
+_compiled_in = _sig("gt_hip_compiled_in", C.c_int, [])
+_direct = _sig("gt_hip_direct", C.c_int, [_vp, _vp, _P(Params), _vp])
+_tree = _sig("gt_hip_tree", C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp, C.c_uint64,
+                                      _vp, _vp, _P(Params), C.c_int, _vp])
+_group_compute = _sig("gt_hip_group_compute", C.c_int,
+                      [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp])
+_walk_tree = _sig("gt_hip_walk_tree", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, C.c_uint64,
+                                                _vp, _vp, _vp, _vp, _vp])
+_last_info = _sig("gt_hip_last_info", C.c_int, [_vp, _P(ListsInfo)])
+_last_timings = _sig("gt_hip_last_timings", C.c_int, [_vp, _P(Timings)])
+_get_tree = _sig("gt_hip_get_tree", C.c_int, [_vp, _vp, _vp, _vp])
+_get_lists = _sig("gt_hip_get_lists", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_state_set = _sig("gt_hip_state_set", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp])
+_state_get = _sig("gt_hip_state_get", C.c_int, [_vp, _vp, _vp, _vp, _vp])
+_prime = _sig("gt_hip_prime", C.c_int, [_vp, _P(GroupConfig)])
+_steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_double, C.c_int])
+_direct_f64_sample = _sig("gt_hip_direct_f64_sample", C.c_int,
+                          [_vp, C.c_uint64, _vp, _vp, C.c_double, C.c_double, C.c_uint64, _vp,
+                           _vp])
+
+
+def compiled_in():
+    return bool(_compiled_in())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
+                pack=-1, exact=False):
+    cfg = GroupConfig()
+    cfg.G, cfg.theta, cfg.eps = G, theta, eps
+    cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
+    cfg.pack, cfg.exact = pack, int(exact)
+    return cfg
+
+
+def make_params(n, n_groups=0, G=1.0, eps=0.05, quadrupole=True):
+    """make_kernel_params (src/gravity_backends.cpp:122-129)."""
+    p = Params()
+    p.n, p.n_groups = n, n_groups
+    p.G = np.float32(G)
+    p.eps2 = np.float32(eps * eps)
+    p.quadrupole = int(quadrupole)
+    return p
+
+
+class Context:
+    """One GPU context (metal::Context's counterpart).  Not thread-safe."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = _create(C.byref(h), device)
+        if rc != OK:
+            msg = _last_error(None).decode()
+            raise RuntimeError(f"hip: {msg or 'no GPU device available'} (status {rc})")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc == OK:
+            return
+        msg = _last_error(self._h).decode()
+        if rc == ERR_ARGUMENT:
+            raise ValueError(msg)
+        if rc == ERR_MEMORY:
+            raise MemoryError(msg)
+        raise RuntimeError(f"{msg} (status {rc})")
+
+    @property
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        self._check(_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    @property
+    def last_gpu_seconds(self):
+        return _last_gpu_seconds(self._h)
+
+    # -- the reference's GPU entry points ------------------------------------------------
+    def direct(self, bodies, G=1.0, eps=0.05):
+        bodies = np.ascontiguousarray(bodies, np.float32)
+        out = np.empty_like(bodies)
+        p = make_params(len(bodies), 0, G, eps)
+        self._check(_direct(self._h, _ptr(bodies), C.byref(p), _ptr(out)))
+        return out
+
+    def tree(self, lists, G=1.0, eps=0.05, quadrupole=True, exact=False, params=None):
+        b = np.ascontiguousarray(lists["bodies"], np.float32)
+        cells = np.ascontiguousarray(lists["cells"], np.float32)
+        cl = np.ascontiguousarray(lists["cell_list"], np.uint32)
+        bl = np.ascontiguousarray(lists["body_list"], np.uint32)
+        groups = np.ascontiguousarray(lists["groups"], np.uint32)
+        p = params if params is not None else make_params(len(b), len(groups), G, eps, quadrupole)
+        out = np.empty((len(b), 4), np.float32)
+        self._check(_tree(self._h, _ptr(b), len(cells), _ptr(cells), len(cl), _ptr(cl), len(bl),
+                          _ptr(bl), _ptr(groups), C.byref(p), int(exact), _ptr(out)))
+        return out
+
+    # -- the whole force evaluation -----------------------------------------------------
+    def group_compute(self, pos, mass, **cfg):
+        n = len(mass)
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        c = make_config(**cfg)
+        self._check(_group_compute(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
+                                   _ptr(pot)))
+        return acc, pot
+
+    def walk_tree(self, geom, links, tree_pos, tree_mass, evaluate=True, **cfg):
+        n = len(tree_mass)
+        geom = _f64(geom)
+        links = np.ascontiguousarray(links, np.int32)
+        tpos = _f64(tree_pos, (n, 3))
+        tmass = _f64(tree_mass)
+        out = np.empty((n, 4), np.float32) if evaluate else None
+        c = make_config(**cfg)
+        self._check(_walk_tree(self._h, C.byref(c), n, len(links), _ptr(geom), _ptr(links),
+                               _ptr(tpos), _ptr(tmass), _ptr(out)))
+        return out
+
+    def info(self):
+        i = ListsInfo()
+        self._check(_last_info(self._h, C.byref(i)))
+        return {k: int(getattr(i, k)) for k, _ in ListsInfo._fields_}
+
+    def timings(self):
+        t = Timings()
+        self._check(_last_timings(self._h, C.byref(t)))
+        return {k: float(getattr(t, k)) for k, _ in Timings._fields_}
+
+    def get_tree(self):
+        inf = self.info()
+        nn, n = inf["n_nodes"], inf["n"]
+        geom = np.empty((nn, 16))
+        links = np.empty((nn, 4), np.int32)
+        order = np.empty(n, np.uint64)
+        self._check(_get_tree(self._h, _ptr(geom), _ptr(links), _ptr(order)))
+        return geom, links, order
+
+    def get_lists(self):
+        inf = self.info()
+        groups = np.empty((inf["n_groups"], 8), np.uint32)
+        cl = np.empty(inf["n_cell_list"], np.uint32)
+        bl = np.empty(inf["n_body_list"], np.uint32)
+        bodies = np.empty((inf["n"], 4), np.float32)
+        cells = np.empty((inf["n_nodes"], 12), np.float32)
+        origin = np.empty(3)
+        self._check(_get_lists(self._h, _ptr(groups), _ptr(cl), _ptr(bl), _ptr(bodies),
+                               _ptr(cells), _ptr(origin)))
+        return dict(groups=groups, cell_list=cl, body_list=bl, bodies=bodies, cells=cells,
+                    origin=origin)
+
+    # -- resident leapfrog ---------------------------------------------------------------
+    def state_set(self, pos, vel, mass):
+        n = len(mass)
+        self._n = n
+        self._check(_state_set(self._h, n, _ptr(_f64(pos, (n, 3))), _ptr(_f64(vel, (n, 3))),
+                               _ptr(_f64(mass))))
+
+    def state_get(self):
+        n = self._n
+        pos, vel, acc = (np.empty((n, 3)) for _ in range(3))
+        pot = np.empty(n)
+        self._check(_state_get(self._h, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(pot)))
+        return pos, vel, acc, pot
+
+    def prime(self, **cfg):
+        c = make_config(**cfg)
+        self._check(_prime(self._h, C.byref(c)))
+
+    def steps(self, nsteps, dt, use_graph=False, **cfg):
+        c = make_config(**cfg)
+        self._check(_steps(self._h, C.byref(c), nsteps, dt, int(use_graph)))
+
+    # -- measurement -------------------------------------------------------------------
+    def direct_f64_sample(self, pos, mass, sample, G=1.0, eps=0.025):
+        n = len(mass)
+        sample = np.ascontiguousarray(sample, np.uint64)
+        acc = np.empty((len(sample), 3))
+        self._check(_direct_f64_sample(self._h, n, _ptr(_f64(pos, (n, 3))), _ptr(_f64(mass)),
+                                       G, eps, len(sample), _ptr(sample), _ptr(acc)))
+        return acc

@@ -1,0 +1,69 @@
+"""Python access to the host ICs generator (galactic-tides_amd/host/gt_ics.h)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgt_host.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run python galactic-tides_amd/build.py")
+_lib = C.CDLL(LIB_PATH)
+
+
+class Model(C.Structure):
+    _fields_ = [("G", C.c_double), ("disk_n", C.c_uint64), ("disk_mass", C.c_double),
+                ("disk_h", C.c_double), ("disk_z0", C.c_double), ("disk_rmax", C.c_double),
+                ("disk_Q", C.c_double), ("disk_Q_radius", C.c_double), ("bulge_n", C.c_uint64),
+                ("bulge_mass", C.c_double), ("bulge_a", C.c_double), ("bulge_rmax", C.c_double),
+                ("halo_n", C.c_uint64), ("halo_mass", C.c_double), ("halo_core", C.c_double),
+                ("halo_cutoff", C.c_double), ("halo_rmax", C.c_double),
+                ("halo_profile", C.c_int), ("halo_a", C.c_double), ("seed", C.c_uint64)]
+
+
+_vp = C.c_void_p
+_lib.gt_ics_default_model.argtypes = [C.POINTER(Model)]
+_lib.gt_ics_galaxy.restype = C.c_uint64
+_lib.gt_ics_galaxy.argtypes = [C.POINTER(Model), C.c_double, C.c_double, _vp, _vp, _vp, _vp]
+_lib.gt_ics_encounter.restype = C.c_uint64
+_lib.gt_ics_encounter.argtypes = [C.POINTER(Model)] + [C.c_double] * 6 + [_vp] * 4
+
+
+def model(**kw):
+    """HernquistModel defaults (hernquist_ics.h:52-82) with overrides; the halo
+    profile is 'reference' (cored, cut off) or 'hernquist' (scale halo_a)."""
+    m = Model()
+    _lib.gt_ics_default_model(C.byref(m))
+    prof = kw.pop("halo_profile", "reference")
+    m.halo_profile = {"reference": 0, "hernquist": 1}[prof]
+    for k, v in kw.items():
+        if not hasattr(m, k):
+            raise KeyError(k)
+        setattr(m, k, v)
+    return m
+
+
+def _arrays(n):
+    return np.empty((n, 3)), np.empty((n, 3)), np.empty(n), np.empty(n, np.uint8)
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+def galaxy(m, incl=0.0, omega=0.0):
+    n = int(m.disk_n + m.bulge_n + m.halo_n)
+    pos, vel, mass, typ = _arrays(n)
+    got = _lib.gt_ics_galaxy(C.byref(m), incl, omega, _p(pos), _p(vel), _p(mass), _p(typ))
+    assert got == n
+    return pos, vel, mass, typ
+
+
+def encounter(m, g1_incl=0.0, g1_omega=0.0, g2_incl=60.0, g2_omega=30.0, rp=2.0, r0=30.0):
+    n = 2 * int(m.disk_n + m.bulge_n + m.halo_n)
+    pos, vel, mass, typ = _arrays(n)
+    got = _lib.gt_ics_encounter(C.byref(m), g1_incl, g1_omega, g2_incl, g2_omega, rp, r0,
+                                _p(pos), _p(vel), _p(mass), _p(typ))
+    if got != n:
+        raise ValueError("encounter: start separation below pericentre")
+    return pos, vel, mass, typ

@@ -1,0 +1,172 @@
+/* gt_hip.h: C ABI of libgt_hip.so, the MI355X (gfx950) gravity backend for
+ * galactic-tides' grouped Barnes-Hut path (device = hip).
+ *
+ * Plain C types only; every entry point returns an int status (GT_HIP_OK or
+ * an error code) and never lets a C++ exception cross the boundary.  The
+ * message for the last failure on a context is gt_hip_last_error(ctx).
+ * A context is not re-entrant: use one per host thread.  All calls are
+ * synchronous: when a call returns, its outputs are in the caller's memory.
+ *
+ * Structs keep the byte layouts of the reference's kernel structs
+ * (include/galactic_tides/kernels/gravity_kernels.h:36-38,51-60,87-105):
+ * gt_float4 16 B, gt_params 32 B, gt_cell 48 B, gt_group 32 B.
+ */
+#ifndef GT_HIP_H
+#define GT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_HIP_ABI_VERSION 1
+
+enum {
+  GT_HIP_OK = 0,
+  GT_HIP_ERR_ARGUMENT = 1,    /* invalid argument (reference: std::invalid_argument) */
+  GT_HIP_ERR_DEVICE = 2,      /* no usable GPU, or a HIP call failed (std::runtime_error) */
+  GT_HIP_ERR_MEMORY = 3,      /* device allocation failed */
+  GT_HIP_ERR_LIMIT = 4,       /* too many particles or list entries for 32-bit indices */
+  GT_HIP_ERR_UNSUPPORTED = 5  /* input outside what this build handles (message says why) */
+};
+
+typedef struct gt_hip_context gt_hip_context;
+
+/* kern::float4 (gravity_kernels.h:36-38): x, y, z and mass (or ax, ay, az, pot). */
+typedef struct {
+  float x, y, z, w;
+} gt_float4;
+
+/* kern::GTParams (gravity_kernels.h:51-60). */
+typedef struct {
+  uint32_t n, n_groups;
+  float G, eps2;
+  uint32_t quadrupole, pad0, pad1, pad2;
+} gt_params;
+
+/* kern::GTCell (gravity_kernels.h:87-91). */
+typedef struct {
+  gt_float4 com_mass, q0, q1;
+} gt_cell;
+
+/* kern::GTGroup (gravity_kernels.h:96-105). */
+typedef struct {
+  uint32_t begin, end, cell_begin, cell_end, body_begin, body_end, pad0, pad1;
+} gt_group;
+
+/* Solver settings: GravityParams (include/galactic_tides/octree.h:13-18) plus
+ * the GroupTreeGravity constructor arguments (gravity_backends.h:62-63). */
+typedef struct {
+  double G;          /* gravitational constant */
+  double theta;      /* opening angle */
+  double eps;        /* Plummer softening length */
+  int quadrupole;    /* 1: cells carry quadrupole moments */
+  int group_size;    /* targets per group, 1..64 */
+  int leaf_size;     /* bodies per leaf (>= 1) */
+  int pack;          /* merge small siblings into groups; < 0: device default (on) */
+  int exact;         /* 1: IEEE 1/sqrt, no contraction, reference summation order:
+                        forces bit-identical to evaluate_group_lists_cpu */
+  int reserved[7];   /* zero */
+} gt_group_config;
+
+/* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
+typedef struct {
+  uint64_t n, n_nodes, depth, n_groups, n_cell_list, n_body_list;
+  uint64_t cell_interactions, body_interactions, max_list, max_group;
+} gt_lists_info;
+
+/* GroupTreeGravity::Timings (gravity_backends.h:54-58), in milliseconds,
+ * measured with HIP events on the context's stream. */
+typedef struct {
+  double tree, lists, evaluate, scatter, total;
+} gt_timings;
+
+/* ---- context (replaces metal::Context, include/galactic_tides/metal_backend.h:23-54) ---- */
+
+/* metal::compiled_in() (metal_backend.h:23): 1 when built with HIP kernels. */
+int gt_hip_compiled_in(void);
+/* metal::Context::Context() (metal_backend.h:27): fails with GT_HIP_ERR_DEVICE
+ * when no GPU is visible. */
+int gt_hip_create(gt_hip_context **ctx, int device);
+void gt_hip_destroy(gt_hip_context *ctx);
+/* Message for the last failing call on ctx ("" if none). */
+const char *gt_hip_last_error(const gt_hip_context *ctx);
+/* metal::Context::device_name() (metal_backend.h:32). */
+int gt_hip_device_name(const gt_hip_context *ctx, char *buf, size_t len);
+/* metal::Context::last_gpu_seconds() (metal_backend.h:47): kernel time of the last call. */
+double gt_hip_last_gpu_seconds(const gt_hip_context *ctx);
+
+/* ---- the reference's two GPU entry points, host arrays in and out ---- */
+
+/* metal::Context::direct (metal_backend.h:35-36, kernel gt_direct,
+ * src/metal/gravity_kernels.metal:18-43): out[i] = (ax, ay, az, pot) of body i
+ * against all others; p->n bodies. */
+int gt_hip_direct(gt_hip_context *ctx, const gt_float4 *bodies, const gt_params *p,
+                  gt_float4 *out);
+/* metal::Context::tree (metal_backend.h:41-44, kernel gt_tree,
+ * src/metal/gravity_kernels.metal:53-112): evaluates grouped interaction lists,
+ * out[i] in tree order.  p->n must equal the number of bodies, p->n_groups the
+ * number of groups; groups may hold at most 64 targets.  exact = 1 gives
+ * results bit-identical to evaluate_group_lists_cpu (gravity_backends.cpp:167). */
+int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
+                const gt_cell *cells, uint64_t n_cell_list, const uint32_t *cell_list,
+                uint64_t n_body_list, const uint32_t *body_list, const gt_group *groups,
+                const gt_params *p, int exact, gt_float4 *out);
+
+/* ---- the whole force evaluation on the GPU ---- */
+
+/* GroupTreeGravity::compute (src/gravity_backends.cpp:206-234) with device = hip:
+ * octree build, group selection, grouped walk, list evaluation and the scatter
+ * back to particle order all run on the GPU.  pos/acc are n x 3 doubles. */
+int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
+                         const double *pos, const double *mass, double *acc, double *pot);
+/* Walk and evaluate on a tree supplied by the caller (the reference's node
+ * layout: geom n_nodes x 16 doubles = center xyz, half, com xyz, mass,
+ * quad xx yy zz xy xz yz, delta, 0; links n_nodes x 4 int32 = first_child,
+ * n_children, begin, end; tree_pos/tree_mass in tree order).  Lists use the
+ * caller's node numbering, so they compare raw with build_group_lists
+ * (src/group_lists.cpp:121) on the same tree.  out (n x 4, tree order) may be NULL. */
+int gt_hip_walk_tree(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
+                     uint64_t n_nodes, const double *geom, const int32_t *links,
+                     const double *tree_pos, const double *tree_mass, gt_float4 *out);
+/* Statistics and phase timings of the last gt_hip_group_compute / walk / step. */
+int gt_hip_last_info(const gt_hip_context *ctx, gt_lists_info *info);
+int gt_hip_last_timings(const gt_hip_context *ctx, gt_timings *t);
+/* Octree accessors (Octree::nodes()/order(), octree.h:77-79) for the last GPU
+ * build: geom/links as above (n_nodes rows, level order), order n entries. */
+int gt_hip_get_tree(gt_hip_context *ctx, double *geom, int32_t *links, uint64_t *order);
+/* GroupLists accessor (GroupTreeGravity::lists(), gravity_backends.h:68) for
+ * the last walk: n_groups groups, the two lists, bodies (n), cells (n_nodes)
+ * and the origin (3 doubles).  Any pointer may be NULL. */
+int gt_hip_get_lists(gt_hip_context *ctx, gt_group *groups, uint32_t *cell_list,
+                     uint32_t *body_list, gt_float4 *bodies, gt_cell *cells, double *origin);
+
+/* ---- device-resident leapfrog (LeapfrogKDK, include/galactic_tides/integrator.h:101-109) ---- */
+
+/* Upload particles (n x 3 pos/vel, n mass) to device memory kept by ctx. */
+int gt_hip_state_set(gt_hip_context *ctx, uint64_t n, const double *pos, const double *vel,
+                     const double *mass);
+/* Download; any pointer may be NULL. */
+int gt_hip_state_get(gt_hip_context *ctx, double *pos, double *vel, double *acc, double *pot);
+/* LeapfrogKDK::prime (integrator.h:104): forces for the resident state. */
+int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
+/* nsteps x LeapfrogKDK::step (src/integrator.cpp:5-16) on the resident state:
+ * FP64 kick and drift kernels around each GPU force evaluation.  use_graph = 1
+ * captures the step once as a hipGraph and replays it. */
+int gt_hip_steps(gt_hip_context *ctx, const gt_group_config *cfg, int nsteps, double dt,
+                 int use_graph);
+
+/* ---- measurement helpers ---- */
+
+/* FP64 direct summation for a sample of targets (apps/galactic_tides_bench.cpp:80-95):
+ * acc is ns x 3.  Used to measure force errors on the GPU box. */
+int gt_hip_direct_f64_sample(gt_hip_context *ctx, uint64_t n, const double *pos,
+                             const double *mass, double G, double eps, uint64_t ns,
+                             const uint64_t *sample, double *acc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
