@@ -42,8 +42,17 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host library: ICs generator (plain C++, no GPU code)
+    host = os.path.join(HERE, "libgt_host.so")
+    cmd = ["g++", "-std=c++17", "-O3", "-fPIC", "-shared", "-ffp-contract=off",
+           "-I", os.path.join(HERE, "host"), "-o", host + ".tmp",
+           os.path.join(HERE, "host", "gt_ics.cpp")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host library build failed:\n{r.stderr}")
+    os.replace(host + ".tmp", host)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", host)
     return LIB
 
 

@@ -814,6 +814,8 @@ int gto_time_group(gto_run *r, uint64_t n, const double *pos, const double *mass
       best4[1] = sec(t0, t1);
       best4[2] = sec(t1, t2);
       best4[3] = sec(t2, t3);
+      best4[4] = static_cast<double>(r->lists.cell_int);
+      best4[5] = static_cast<double>(r->lists.body_int);
     }
   }
   return 0;

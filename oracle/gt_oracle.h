@@ -70,8 +70,9 @@ int gto_group_gravity(gto_run *r, uint64_t n, const double *pos, const double *m
 int gto_kdk_group(gto_run *r, uint64_t n, double *pos, double *vel, const double *mass,
                   double G, double theta, double eps, int quadrupole, int group_size,
                   int leaf_size, int pack, int nsteps, double dt, double *acc, double *pot);
-/* Wall time of `repeat` GroupTreeGravity::compute calls after one warm-up: best,
- * and the tree / lists / evaluate split of the best (seconds). */
+/* Wall time of `repeat` GroupTreeGravity::compute calls after one warm-up: best4[0]
+ * the best, [1..3] its tree / lists / evaluate split (seconds), [4..5] its cell and
+ * body interactions (best4 holds 6 doubles). */
 int gto_time_group(gto_run *r, uint64_t n, const double *pos, const double *mass, double G,
                    double theta, double eps, int quadrupole, int group_size, int leaf_size,
                    int pack, int repeat, double *best4);

@@ -208,13 +208,14 @@ class Checker:
 
     def time_group(self, pos, mass, G=1.0, theta=0.75, eps=0.025, quadrupole=True,
                    group_size=32, leaf_size=8, pack=True, repeat=3):
-        best = np.zeros(4)
+        best = np.zeros(6)
         rc = self._time_group(self.h, len(mass), np.ascontiguousarray(pos, np.float64),
                               np.ascontiguousarray(mass, np.float64), G, theta, eps,
                               int(quadrupole), group_size, leaf_size, int(pack), repeat, best)
         if rc:
             raise RuntimeError("timing failed")
-        return dict(total=best[0], tree=best[1], lists=best[2], evaluate=best[3])
+        return dict(total=best[0], tree=best[1], lists=best[2], evaluate=best[3],
+                    interactions=best[4] + best[5])
 
     def max_threads(self):
         return int(self._max_threads())

@@ -307,6 +307,8 @@ int gtref_time_group(gtref_run *, uint64_t n, const double *pos, const double *m
         best4[1] = g.timings().tree;
         best4[2] = g.timings().lists;
         best4[3] = g.timings().evaluate;
+        best4[4] = static_cast<double>(g.stats().cell_interactions);
+        best4[5] = static_cast<double>(g.stats().particle_interactions);
       }
     }
   } catch (...) {
