@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--halo", default="hernquist", choices=["hernquist", "reference"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--exact", action="store_true", help="bit-exact evaluation mode")
+    ap.add_argument("--eval-waves", type=int, default=0, help="tuning: 2 or 3 (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -70,7 +71,7 @@ def main():
 
     import gt_hip
     cfg = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
-               pack=1, exact=args.exact)
+               pack=1, exact=args.exact, eval_waves=args.eval_waves)
     t0 = time.time()
     pos, vel, mass, _ = encounter_2m(args.halo)
     n = len(mass)
@@ -128,8 +129,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        ccfg = dict(cfg)
-        kind, cores, best, wall = cpu_baseline(pos, mass, ccfg)
+        kind, cores, best, wall = cpu_baseline(pos, mass, cfg)
         cpu = {"value": round(best["interactions"] / best["total"], 1), "unit": "interactions/s",
                "cores": cores, "kind": kind,
                "ms_per_force_eval": round(best["total"] * 1e3, 1),

@@ -27,11 +27,16 @@ struct gt_hip_context {
   DBuf<double> up_geom, up_tpos, up_tmass;
   DBuf<int4> up_lnk;
   // lists of the last walk
-  DBuf<WalkNode> wn;
+  DBuf<WalkRec> recs;
+  DBuf<uint2> span;
   DBuf<Cell> cells;
   DBuf<float4> bodies, out;
-  DBuf<uint32_t> parent, skip, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
+  DBuf<uint32_t> parent, skip, depth, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
       body_list;
+  // per-group list slots of the one-pass walk, from the previous evaluation's
+  // longest lists (0 before the first)
+  uint32_t cap_cells = 0, cap_bodies = 0;
+  uint64_t walk_passes = 0, walk_fallbacks = 0;
   DBuf<gt_group> groups;
   DBuf<unsigned long long> scan_tmp, misc;
   uint32_t lists_n = 0, lists_nn = 0;
@@ -106,7 +111,10 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Group selection, walk and list statistics for a tree held in device memory.
-// Leaves groups, lists, bodies and cells in the context.
+// Leaves groups, lists, bodies and cells in the context.  The walk writes each
+// group's lists into slots sized from the previous evaluation (one pass); if a
+// group's lists do not fit, its counts say so and the lists are rewritten at
+// exact scanned offsets (the same walk again, so the contents are the same).
 void run_lists(gt_hip_context *c, const gt_group_config *cfg, const double *geom,
                const int4 *lnk, const double *tpos, const double *tmass, uint32_t n,
                uint32_t nn) {
@@ -118,59 +126,76 @@ void run_lists(gt_hip_context *c, const gt_group_config *cfg, const double *geom
   c->info.n = n;
   c->info.n_nodes = nn;
   if (n == 0) return;
-  WalkNode *wn = c->wn.ensure(nn);
+  if (nn >= kRecIndexMask)
+    fail(GT_HIP_ERR_LIMIT, "too many tree nodes for the packed walk records (2^26 - 1)");
+  unsigned long long *misc = c->misc.ensure(16);  // [0..2] totals, [3] error, [4..11] stats
+  launch_fill_u64(misc, 16, 0ull, s);
+  uint32_t *werr = reinterpret_cast<uint32_t *>(misc + 3);
+  uint32_t *parent = c->parent.ensure(nn), *skip = c->skip.ensure(nn), *depth = c->depth.ensure(nn);
+  launch_link_up(lnk, nn, parent, skip, depth, s);
+  WalkRec *recs = c->recs.ensure(nn);
+  uint2 *span = c->span.ensure(nn);
   Cell *cells = c->cells.ensure(nn);
   float4 *bodies = c->bodies.ensure(n);
-  launch_prep_lists(geom, tpos, tmass, n, nn, cfg->theta, wn, cells, bodies, s);
-  uint32_t *parent = c->parent.ensure(nn), *skip = c->skip.ensure(nn);
-  launch_link_up(lnk, nn, parent, skip, s);
+  launch_prep_lists(geom, lnk, skip, depth, tpos, tmass, n, nn, cfg->theta, recs, span, cells,
+                    bodies, werr, s);
 
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
   uint32_t *flags = c->flags.ensure(n), *offs = c->offs.ensure(n);
-  unsigned long long *misc = c->misc.ensure(8);  // [0..2] totals, [3] error flag, [4..7] stats
-  launch_fill_u64(misc, 8, 0ull, s);
-  uint32_t *werr = reinterpret_cast<uint32_t *>(misc + 3);
   unsigned long long *stmp = c->scan_tmp.ensure(scan_tmp_words(n) + 8);
   launch_group_starts(lnk, nn, n, static_cast<uint32_t>(cfg->group_size), pack, flags, s);
   exclusive_scan_u32(flags, offs, n, misc, stmp, s);
-  unsigned long long hm[3];
-  download(hm, misc, 1, s);
+  unsigned long long hm[12];
+  download(hm, misc, 4, s);
   GTH_CHECK(hipStreamSynchronize(s));
+  if (hm[3] & 2u) fail(GT_HIP_ERR_ARGUMENT, "tree: a node's half size is not root half / 2^depth");
   const uint32_t ng = static_cast<uint32_t>(hm[0]);
   gt_group *groups = c->groups.ensure(ng);
   uint32_t *starts = c->starts.ensure(ng);
   launch_make_groups(flags, offs, n, ng, starts, groups, s);
 
   uint32_t *cc = c->ccount.ensure(ng), *bc = c->bcount.ensure(ng);
-  uint32_t *co = c->coff.ensure(ng), *bo = c->boff.ensure(ng);
-  launch_walk(false, wn, lnk, skip, tpos, groups, ng, nullptr, nullptr, cc, bc, nullptr, nullptr,
-              nn, werr, s);
-  stmp = c->scan_tmp.ensure(scan_tmp_words(ng) + 8);
-  exclusive_scan_u32(cc, co, ng, misc + 1, stmp, s);
-  exclusive_scan_u32(bc, bo, ng, misc + 2, stmp, s);
-  unsigned long long hm4[4];
-  download(hm4, misc, 4, s);
+  uint32_t cap_c = c->cap_cells, cap_b = c->cap_bodies;
+  if (static_cast<uint64_t>(ng) * cap_c > 0xffffffffull ||
+      static_cast<uint64_t>(ng) * cap_b > 0xffffffffull)
+    cap_c = cap_b = 0;
+  uint32_t *cl = c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(ng) * cap_c, 1));
+  uint32_t *bl = c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(ng) * cap_b, 1));
+  const int lanes = cfg->walk_lanes == 16 || cfg->walk_lanes == 32 || cfg->walk_lanes == 64
+                        ? cfg->walk_lanes
+                        : 1;
+  launch_walk(kWalkCapacity, lanes, recs, span, tpos, groups, ng, nn, geom, cap_c, cap_b, nullptr,
+              nullptr, cc, bc, cl, bl, werr, s);
+  launch_list_stats(cc, bc, groups, ng, misc + 4, s);
+  download(hm, misc, 12, s);
   GTH_CHECK(hipStreamSynchronize(s));
-  hm[1] = hm4[1];
-  hm[2] = hm4[2];
-  if (hm4[3]) fail(GT_HIP_ERR_DEVICE, "group walk: malformed tree (walk did not terminate)");
-  if (hm[1] > 0xffffffffull || hm[2] > 0xffffffffull)
+  ++c->walk_passes;
+  if (hm[3] & 1u) fail(GT_HIP_ERR_DEVICE, "group walk: malformed tree (walk did not terminate)");
+  const unsigned long long most_c = hm[8], most_b = hm[9], all_c = hm[10], all_b = hm[11];
+  if (all_c > 0xffffffffull || all_b > 0xffffffffull)
     fail(GT_HIP_ERR_LIMIT, "interaction lists too long for 32 bit offsets");
-  uint32_t *cl = c->cell_list.ensure(std::max<unsigned long long>(hm[1], 1));
-  uint32_t *bl = c->body_list.ensure(std::max<unsigned long long>(hm[2], 1));
-  launch_walk(true, wn, lnk, skip, tpos, groups, ng, co, bo, nullptr, nullptr, cl, bl, nn, werr,
-              s);
-  launch_list_stats(groups, ng, misc + 4, s);
-  unsigned long long st[4];
-  download(st, misc + 4, 4, s);
-  GTH_CHECK(hipStreamSynchronize(s));
+  if (most_c > cap_c || most_b > cap_b) {
+    // Some group did not fit its slots: place every group at scanned offsets.
+    ++c->walk_fallbacks;
+    uint32_t *co = c->coff.ensure(ng), *bo = c->boff.ensure(ng);
+    stmp = c->scan_tmp.ensure(scan_tmp_words(ng) + 8);
+    exclusive_scan_u32(cc, co, ng, misc + 1, stmp, s);
+    exclusive_scan_u32(bc, bo, ng, misc + 2, stmp, s);
+    cl = c->cell_list.ensure(std::max<unsigned long long>(all_c, 1));
+    bl = c->body_list.ensure(std::max<unsigned long long>(all_b, 1));
+    launch_walk(kWalkExact, lanes, recs, span, tpos, groups, ng, nn, geom, 0, 0, co, bo, cc, bc, cl,
+                bl, werr, s);
+  }
+  // Slots for the next evaluation: a quarter more than this one's longest lists.
+  c->cap_cells = static_cast<uint32_t>(std::min<unsigned long long>(most_c + most_c / 4 + 32, 1u << 30));
+  c->cap_bodies = static_cast<uint32_t>(std::min<unsigned long long>(most_b + most_b / 4 + 32, 1u << 30));
   c->info.n_groups = ng;
-  c->info.n_cell_list = hm[1];
-  c->info.n_body_list = hm[2];
-  c->info.cell_interactions = st[0];
-  c->info.body_interactions = st[1];
-  c->info.max_list = st[2];
-  c->info.max_group = st[3];
+  c->info.n_cell_list = all_c;
+  c->info.n_body_list = all_b;
+  c->info.cell_interactions = hm[4];
+  c->info.body_interactions = hm[5];
+  c->info.max_list = hm[6];
+  c->info.max_group = hm[7];
 }
 
 void run_eval(gt_hip_context *c, const gt_group_config *cfg) {
@@ -179,7 +204,8 @@ void run_eval(gt_hip_context *c, const gt_group_config *cfg) {
   const float eps2 = static_cast<float>(cfg->eps * cfg->eps);
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), static_cast<uint32_t>(c->info.n_groups), G, eps2,
-              cfg->quadrupole ? 1u : 0u, cfg->exact != 0, out, c->stream);
+              cfg->quadrupole ? 1u : 0u, cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, out,
+              c->stream);
 }
 
 // One force evaluation of particles already in device memory: build, lists,
@@ -328,7 +354,7 @@ int gt_hip_tree(gt_hip_context *c, const gt_float4 *bodies, uint64_t n_cells,
     upload(dbl, body_list, n_body_list, s);
     upload(dg, groups, ng, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
-    launch_eval(db, dc, dcl, dbl, dg, ng, p->G, p->eps2, p->quadrupole, exact != 0, dout, s);
+    launch_eval(db, dc, dcl, dbl, dg, ng, p->G, p->eps2, p->quadrupole, exact != 0, 2, dout, s);
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     download(reinterpret_cast<float4 *>(out), dout, n, s);
     GTH_CHECK(hipStreamSynchronize(s));
@@ -451,13 +477,40 @@ int gt_hip_get_lists(gt_hip_context *c, gt_group *groups, uint32_t *cell_list,
     hipStream_t s = c->stream;
     const gt_lists_info &I = c->info;
     if (c->lists_n == 0) return;
-    if (groups) download(groups, c->groups.get(), I.n_groups, s);
-    if (cell_list) download(cell_list, c->cell_list.get(), I.n_cell_list, s);
-    if (body_list) download(body_list, c->body_list.get(), I.n_body_list, s);
+    // The device lists may sit in per-group slots with gaps between groups:
+    // hand them out packed, as build_group_lists lays them out.
+    const uint32_t ng = static_cast<uint32_t>(I.n_groups);
+    std::vector<gt_group> g(ng);
+    download(g.data(), c->groups.get(), ng, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+    uint64_t cext = 0, bext = 0;
+    for (const auto &G : g) {
+      cext = std::max<uint64_t>(cext, G.cell_end);
+      bext = std::max<uint64_t>(bext, G.body_end);
+    }
+    std::vector<uint32_t> cl(cell_list ? cext : 0), bl(body_list ? bext : 0);
+    if (cell_list) download(cl.data(), c->cell_list.get(), cext, s);
+    if (body_list) download(bl.data(), c->body_list.get(), bext, s);
     if (bodies) download(reinterpret_cast<float4 *>(bodies), c->bodies.get(), c->lists_n, s);
     if (cells) download(reinterpret_cast<Cell *>(cells), c->cells.get(), c->lists_nn, s);
     if (origin) download(origin, c->lists_geom, 3, s);
     GTH_CHECK(hipStreamSynchronize(s));
+    uint32_t nc = 0, nb = 0;
+    for (uint32_t k = 0; k < ng; ++k) {
+      gt_group G = g[k];
+      if (cell_list)
+        std::copy(cl.begin() + G.cell_begin, cl.begin() + G.cell_end, cell_list + nc);
+      if (body_list)
+        std::copy(bl.begin() + G.body_begin, bl.begin() + G.body_end, body_list + nb);
+      const uint32_t c_len = G.cell_end - G.cell_begin, b_len = G.body_end - G.body_begin;
+      G.cell_begin = nc;
+      G.cell_end = nc + c_len;
+      G.body_begin = nb;
+      G.body_end = nb + b_len;
+      nc += c_len;
+      nb += b_len;
+      if (groups) groups[k] = G;
+    }
   });
 }
 

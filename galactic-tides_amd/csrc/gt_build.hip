@@ -75,27 +75,42 @@ __global__ __launch_bounds__(kBlock) void bbox_partial(const double *__restrict_
 }
 
 // root[0..2] centre, root[3] half (src/octree.cpp:324-332).
-__global__ void bbox_final(const double *__restrict__ part, uint32_t nparts,
-                           double *__restrict__ root) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kBlock) void bbox_final(const double *__restrict__ part,
+                                                     uint32_t nparts, double *__restrict__ root) {
+  __shared__ double r[6][kBlock];
   double lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
     lo[a] = part[a];
     hi[a] = part[3 + a];
   }
-  for (uint32_t b = 1; b < nparts; ++b)
+  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock)
     for (int a = 0; a < 3; ++a) {
       const double l = part[6 * b + a], h = part[6 * b + 3 + a];
       lo[a] = l < lo[a] ? l : lo[a];
       hi[a] = hi[a] < h ? h : hi[a];
     }
+  for (int a = 0; a < 3; ++a) {
+    r[a][threadIdx.x] = lo[a];
+    r[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(w))
+      for (int a = 0; a < 3; ++a) {
+        const double l = r[a][threadIdx.x + w], h = r[3 + a][threadIdx.x + w];
+        if (l < r[a][threadIdx.x]) r[a][threadIdx.x] = l;
+        if (r[3 + a][threadIdx.x] < h) r[3 + a][threadIdx.x] = h;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   double half = 0.0;
   for (int a = 0; a < 3; ++a) {
-    const double e = 0.5 * (hi[a] - lo[a]);
+    const double e = 0.5 * (r[3 + a][0] - r[a][0]);
     half = half < e ? e : half;
   }
   half = half * (1.0 + 1e-9) + 1e-300;
-  for (int a = 0; a < 3; ++a) root[a] = 0.5 * (lo[a] + hi[a]);
+  for (int a = 0; a < 3; ++a) root[a] = 0.5 * (r[a][0] + r[3 + a][0]);
   root[3] = half;
 }
 
@@ -416,7 +431,7 @@ void TreeBuilder::build(const double *pos, const double *mass, uint32_t n, int l
   root_.ensure(4);
   hipLaunchKernelGGL(bbox_partial, dim3(nparts), dim3(kBlock), 0, s, pos, n, part);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, part, nparts, root_.get());
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, root_.get());
   GTH_LAUNCH_CHECK();
 
   // 2-3. octant paths, sorted

@@ -16,8 +16,10 @@
 //    keeping per-target partial sums in registers; a cross-lane reduction
 //    ends the group.  Lanes are busy whenever the list is longer than 64
 //    entries, whatever the group's fill, and the targets' coordinates are
-//    scalar operands.  v_rsq_f32 and FMA contraction; G is folded into the
-//    entry's mass and quadrupole once per entry.
+//    read from LDS as broadcasts.  Group sizes are bucketed (8/16/24/32
+//    targets per batch) so each path is branch-free over its targets.
+//    v_rsq_f32 and FMA chains; G is folded into the entry's mass and
+//    quadrupole once per entry; per-target sums are reduced with DPP.
 #include "gt_internal.h"
 
 namespace gth {
@@ -120,125 +122,203 @@ __global__ __launch_bounds__(kBlock) void eval_exact(
 
 // ---- fast: entries across lanes, targets uniform ------------------------------------
 
-__device__ inline float wave_sum(float v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+// Sum over the 64 lanes with DPP moves (no LDS traffic): pairs, quads, half
+// rows and rows by permutes within a row, then rows combined with the row
+// broadcasts; the total ends in lane 63.  Fixed order, so deterministic.
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_add(float v) {
+  const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+  return v + __int_as_float(moved);
+}
+
+__device__ inline float wave_total(float v) {
+  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xf>(v);  // row_mirror
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Position used for lanes past the end of a list chunk: far enough that the
 // zero-mass entry contributes exactly zero without overflowing r^2.
 constexpr float kFar = 1.0e18f;
 
-template <int TB>
-__global__ __launch_bounds__(kBlock) void eval_fast(
+// A list entry as one lane holds it: position, -G m and (cells) G Q.
+struct Entry {
+  float x, y, z, ngm;
+  float q0, q1, q2, q3, q4, q5;
+};
+
+template <bool QUAD>
+__device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, bool valid, float G) {
+  Entry e = {kFar, kFar, kFar, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (valid) {
+    const float4 cm = cells[idx].cm;
+    e.x = cm.x;
+    e.y = cm.y;
+    e.z = cm.z;
+    e.ngm = -G * cm.w;
+    if (QUAD) {
+      const float4 a = cells[idx].q0;
+      const float2 b = *reinterpret_cast<const float2 *>(&cells[idx].q1);
+      e.q0 = G * a.x;
+      e.q1 = G * a.y;
+      e.q2 = G * a.z;
+      e.q3 = G * a.w;
+      e.q4 = G * b.x;
+      e.q5 = G * b.y;
+    }
+  }
+  return e;
+}
+
+// Keeps the compiler from hoisting the LDS target reads out of the chunk
+// loops (which would hold every target's coordinates in registers): the
+// offset is zero, but only known to be so at run time.
+__device__ inline uint32_t opaque_zero() {
+  uint32_t z = 0;
+  asm volatile("" : "+s"(z));
+  return z;
+}
+
+// One batch of TB targets (the group's targets tb .. tb+nt-1, nt <= TB) against
+// the whole list.  Targets sit in LDS and are read as broadcasts; lane k holds
+// list entry base + k.  Targets past nt repeat the last one and are not stored.
+// The list gathers are software-pipelined: the index of chunk c+2 and the
+// entry of chunk c+1 are in flight while chunk c is evaluated.
+template <int TB, bool QUAD>
+__device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
+                                  const uint32_t *__restrict__ cell_list,
+                                  const uint32_t *__restrict__ body_list, const gt_group &g,
+                                  uint32_t tb, uint32_t nt, float G, float eps2, float4 *tgt,
+                                  uint32_t lane, float4 *__restrict__ out) {
+  if (lane < TB) tgt[lane] = bodies[tb + min(lane, nt - 1)];
+  __builtin_amdgcn_wave_barrier();
+  float ax[TB], ay[TB], az[TB], ap[TB];
+#pragma unroll
+  for (int t = 0; t < TB; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+
+  {
+    const uint32_t cb = g.cell_begin, ce = g.cell_end;
+    uint32_t k = cb + lane;
+    uint32_t idx = k < ce ? cell_list[k] : 0u;
+    uint32_t idx_next = k + kWave < ce ? cell_list[k + kWave] : 0u;
+    Entry e = load_cell<QUAD>(cells, idx, k < ce, G);
+    for (uint32_t base = cb; base < ce; base += kWave) {
+      const uint32_t kn = base + kWave + lane;
+      const Entry en = load_cell<QUAD>(cells, idx_next, kn < ce, G);
+      const uint32_t k2 = kn + kWave;
+      idx_next = k2 < ce ? cell_list[k2] : 0u;
+      const uint32_t z = opaque_zero();
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const float4 p = tgt[t + z];
+        const float dx = p.x - e.x, dy = p.y - e.y, dz = p.z - e.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float inv3 = inv * inv2;
+        const float m3 = e.ngm * inv3;  // -G m / r^3
+        if (QUAD) {
+          const float qx = fmaf(e.q0, dx, fmaf(e.q3, dy, e.q4 * dz));
+          const float qy = fmaf(e.q3, dx, fmaf(e.q1, dy, e.q5 * dz));
+          const float qz = fmaf(e.q4, dx, fmaf(e.q5, dy, e.q2 * dz));
+          const float rqr = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+          const float inv5 = inv3 * inv2;
+          const float h = rqr * inv5;
+          const float s = fmaf(-2.5f * h, inv2, m3);
+          ax[t] = fmaf(s, dx, fmaf(qx, inv5, ax[t]));
+          ay[t] = fmaf(s, dy, fmaf(qy, inv5, ay[t]));
+          az[t] = fmaf(s, dz, fmaf(qz, inv5, az[t]));
+          ap[t] = fmaf(-0.5f, h, fmaf(e.ngm, inv, ap[t]));
+        } else {
+          ax[t] = fmaf(m3, dx, ax[t]);
+          ay[t] = fmaf(m3, dy, ay[t]);
+          az[t] = fmaf(m3, dz, az[t]);
+          ap[t] = fmaf(e.ngm, inv, ap[t]);
+        }
+      }
+      e = en;
+    }
+  }
+
+  {
+    const float inf = __builtin_inff();
+    const uint32_t bb = g.body_begin, be = g.body_end;
+    uint32_t k = bb + lane;
+    uint32_t j = k < be ? body_list[k] : kNone;
+    uint32_t j_next = k + kWave < be ? body_list[k + kWave] : kNone;
+    float4 b = j != kNone ? bodies[j] : make_float4(kFar, kFar, kFar, 0.0f);
+    for (uint32_t base = bb; base < be; base += kWave) {
+      const float4 bn = j_next != kNone ? bodies[j_next] : make_float4(kFar, kFar, kFar, 0.0f);
+      const uint32_t k2 = base + 2 * kWave + lane;
+      const uint32_t j2 = k2 < be ? body_list[k2] : kNone;
+      const float ngm = -G * b.w;
+      const uint32_t z = opaque_zero();
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const float4 p = tgt[t + z];
+        const float dx = p.x - b.x, dy = p.y - b.y, dz = p.z - b.z;
+        float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        r2 = (j == tb + t) ? inf : r2;  // a body never feels itself
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float gi = ngm * inv;
+        const float g3 = gi * inv * inv;
+        ax[t] = fmaf(g3, dx, ax[t]);
+        ay[t] = fmaf(g3, dy, ay[t]);
+        az[t] = fmaf(g3, dz, az[t]);
+        ap[t] += gi;
+      }
+      b = bn;
+      j = j_next;
+      j_next = j2;
+    }
+  }
+
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f, rp = 0.0f;
+#pragma unroll
+  for (int t = 0; t < TB; ++t) {
+    const float sx = wave_total(ax[t]), sy = wave_total(ay[t]), sz = wave_total(az[t]),
+                sp = wave_total(ap[t]);
+    if (lane == static_cast<uint32_t>(t)) {
+      rx = sx;
+      ry = sy;
+      rz = sz;
+      rp = sp;
+    }
+  }
+  if (lane < nt) out[tb + lane] = make_float4(rx, ry, rz, rp);
+  __builtin_amdgcn_wave_barrier();  // targets of the next batch overwrite tgt
+}
+
+// WAVES: waves per SIMD the register budget is cut for (2: no spills; 3: a
+// few spilled registers for more latency hiding).  A tuning parameter.
+template <bool QUAD, int WAVES>
+__global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
     const uint32_t *__restrict__ cell_list, const uint32_t *__restrict__ body_list,
-    const gt_group *__restrict__ groups, uint32_t n_groups, float G, float eps2, uint32_t quad,
+    const gt_group *__restrict__ groups, uint32_t n_groups, float G, float eps2,
     float4 *__restrict__ out) {
+  __shared__ float4 targets[kBlock / kWave][32];
   const uint32_t gi = uniform_wave_id();
   if (gi >= n_groups) return;
   const uint32_t lane = threadIdx.x % kWave;
+  float4 *tgt = targets[threadIdx.x / kWave];
   const gt_group g = groups[gi];
   const uint32_t ntot = g.end - g.begin;
-  const float inf = __builtin_inff();
-
-  for (uint32_t t0 = 0; t0 < ntot; t0 += TB) {
-    const uint32_t nt = min(static_cast<uint32_t>(TB), ntot - t0);
+  for (uint32_t t0 = 0; t0 < ntot; t0 += 32) {
+    const uint32_t nt = min(32u, ntot - t0);
     const uint32_t tb = g.begin + t0;
-    float ax[TB], ay[TB], az[TB], ap[TB];
-#pragma unroll
-    for (int t = 0; t < TB; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
-
-    for (uint32_t base = g.cell_begin; base < g.cell_end; base += kWave) {
-      const uint32_t k = base + lane;
-      float cx = kFar, cy = kFar, cz = kFar, gm = 0.0f;
-      float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f, q5 = 0.0f;
-      if (k < g.cell_end) {
-        const Cell c = cells[cell_list[k]];
-        cx = c.cm.x;
-        cy = c.cm.y;
-        cz = c.cm.z;
-        gm = G * c.cm.w;
-        if (quad) {
-          q0 = G * c.q0.x;
-          q1 = G * c.q0.y;
-          q2 = G * c.q0.z;
-          q3 = G * c.q0.w;
-          q4 = G * c.q1.x;
-          q5 = G * c.q1.y;
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < TB; ++t) {
-        if (t < static_cast<int>(nt)) {
-          const float4 p = bodies[tb + t];
-          const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-          const float inv = __builtin_amdgcn_rsqf(r2);
-          const float inv2 = inv * inv;
-          const float inv3 = inv * inv2;
-          const float m3 = gm * inv3;
-          const float qx = q0 * dx + q3 * dy + q4 * dz;
-          const float qy = q3 * dx + q1 * dy + q5 * dz;
-          const float qz = q4 * dx + q5 * dy + q2 * dz;
-          const float rqr = dx * qx + dy * qy + dz * qz;
-          const float inv5 = inv3 * inv2;
-          const float h = rqr * inv5;
-          const float s = -2.5f * h * inv2 - m3;
-          ax[t] += qx * inv5 + s * dx;
-          ay[t] += qy * inv5 + s * dy;
-          az[t] += qz * inv5 + s * dz;
-          ap[t] -= gm * inv + 0.5f * h;
-        }
-      }
-    }
-
-    for (uint32_t base = g.body_begin; base < g.body_end; base += kWave) {
-      const uint32_t k = base + lane;
-      float bx = kFar, by = kFar, bz = kFar, gm = 0.0f;
-      uint32_t j = kNone;
-      if (k < g.body_end) {
-        j = body_list[k];
-        const float4 b = bodies[j];
-        bx = b.x;
-        by = b.y;
-        bz = b.z;
-        gm = G * b.w;
-      }
-#pragma unroll
-      for (int t = 0; t < TB; ++t) {
-        if (t < static_cast<int>(nt)) {
-          const float4 p = bodies[tb + t];
-          const float dx = p.x - bx, dy = p.y - by, dz = p.z - bz;
-          float r2 = dx * dx + dy * dy + dz * dz + eps2;
-          r2 = (j == tb + t) ? inf : r2;  // a body never feels itself
-          const float inv = __builtin_amdgcn_rsqf(r2);
-          const float gi1 = gm * inv;
-          const float g3 = gi1 * inv * inv;
-          ax[t] -= g3 * dx;
-          ay[t] -= g3 * dy;
-          az[t] -= g3 * dz;
-          ap[t] -= gi1;
-        }
-      }
-    }
-
-    float rx = 0.0f, ry = 0.0f, rz = 0.0f, rp = 0.0f;
-#pragma unroll
-    for (int t = 0; t < TB; ++t) {
-      if (t < static_cast<int>(nt)) {
-        const float sx = wave_sum(ax[t]), sy = wave_sum(ay[t]), sz = wave_sum(az[t]),
-                    sp = wave_sum(ap[t]);
-        if (lane == static_cast<uint32_t>(t)) {
-          rx = sx;
-          ry = sy;
-          rz = sz;
-          rp = sp;
-        }
-      }
-    }
-    if (lane < nt) out[tb + lane] = make_float4(rx, ry, rz, rp);
+    if (nt > 24)
+      eval_batch<32, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
+    else if (nt > 16)
+      eval_batch<24, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
+    else if (nt > 8)
+      eval_batch<16, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
+    else
+      eval_batch<8, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
   }
 }
 
@@ -246,16 +326,24 @@ __global__ __launch_bounds__(kBlock) void eval_fast(
 
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups, uint32_t n_groups, float G,
-                 float eps2, uint32_t quad, bool exact, float4 *out, hipStream_t s) {
+                 float eps2, uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s) {
   if (n_groups == 0) return;
   const uint32_t waves_per_block = kBlock / kWave;
   const uint32_t blocks = (n_groups + waves_per_block - 1) / waves_per_block;
+  const dim3 grid(blocks), block(kBlock);
+#define GTH_EVAL_ARGS bodies, cells, cell_list, body_list, groups, n_groups, G, eps2, out
   if (exact)
-    hipLaunchKernelGGL(eval_exact, dim3(blocks), dim3(kBlock), 0, s, bodies, cells, cell_list,
-                       body_list, groups, n_groups, G, eps2, quad, out);
+    hipLaunchKernelGGL(eval_exact, grid, block, 0, s, bodies, cells, cell_list, body_list, groups,
+                       n_groups, G, eps2, quad, out);
+  else if (quad && waves == 3)
+    hipLaunchKernelGGL((eval_fast<true, 3>), grid, block, 0, s, GTH_EVAL_ARGS);
+  else if (quad)
+    hipLaunchKernelGGL((eval_fast<true, 2>), grid, block, 0, s, GTH_EVAL_ARGS);
+  else if (waves == 3)
+    hipLaunchKernelGGL((eval_fast<false, 3>), grid, block, 0, s, GTH_EVAL_ARGS);
   else
-    hipLaunchKernelGGL(eval_fast<32>, dim3(blocks), dim3(kBlock), 0, s, bodies, cells,
-                       cell_list, body_list, groups, n_groups, G, eps2, quad, out);
+    hipLaunchKernelGGL((eval_fast<false, 2>), grid, block, 0, s, GTH_EVAL_ARGS);
+#undef GTH_EVAL_ARGS
   GTH_LAUNCH_CHECK();
 }
 

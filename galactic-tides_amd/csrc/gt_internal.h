@@ -79,17 +79,39 @@ static_assert(sizeof(gt_cell) == 48 && sizeof(gt_group) == 32 && sizeof(gt_param
                   sizeof(gt_float4) == 16,
               "reference struct layouts");
 
-// Node as the group walk reads it: geometry for the FP64 opening test.
-struct WalkNode {
-  double cx, cy, cz, half;
-  double mx, my, mz, crit;
+// One node as the group walk reads it (64 bytes, half a cache line): FP64
+// centre, centre of mass and opening distance, and `bits` packing the first
+// child (bits 0-25), the skip node (26-51, all ones = none), the child count
+// (52-55), the depth (56-61; half = root half / 2^depth) and whether the node
+// holds at most kBodiesIfAtMost bodies (62).
+struct WalkRec {
+  double cx, cy, cz;
+  double mx, my, mz;
+  double crit;
+  uint64_t bits;
 };
-static_assert(sizeof(WalkNode) == 64, "WalkNode layout");
+static_assert(sizeof(WalkRec) == 64, "WalkRec layout");
+constexpr uint32_t kRecIndexMask = (1u << 26) - 1;  // node indices in WalkRec::bits
+constexpr int kWalkExact = 0;     // lists written at scanned offsets
+constexpr int kWalkCapacity = 1;  // lists written into fixed per-group slots
 
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr int kGeomStride = 16;  // doubles per node in the host-compatible geometry rows
 constexpr int kMaxGroup = 64;    // targets per group: one wave64
 constexpr uint32_t kBodiesIfAtMost = 2;  // accepted cells this small are listed as bodies
+
+// ---- XCD-aware block order ---------------------------------------------------------
+
+// Workgroups of a launch are dealt to the 8 XCDs in turn.  Launching
+// xcd_grid(n) blocks and renumbering with xcd_block gives XCD x the logical
+// blocks [x * q, (x + 1) * q), q = ceil(n / 8): a contiguous run of the work,
+// so neighbouring groups (which read the same tree nodes and cells) share an
+// XCD's L2.  Logical blocks >= n must exit.
+constexpr uint32_t kXcds = 8;
+inline uint32_t xcd_grid(uint32_t n) { return (n + kXcds - 1) / kXcds * kXcds; }
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
+  return (b % kXcds) * (grid / kXcds) + b / kXcds;
+}
 
 // ---- launchers (defined in the .hip sources) -------------------------------------
 
@@ -109,7 +131,7 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t
 // gt_eval.hip
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups, uint32_t n_groups, float G,
-                 float eps2, uint32_t quad, bool exact, float4 *out, hipStream_t s);
+                 float eps2, uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s);
 
 // gt_direct.hip
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
@@ -121,21 +143,23 @@ void launch_direct_f64_sample(const double *pos, const double *mass, uint64_t n,
 // gt_walk.hip
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
-void launch_prep_lists(const double *geom, const double *tpos, const double *tmass, uint32_t n,
-                       uint32_t n_nodes, double theta, WalkNode *wn, Cell *cells, float4 *bodies,
-                       hipStream_t s);
 void launch_link_up(const int4 *lnk, uint32_t n_nodes, uint32_t *parent, uint32_t *skip,
-                    hipStream_t s);
+                    uint32_t *depth, hipStream_t s);
+void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
+                       const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
+                       uint32_t n_nodes, double theta, WalkRec *recs, uint2 *span, Cell *cells,
+                       float4 *bodies, uint32_t *error, hipStream_t s);
 void launch_group_starts(const int4 *lnk, uint32_t n_nodes, uint32_t n, uint32_t gs, int pack,
                          uint32_t *flags, hipStream_t s);
 void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n, uint32_t ng,
                         uint32_t *starts, gt_group *groups, hipStream_t s);
-void launch_walk(bool fill, const WalkNode *wn, const int4 *lnk, const uint32_t *skip,
-                 const double *tpos, gt_group *groups, uint32_t ng, const uint32_t *cell_off,
-                 const uint32_t *body_off, uint32_t *cell_count, uint32_t *body_count,
-                 uint32_t *cell_list, uint32_t *body_list, uint32_t nn, uint32_t *error,
-                 hipStream_t s);
-void launch_list_stats(const gt_group *groups, uint32_t ng, unsigned long long *stats,
+void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
+                 gt_group *groups, uint32_t ng, uint32_t nn, const double *root, uint32_t cap_c,
+                 uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
+                 uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
+                 uint32_t *body_list, uint32_t *error, hipStream_t s);
+void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
+                       const gt_group *groups, uint32_t ng, unsigned long long *stats,
                        hipStream_t s);
 
 // gt_step.hip

@@ -1,24 +1,32 @@
 // Interaction lists on the GPU (reference: build_group_lists,
 // src/group_lists.cpp:121-310).
 //
-//  prep_lists    FP32 bodies and cells relative to the root centre, and the
-//                walk nodes with crit = 2 half / theta + delta (:143-166).
-//  link_up       parent of every node, then `skip`: the node a LIFO depth-
-//                first walk pops after finishing a node's subtree (previous
-//                sibling, else the parent's skip).  With it the walk needs
-//                no stack: open = go to the last child, done = go to skip.
+//  link_up       parent of every node, then each node's depth and `skip`: the
+//                node a LIFO depth-first walk pops after finishing the node's
+//                subtree (previous sibling, else the parent's skip).  With it
+//                the walk needs no stack: open = go to the last child, done =
+//                go to skip.
+//  prep_lists    FP32 bodies and cells relative to the root centre (:143-166),
+//                and one 64-byte walk record per node: FP64 centre, centre of
+//                mass and crit = 2 half / theta + delta, plus one word packing
+//                first child, skip, child count, depth and "at most two bodies".
+//                A node visit of the walk reads exactly one record (half a
+//                cache line); half is root_half / 2^depth, which is exactly the
+//                reference's repeated halving.
 //  group_starts  group selection (:168-216) as flags on tree positions: every
 //                node holding more than group_size bodies is one the
 //                reference's depth-first `visit` reaches, and each such node
 //                marks where its runs of small children (packed greedily while
 //                they fit) and its chunks start.  A scan turns the flags into
 //                groups in tree order.
-//  walk<FILL>    one thread per group: group bounding box, then the depth-
+//  walk<MODE>    one thread per group: group bounding box, then the depth-
 //                first walk with the FP64 opening test (:227-268) evaluated
 //                in the reference's operation order with no FMA contraction,
 //                so accept/open decisions, and hence the lists, are identical.
-//                A counting pass sizes the lists, a scan places them and a
-//                filling pass writes them.
+//                MODE kWalkCapacity writes each group's lists into fixed-size
+//                slots (sized from the previous evaluation) and counts; groups
+//                that do not fit are caught by the counts, and then the lists
+//                are rewritten by MODE kWalkExact at scanned offsets.
 //  list_stats    interaction counts, longest list and largest group (:290-309).
 #include "gt_internal.h"
 
@@ -26,15 +34,64 @@ namespace gth {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWalkBlock = 64;
 
-inline uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+inline uint32_t blocks_for(size_t n, int b = kBlock) {
+  return static_cast<uint32_t>((n + b - 1) / b);
+}
+
+__device__ inline uint64_t pack_walk_bits(uint32_t first_child, uint32_t skip, uint32_t nch,
+                                          uint32_t depth, bool small) {
+  return static_cast<uint64_t>(first_child & kRecIndexMask) |
+         (static_cast<uint64_t>(skip & kRecIndexMask) << 26) |
+         (static_cast<uint64_t>(nch & 15u) << 52) | (static_cast<uint64_t>(depth & 63u) << 56) |
+         (static_cast<uint64_t>(small ? 1u : 0u) << 62);
+}
+
+__global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ lnk,
+                                                      uint32_t n_nodes,
+                                                      uint32_t *__restrict__ parent) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n_nodes) return;
+  if (p == 0) parent[0] = kNone;
+  const int4 L = lnk[p];
+  for (int c = 0; c < L.y; ++c) parent[L.x + c] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void set_skip_depth(const int4 *__restrict__ lnk,
+                                                         const uint32_t *__restrict__ parent,
+                                                         uint32_t n_nodes,
+                                                         uint32_t *__restrict__ skip,
+                                                         uint32_t *__restrict__ depth) {
+  const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
+  if (x0 >= n_nodes) return;
+  uint32_t d = 0;
+  for (uint32_t x = x0; parent[x] != kNone && d <= 64; x = parent[x]) ++d;
+  depth[x0] = d;
+  // Climb while x is the first child of its parent: its subtree ends its parent's.
+  uint32_t x = x0;
+  for (;;) {
+    const uint32_t p = parent[x];
+    if (p == kNone) {
+      skip[x0] = kNone;
+      return;
+    }
+    if (static_cast<uint32_t>(lnk[p].x) != x) {
+      skip[x0] = x - 1;
+      return;
+    }
+    x = p;
+  }
+}
 
 #pragma clang fp contract(off)
 
 __global__ __launch_bounds__(kBlock) void prep_lists(
-    const double *__restrict__ geom, const double *__restrict__ tpos,
-    const double *__restrict__ tmass, uint32_t n, uint32_t n_nodes, double inv_theta,
-    WalkNode *__restrict__ wn, Cell *__restrict__ cells, float4 *__restrict__ bodies) {
+    const double *__restrict__ geom, const int4 *__restrict__ lnk,
+    const uint32_t *__restrict__ skip, const uint32_t *__restrict__ depth,
+    const double *__restrict__ tpos, const double *__restrict__ tmass, uint32_t n,
+    uint32_t n_nodes, double inv_theta, WalkRec *__restrict__ recs, uint2 *__restrict__ span,
+    Cell *__restrict__ cells, float4 *__restrict__ bodies, uint32_t *__restrict__ error) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   const double ox = geom[0], oy = geom[1], oz = geom[2];
   if (k < n) {
@@ -52,49 +109,255 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
                        static_cast<float>(g[10]), static_cast<float>(g[11]));
     c.q1 = make_float4(static_cast<float>(g[12]), static_cast<float>(g[13]), 0.0f, 0.0f);
     cells[k] = c;
-    WalkNode w;
+    const int4 L = lnk[k];
+    const uint32_t d = depth[k];
+    // The packed record keeps half as root_half / 2^depth: check that it is.
+    if (d > 63 || ldexp(geom[3], -static_cast<int>(d)) != g[3]) atomicOr(error, 2u);
+    WalkRec w;
     w.cx = g[0];
     w.cy = g[1];
     w.cz = g[2];
-    w.half = g[3];
     w.mx = g[4];
     w.my = g[5];
     w.mz = g[6];
     w.crit = 2.0 * g[3] * inv_theta + g[14];
-    wn[k] = w;
+    w.bits = pack_walk_bits(static_cast<uint32_t>(L.x), skip[k], static_cast<uint32_t>(L.y), d,
+                            static_cast<uint32_t>(L.w - L.z) <= kBodiesIfAtMost);
+    recs[k] = w;
+    span[k] = make_uint2(static_cast<uint32_t>(L.z), static_cast<uint32_t>(L.w));
   }
 }
 
-__global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ lnk,
-                                                      uint32_t n_nodes,
-                                                      uint32_t *__restrict__ parent) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n_nodes) return;
-  if (p == 0) parent[0] = kNone;
-  const int4 L = lnk[p];
-  for (int c = 0; c < L.y; ++c) parent[L.x + c] = p;
+template <int MODE>
+__global__ __launch_bounds__(kWalkBlock) void walk(
+    const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
+    const double *__restrict__ tpos, gt_group *__restrict__ groups, uint32_t ng, uint32_t nn,
+    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b, const uint32_t *__restrict__ cell_off,
+    const uint32_t *__restrict__ body_off, uint32_t *__restrict__ cell_count,
+    uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
+    uint32_t *__restrict__ body_list, uint32_t *__restrict__ error) {
+  const uint32_t g = blockIdx.x * kWalkBlock + threadIdx.x;
+  if (g >= ng) return;
+  const uint32_t gb = groups[g].begin, ge = groups[g].end;
+  const double root_half = root[3];
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = tpos[3 * static_cast<size_t>(gb) + a];
+  for (uint32_t k = gb; k < ge; ++k)
+    for (int a = 0; a < 3; ++a) {
+      const double v = tpos[3 * static_cast<size_t>(k) + a];
+      lo[a] = (v < lo[a]) ? v : lo[a];  // std::min / std::max semantics
+      hi[a] = (hi[a] < v) ? v : hi[a];
+    }
+  const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]),
+               gcz = 0.5 * (lo[2] + hi[2]);
+  const double ghx = 0.5 * (hi[0] - lo[0]), ghy = 0.5 * (hi[1] - lo[1]),
+               ghz = 0.5 * (hi[2] - lo[2]);
+  uint32_t cpos, bpos;
+  if (MODE == kWalkExact) {
+    cpos = cell_off[g];
+    bpos = body_off[g];
+  } else {
+    cpos = g * cap_c;
+    bpos = g * cap_b;
+  }
+  uint32_t nc = 0, nb = 0;
+  uint32_t cur = 0;
+  // A depth-first walk visits each node at most once: more visits than nodes,
+  // or an index past the end, means a malformed tree.  Stop and report it.
+  uint32_t budget = nn;
+  while (cur != kNone) {
+    if (cur >= nn || budget-- == 0) {
+      atomicOr(error, 1u);
+      break;
+    }
+    const WalkRec r = recs[cur];
+    const uint32_t fc = static_cast<uint32_t>(r.bits) & kRecIndexMask;
+    uint32_t sk = static_cast<uint32_t>(r.bits >> 26) & kRecIndexMask;
+    sk = sk == kRecIndexMask ? kNone : sk;
+    const uint32_t nch = static_cast<uint32_t>(r.bits >> 52) & 15u;
+    const int depth = static_cast<int>((r.bits >> 56) & 63u);
+    const bool small = (r.bits >> 62) & 1u;
+    const double half = ldexp(root_half, -depth);
+    bool overlap = true;
+    double d2 = 0.0;
+    {
+      if (fabs(r.cx - gcx) > half * 1.0001 + ghx) overlap = false;
+      const double d = fabs(r.mx - gcx) - ghx;
+      if (d > 0) d2 += d * d;
+    }
+    {
+      if (fabs(r.cy - gcy) > half * 1.0001 + ghy) overlap = false;
+      const double d = fabs(r.my - gcy) - ghy;
+      if (d > 0) d2 += d * d;
+    }
+    {
+      if (fabs(r.cz - gcz) > half * 1.0001 + ghz) overlap = false;
+      const double d = fabs(r.mz - gcz) - ghz;
+      if (d > 0) d2 += d * d;
+    }
+    const bool accept = !overlap && d2 > r.crit * r.crit;
+    if (accept && !small) {
+      if (MODE == kWalkExact || nc < cap_c) cell_list[cpos + nc] = cur;
+      ++nc;
+      cur = sk;
+    } else if (accept || nch == 0) {
+      const uint2 sp = span[cur];
+      for (uint32_t j = sp.x; j < sp.y; ++j) {
+        const uint32_t at = nb + (j - sp.x);
+        if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = j;
+      }
+      nb += sp.y - sp.x;
+      cur = sk;
+    } else {
+      cur = fc + nch - 1;
+    }
+  }
+  cell_count[g] = nc;
+  body_count[g] = nb;
+  gt_group G = groups[g];
+  G.cell_begin = cpos;
+  G.cell_end = cpos + nc;
+  G.body_begin = bpos;
+  G.body_end = bpos + nb;
+  groups[g] = G;
 }
 
-__global__ __launch_bounds__(kBlock) void set_skip(const int4 *__restrict__ lnk,
-                                                   const uint32_t *__restrict__ parent,
-                                                   uint32_t n_nodes, uint32_t *__restrict__ skip) {
-  const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
-  if (x0 >= n_nodes) return;
-  uint32_t x = x0;
-  // Climb while x is the first child of its parent: its subtree ends its parent's.
+// Wave-coherent walk: LANES consecutive (spatially adjacent) groups share one
+// depth-first traversal of the union of their trees.  The node being visited
+// is the same for the whole wave, so its record comes through the scalar cache
+// into SGPRs, together with the records of both possible next nodes (skip and
+// last child).  Each lane applies its own group's FP64 test; a lane that
+// accepts a node (or takes a leaf's bodies) sleeps until the traversal leaves
+// that node's subtree (reaches its skip node); the wave descends when any
+// awake lane opens the node.  A lane is awake exactly at the nodes its own
+// depth-first walk visits, in the same order, so each lane's lists are the
+// reference's.
+constexpr uint32_t kAwake = 0xfffffffeu;
+
+template <int MODE, int LANES>
+__global__ __launch_bounds__(kBlock) void walk_coherent(
+    const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
+    const double *__restrict__ tpos, gt_group *__restrict__ groups, uint32_t ng, uint32_t nn,
+    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b,
+    const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
+    uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
+    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
+    uint32_t *__restrict__ error, uint32_t nblocks) {
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t lane = threadIdx.x % 64;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * (kBlock / 64) + threadIdx.x / 64);
+  if (blk >= nblocks || wave * LANES >= ng) return;  // the whole wave leaves together
+  const uint32_t g = wave * LANES + lane;
+  const bool mine = lane < static_cast<uint32_t>(LANES) && g < ng;
+  const double root_half = root[3];
+  double gcx = 0.0, gcy = 0.0, gcz = 0.0, ghx = 0.0, ghy = 0.0, ghz = 0.0;
+  uint32_t cpos = 0, bpos = 0;
+  if (mine) {
+    const uint32_t gb = groups[g].begin, ge = groups[g].end;
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = tpos[3 * static_cast<size_t>(gb) + a];
+    for (uint32_t k = gb; k < ge; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const double v = tpos[3 * static_cast<size_t>(k) + a];
+        lo[a] = (v < lo[a]) ? v : lo[a];  // std::min / std::max semantics
+        hi[a] = (hi[a] < v) ? v : hi[a];
+      }
+    gcx = 0.5 * (lo[0] + hi[0]);
+    gcy = 0.5 * (lo[1] + hi[1]);
+    gcz = 0.5 * (lo[2] + hi[2]);
+    ghx = 0.5 * (hi[0] - lo[0]);
+    ghy = 0.5 * (hi[1] - lo[1]);
+    ghz = 0.5 * (hi[2] - lo[2]);
+    if (MODE == kWalkExact) {
+      cpos = cell_off[g];
+      bpos = body_off[g];
+    } else {
+      cpos = g * cap_c;
+      bpos = g * cap_b;
+    }
+  }
+  uint32_t resume = mine ? kAwake : kNone;
+  uint32_t nc = 0, nb = 0;
+  uint32_t cur = 0;
+  WalkRec r = recs[0];
+  uint32_t budget = nn;
   for (;;) {
-    const uint32_t p = parent[x];
-    if (p == kNone) {
-      skip[x0] = kNone;
-      return;
+    if (budget-- == 0) {
+      atomicOr(error, 1u);
+      break;
     }
-    if (static_cast<uint32_t>(lnk[p].x) != x) {
-      skip[x0] = x - 1;
-      return;
+    const uint32_t fc = static_cast<uint32_t>(r.bits) & kRecIndexMask;
+    uint32_t sk = static_cast<uint32_t>(r.bits >> 26) & kRecIndexMask;
+    sk = sk == kRecIndexMask ? kNone : sk;
+    const uint32_t nch = static_cast<uint32_t>(r.bits >> 52) & 15u;
+    const int depth = static_cast<int>((r.bits >> 56) & 63u);
+    const bool small = (r.bits >> 62) & 1u;
+    const uint32_t last = fc + nch - 1;
+    // Both possible next records are requested before this node is tested.
+    WalkRec rs = r, rc = r;
+    if (sk < nn) rs = recs[sk];
+    if (nch != 0 && last < nn) rc = recs[last];
+    if (resume == cur) resume = kAwake;
+    bool opens = false;
+    if (resume == kAwake) {
+      const double half = ldexp(root_half, -depth);
+      bool overlap = true;
+      double d2 = 0.0;
+      {
+        if (fabs(r.cx - gcx) > half * 1.0001 + ghx) overlap = false;
+        const double d = fabs(r.mx - gcx) - ghx;
+        if (d > 0) d2 += d * d;
+      }
+      {
+        if (fabs(r.cy - gcy) > half * 1.0001 + ghy) overlap = false;
+        const double d = fabs(r.my - gcy) - ghy;
+        if (d > 0) d2 += d * d;
+      }
+      {
+        if (fabs(r.cz - gcz) > half * 1.0001 + ghz) overlap = false;
+        const double d = fabs(r.mz - gcz) - ghz;
+        if (d > 0) d2 += d * d;
+      }
+      const bool accept = !overlap && d2 > r.crit * r.crit;
+      if (accept && !small) {
+        if (MODE == kWalkExact || nc < cap_c) cell_list[cpos + nc] = cur;
+        ++nc;
+        resume = sk;
+      } else if (accept || nch == 0) {
+        const uint2 sp = span[cur];
+        for (uint32_t j = sp.x; j < sp.y; ++j) {
+          const uint32_t at = nb + (j - sp.x);
+          if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = j;
+        }
+        nb += sp.y - sp.x;
+        resume = sk;
+      } else {
+        opens = true;
+      }
     }
-    x = p;
+    const bool descend = __ballot(opens) != 0;
+    const uint32_t next = __builtin_amdgcn_readfirstlane(descend ? last : sk);
+    if (next == kNone) break;
+    if (next >= nn) {
+      atomicOr(error, 1u);
+      break;
+    }
+    r = descend ? rc : rs;
+    cur = next;
+  }
+  if (mine) {
+    cell_count[g] = nc;
+    body_count[g] = nb;
+    gt_group G = groups[g];
+    G.cell_begin = cpos;
+    G.cell_end = cpos + nc;
+    G.body_begin = bpos;
+    G.body_end = bpos + nb;
+    groups[g] = G;
   }
 }
+
+#pragma clang fp contract(fast)
 
 __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ lnk,
                                                        uint32_t n_nodes, uint32_t gs, int pack,
@@ -147,124 +410,46 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
   groups[g] = G;
 }
 
-template <bool FILL>
-__global__ __launch_bounds__(kBlock) void walk(const WalkNode *__restrict__ wn,
-                                               const int4 *__restrict__ lnk,
-                                               const uint32_t *__restrict__ skip,
-                                               const double *__restrict__ tpos,
-                                               gt_group *__restrict__ groups, uint32_t ng,
-                                               const uint32_t *__restrict__ cell_off,
-                                               const uint32_t *__restrict__ body_off,
-                                               uint32_t *__restrict__ cell_count,
-                                               uint32_t *__restrict__ body_count,
-                                               uint32_t *__restrict__ cell_list,
-                                               uint32_t *__restrict__ body_list, uint32_t nn,
-                                               uint32_t *__restrict__ error) {
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  if (g >= ng) return;
-  const uint32_t gb = groups[g].begin, ge = groups[g].end;
-  double lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = tpos[3 * static_cast<size_t>(gb) + a];
-  for (uint32_t k = gb; k < ge; ++k)
-    for (int a = 0; a < 3; ++a) {
-      const double v = tpos[3 * static_cast<size_t>(k) + a];
-      lo[a] = (v < lo[a]) ? v : lo[a];  // std::min / std::max semantics
-      hi[a] = (hi[a] < v) ? v : hi[a];
-    }
-  const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]),
-               gcz = 0.5 * (lo[2] + hi[2]);
-  const double ghx = 0.5 * (hi[0] - lo[0]), ghy = 0.5 * (hi[1] - lo[1]),
-               ghz = 0.5 * (hi[2] - lo[2]);
-  uint32_t nc = 0, nb = 0;
-  uint32_t cpos = 0, bpos = 0;
-  if (FILL) {
-    cpos = cell_off[g];
-    bpos = body_off[g];
-  }
-  uint32_t cur = 0;
-  // A depth-first walk visits each node at most once: more visits than nodes,
-  // or an index past the end, means a malformed tree.  Stop and report it.
-  uint32_t budget = nn;
-  while (cur != kNone) {
-    if (cur >= nn || budget-- == 0) {
-      atomicOr(error, 1u);
-      break;
-    }
-    const WalkNode nd = wn[cur];
-    const int4 L = lnk[cur];
-    bool overlap = true;
-    double d2 = 0.0;
-    {
-      if (fabs(nd.cx - gcx) > nd.half * 1.0001 + ghx) overlap = false;
-      const double d = fabs(nd.mx - gcx) - ghx;
-      if (d > 0) d2 += d * d;
-    }
-    {
-      if (fabs(nd.cy - gcy) > nd.half * 1.0001 + ghy) overlap = false;
-      const double d = fabs(nd.my - gcy) - ghy;
-      if (d > 0) d2 += d * d;
-    }
-    {
-      if (fabs(nd.cz - gcz) > nd.half * 1.0001 + ghz) overlap = false;
-      const double d = fabs(nd.mz - gcz) - ghz;
-      if (d > 0) d2 += d * d;
-    }
-    const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-    const bool accept = !overlap && d2 > nd.crit * nd.crit;
-    if (accept && e - b > kBodiesIfAtMost) {
-      if (FILL) cell_list[cpos + nc] = cur;
-      ++nc;
-      cur = skip[cur];
-    } else if (accept || L.y == 0) {
-      if (FILL)
-        for (uint32_t j = b; j < e; ++j) body_list[bpos + nb + (j - b)] = j;
-      nb += e - b;
-      cur = skip[cur];
-    } else {
-      cur = static_cast<uint32_t>(L.x + L.y - 1);
-    }
-  }
-  if (!FILL) {
-    cell_count[g] = nc;
-    body_count[g] = nb;
-  } else {
-    gt_group G = groups[g];
-    G.cell_begin = cpos;
-    G.cell_end = cpos + nc;
-    G.body_begin = bpos;
-    G.body_end = bpos + nb;
-    groups[g] = G;
-  }
-}
-
-#pragma clang fp contract(fast)
-
-// stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group
-__global__ __launch_bounds__(kBlock) void list_stats(const gt_group *__restrict__ groups,
+// stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group,
+// [4] most cells of a group, [5] most bodies of a group, [6] all cells, [7] all bodies
+__global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict__ cell_count,
+                                                     const uint32_t *__restrict__ body_count,
+                                                     const gt_group *__restrict__ groups,
                                                      uint32_t ng,
                                                      unsigned long long *__restrict__ stats) {
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  unsigned long long ci = 0, bi = 0, ml = 0, mg = 0;
+  unsigned long long ci = 0, bi = 0, ml = 0, mg = 0, mc = 0, mb = 0, sc = 0, sb = 0;
   if (g < ng) {
-    const gt_group G = groups[g];
-    const unsigned long long t = G.end - G.begin, c = G.cell_end - G.cell_begin,
-                             b = G.body_end - G.body_begin;
+    const unsigned long long t = groups[g].end - groups[g].begin, c = cell_count[g],
+                             b = body_count[g];
     ci = t * c;
     bi = t * b - t;
     ml = c + b;
     mg = t;
+    mc = c;
+    mb = b;
+    sc = c;
+    sb = b;
   }
   for (int off = 32; off > 0; off >>= 1) {
     ci += __shfl_xor(ci, off, 64);
     bi += __shfl_xor(bi, off, 64);
     ml = max(ml, __shfl_xor(ml, off, 64));
     mg = max(mg, __shfl_xor(mg, off, 64));
+    mc = max(mc, __shfl_xor(mc, off, 64));
+    mb = max(mb, __shfl_xor(mb, off, 64));
+    sc += __shfl_xor(sc, off, 64);
+    sb += __shfl_xor(sb, off, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&stats[0], ci);
     atomicAdd(&stats[1], bi);
     atomicMax(&stats[2], ml);
     atomicMax(&stats[3], mg);
+    atomicMax(&stats[4], mc);
+    atomicMax(&stats[5], mb);
+    atomicAdd(&stats[6], sc);
+    atomicAdd(&stats[7], sb);
   }
 }
 
@@ -292,24 +477,25 @@ void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipS
   GTH_LAUNCH_CHECK();
 }
 
-void launch_prep_lists(const double *geom, const double *tpos, const double *tmass, uint32_t n,
-                       uint32_t n_nodes, double theta, WalkNode *wn, Cell *cells, float4 *bodies,
-                       hipStream_t s) {
-  const uint32_t m = n > n_nodes ? n : n_nodes;
-  if (!m) return;
-  hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, tpos, tmass, n,
-                     n_nodes, 1.0 / theta, wn, cells, bodies);
-  GTH_LAUNCH_CHECK();
-}
-
 void launch_link_up(const int4 *lnk, uint32_t n_nodes, uint32_t *parent, uint32_t *skip,
-                    hipStream_t s) {
+                    uint32_t *depth, hipStream_t s) {
   if (!n_nodes) return;
   hipLaunchKernelGGL(set_parents, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, n_nodes,
                      parent);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(set_skip, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, parent, n_nodes,
-                     skip);
+  hipLaunchKernelGGL(set_skip_depth, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, parent,
+                     n_nodes, skip, depth);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
+                       const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
+                       uint32_t n_nodes, double theta, WalkRec *recs, uint2 *span, Cell *cells,
+                       float4 *bodies, uint32_t *error, hipStream_t s) {
+  const uint32_t m = n > n_nodes ? n : n_nodes;
+  if (!m) return;
+  hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
+                     tpos, tmass, n, n_nodes, 1.0 / theta, recs, span, cells, bodies, error);
   GTH_LAUNCH_CHECK();
 }
 
@@ -333,28 +519,56 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
   GTH_LAUNCH_CHECK();
 }
 
-void launch_walk(bool fill, const WalkNode *wn, const int4 *lnk, const uint32_t *skip,
-                 const double *tpos, gt_group *groups, uint32_t ng, const uint32_t *cell_off,
-                 const uint32_t *body_off, uint32_t *cell_count, uint32_t *body_count,
-                 uint32_t *cell_list, uint32_t *body_list, uint32_t nn, uint32_t *error,
-                 hipStream_t s) {
+void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
+                 gt_group *groups, uint32_t ng, uint32_t nn, const double *root, uint32_t cap_c,
+                 uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
+                 uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
+                 uint32_t *body_list, uint32_t *error, hipStream_t s) {
   if (!ng) return;
-  if (fill)
-    hipLaunchKernelGGL(walk<true>, dim3(blocks_for(ng)), dim3(kBlock), 0, s, wn, lnk, skip, tpos,
-                       groups, ng, cell_off, body_off, cell_count, body_count, cell_list,
-                       body_list, nn, error);
-  else
-    hipLaunchKernelGGL(walk<false>, dim3(blocks_for(ng)), dim3(kBlock), 0, s, wn, lnk, skip, tpos,
-                       groups, ng, cell_off, body_off, cell_count, body_count, cell_list,
-                       body_list, nn, error);
+  if (lanes == 1) {
+    if (mode == kWalkExact)
+      hipLaunchKernelGGL(walk<kWalkExact>, dim3(blocks_for(ng, kWalkBlock)), dim3(kWalkBlock), 0,
+                         s, recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off,
+                         body_off, cell_count, body_count, cell_list, body_list, error);
+    else
+      hipLaunchKernelGGL(walk<kWalkCapacity>, dim3(blocks_for(ng, kWalkBlock)), dim3(kWalkBlock),
+                         0, s, recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off,
+                         body_off, cell_count, body_count, cell_list, body_list, error);
+    GTH_LAUNCH_CHECK();
+    return;
+  }
+  const uint32_t waves = static_cast<uint32_t>((ng + lanes - 1) / lanes);
+  const uint32_t nblocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+  const dim3 grid(xcd_grid(nblocks)), block(kBlock);
+#define GTH_WALK_ARGS \
+  recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off, cell_count, \
+      body_count, cell_list, body_list, error, nblocks
+  if (mode == kWalkExact) {
+    if (lanes == 16)
+      hipLaunchKernelGGL((walk_coherent<kWalkExact, 16>), grid, block, 0, s, GTH_WALK_ARGS);
+    else if (lanes == 32)
+      hipLaunchKernelGGL((walk_coherent<kWalkExact, 32>), grid, block, 0, s, GTH_WALK_ARGS);
+    else
+      hipLaunchKernelGGL((walk_coherent<kWalkExact, 64>), grid, block, 0, s, GTH_WALK_ARGS);
+  } else {
+    if (lanes == 16)
+      hipLaunchKernelGGL((walk_coherent<kWalkCapacity, 16>), grid, block, 0, s, GTH_WALK_ARGS);
+    else if (lanes == 32)
+      hipLaunchKernelGGL((walk_coherent<kWalkCapacity, 32>), grid, block, 0, s, GTH_WALK_ARGS);
+    else
+      hipLaunchKernelGGL((walk_coherent<kWalkCapacity, 64>), grid, block, 0, s, GTH_WALK_ARGS);
+  }
+#undef GTH_WALK_ARGS
   GTH_LAUNCH_CHECK();
 }
 
-void launch_list_stats(const gt_group *groups, uint32_t ng, unsigned long long *stats,
+void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
+                       const gt_group *groups, uint32_t ng, unsigned long long *stats,
                        hipStream_t s) {
-  launch_fill_u64(stats, 4, 0ull, s);
+  launch_fill_u64(stats, 8, 0ull, s);
   if (!ng) return;
-  hipLaunchKernelGGL(list_stats, dim3(blocks_for(ng)), dim3(kBlock), 0, s, groups, ng, stats);
+  hipLaunchKernelGGL(list_stats, dim3(blocks_for(ng)), dim3(kBlock), 0, s, cell_count, body_count,
+                     groups, ng, stats);
   GTH_LAUNCH_CHECK();
 }
 

@@ -24,7 +24,8 @@ OK, ERR_ARGUMENT, ERR_DEVICE, ERR_MEMORY, ERR_LIMIT, ERR_UNSUPPORTED = range(6)
 class GroupConfig(C.Structure):
     _fields_ = [("G", C.c_double), ("theta", C.c_double), ("eps", C.c_double),
                 ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
-                ("pack", C.c_int), ("exact", C.c_int), ("reserved", C.c_int * 7)]
+                ("pack", C.c_int), ("exact", C.c_int), ("eval_waves", C.c_int),
+                ("walk_lanes", C.c_int), ("reserved", C.c_int * 5)]
 
 
 class Params(C.Structure):
@@ -96,11 +97,11 @@ def _f64(a, shape=None):
 
 
 def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
-                pack=-1, exact=False):
+                pack=-1, exact=False, eval_waves=0, walk_lanes=0):
     cfg = GroupConfig()
     cfg.G, cfg.theta, cfg.eps = G, theta, eps
     cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
-    cfg.pack, cfg.exact = pack, int(exact)
+    cfg.pack, cfg.exact, cfg.eval_waves, cfg.walk_lanes = pack, int(exact), eval_waves, walk_lanes
     return cfg
 
 

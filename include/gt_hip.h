@@ -68,7 +68,11 @@ typedef struct {
   int pack;          /* merge small siblings into groups; < 0: device default (on) */
   int exact;         /* 1: IEEE 1/sqrt, no contraction, reference summation order:
                         forces bit-identical to evaluate_group_lists_cpu */
-  int reserved[7];   /* zero */
+  int eval_waves;    /* tuning: waves per SIMD the list-evaluation kernel is built
+                        for (2 or 3; 0 = library default, 3) */
+  int walk_lanes;    /* tuning: 16, 32 or 64 groups share one wave-coherent walk;
+                        0 or 1 (default): one independent walk per lane */
+  int reserved[5];   /* zero */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */

@@ -112,3 +112,21 @@ def test_group_size_limit(hip, oracle):
     pos, mass = oracle.test_system(0, 100, 1)
     with pytest.raises(ValueError):
         hip.group_compute(pos, mass, group_size=65)
+
+
+@pytest.mark.parametrize("lanes", [1, 16, 32, 64])
+@pytest.mark.parametrize("gs,pack,theta", [(32, 1, 0.75), (7, 0, 0.5), (64, 1, 1.0)])
+def test_walk_variants_give_identical_lists(hip, oracle, lanes, gs, pack, theta):
+    # one independent walk per lane, or 16/32/64 groups sharing one traversal
+    pos, mass = oracle.test_system(0, 20000, 9)
+    oracle.build(pos, mass, 8)
+    oracle.lists(theta, gs, pack)
+    want = oracle.lists_get()
+    geom, links = oracle.nodes()
+    tpos, tmass = oracle.tree_positions()
+    for _ in range(2):  # first call: exact offsets; second: per-group slots
+        hip.walk_tree(geom, links, tpos, tmass, evaluate=False, theta=theta, group_size=gs,
+                      pack=pack, walk_lanes=lanes)
+        got = hip.get_lists()
+        for k in ("groups", "cell_list", "body_list"):
+            assert _same(got[k], want[k]), k
