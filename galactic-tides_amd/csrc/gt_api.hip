@@ -243,7 +243,8 @@ int gt_hip_create(gt_hip_context **out, int device) {
   if (!out) return GT_HIP_ERR_ARGUMENT;
   *out = nullptr;
   gt_hip_context *c = nullptr;
-  const int rc = guarded(nullptr, [&] {
+  int rc = GT_HIP_OK;
+  try {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
       fail(GT_HIP_ERR_DEVICE, "hip: no GPU device available");
@@ -255,12 +256,15 @@ int gt_hip_create(gt_hip_context **out, int device) {
     GTH_CHECK(hipGetDeviceProperties(&c->prop, device));
     GTH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) GTH_CHECK(hipEventCreate(&e));
-  });
+    g_create_error.clear();
+  } catch (const Error &e) {
+    g_create_error = e.what();
+    rc = e.code;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    rc = GT_HIP_ERR_DEVICE;
+  }
   if (rc != GT_HIP_OK) {
-    try {
-      g_create_error = "hip context creation failed";
-    } catch (...) {
-    }
     delete c;
     return rc;
   }

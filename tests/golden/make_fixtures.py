@@ -1,0 +1,152 @@
+"""Writes the golden vectors in tests/golden/ from the reference itself.
+
+Runs only where the reference build exists (oracle/_ref/libgt_ref.so, built
+by `make -C oracle ref` from /root/reference): every expected value below is
+produced by the reference's own code (Octree::build, build_group_lists,
+evaluate_group_lists_cpu, LeapfrogKDK with GroupTreeGravity, make_hernquist_galaxy).
+The tests compare the CPU oracle and the GPU library against these files.
+
+    python tests/golden/make_fixtures.py
+
+Outputs:
+  tiny.npz       full inputs and outputs for three small systems
+  configs.json   SHA-256 digests and statistics for the 50k and 200k galaxies,
+                 a small encounter, and 10 leapfrog steps on the 50k galaxy
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from pyoracle import Checker, canonical_ids  # noqa: E402
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def model_vector(disk_n, bulge_n, halo_n, seed, eps=0.025, theta=0.75):
+    """HernquistModel fields in ref_shim.cpp's order, with the configurations'
+    bulge mass 0.333 and cut-off radius 2."""
+    return np.array([1.0, disk_n, 1.0, 1.0, 0.2, 10.0, 1.5, 2.43, bulge_n, 0.333, 0.2, 2.0,
+                     halo_n, 5.8, 1.0, 10.0, 30.0, 0.95, seed, eps, theta], np.float64)
+
+
+def ref_galaxy(ref, disk_n, bulge_n, halo_n, seed):
+    n = disk_n + bulge_n + halo_n
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    typ = np.zeros(n, np.uint8)
+    ref._galaxy(model_vector(disk_n, bulge_n, halo_n, seed), 0.0, 0.0, pos, vel, mass, typ)
+    return pos, vel, mass, typ
+
+
+def ref_encounter(ref, disk_n, bulge_n, halo_n, seed):
+    n = 2 * (disk_n + bulge_n + halo_n)
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    typ = np.zeros(n, np.uint8)
+    ref._encounter(model_vector(disk_n, bulge_n, halo_n, seed), 0.0, 0.0, 60.0, 30.0, 2.0, 30.0,
+                   pos, vel, mass, typ)
+    return pos, vel, mass, typ
+
+
+def tree_and_lists(ref, pos, mass, leaf, theta, gs, pack, eps):
+    ref.build(pos, mass, leaf)
+    geom, links = ref.nodes()
+    order = ref.order()
+    ref.lists(theta, gs, pack)
+    L = ref.lists_get()
+    out = ref.eval(1.0, eps, True)
+    return geom, links, order, L, out, ref.info()
+
+
+def summary(ref, pos, mass, theta=0.75, gs=32, pack=True, eps=0.025, leaf=8):
+    geom, links, order, L, out, info = tree_and_lists(ref, pos, mass, leaf, theta, gs, pack, eps)
+    canon = canonical_ids(links)
+    inv = np.argsort(canon)
+    fc = np.where(links[:, 1] > 0, canon[np.maximum(links[:, 0], 0)], -1)
+    clinks = np.stack([fc, links[:, 1], links[:, 2], links[:, 3]], 1)[inv].astype(np.int64)
+    return {
+        "positions": digest(pos), "masses": digest(mass),
+        "n_nodes": info["n_nodes"], "depth": info["depth"], "n_groups": info["n_groups"],
+        "n_cell_list": info["n_cell_list"], "n_body_list": info["n_body_list"],
+        "cell_interactions": info["cell_interactions"],
+        "body_interactions": info["body_interactions"],
+        "max_list": info["max_list"], "max_group": info["max_group"],
+        "order": digest(order.astype(np.uint64)),
+        "tree_geometry": digest(geom[inv]), "tree_links": digest(clinks),
+        "groups": digest(L["groups"]), "body_list": digest(L["body_list"]),
+        "cell_list_canonical": digest(canon[L["cell_list"]].astype(np.int64)),
+        "forces_fp32": digest(out),
+    }
+
+
+def main():
+    ref = Checker("ref")
+    tiny = {}
+    # 1) the reference tests' own small systems
+    cases = {
+        "mock307": (1, 307, 3, 4, 0.02),     # test_metal_mock.cpp:19-31,72-101 (leaf 4)
+        "kernels3000": (0, 3000, 11, 8, 0.01),  # test_gpu_kernels.cpp:29-47
+    }
+    oracle = Checker("oracle")
+    for name, (kind, n, seed, leaf, eps) in cases.items():
+        pos, mass = oracle.test_system(kind, n, seed)
+        for pack in (0, 1):
+            geom, links, order, L, out, info = tree_and_lists(ref, pos, mass, leaf, 0.7, 32,
+                                                              pack, eps)
+            key = f"{name}_p{pack}"
+            tiny[f"{key}_pos"], tiny[f"{key}_mass"] = pos, mass
+            tiny[f"{key}_geom"], tiny[f"{key}_links"], tiny[f"{key}_order"] = geom, links, order
+            for k in ("groups", "cell_list", "body_list"):
+                tiny[f"{key}_{k}"] = L[k]
+            tiny[f"{key}_out"] = out
+            tiny[f"{key}_eps"] = np.array([eps])
+            tiny[f"{key}_leaf"] = np.array([leaf])
+    # 2) coincident particles (test_gpu_kernels.cpp:276-288): a leaf bigger than a group
+    pos = np.zeros((150, 3))
+    pos[:100] = (1.0, 2.0, 3.0)
+    i = np.arange(50, dtype=np.float64)
+    pos[100:] = np.stack([i, -i, 0.5 * i], 1)
+    mass = np.ones(150)
+    for pack in (0, 1):
+        geom, links, order, L, out, info = tree_and_lists(ref, pos, mass, 8, 0.6, 32, pack, 0.05)
+        key = f"coincident_p{pack}"
+        tiny[f"{key}_pos"], tiny[f"{key}_mass"] = pos, mass
+        tiny[f"{key}_geom"], tiny[f"{key}_links"], tiny[f"{key}_order"] = geom, links, order
+        for k in ("groups", "cell_list", "body_list"):
+            tiny[f"{key}_{k}"] = L[k]
+        tiny[f"{key}_out"] = out
+        tiny[f"{key}_eps"] = np.array([0.05])
+        tiny[f"{key}_leaf"] = np.array([8])
+    np.savez_compressed(os.path.join(HERE, "tiny.npz"), **tiny)
+
+    configs = {}
+    # 3) the 50k and 200k galaxies (reference halo profile), theta 0.75, gs 32, packed
+    for name, (d, b, h, seed) in {"galaxy50k": (20000, 5000, 25000, 1),
+                                  "galaxy200k": (80000, 20000, 100000, 2)}.items():
+        pos, vel, mass, typ = ref_galaxy(ref, d, b, h, seed)
+        configs[name] = summary(ref, pos, mass)
+        configs[name]["model"] = dict(disk_n=d, bulge_n=b, halo_n=h, seed=seed)
+        print(name, configs[name]["n_nodes"], configs[name]["n_groups"], flush=True)
+    # 4) a small encounter (positions only: the orbit set-up and the second seed)
+    pos, vel, mass, typ = ref_encounter(ref, 4000, 1000, 5000, 3)
+    configs["encounter20k"] = {"positions": digest(pos), "masses": digest(mass),
+                               "model": dict(disk_n=4000, bulge_n=1000, halo_n=5000, seed=3)}
+    # 5) leapfrog: the 50k galaxy's positions at rest, 10 steps of 0.01
+    pos, vel, mass, typ = ref_galaxy(ref, 20000, 5000, 25000, 1)
+    p, v, a, phi = ref.kdk_group(pos, np.zeros_like(pos), mass, 10, 0.01, 1.0, 0.75, 0.025, True,
+                                 32, 8, True)
+    configs["kdk50k"] = {"steps": 10, "dt": 0.01, "pos": digest(p), "vel": digest(v),
+                         "acc": digest(a), "pot": digest(phi)}
+    with open(os.path.join(HERE, "configs.json"), "w") as f:
+        json.dump(configs, f, indent=1, sort_keys=True)
+    print("wrote tiny.npz and configs.json")
+
+
+if __name__ == "__main__":
+    main()
