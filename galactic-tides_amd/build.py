@@ -51,6 +51,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"host library build failed:\n{r.stderr}")
     os.replace(host + ".tmp", host)
+    # C++ test program of the host wrapper (tests/cpp), linked against the library
+    exe = os.path.join(HERE, "test_hip_backend")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"), "-o", exe + ".tmp",
+           os.path.join(ROOT, "tests", "cpp", "test_hip_backend.cpp"), "-L", HERE, "-lgt_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", LIB, "and", host)
     return LIB

@@ -5,13 +5,16 @@
 // or is too small to split (:169-173).  The same tree falls out of a sort:
 //
 //  1. root cube from the bounding box (:313-334);
-//  2. for every particle, its octant path: 21 levels of 3-bit octants found
+//  2. for every particle, its octant path: 3-bit octants, 21 levels per word, found
 //     by comparing against node centres that are updated with exactly the
 //     reference's FP64 operations (centre +- half/2, :258-266), so every
 //     comparison matches the reference's; also the first depth at which the
 //     path's node is degenerate (:172);
 //  3. a stable radix sort of the 63-bit paths (equal paths keep particle
-//     order, as the counting sorts do);
+//     order, as the counting sorts do); when a cluster of more than leaf_size
+//     bodies shares all the levels computed so far, the next 21 levels are
+//     computed and the order refined (rare: coincident or near-coincident
+//     particles; at most three words cover the 60-level limit);
 //  4. lcp[i] = levels shared by sorted neighbours i, i+1.  The depth-d node
 //     containing p holds more than leaf_size bodies iff some window of
 //     leaf_size + 1 consecutive particles around p shares >= d levels, so p's
@@ -120,8 +123,13 @@ __device__ inline bool degenerate(double cx, double cy, double cz, double half) 
   return half < 1e-12 * (fabs(cx) + fabs(cy) + fabs(cz) + 1e-300);
 }
 
+// Digits of levels [21 word, 21 word + 21) of every particle's octant path
+// (the descent from the root replays all earlier levels), and the first depth
+// <= 21 (word + 1) at which the path's node is degenerate.  Word 0 also seeds
+// the sort: keys[i] = path, vals[i] = i.
 __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict__ pos, uint32_t n,
-                                                       const double *__restrict__ root,
+                                                       const double *__restrict__ root, int word,
+                                                       uint64_t *__restrict__ wkey,
                                                        uint64_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals,
                                                        uint8_t *__restrict__ dg) {
@@ -132,37 +140,62 @@ __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict_
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   uint64_t key = 0;
   uint8_t first_dg = kNoDegenerate;
-  for (int l = 0; l < kLevels; ++l) {
+  const int first = kLevels * word, last = first + kLevels;
+  for (int l = 0; l < last; ++l) {
     if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = static_cast<uint8_t>(l);
     const uint32_t o = (x >= cx ? 1u : 0u) | (y >= cy ? 2u : 0u) | (z >= cz ? 4u : 0u);
-    key = (key << 3) | o;
+    if (l >= first) key = (key << 3) | o;
     const double h2 = 0.5 * h;
     cx = cx + ((o & 1u) ? h2 : -h2);
     cy = cy + ((o & 2u) ? h2 : -h2);
     cz = cz + ((o & 4u) ? h2 : -h2);
     h = h2;
   }
-  if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = kLevels;
-  keys[i] = key;
-  vals[i] = i;
+  if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = static_cast<uint8_t>(last);
+  wkey[i] = key;
   dg[i] = first_dg;
+  if (keys) {
+    keys[i] = key;
+    vals[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gather_keys(const uint64_t *__restrict__ wkey,
+                                                      const uint32_t *__restrict__ vals, uint32_t n,
+                                                      uint64_t *__restrict__ keys) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p < n) keys[p] = wkey[vals[p]];
 }
 
 // ---- 4. shared prefixes and leaf depths ---------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restrict__ keys,
+// Levels shared by the paths of sorted neighbours p, p+1 (at most 21 words).
+__global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restrict__ w0,
+                                                        const uint64_t *__restrict__ w1,
+                                                        const uint64_t *__restrict__ w2, int words,
+                                                        const uint32_t *__restrict__ vals,
                                                         uint32_t n, uint8_t *__restrict__ lcp) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i + 1 >= n) return;
-  const uint64_t d = keys[i] ^ keys[i + 1];
-  lcp[i] = d ? static_cast<uint8_t>((__clzll(d) - 1) / 3) : static_cast<uint8_t>(kLevels);
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p + 1 >= n) return;
+  const uint32_t a = vals[p], b = vals[p + 1];
+  const uint64_t *w[3] = {w0, w1, w2};
+  int common = kLevels * words;
+  for (int k = 0; k < words; ++k) {
+    const uint64_t d = w[k][a] ^ w[k][b];
+    if (d) {
+      common = kLevels * k + (__clzll(d) - 1) / 3;
+      break;
+    }
+  }
+  lcp[p] = static_cast<uint8_t>(common);
 }
 
 // Leaf depth of sorted position p and the number of nodes beginning at p.
 __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict__ lcp,
                                                       const uint32_t *__restrict__ vals,
                                                       const uint8_t *__restrict__ dg, uint32_t n,
-                                                      uint32_t leaf, uint8_t *__restrict__ ldep,
+                                                      uint32_t leaf, int cap,
+                                                      uint8_t *__restrict__ ldep,
                                                       uint32_t *__restrict__ starts,
                                                       uint32_t *__restrict__ unresolved) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -173,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
     const uint32_t jlo = p >= leaf ? p - leaf : 0u;
     const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
     for (uint32_t j = jlo; j <= jhi; ++j) {
-      int w = kLevels;
+      int w = cap;
       for (uint32_t k = j; k < j + leaf; ++k) w = min(w, static_cast<int>(lcp[k]));
       best = max(best, w);
     }
@@ -181,8 +214,8 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
   int d = best + 1;
   d = min(d, static_cast<int>(dg[vals[p]]));
   d = min(d, kDepthLimit);
-  if (d > kLevels) atomicOr(unresolved, 1u);
-  ldep[p] = static_cast<uint8_t>(min(d, kLevels));
+  if (d > cap) atomicOr(unresolved, 1u);
+  ldep[p] = static_cast<uint8_t>(min(d, cap));
   const int first = p == 0 ? 0 : static_cast<int>(lcp[p - 1]) + 1;
   starts[p] = d >= first ? static_cast<uint32_t>(d - first + 1) : 0u;
 }
@@ -218,18 +251,22 @@ __global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restric
 
 __global__ __launch_bounds__(kBlock) void link_nodes(
     const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
-    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ keys,
-    const double *__restrict__ root, uint32_t n_nodes, double *__restrict__ geom,
+    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
+    const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
+    const uint32_t *__restrict__ vals, const double *__restrict__ root, uint32_t n_nodes,
+    double *__restrict__ geom,
     int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild) {
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t d = static_cast<uint32_t>(node_depth[x]);
   const uint32_t b = node_begin[x];
   // Centre and half size: replay the octant path of the node's first body.
-  const uint64_t key = keys[b];
+  const uint32_t i = vals[b];
+  const uint64_t *w[3] = {w0, w1, w2};
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   for (uint32_t l = 0; l < d; ++l) {
-    const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l))) & 7u;
+    const uint64_t key = w[l / kLevels][i];
+    const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
     const double h2 = 0.5 * h;
     cx = cx + ((o & 1u) ? h2 : -h2);
     cy = cy + ((o & 2u) ? h2 : -h2);
@@ -434,36 +471,54 @@ void TreeBuilder::build(const double *pos, const double *mass, uint32_t n, int l
   hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, root_.get());
   GTH_LAUNCH_CHECK();
 
-  // 2-3. octant paths, sorted
+  // 2-4. octant paths, sorted, and leaf depths.  One 21-level word of path
+  // is enough unless a cluster of more than leaf_size bodies shares all 21
+  // levels: then the next word is computed and the order refined (sorting
+  // stably by the new word, then by each earlier one), up to three words.
   uint64_t *keys = keys_.ensure(n);
   uint32_t *vals = vals_.ensure(n);
   uint8_t *dg = dg_.ensure(n);
-  hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
-                     keys, vals, dg);
+  uint64_t *w[3] = {word0_.ensure(n), nullptr, nullptr};
+  hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(), 0,
+                     w[0], keys, vals, dg);
   GTH_LAUNCH_CHECK();
   radix_sort_u64(keys, vals, keys_alt_.ensure(n), vals_alt_.ensure(n), n, 3 * kLevels,
                  sort_tmp_.ensure(radix_tmp_words(n)), s);
-
-  // 4. shared prefixes, leaf depths, node counts per start position
   uint8_t *lcp = lcp_.ensure(n);
   uint8_t *ldep = ldep_.ensure(n);
   uint32_t *starts = starts_.ensure(n);
   uint32_t *offs = offs_.ensure(n);
   unsigned long long *misc = misc_.ensure(4);  // [0] node total, [1] unresolved flag
-  launch_fill_u64(misc, 4, 0ull, s);
-  hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, keys, n, lcp);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
-                     ldep, starts, reinterpret_cast<uint32_t *>(misc + 1));
-  GTH_LAUNCH_CHECK();
-  exclusive_scan_u32(starts, offs, n, misc, scan_tmp_.ensure(scan_tmp_words(n)), s);
+  int words = 1;
   unsigned long long host_misc[2];
-  GTH_CHECK(hipMemcpyAsync(host_misc, misc, sizeof(host_misc), hipMemcpyDeviceToHost, s));
-  GTH_CHECK(hipStreamSynchronize(s));
-  if (host_misc[1] & 1ull)
-    fail(GT_HIP_ERR_UNSUPPORTED,
-         "octree build: a cluster of more than leaf_size bodies shares 21 octant levels; "
-         "deeper trees are not supported by the GPU builder yet");
+  for (;;) {
+    launch_fill_u64(misc, 4, 0ull, s);
+    hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
+                       words, vals, n, lcp);
+    GTH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n,
+                       leaf, kLevels * words, ldep, starts,
+                       reinterpret_cast<uint32_t *>(misc + 1));
+    GTH_LAUNCH_CHECK();
+    exclusive_scan_u32(starts, offs, n, misc, scan_tmp_.ensure(scan_tmp_words(n)), s);
+    GTH_CHECK(hipMemcpyAsync(host_misc, misc, sizeof(host_misc), hipMemcpyDeviceToHost, s));
+    GTH_CHECK(hipStreamSynchronize(s));
+    if (!(host_misc[1] & 1ull)) break;
+    if (words == 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
+    DBuf<uint64_t> &wb = words == 1 ? word1_ : word2_;
+    w[words] = wb.ensure(n);
+    hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
+                       words, w[words], static_cast<uint64_t *>(nullptr),
+                       static_cast<uint32_t *>(nullptr), dg);
+    GTH_LAUNCH_CHECK();
+    for (int k = words; k >= 0; --k) {
+      hipLaunchKernelGGL(gather_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[k], vals, n, keys);
+      GTH_LAUNCH_CHECK();
+      radix_sort_u64(keys, vals, keys_alt_.ensure(n), vals_alt_.ensure(n), n, 3 * kLevels,
+                     sort_tmp_.ensure(radix_tmp_words(n)), s);
+    }
+    ++words;
+  }
   if (host_misc[0] > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "octree build: too many nodes");
   const uint32_t nn = static_cast<uint32_t>(host_misc[0]);
   n_nodes_ = nn;
@@ -487,7 +542,7 @@ void TreeBuilder::build(const double *pos, const double *mass, uint32_t n, int l
   uint32_t *nchild = nchild_.ensure(nn);
   launch_fill_u32(nchild, nn, 0u, s);
   hipLaunchKernelGGL(link_nodes, dim3(blocks_for(nn)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
-                     keys, root_.get(), nn, geom, lnk, parent, nchild);
+                     w[0], w[1], w[2], vals, root_.get(), nn, geom, lnk, parent, nchild);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(finish_links, dim3(blocks_for(nn)), dim3(kBlock), 0, s, parent, nchild, nbeg,
                      nn, lnk);

@@ -88,3 +88,15 @@ def test_tree_refuses_wide_groups_and_bad_lists(hip, oracle):
     bad["cell_list"] = cl
     with pytest.raises(ValueError):
         hip.tree(bad, 1.0, 0.01)
+
+
+def test_cpp_host_wrapper_program():
+    # tests/cpp/test_hip_backend.cpp (built by galactic-tides_amd/build.py):
+    # the C++ face of the library, checked in the reference's test_metal.cpp style
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "galactic-tides_amd", "test_hip_backend")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
