@@ -38,6 +38,19 @@ def encounter_2m(halo="hernquist"):
     return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
 
 
+def measured_traffic(kernel="eval_fast"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "eval_fast_traffic_encounter2m.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pos, mass, cfg, budget_s=30.0):
     """The reference's CPU group solver (packed lists, OpenMP on every host
     core) timed on the same particles: one warm-up and best of up to three
@@ -124,7 +137,8 @@ def main():
     # vector ALUs; pairwise gravity has no matrix operation to put on MFMA).
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": "eval_fast", "kernel_ms": round(eval_ms, 4),
+                "traffic": measured_traffic(), "kernel": "eval_fast",
+                "kernel_ms": round(eval_ms, 4),
                 "flops_per_launch": flops, "compute": "fp32 vector ALU"}
 
     cpu = None

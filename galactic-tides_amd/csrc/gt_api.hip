@@ -187,8 +187,11 @@ void run_lists(gt_hip_context *c, const gt_group_config *cfg, const double *geom
                 bl, werr, s);
   }
   // Slots for the next evaluation: a quarter more than this one's longest lists.
-  c->cap_cells = static_cast<uint32_t>(std::min<unsigned long long>(most_c + most_c / 4 + 32, 1u << 30));
-  c->cap_bodies = static_cast<uint32_t>(std::min<unsigned long long>(most_b + most_b / 4 + 32, 1u << 30));
+  // (multiples of 4, so that slots start 16-byte aligned for the walk's vector stores)
+  c->cap_cells = static_cast<uint32_t>(
+      std::min<unsigned long long>((most_c + most_c / 4 + 32 + 3) & ~3ull, 1u << 30));
+  c->cap_bodies = static_cast<uint32_t>(
+      std::min<unsigned long long>((most_b + most_b / 4 + 32 + 3) & ~3ull, 1u << 30));
   c->info.n_groups = ng;
   c->info.n_cell_list = all_c;
   c->info.n_body_list = all_b;

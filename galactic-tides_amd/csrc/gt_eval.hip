@@ -16,7 +16,7 @@
 //    keeping per-target partial sums in registers; a cross-lane reduction
 //    ends the group.  Lanes are busy whenever the list is longer than 64
 //    entries, whatever the group's fill, and the targets' coordinates are
-//    read from LDS as broadcasts.  Group sizes are bucketed (8/16/24/32
+//    read from LDS as broadcasts.  Group sizes are bucketed (multiples of 4
 //    targets per batch) so each path is branch-free over its targets.
 //    v_rsq_f32 and FMA chains; G is folded into the entry's mass and
 //    quadrupole once per entry; per-target sums are reduced with DPP.
@@ -302,7 +302,10 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const gt_group *__restrict__ groups, uint32_t n_groups, float G, float eps2,
     float4 *__restrict__ out) {
   __shared__ float4 targets[kBlock / kWave][32];
-  const uint32_t gi = uniform_wave_id();
+  // XCD-aware: each XCD evaluates a contiguous run of (spatially adjacent)
+  // groups, whose lists gather the same cells, so its L2 holds them.
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t gi = __builtin_amdgcn_readfirstlane(blk * (kBlock / kWave) + threadIdx.x / kWave);
   if (gi >= n_groups) return;
   const uint32_t lane = threadIdx.x % kWave;
   float4 *tgt = targets[threadIdx.x / kWave];
@@ -311,14 +314,21 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   for (uint32_t t0 = 0; t0 < ntot; t0 += 32) {
     const uint32_t nt = min(32u, ntot - t0);
     const uint32_t tb = g.begin + t0;
-    if (nt > 24)
-      eval_batch<32, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
-    else if (nt > 16)
-      eval_batch<24, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
-    else if (nt > 8)
-      eval_batch<16, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
-    else
-      eval_batch<8, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out);
+#define GTH_BATCH(TB) \
+  eval_batch<TB, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out)
+    // Targets per batch rounded up to a multiple of 4: each path is
+    // branch-free over its targets, and at most 3 targets are padding.
+    switch ((nt + 3) / 4) {
+      case 1: GTH_BATCH(4); break;
+      case 2: GTH_BATCH(8); break;
+      case 3: GTH_BATCH(12); break;
+      case 4: GTH_BATCH(16); break;
+      case 5: GTH_BATCH(20); break;
+      case 6: GTH_BATCH(24); break;
+      case 7: GTH_BATCH(28); break;
+      default: GTH_BATCH(32); break;
+    }
+#undef GTH_BATCH
   }
 }
 
@@ -330,7 +340,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
   if (n_groups == 0) return;
   const uint32_t waves_per_block = kBlock / kWave;
   const uint32_t blocks = (n_groups + waves_per_block - 1) / waves_per_block;
-  const dim3 grid(blocks), block(kBlock);
+  const dim3 grid(exact ? blocks : xcd_grid(blocks)), block(kBlock);
 #define GTH_EVAL_ARGS bodies, cells, cell_list, body_list, groups, n_groups, G, eps2, out
   if (exact)
     hipLaunchKernelGGL(eval_exact, grid, block, 0, s, bodies, cells, cell_list, body_list, groups,

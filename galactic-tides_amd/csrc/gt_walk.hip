@@ -135,9 +135,10 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b, const uint32_t *__restrict__ cell_off,
     const uint32_t *__restrict__ body_off, uint32_t *__restrict__ cell_count,
     uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
-    uint32_t *__restrict__ body_list, uint32_t *__restrict__ error) {
-  const uint32_t g = blockIdx.x * kWalkBlock + threadIdx.x;
-  if (g >= ng) return;
+    uint32_t *__restrict__ body_list, uint32_t *__restrict__ error, uint32_t nblocks) {
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t g = blk * kWalkBlock + threadIdx.x;
+  if (blk >= nblocks || g >= ng) return;
   const uint32_t gb = groups[g].begin, ge = groups[g].end;
   const double root_half = root[3];
   double lo[3], hi[3];
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     bpos = g * cap_b;
   }
   uint32_t nc = 0, nb = 0;
+  uint32_t cbuf[4] = {0, 0, 0, 0};  // pending cell ids, indexed by list position mod 4
   uint32_t cur = 0;
   // A depth-first walk visits each node at most once: more visits than nodes,
   // or an index past the end, means a malformed tree.  Stop and report it.
@@ -197,7 +199,19 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     }
     const bool accept = !overlap && d2 > r.crit * r.crit;
     if (accept && !small) {
-      if (MODE == kWalkExact || nc < cap_c) cell_list[cpos + nc] = cur;
+      // Cell ids go out four at a time (one 16-byte store) once the slot is
+      // 16-byte aligned: fewer partial cache-line writes.
+      const uint32_t at = cpos + nc;
+      if (MODE == kWalkExact || nc < cap_c) {
+        cbuf[at & 3u] = cur;
+        if ((at & 3u) == 3u) {
+          if (at >= 3u && at - 3u >= cpos)
+            *reinterpret_cast<uint4 *>(cell_list + (at - 3u)) =
+                make_uint4(cbuf[0], cbuf[1], cbuf[2], cbuf[3]);
+          else
+            for (uint32_t q = cpos; q <= at; ++q) cell_list[q] = cbuf[q & 3u];
+        }
+      }
       ++nc;
       cur = sk;
     } else if (accept || nch == 0) {
@@ -211,6 +225,14 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     } else {
       cur = fc + nch - 1;
     }
+  }
+  // the ids still pending (the last partial group of four)
+  {
+    const uint32_t stored = MODE == kWalkExact ? nc : min(nc, cap_c);
+    const uint32_t end = cpos + stored;
+    uint32_t first = end & ~3u;
+    if (first < cpos) first = cpos;
+    for (uint32_t q = first; q < end; ++q) cell_list[q] = cbuf[q & 3u];
   }
   cell_count[g] = nc;
   body_count[g] = nb;
@@ -526,14 +548,15 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, co
                  uint32_t *body_list, uint32_t *error, hipStream_t s) {
   if (!ng) return;
   if (lanes == 1) {
+    const uint32_t nb = blocks_for(ng, kWalkBlock);
     if (mode == kWalkExact)
-      hipLaunchKernelGGL(walk<kWalkExact>, dim3(blocks_for(ng, kWalkBlock)), dim3(kWalkBlock), 0,
-                         s, recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off,
-                         body_off, cell_count, body_count, cell_list, body_list, error);
+      hipLaunchKernelGGL(walk<kWalkExact>, dim3(xcd_grid(nb)), dim3(kWalkBlock), 0, s, recs, span,
+                         tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off, cell_count,
+                         body_count, cell_list, body_list, error, nb);
     else
-      hipLaunchKernelGGL(walk<kWalkCapacity>, dim3(blocks_for(ng, kWalkBlock)), dim3(kWalkBlock),
-                         0, s, recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off,
-                         body_off, cell_count, body_count, cell_list, body_list, error);
+      hipLaunchKernelGGL(walk<kWalkCapacity>, dim3(xcd_grid(nb)), dim3(kWalkBlock), 0, s, recs,
+                         span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off,
+                         cell_count, body_count, cell_list, body_list, error, nb);
     GTH_LAUNCH_CHECK();
     return;
   }
