@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--exact", action="store_true", help="bit-exact evaluation mode")
     ap.add_argument("--eval-waves", type=int, default=0, help="tuning: 2 or 3 (0: default)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch each step's kernels directly instead of from captured graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -95,28 +97,27 @@ def main():
     ctx.state_set(pos, vel, mass)
     ctx.prime(**cfg)
     dt = 0.01
-    for _ in range(args.warmup):
-        ctx.steps(1, dt, **cfg)
+    graph = not args.eager
+    if args.warmup:
+        ctx.steps(args.warmup, dt, use_graph=graph, **cfg)
 
-    inter, ev_ms, force_ms = [], [], []
+    # The K timed steps: one call, which launches them back to back (from
+    # captured graphs unless --eager) and synchronises once at the end.
+    st0 = ctx.pipeline_stats()
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.steps(1, dt, **cfg)
-        inf = ctx.info()
-        tm = ctx.timings()
-        inter.append(inf["cell_interactions"] + inf["body_interactions"])
-        ev_ms.append(tm["evaluate"])
-        force_ms.append(tm["total"])
+    ctx.steps(args.steps, dt, use_graph=graph, **cfg)
     elapsed = time.perf_counter() - t_start
+    st1 = ctx.pipeline_stats()
     ms_per_step = 1e3 * elapsed / args.steps
     inf = ctx.info()
-    tm = ctx.timings()
-    interactions = float(np.mean(inter))
+    tm = ctx.timings()  # per-step means over the timed steps (HIP events)
+    interactions = (st1["interactions"] - st0["interactions"]) / args.steps
     value = interactions / (ms_per_step * 1e-3)
-    print(f"# step {ms_per_step:.2f} ms, force eval (GPU events) {np.mean(force_ms):.2f} ms: "
+    pipeline = {k: st1[k] - st0[k] for k in st1}
+    print(f"# step {ms_per_step:.2f} ms, force eval (GPU events) {tm['total']:.2f} ms: "
           f"tree {tm['tree']:.2f} lists {tm['lists']:.2f} evaluate {tm['evaluate']:.2f} "
-          f"scatter {tm['scatter']:.2f}; {inf['n_groups']} groups, {inf['n_nodes']} nodes, "
-          f"max list {inf['max_list']}", file=sys.stderr, flush=True)
+          f"scatter+kick {tm['scatter']:.2f}; {inf['n_groups']} groups, {inf['n_nodes']} nodes, "
+          f"max list {inf['max_list']}; {pipeline}", file=sys.stderr, flush=True)
 
     # Force error against FP64 direct summation on a seeded sample of 1000.
     p_now, _, a_now, _ = ctx.state_get()
@@ -131,7 +132,7 @@ def main():
     # flops of the last evaluation / its kernel time measured with HIP events
     # on the stream it runs on.
     flops = CELL_FLOPS * inf["cell_interactions"] + BODY_FLOPS * inf["body_interactions"]
-    eval_ms = float(np.mean(ev_ms))
+    eval_ms = tm["evaluate"]
     achieved = flops / (eval_ms * 1e-3) / 1e12
     # Compute-bound: priced against the FP32 peak (the kernel runs on the
     # vector ALUs; pairwise gravity has no matrix operation to put on MFMA).
@@ -160,7 +161,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
-        "ms_per_force_eval": round(float(np.mean(force_ms)), 3),
+        "ms_per_force_eval": round(tm["total"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -169,7 +170,7 @@ def main():
                 "the reference ICs generator)",
         "config": {"workload": "encounter-2M", "n": n, "theta": 0.75, "group_size": 32,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
-                   "exact": bool(args.exact), "parallelism": "single GPU"},
+                   "exact": bool(args.exact), "graphs": graph, "parallelism": "single GPU"},
         "force_error": {"median": med, "p99": p99, "sample": 1000},
         "interactions_per_step": interactions,
         "roofline": roofline,

@@ -12,6 +12,44 @@
 
 using namespace gth;
 
+// What the previous evaluation on a context needed; bounds for the next one
+// are derived from it.  Zero / -1 / false mean "not known".
+struct History {
+  uint32_t n = 0;  // particles of the evaluation described
+  int words = 1;
+  uint32_t nodes = 0;
+  int depth = -1;
+  uint32_t groups = 0;
+  bool lists_known = false;
+  uint64_t most_c = 0, most_b = 0;
+};
+
+// Settings a captured step graph was made for.
+struct GraphKey {
+  Plan plan;
+  double G = 0, theta = 0, eps = 0, dt = 0;
+  int quadrupole = 0, group_size = 0, leaf_size = 0, pack = 0, exact = 0, eval_waves = 0,
+      walk_lanes = 0;
+  uint64_t generation = 0;
+};
+
+bool same_plan(const Plan &a, const Plan &b) {
+  return a.n == b.n && a.words == b.words && a.node_cap == b.node_cap &&
+         a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
+         a.cap_b == b.cap_b;
+}
+
+bool same_key(const GraphKey &a, const GraphKey &b) {
+  return same_plan(a.plan, b.plan) && a.G == b.G && a.theta == b.theta && a.eps == b.eps &&
+         a.dt == b.dt && a.quadrupole == b.quadrupole && a.group_size == b.group_size &&
+         a.leaf_size == b.leaf_size && a.pack == b.pack && a.exact == b.exact &&
+         a.eval_waves == b.eval_waves && a.walk_lanes == b.walk_lanes &&
+         a.generation == b.generation;
+}
+
+constexpr int kTimedSteps = 64;  // graph steps whose phases are timed with events
+constexpr int kPhases = 4;       // tree, lists, evaluation, scatter + kick
+
 struct gt_hip_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -20,9 +58,16 @@ struct gt_hip_context {
   double gpu_seconds = 0.0;
   gt_lists_info info{};
   gt_timings tm{};
+  gt_pipeline_stats pstats{};
   hipDeviceProp_t prop{};
 
   TreeBuilder tree;
+  // device frame of the current evaluation, and its host copy
+  DBuf<unsigned long long> frame, raw_frame, misc;
+  unsigned long long fw[frame::kWords] = {};
+  History hist;
+  Plan plan;  // bounds of the last evaluation
+  uint32_t debug_shrink = 0;  // test hook: list slots this much too short
   // caller-supplied tree (gt_hip_walk_tree)
   DBuf<double> up_geom, up_tpos, up_tmass;
   DBuf<int4> up_lnk;
@@ -33,12 +78,8 @@ struct gt_hip_context {
   DBuf<float4> bodies, out;
   DBuf<uint32_t> parent, skip, depth, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
       body_list;
-  // per-group list slots of the one-pass walk, from the previous evaluation's
-  // longest lists (0 before the first)
-  uint32_t cap_cells = 0, cap_bodies = 0;
-  uint64_t walk_passes = 0, walk_fallbacks = 0;
   DBuf<gt_group> groups;
-  DBuf<unsigned long long> scan_tmp, misc;
+  DBuf<unsigned long long> scan_tmp;
   uint32_t lists_n = 0, lists_nn = 0;
   const double *lists_geom = nullptr;  // geometry rows the last lists refer to
   // particle state (resident leapfrog) and inputs of single evaluations
@@ -46,6 +87,11 @@ struct gt_hip_context {
   DBuf<double> in_pos, in_mass, in_acc, in_pot;
   uint64_t n_state = 0;
   bool state_forces = false;
+  // captured step graphs
+  hipGraphExec_t step_exec[kPhases] = {};
+  bool step_valid = false;
+  GraphKey step_key;
+  std::vector<hipEvent_t> step_ev;  // kTimedSteps x (kPhases + 1)
   // raw-array entry points and sampling
   DBuf<float4> raw_bodies, raw_out;
   DBuf<Cell> raw_cells;
@@ -110,130 +156,460 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// Group selection, walk and list statistics for a tree held in device memory.
-// Leaves groups, lists, bodies and cells in the context.  The walk writes each
-// group's lists into slots sized from the previous evaluation (one pass); if a
-// group's lists do not fit, its counts say so and the lists are rewritten at
-// exact scanned offsets (the same walk again, so the contents are the same).
-void run_lists(gt_hip_context *c, const gt_group_config *cfg, const double *geom,
-               const int4 *lnk, const double *tpos, const double *tmass, uint32_t n,
-               uint32_t nn) {
-  hipStream_t s = c->stream;
-  c->lists_n = n;
-  c->lists_nn = nn;
-  c->lists_geom = geom;
-  c->info = gt_lists_info{};
-  c->info.n = n;
-  c->info.n_nodes = nn;
-  if (n == 0) return;
-  if (nn >= kRecIndexMask)
-    fail(GT_HIP_ERR_LIMIT, "too many tree nodes for the packed walk records (2^26 - 1)");
-  unsigned long long *misc = c->misc.ensure(16);  // [0..2] totals, [3] error, [4..11] stats
-  launch_fill_u64(misc, 16, 0ull, s);
-  uint32_t *werr = reinterpret_cast<uint32_t *>(misc + 3);
-  uint32_t *parent = c->parent.ensure(nn), *skip = c->skip.ensure(nn), *depth = c->depth.ensure(nn);
-  launch_link_up(lnk, nn, parent, skip, depth, s);
-  WalkRec *recs = c->recs.ensure(nn);
-  uint2 *span = c->span.ensure(nn);
-  Cell *cells = c->cells.ensure(nn);
-  float4 *bodies = c->bodies.ensure(n);
-  launch_prep_lists(geom, lnk, skip, depth, tpos, tmass, n, nn, cfg->theta, recs, span, cells,
-                    bodies, werr, s);
+// ---- bounds --------------------------------------------------------------------
 
-  const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  uint32_t *flags = c->flags.ensure(n), *offs = c->offs.ensure(n);
-  unsigned long long *stmp = c->scan_tmp.ensure(scan_tmp_words(n) + 8);
-  launch_group_starts(lnk, nn, n, static_cast<uint32_t>(cfg->group_size), pack, flags, s);
-  exclusive_scan_u32(flags, offs, n, misc, stmp, s);
-  unsigned long long hm[12];
-  download(hm, misc, 4, s);
-  GTH_CHECK(hipStreamSynchronize(s));
-  if (hm[3] & 2u) fail(GT_HIP_ERR_ARGUMENT, "tree: a node's half size is not root half / 2^depth");
-  const uint32_t ng = static_cast<uint32_t>(hm[0]);
-  gt_group *groups = c->groups.ensure(ng);
-  uint32_t *starts = c->starts.ensure(ng);
-  launch_make_groups(flags, offs, n, ng, starts, groups, s);
+constexpr uint32_t kMaxNodes = kRecIndexMask - 1;  // node ids must fit the walk records
 
-  uint32_t *cc = c->ccount.ensure(ng), *bc = c->bcount.ensure(ng);
-  uint32_t cap_c = c->cap_cells, cap_b = c->cap_bodies;
-  if (static_cast<uint64_t>(ng) * cap_c > 0xffffffffull ||
-      static_cast<uint64_t>(ng) * cap_b > 0xffffffffull)
-    cap_c = cap_b = 0;
-  uint32_t *cl = c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(ng) * cap_c, 1));
-  uint32_t *bl = c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(ng) * cap_b, 1));
-  const int lanes = cfg->walk_lanes == 16 || cfg->walk_lanes == 32 || cfg->walk_lanes == 64
-                        ? cfg->walk_lanes
-                        : 1;
-  launch_walk(kWalkCapacity, lanes, recs, span, tpos, groups, ng, nn, geom, cap_c, cap_b, nullptr,
-              nullptr, cc, bc, cl, bl, werr, s);
-  launch_list_stats(cc, bc, groups, ng, misc + 4, s);
-  download(hm, misc, 12, s);
-  GTH_CHECK(hipStreamSynchronize(s));
-  ++c->walk_passes;
-  if (hm[3] & 1u) fail(GT_HIP_ERR_DEVICE, "group walk: malformed tree (walk did not terminate)");
-  const unsigned long long most_c = hm[8], most_b = hm[9], all_c = hm[10], all_b = hm[11];
-  if (all_c > 0xffffffffull || all_b > 0xffffffffull)
-    fail(GT_HIP_ERR_LIMIT, "interaction lists too long for 32 bit offsets");
-  if (most_c > cap_c || most_b > cap_b) {
-    // Some group did not fit its slots: place every group at scanned offsets.
-    ++c->walk_fallbacks;
-    uint32_t *co = c->coff.ensure(ng), *bo = c->boff.ensure(ng);
-    stmp = c->scan_tmp.ensure(scan_tmp_words(ng) + 8);
-    exclusive_scan_u32(cc, co, ng, misc + 1, stmp, s);
-    exclusive_scan_u32(bc, bo, ng, misc + 2, stmp, s);
-    cl = c->cell_list.ensure(std::max<unsigned long long>(all_c, 1));
-    bl = c->body_list.ensure(std::max<unsigned long long>(all_b, 1));
-    launch_walk(kWalkExact, lanes, recs, span, tpos, groups, ng, nn, geom, 0, 0, co, bo, cc, bc, cl,
-                bl, werr, s);
-  }
-  // Slots for the next evaluation: a quarter more than this one's longest lists.
-  // (multiples of 4, so that slots start 16-byte aligned for the walk's vector stores)
-  c->cap_cells = static_cast<uint32_t>(
-      std::min<unsigned long long>((most_c + most_c / 4 + 32 + 3) & ~3ull, 1u << 30));
-  c->cap_bodies = static_cast<uint32_t>(
-      std::min<unsigned long long>((most_b + most_b / 4 + 32 + 3) & ~3ull, 1u << 30));
-  c->info.n_groups = ng;
-  c->info.n_cell_list = all_c;
-  c->info.n_body_list = all_b;
-  c->info.cell_interactions = hm[4];
-  c->info.body_interactions = hm[5];
-  c->info.max_list = hm[6];
-  c->info.max_group = hm[7];
+uint32_t grown(uint64_t x, uint64_t extra, uint64_t cap) {
+  return static_cast<uint32_t>(std::min<uint64_t>(x + x / 4 + extra, cap));
 }
 
-void run_eval(gt_hip_context *c, const gt_group_config *cfg) {
-  float4 *out = c->out.ensure(std::max<uint32_t>(c->lists_n, 1));
+// List slot for groups whose longest list so far is `most`: a quarter more, a
+// multiple of 4 (slots start 16-byte aligned for the walk's vector stores).
+uint32_t slot_for(uint64_t most) {
+  return static_cast<uint32_t>(std::min<uint64_t>((most + most / 4 + 32 + 3) & ~3ull, 1u << 30));
+}
+
+// Whether the bounds p still hold what the history says with some headroom
+// (then they are kept, so buffers and captured graphs stay as they are).
+bool plan_holds(const Plan &p, const History &h, uint32_t n) {
+  if (p.n != n || h.n != n || p.words < h.words || h.nodes == 0 || h.depth < 0 ||
+      h.groups == 0 || !h.lists_known)
+    return false;
+  if (static_cast<uint64_t>(h.nodes) + h.nodes / 16 > p.node_cap) return false;
+  if (h.depth + 1 > p.depth_cap && p.depth_cap < 60) return false;
+  if (p.group_cap < n && static_cast<uint64_t>(h.groups) + h.groups / 16 > p.group_cap) return false;
+  if (h.most_c + h.most_c / 16 + 8 > p.cap_c || h.most_b + h.most_b / 16 + 8 > p.cap_b)
+    return false;
+  return true;
+}
+
+// Whether the history describes every size of an evaluation of n particles.
+bool sizes_known(const History &h, uint32_t n) {
+  return h.n == n && h.nodes && h.depth >= 0 && h.groups && h.lists_known;
+}
+
+// Bounds for an evaluation of n particles: the last ones while they hold,
+// else new ones from the history with headroom (kept in c->plan).
+Plan bounds_for(gt_hip_context *c, uint32_t n) {
+  const History &h = c->hist;
+  if (plan_holds(c->plan, h, n)) return c->plan;
+  Plan p;
+  p.n = n;
+  const bool same = h.n == n;
+  p.words = same ? h.words : 1;
+  p.node_cap = same && h.nodes ? grown(h.nodes, 1024, kMaxNodes)
+                               : static_cast<uint32_t>(std::min<uint64_t>(n + 1024ull, kMaxNodes));
+  p.depth_cap = same && h.depth >= 0 ? std::min(60, h.depth + 3) : 60;
+  p.group_cap = same && h.groups ? grown(h.groups, 64, n) : n;
+  if (h.lists_known) {
+    p.cap_c = slot_for(h.most_c);
+    p.cap_b = slot_for(h.most_b);
+  }
+  if (static_cast<uint64_t>(p.group_cap) * p.cap_c > 0xffffffffull ||
+      static_cast<uint64_t>(p.group_cap) * p.cap_b > 0xffffffffull)
+    p.cap_c = p.cap_b = 0;
+  c->plan = p;
+  return p;
+}
+
+Plan plan_for(gt_hip_context *c, uint32_t n) {
+  Plan p = bounds_for(c, n);
+  const History &h = c->hist;
+  if (c->debug_shrink && h.lists_known) {  // test hook: slots too short
+    const uint64_t k = c->debug_shrink;
+    p.cap_c = h.most_c > k ? static_cast<uint32_t>(h.most_c - k) & ~3u : 0u;
+    p.cap_b = h.most_b > k ? static_cast<uint32_t>(h.most_b - k) & ~3u : 0u;
+  }
+  return p;
+}
+
+// What an evaluation that ended with status st says about the sizes needed.
+void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
+  const unsigned long long *fw = c->fw;
+  History &h = c->hist;
+  if (h.n != p.n) h = History{};
+  h.n = p.n;
+  h.words = p.words;
+  if (st & kAbortPaths) {
+    if (p.words >= 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
+    h.words = p.words + 1;
+  }
+  if (fw[frame::kNodes] > kMaxNodes)
+    fail(GT_HIP_ERR_LIMIT, "too many tree nodes for the packed walk records (2^26 - 1)");
+  h.nodes = static_cast<uint32_t>(fw[frame::kNodes]);
+  if (!(st & kAbortNodes)) {
+    h.depth = static_cast<int>(fw[frame::kDepth]);
+    h.groups = static_cast<uint32_t>(fw[frame::kGroups]);
+  }
+  if ((st & ~static_cast<unsigned long long>(kAbortLists)) == 0) {
+    h.lists_known = true;
+    h.most_c = fw[frame::kStats + 4];
+    h.most_b = fw[frame::kStats + 5];
+  }
+}
+
+void read_frame(gt_hip_context *c) {
+  download(c->fw, c->frame.get(), frame::kWords, c->stream);
+  GTH_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// Walk errors: a malformed tree, or node half sizes that are not the root's
+// halved (code: the error for a caller-supplied tree, or an internal one).
+void check_walk(const gt_hip_context *c, unsigned long long st, int code) {
+  if (c->fw[frame::kWalkError] & 2ull)
+    fail(code, "tree: a node's half size is not root half / 2^depth");
+  if (st & kAbortWalk) fail(code, "group walk: malformed tree (walk did not terminate)");
+}
+
+// ---- the phases of one evaluation, enqueued with no host round trip --------------
+
+// Every device buffer an evaluation with bounds p needs.
+void reserve_all(gt_hip_context *c, const Plan &p) {
+  const size_t n = std::max<uint32_t>(p.n, 1), nn = std::max<uint32_t>(p.node_cap, 1),
+               gc = std::max<uint32_t>(p.group_cap, 1);
+  c->tree.reserve(p);
+  c->parent.ensure(nn);
+  c->skip.ensure(nn);
+  c->depth.ensure(nn);
+  c->recs.ensure(nn);
+  c->span.ensure(nn);
+  c->cells.ensure(nn);
+  c->bodies.ensure(n);
+  c->flags.ensure(n);
+  c->offs.ensure(n);
+  c->starts.ensure(n);
+  c->scan_tmp.ensure(scan_tmp_words(n) + 8);
+  c->groups.ensure(gc);
+  c->ccount.ensure(gc);
+  c->bcount.ensure(gc);
+  c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_c, 1));
+  c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_b, 1));
+  c->out.ensure(n);
+}
+
+int walk_lanes(const gt_group_config *cfg) {
+  return cfg->walk_lanes == 16 || cfg->walk_lanes == 32 || cfg->walk_lanes == 64 ? cfg->walk_lanes
+                                                                                  : 1;
+}
+
+// Clears the counts of the frame (not its status or step counter).
+void clear_counts(unsigned long long *frame, hipStream_t s) {
+  launch_fill_u64(frame + frame::kNodes, frame::kStats + 8 - frame::kNodes, 0ull, s);
+}
+
+// Group selection, walk into per-group slots, statistics and the status check,
+// for a tree in device memory whose node count is frame[kNodes].
+void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                   const double *geom, const int4 *lnk, const double *tpos, const double *tmass) {
+  hipStream_t s = c->stream;
+  unsigned long long *frame = c->frame.get();
+  reserve_all(c, p);
+  const uint32_t n = p.n, cap = p.node_cap;
+  launch_link_up(lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
+  launch_prep_lists(geom, lnk, c->skip.get(), c->depth.get(), tpos, tmass, n, frame, cap,
+                    cfg->theta, c->recs.get(), c->span.get(), c->cells.get(), c->bodies.get(), s);
+  const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
+  launch_group_starts(lnk, frame, cap, n, static_cast<uint32_t>(cfg->group_size), pack,
+                      c->flags.get(), s);
+  exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
+                     s);
+  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, c->starts.get(),
+                     c->groups.get(), s);
+  launch_walk(kWalkCapacity, walk_lanes(cfg), c->recs.get(), c->span.get(), tpos, c->groups.get(),
+              frame, p.group_cap, cap, geom, p.cap_c, p.cap_b, nullptr, nullptr, c->ccount.get(),
+              c->bcount.get(), c->cell_list.get(), c->body_list.get(), s);
+  launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s);
+  launch_check_frame(frame, p, true, s);
+}
+
+// The lists again at exact (scanned) offsets, for an evaluation whose only
+// problem was slots too small.  Needs the counts of the capacity walk.
+void lists_exact(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, const double *geom,
+                 const double *tpos) {
+  hipStream_t s = c->stream;
+  unsigned long long *frame = c->frame.get();
+  const unsigned long long ng = c->fw[frame::kGroups], all_c = c->fw[frame::kStats + 6],
+                           all_b = c->fw[frame::kStats + 7];
+  if (all_c > 0xffffffffull || all_b > 0xffffffffull)
+    fail(GT_HIP_ERR_LIMIT, "interaction lists too long for 32 bit offsets");
+  ++c->pstats.list_rewrites;
+  uint32_t *co = c->coff.ensure(std::max<unsigned long long>(ng, 1));
+  uint32_t *bo = c->boff.ensure(std::max<unsigned long long>(ng, 1));
+  unsigned long long *stmp = c->scan_tmp.ensure(scan_tmp_words(ng) + 8);
+  unsigned long long *tot = c->misc.ensure(4);
+  exclusive_scan_u32(c->ccount.get(), co, ng, tot, stmp, s);
+  exclusive_scan_u32(c->bcount.get(), bo, ng, tot + 1, stmp, s);
+  uint32_t *cl = c->cell_list.ensure(std::max<unsigned long long>(all_c, 1));
+  uint32_t *bl = c->body_list.ensure(std::max<unsigned long long>(all_b, 1));
+  launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+  launch_walk(kWalkExact, walk_lanes(cfg), c->recs.get(), c->span.get(), tpos, c->groups.get(),
+              frame, p.group_cap, p.node_cap, geom, 0, 0, co, bo, c->ccount.get(), c->bcount.get(),
+              cl, bl, s);
+  launch_check_frame(frame, p, false, s);
+}
+
+void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
   const float G = static_cast<float>(cfg->G);
   const float eps2 = static_cast<float>(cfg->eps * cfg->eps);
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
-              c->groups.get(), static_cast<uint32_t>(c->info.n_groups), G, eps2,
-              cfg->quadrupole ? 1u : 0u, cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, out,
-              c->stream);
+              c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
+              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), c->stream);
 }
 
-// One force evaluation of particles already in device memory: build, lists,
-// evaluate, scatter.  Phase times go to c->tm.
+// After a complete evaluation: sizes into the history and the statistics
+// (counted in the pipeline counters unless `count` is false).
+void finish(gt_hip_context *c, const Plan &p, const double *geom, bool count = true) {
+  const unsigned long long *fw = c->fw;
+  learn(c, p, 0);
+  const uint32_t nn = static_cast<uint32_t>(fw[frame::kNodes]);
+  c->info = gt_lists_info{};
+  c->info.n = p.n;
+  c->info.n_nodes = nn;
+  c->info.depth = fw[frame::kDepth];
+  c->info.n_groups = fw[frame::kGroups];
+  c->info.cell_interactions = fw[frame::kStats + 0];
+  c->info.body_interactions = fw[frame::kStats + 1];
+  c->info.max_list = fw[frame::kStats + 2];
+  c->info.max_group = fw[frame::kStats + 3];
+  c->info.n_cell_list = fw[frame::kStats + 6];
+  c->info.n_body_list = fw[frame::kStats + 7];
+  c->lists_n = p.n;
+  c->lists_nn = nn;
+  c->lists_geom = geom;
+  if (count) {
+    ++c->pstats.evaluations;
+    c->pstats.interactions += c->info.cell_interactions + c->info.body_interactions;
+  }
+}
+
+// One force evaluation of particles in device memory: build, lists, evaluate,
+// scatter, enqueued with no host round trip, then one synchronisation to read
+// the status.  Slots too small: the lists are rewritten at exact offsets and
+// evaluated.  Any other bound too small: everything again with the sizes the
+// status reported.  Phase times go to c->tm.
 void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos,
                 const double *mass, uint32_t n, double *acc, double *pot) {
   hipStream_t s = c->stream;
-  GTH_CHECK(hipEventRecord(c->ev[0], s));
-  c->tree.build(pos, mass, n, cfg->leaf_size, s);
-  GTH_CHECK(hipEventRecord(c->ev[1], s));
-  run_lists(c, cfg, c->tree.geom(), c->tree.links(), c->tree.tree_pos(), c->tree.tree_mass(), n,
-            c->tree.n_nodes());
-  c->info.depth = static_cast<uint64_t>(c->tree.depth());
-  GTH_CHECK(hipEventRecord(c->ev[2], s));
-  run_eval(c, cfg);
-  GTH_CHECK(hipEventRecord(c->ev[3], s));
-  launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, s);
-  GTH_CHECK(hipEventRecord(c->ev[4], s));
-  GTH_CHECK(hipStreamSynchronize(s));
+  unsigned long long *frame = c->frame.get();
+  for (int attempt = 0;; ++attempt) {
+    if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "force evaluation: bounds did not settle");
+    const Plan p = plan_for(c, n);
+    const double *geom = c->tree.geom();
+    launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+    clear_counts(frame, s);
+    GTH_CHECK(hipEventRecord(c->ev[0], s));
+    c->tree.enqueue(pos, mass, p, cfg->leaf_size, frame, s);
+    geom = c->tree.geom();
+    GTH_CHECK(hipEventRecord(c->ev[1], s));
+    enqueue_lists(c, cfg, p, geom, c->tree.links(), c->tree.tree_pos(), c->tree.tree_mass());
+    GTH_CHECK(hipEventRecord(c->ev[2], s));
+    enqueue_eval(c, cfg, p);
+    GTH_CHECK(hipEventRecord(c->ev[3], s));
+    launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+    GTH_CHECK(hipEventRecord(c->ev[4], s));
+    read_frame(c);
+    unsigned long long st = c->fw[frame::kStatus];
+    check_walk(c, st, GT_HIP_ERR_DEVICE);
+    if (st == kAbortLists) {
+      learn(c, p, st);
+      lists_exact(c, cfg, p, geom, c->tree.tree_pos());
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      enqueue_eval(c, cfg, p);
+      GTH_CHECK(hipEventRecord(c->ev[3], s));
+      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      GTH_CHECK(hipEventRecord(c->ev[4], s));
+      read_frame(c);
+      st = c->fw[frame::kStatus];
+      check_walk(c, st, GT_HIP_ERR_DEVICE);
+      if (st) fail(GT_HIP_ERR_DEVICE, "group walk: exact lists incomplete");
+    }
+    if (st == 0) {
+      finish(c, p, geom);
+      c->tree.set_counts(static_cast<uint32_t>(c->info.n_nodes), static_cast<int>(c->info.depth));
+      break;
+    }
+    learn(c, p, st);
+    ++c->pstats.reruns;
+  }
   c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
   c->tm.lists = elapsed_ms(c->ev[1], c->ev[2]);
   c->tm.evaluate = elapsed_ms(c->ev[2], c->ev[3]);
   c->tm.scatter = elapsed_ms(c->ev[3], c->ev[4]);
   c->tm.total = elapsed_ms(c->ev[0], c->ev[4]);
   c->gpu_seconds = c->tm.total * 1e-3;
+}
+
+// ---- leapfrog steps ---------------------------------------------------------------
+
+void eager_step(gt_hip_context *c, const gt_group_config *cfg, double dt) {
+  const uint64_t n = c->n_state;
+  const double h = 0.5 * dt;
+  unsigned long long *status = c->frame.get() + frame::kStatus;
+  launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt, status, c->stream);
+  force_eval(c, cfg, c->pos.get(), c->mass.get(), static_cast<uint32_t>(n), c->acc.get(),
+             c->pot.get());
+  launch_kick(c->vel.get(), c->acc.get(), n, h, status, c->stream);
+}
+
+void destroy_step_graphs(gt_hip_context *c) {
+  for (auto &e : c->step_exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  c->step_valid = false;
+}
+
+// The kernels of one leapfrog step in phase `phase` (all of them read the
+// frame's status, so a step after one that outgrew its bounds does nothing
+// to the state).
+void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, double dt,
+                        int phase) {
+  hipStream_t s = c->stream;
+  unsigned long long *frame = c->frame.get();
+  const uint64_t n = c->n_state;
+  const double h = 0.5 * dt;
+  switch (phase) {
+    case 0:
+      launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt,
+                        frame + frame::kStatus, s);
+      clear_counts(frame, s);
+      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, frame, s);
+      break;
+    case 1:
+      enqueue_lists(c, cfg, p, c->tree.geom(), c->tree.links(), c->tree.tree_pos(),
+                    c->tree.tree_mass());
+      break;
+    case 2:
+      enqueue_eval(c, cfg, p);
+      break;
+    default:
+      launch_scatter(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
+                     c->pot.get(), frame + frame::kStatus, s);
+      launch_kick(c->vel.get(), c->acc.get(), n, h, frame + frame::kStatus, s);
+      launch_count_step(frame, s);
+      break;
+  }
+}
+
+GraphKey key_for(const gt_group_config *cfg, const Plan &p, double dt) {
+  GraphKey k;
+  k.plan = p;
+  k.G = cfg->G;
+  k.theta = cfg->theta;
+  k.eps = cfg->eps;
+  k.dt = dt;
+  k.quadrupole = cfg->quadrupole != 0;
+  k.group_size = cfg->group_size;
+  k.leaf_size = cfg->leaf_size;
+  k.pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
+  k.exact = cfg->exact != 0;
+  k.eval_waves = cfg->eval_waves == 2 ? 2 : 3;
+  k.walk_lanes = walk_lanes(cfg);
+  k.generation = alloc_generation;
+  return k;
+}
+
+// Captures the step's four phases as graphs for bounds p (unless the ones held
+// were made for the same bounds, settings and buffers).
+void prepare_step_graphs(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, double dt) {
+  reserve_all(c, p);
+  const GraphKey key = key_for(cfg, p, dt);
+  if (c->step_valid && same_key(c->step_key, key)) return;
+  destroy_step_graphs(c);
+  hipStream_t s = c->stream;
+  for (int phase = 0; phase < kPhases; ++phase) {
+    GTH_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_step_phase(c, cfg, p, dt, phase);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hipGraph_t g = nullptr;
+    GTH_CHECK(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&c->step_exec[phase], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    GTH_CHECK(e);
+  }
+  if (alloc_generation != key.generation)
+    fail(GT_HIP_ERR_DEVICE, "step graph: a buffer was allocated during capture");
+  c->step_key = key;
+  c->step_valid = true;
+  ++c->pstats.graph_captures;
+  if (c->step_ev.empty()) {
+    c->step_ev.assign(static_cast<size_t>(kTimedSteps) * (kPhases + 1), nullptr);
+    for (auto &e : c->step_ev) GTH_CHECK(hipEventCreate(&e));
+  }
+}
+
+// nsteps leapfrog steps from captured graphs, launched back to back; a step
+// that outgrows the bounds stops the rest (their kernels see the status), is
+// finished here without graphs, and the remaining steps go on from new graphs.
+void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, double dt) {
+  hipStream_t s = c->stream;
+  unsigned long long *frame = c->frame.get();
+  const uint32_t n = static_cast<uint32_t>(c->n_state);
+  int done_total = 0;
+  while (done_total < nsteps) {
+    if (!sizes_known(c->hist, n)) {  // sizes not known yet: one step without graphs
+      eager_step(c, cfg, dt);
+      ++done_total;
+      continue;
+    }
+    const Plan p = plan_for(c, n);
+    prepare_step_graphs(c, cfg, p, dt);
+    const int k = nsteps - done_total;
+    launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+    launch_fill_u64(frame + frame::kStepsDone, 2, 0ull, s);  // and kStepInteractions
+    for (int i = 0; i < k; ++i) {
+      hipEvent_t *ev = i < kTimedSteps ? &c->step_ev[static_cast<size_t>(i) * (kPhases + 1)]
+                                       : nullptr;
+      if (ev) GTH_CHECK(hipEventRecord(ev[0], s));
+      for (int phase = 0; phase < kPhases; ++phase) {
+        GTH_CHECK(hipGraphLaunch(c->step_exec[phase], s));
+        if (ev) GTH_CHECK(hipEventRecord(ev[phase + 1], s));
+      }
+    }
+    read_frame(c);
+    const int done = static_cast<int>(c->fw[frame::kStepsDone]);
+    const unsigned long long st = c->fw[frame::kStatus];
+    c->pstats.graph_steps += done;
+    c->pstats.evaluations += done;
+    c->pstats.interactions += c->fw[frame::kStepInteractions];
+    if (done > 0) {
+      double t[kPhases] = {}, total = 0.0;
+      const int timed = std::min(done, kTimedSteps);
+      for (int i = 0; i < timed; ++i) {
+        hipEvent_t *ev = &c->step_ev[static_cast<size_t>(i) * (kPhases + 1)];
+        for (int phase = 0; phase < kPhases; ++phase) t[phase] += elapsed_ms(ev[phase], ev[phase + 1]);
+        total += elapsed_ms(ev[0], ev[kPhases]);
+      }
+      c->tm.tree = t[0] / timed;
+      c->tm.lists = t[1] / timed;
+      c->tm.evaluate = t[2] / timed;
+      c->tm.scatter = t[3] / timed;
+      c->tm.total = total / timed;
+      c->gpu_seconds = c->tm.total * 1e-3;
+    }
+    done_total += done;
+    if (st == 0) {
+      // counts of the last step
+      finish(c, p, c->tree.geom(), false);
+      c->tree.set_counts(static_cast<uint32_t>(c->info.n_nodes), static_cast<int>(c->info.depth));
+      continue;
+    }
+    check_walk(c, st, GT_HIP_ERR_DEVICE);
+    // Step done_total ran its kick and drift only: its forces and second kick here.
+    ++c->pstats.graph_stops;
+    learn(c, p, st);
+    force_eval(c, cfg, c->pos.get(), c->mass.get(), n, c->acc.get(), c->pot.get());
+    launch_kick(c->vel.get(), c->acc.get(), n, 0.5 * dt, frame + frame::kStatus, s);
+    ++done_total;
+  }
 }
 
 }  // namespace
@@ -259,6 +635,9 @@ int gt_hip_create(gt_hip_context **out, int device) {
     GTH_CHECK(hipGetDeviceProperties(&c->prop, device));
     GTH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) GTH_CHECK(hipEventCreate(&e));
+    launch_fill_u64(c->frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
+    launch_fill_u64(c->raw_frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
+    GTH_CHECK(hipStreamSynchronize(c->stream));
     g_create_error.clear();
   } catch (const Error &e) {
     g_create_error = e.what();
@@ -279,6 +658,9 @@ void gt_hip_destroy(gt_hip_context *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_step_graphs(c);
+  for (auto &e : c->step_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -360,8 +742,11 @@ int gt_hip_tree(gt_hip_context *c, const gt_float4 *bodies, uint64_t n_cells,
     upload(dcl, cell_list, n_cell_list, s);
     upload(dbl, body_list, n_body_list, s);
     upload(dg, groups, ng, s);
+    unsigned long long *rf = c->raw_frame.get();
+    launch_fill_u64(rf, frame::kWords, 0ull, s);
+    launch_fill_u64(rf + frame::kGroups, 1, ng, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
-    launch_eval(db, dc, dcl, dbl, dg, ng, p->G, p->eps2, p->quadrupole, exact != 0, 2, dout, s);
+    launch_eval(db, dc, dcl, dbl, dg, rf, ng, p->G, p->eps2, p->quadrupole, exact != 0, 2, dout, s);
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     download(reinterpret_cast<float4 *>(out), dout, n, s);
     GTH_CHECK(hipStreamSynchronize(s));
@@ -404,6 +789,8 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
       fail(GT_HIP_ERR_LIMIT, "too many particles or nodes for the 32 bit GPU indices");
     require(n == 0 || (n_nodes > 0 && geom && links && tree_pos && tree_mass),
             "tree arrays are NULL");
+    if (n_nodes >= kRecIndexMask)
+      fail(GT_HIP_ERR_LIMIT, "too many tree nodes for the packed walk records (2^26 - 1)");
     // The walk trusts the links: check them here so a bad tree is an error,
     // not an out-of-bounds access on the GPU.
     for (uint64_t k = 0; k < n_nodes; ++k) {
@@ -435,16 +822,51 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
     upload(dl, reinterpret_cast<const int4 *>(links), n_nodes, s);
     upload(dp, tree_pos, 3 * n, s);
     upload(dm, tree_mass, n, s);
-    GTH_CHECK(hipEventRecord(c->ev[1], s));
-    run_lists(c, cfg, dg, dl, dp, dm, static_cast<uint32_t>(n), static_cast<uint32_t>(n_nodes));
+    c->info = gt_lists_info{};
+    c->tm = gt_timings{};
+    if (n == 0) {
+      GTH_CHECK(hipStreamSynchronize(s));
+      c->lists_n = 0;
+      return;
+    }
+    unsigned long long *frame = c->frame.get();
+    Plan p = plan_for(c, static_cast<uint32_t>(n));
+    p.node_cap = static_cast<uint32_t>(n_nodes);
+    for (int attempt = 0;; ++attempt) {
+      if (attempt == 4) fail(GT_HIP_ERR_DEVICE, "group walk: bounds did not settle");
+      launch_fill_u64(frame, frame::kStats + 8, 0ull, s);
+      launch_fill_u64(frame + frame::kNodes, 1, n_nodes, s);
+      GTH_CHECK(hipEventRecord(c->ev[1], s));
+      enqueue_lists(c, cfg, p, dg, dl, dp, dm);
+      read_frame(c);
+      unsigned long long st = c->fw[frame::kStatus];
+      check_walk(c, st, GT_HIP_ERR_ARGUMENT);
+      if (st == kAbortLists) {
+        learn(c, p, st);
+        lists_exact(c, cfg, p, dg, dp);
+        read_frame(c);
+        st = c->fw[frame::kStatus];
+        check_walk(c, st, GT_HIP_ERR_ARGUMENT);
+        if (st) fail(GT_HIP_ERR_DEVICE, "group walk: exact lists incomplete");
+      }
+      if (st == 0) break;
+      // more groups than the bound: walk again with room for all of them
+      p.group_cap = static_cast<uint32_t>(
+          std::min<unsigned long long>(std::max<unsigned long long>(c->fw[frame::kGroups], 1), n));
+      p.cap_c = p.cap_b = 0;
+      ++c->pstats.reruns;
+    }
+    finish(c, p, dg);
+    // The depth of a caller's tree is not known here: the next GPU build
+    // starts from default bounds.
+    c->hist.depth = -1;
     GTH_CHECK(hipEventRecord(c->ev[2], s));
-    if (out && n) {
-      run_eval(c, cfg);
+    if (out) {
+      enqueue_eval(c, cfg, p);
       download(reinterpret_cast<float4 *>(out), c->out.get(), n, s);
     }
     GTH_CHECK(hipEventRecord(c->ev[3], s));
     GTH_CHECK(hipStreamSynchronize(s));
-    c->tm = gt_timings{};
     c->tm.lists = elapsed_ms(c->ev[1], c->ev[2]);
     c->tm.evaluate = elapsed_ms(c->ev[2], c->ev[3]);
     c->tm.total = c->tm.lists + c->tm.evaluate;
@@ -576,18 +998,31 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     check_config(cfg);
     require(nsteps >= 0, "nsteps must be non-negative");
     require(c->state_forces, "forces are not current: call gt_hip_prime first");
-    (void)use_graph;
     GTH_CHECK(hipSetDevice(c->device));
     const uint64_t n = c->n_state;
-    const double h = 0.5 * dt;
-    for (int step = 0; step < nsteps && n; ++step) {
-      launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt, c->stream);
-      force_eval(c, cfg, c->pos.get(), c->mass.get(), static_cast<uint32_t>(n), c->acc.get(),
-                 c->pot.get());
-      launch_kick(c->vel.get(), c->acc.get(), n, h, c->stream);
-    }
+    if (n == 0 || nsteps == 0) return;
+    // Until the steps have finished the state is between steps: a failure
+    // leaves it so, and the forces must be recomputed (gt_hip_prime).
+    c->state_forces = false;
+    if (use_graph)
+      graph_steps(c, cfg, nsteps, dt);
+    else
+      for (int step = 0; step < nsteps; ++step) eager_step(c, cfg, dt);
     GTH_CHECK(hipStreamSynchronize(c->stream));
+    c->state_forces = true;
   });
+}
+
+int gt_hip_debug_shrink_slots(gt_hip_context *c, uint32_t shrink) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  c->debug_shrink = shrink;
+  return GT_HIP_OK;
+}
+
+int gt_hip_pipeline_stats(const gt_hip_context *c, gt_pipeline_stats *st) {
+  if (!c || !st) return GT_HIP_ERR_ARGUMENT;
+  *st = c->pstats;
+  return GT_HIP_OK;
 }
 
 int gt_hip_direct_f64_sample(gt_hip_context *c, uint64_t n, const double *pos,

@@ -14,24 +14,38 @@ namespace gth {
 // positions and masses.
 class TreeBuilder {
  public:
-  void build(const double *pos, const double *mass, uint32_t n, int leaf_size, hipStream_t s);
+  // Allocates everything an evaluation with these bounds needs (so that a
+  // later enqueue with the same plan allocates nothing, as graph capture needs).
+  void reserve(const Plan &p);
+  // Enqueues the build on s with no host round trip.  The node total, the
+  // depth and the "more path words needed" flag go to the frame; the host
+  // reads them after the evaluation and hands the node count back with
+  // set_counts.  A tree with more nodes than p.node_cap is not built.
+  void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
+               unsigned long long *frame, hipStream_t s);
+  void set_counts(uint32_t n_nodes, int depth) {
+    n_nodes_ = n_nodes;
+    depth_ = depth;
+  }
   uint32_t n() const { return n_; }
   uint32_t n_nodes() const { return n_nodes_; }
+  uint32_t node_cap() const { return node_cap_; }
   int depth() const { return depth_; }
   const double *geom() const { return geom_.get(); }
   const int4 *links() const { return lnk_.get(); }
   const uint32_t *order() const { return vals_.get(); }
   const double *tree_pos() const { return tpos_.get(); }
   const double *tree_mass() const { return tmass_.get(); }
+  const double *root() const { return root_.get(); }
 
  private:
-  uint32_t n_ = 0, n_nodes_ = 0;
+  uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0;
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, ndep_, ndep_alt_, word0_, word1_, word2_;
   DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, nchild_;
   DBuf<uint8_t> dg_, lcp_, ldep_;
-  DBuf<unsigned long long> sort_tmp_, scan_tmp_, misc_;
+  DBuf<unsigned long long> sort_tmp_, scan_tmp_;
   DBuf<int4> lnk_;
 };
 

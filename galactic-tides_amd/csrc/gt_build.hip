@@ -214,8 +214,12 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
   int d = best + 1;
   d = min(d, static_cast<int>(dg[vals[p]]));
   d = min(d, kDepthLimit);
+  // Deeper than the path words computed: reported, and the tree is cut at
+  // `cap` levels (nodes exist only where their octant digits do), to be built
+  // again with another word.
   if (d > cap) atomicOr(unresolved, 1u);
-  ldep[p] = static_cast<uint8_t>(min(d, cap));
+  d = min(d, cap);
+  ldep[p] = static_cast<uint8_t>(d);
   const int first = p == 0 ? 0 : static_cast<int>(lcp[p - 1]) + 1;
   starts[p] = d >= first ? static_cast<uint32_t>(d - first + 1) : 0u;
 }
@@ -225,10 +229,12 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
 __global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__ lcp,
                                                      const uint32_t *__restrict__ starts,
                                                      const uint32_t *__restrict__ offs, uint32_t n,
+                                                     const unsigned long long *__restrict__ frame,
+                                                     uint32_t node_cap,
                                                      uint64_t *__restrict__ node_depth,
                                                      uint32_t *__restrict__ node_begin) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n) return;
+  if (p >= n || frame[frame::kNodes] > node_cap) return;  // over capacity: no tree
   const uint32_t c = starts[p];
   const uint32_t first = p == 0 ? 0u : static_cast<uint32_t>(lcp[p - 1]) + 1u;
   for (uint32_t t = 0; t < c; ++t) {
@@ -237,14 +243,21 @@ __global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__
   }
 }
 
+// lstart[d] = first node of level d (levels past the deepest stay 0), and the
+// depth of the tree into the frame.
 __global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restrict__ node_depth,
-                                                       uint32_t n_nodes,
+                                                       unsigned long long *__restrict__ frame,
+                                                       uint32_t node_cap,
                                                        uint32_t *__restrict__ lstart) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t d = static_cast<uint32_t>(node_depth[x]);
   if (x == 0 || static_cast<uint32_t>(node_depth[x - 1]) != d) lstart[d] = x;
-  if (x == n_nodes - 1) lstart[d + 1] = n_nodes;
+  if (x == n_nodes - 1) {
+    lstart[d + 1] = n_nodes;
+    frame[frame::kDepth] = d;
+  }
 }
 
 // ---- 6. links, geometry, leaf order ------------------------------------------------
@@ -253,9 +266,10 @@ __global__ __launch_bounds__(kBlock) void link_nodes(
     const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
     const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
     const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
-    const uint32_t *__restrict__ vals, const double *__restrict__ root, uint32_t n_nodes,
-    double *__restrict__ geom,
+    const uint32_t *__restrict__ vals, const double *__restrict__ root,
+    const unsigned long long *__restrict__ frame, uint32_t node_cap, double *__restrict__ geom,
     int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t d = static_cast<uint32_t>(node_depth[x]);
@@ -300,7 +314,9 @@ __global__ __launch_bounds__(kBlock) void link_nodes(
 __global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restrict__ parent,
                                                        const uint32_t *__restrict__ nchild,
                                                        const uint32_t *__restrict__ node_begin,
-                                                       uint32_t n_nodes, int4 *__restrict__ lnk) {
+                                                       const unsigned long long *__restrict__ frame,
+                                                       uint32_t node_cap, int4 *__restrict__ lnk) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t p = parent[x];
@@ -312,8 +328,10 @@ __global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restric
 // (for a last child) its parent's end; the root ends at n.
 __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__ parent,
                                                     const uint32_t *__restrict__ node_begin,
-                                                    uint32_t n_nodes, uint32_t n,
+                                                    const unsigned long long *__restrict__ frame,
+                                                    uint32_t node_cap, uint32_t n,
                                                     int4 *__restrict__ lnk) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
   if (x0 >= n_nodes) return;
   uint32_t x = x0;
@@ -333,8 +351,11 @@ __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__
 }
 
 // Bodies of a leaf back into particle order (insertion sort; leaves are small).
-__global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ lnk, uint32_t n_nodes,
+__global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ lnk,
+                                                     const unsigned long long *__restrict__ frame,
+                                                     uint32_t node_cap,
                                                      uint32_t *__restrict__ vals) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const int4 L = lnk[x];
@@ -365,14 +386,12 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
 
 // ---- 7. moments, one level at a time -------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void level_moments(uint32_t first, uint32_t last,
-                                                        uint32_t n_nodes,
-                                                        const int4 *__restrict__ lnk,
-                                                        const double *__restrict__ tpos,
-                                                        const double *__restrict__ tmass,
-                                                        double *__restrict__ geom) {
-  const uint32_t x = first + blockIdx.x * kBlock + threadIdx.x;
-  if (x >= last) return;
+// Moments of node x from its bodies (leaf) or its children, which are on the
+// level below and already done.
+__device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__restrict__ lnk,
+                                    const double *__restrict__ tpos,
+                                    const double *__restrict__ tmass,
+                                    double *__restrict__ geom) {
   const int4 L = lnk[x];
   double *g = geom + static_cast<size_t>(kGeomStride) * x;
   const double ccx = g[0], ccy = g[1], ccz = g[2];
@@ -450,126 +469,162 @@ __global__ __launch_bounds__(kBlock) void level_moments(uint32_t first, uint32_t
   g[14] = __dsqrt_rn(ex * ex + ey * ey + ez * ez);
 }
 
+// Every node of level d: lstart[d] .. lstart[d + 1] (empty for levels past the
+// deepest, whose lstart entries are 0).
+__global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *__restrict__ lstart,
+                                                        const unsigned long long *__restrict__ frame,
+                                                        uint32_t node_cap,
+                                                        const int4 *__restrict__ lnk,
+                                                        const double *__restrict__ tpos,
+                                                        const double *__restrict__ tmass,
+                                                        double *__restrict__ geom) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
+  for (uint32_t x = first + blockIdx.x * kBlock + threadIdx.x; x < last; x += gridDim.x * kBlock)
+    node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+}
+
 #pragma clang fp contract(fast)
 
 }  // namespace
 
-void TreeBuilder::build(const double *pos, const double *mass, uint32_t n, int leaf_size,
-                        hipStream_t s) {
+void TreeBuilder::reserve(const Plan &p) {
+  const size_t n = std::max<uint32_t>(p.n, 1), nn = std::max<uint32_t>(p.node_cap, 1);
+  const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
+  part_.ensure(6 * static_cast<size_t>(nparts));
+  root_.ensure(4);
+  keys_.ensure(n);
+  vals_.ensure(n);
+  keys_alt_.ensure(n);
+  vals_alt_.ensure(n);
+  dg_.ensure(n);
+  word0_.ensure(n);
+  if (p.words > 1) word1_.ensure(n);
+  if (p.words > 2) word2_.ensure(n);
+  sort_tmp_.ensure(radix_tmp_words(std::max(n, nn)));
+  lcp_.ensure(n);
+  ldep_.ensure(n);
+  starts_.ensure(n);
+  offs_.ensure(n);
+  scan_tmp_.ensure(scan_tmp_words(n));
+  ndep_.ensure(nn);
+  nbeg_.ensure(nn);
+  ndep_alt_.ensure(nn);
+  nbeg_alt_.ensure(nn);
+  lstart_.ensure(kDepthLimit + 2);
+  geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
+  lnk_.ensure(nn);
+  parent_.ensure(nn);
+  nchild_.ensure(nn);
+  tpos_.ensure(3 * n);
+  tmass_.ensure(n);
+}
+
+void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
+                          unsigned long long *frame, hipStream_t s) {
+  const uint32_t n = p.n;
   n_ = n;
-  n_nodes_ = 0;
-  depth_ = 0;
+  node_cap_ = p.node_cap;
   if (n == 0) return;
+  if (p.words < 1 || p.words > 3) fail(GT_HIP_ERR_ARGUMENT, "octree build: 1 to 3 path words");
+  if (p.depth_cap < 0 || p.depth_cap > kDepthLimit)
+    fail(GT_HIP_ERR_ARGUMENT, "octree build: depth bound out of range");
+  reserve(p);
   const uint32_t leaf = static_cast<uint32_t>(leaf_size < 1 ? 1 : leaf_size);
+  const uint32_t cap = p.node_cap;
 
   // 1. root cube
   const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
-  double *part = part_.ensure(6 * static_cast<size_t>(nparts));
-  root_.ensure(4);
+  double *part = part_.get();
   hipLaunchKernelGGL(bbox_partial, dim3(nparts), dim3(kBlock), 0, s, pos, n, part);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, root_.get());
   GTH_LAUNCH_CHECK();
 
-  // 2-4. octant paths, sorted, and leaf depths.  One 21-level word of path
-  // is enough unless a cluster of more than leaf_size bodies shares all 21
-  // levels: then the next word is computed and the order refined (sorting
-  // stably by the new word, then by each earlier one), up to three words.
-  uint64_t *keys = keys_.ensure(n);
-  uint32_t *vals = vals_.ensure(n);
-  uint8_t *dg = dg_.ensure(n);
-  uint64_t *w[3] = {word0_.ensure(n), nullptr, nullptr};
+  // 2-3. octant paths (p.words 21-level words of each) and their stable sort:
+  // by the last word first, then by each earlier one.  One word is enough
+  // unless a cluster of more than leaf_size bodies shares all 21 levels; the
+  // leaf depths report it, and the host builds again with another word.
+  uint64_t *keys = keys_.get();
+  uint32_t *vals = vals_.get();
+  uint8_t *dg = dg_.get();
+  uint64_t *w[3] = {word0_.get(), p.words > 1 ? word1_.get() : nullptr,
+                    p.words > 2 ? word2_.get() : nullptr};
   hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(), 0,
                      w[0], keys, vals, dg);
   GTH_LAUNCH_CHECK();
-  radix_sort_u64(keys, vals, keys_alt_.ensure(n), vals_alt_.ensure(n), n, 3 * kLevels,
-                 sort_tmp_.ensure(radix_tmp_words(n)), s);
-  uint8_t *lcp = lcp_.ensure(n);
-  uint8_t *ldep = ldep_.ensure(n);
-  uint32_t *starts = starts_.ensure(n);
-  uint32_t *offs = offs_.ensure(n);
-  unsigned long long *misc = misc_.ensure(4);  // [0] node total, [1] unresolved flag
-  int words = 1;
-  unsigned long long host_misc[2];
-  for (;;) {
-    launch_fill_u64(misc, 4, 0ull, s);
-    hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
-                       words, vals, n, lcp);
-    GTH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n,
-                       leaf, kLevels * words, ldep, starts,
-                       reinterpret_cast<uint32_t *>(misc + 1));
-    GTH_LAUNCH_CHECK();
-    exclusive_scan_u32(starts, offs, n, misc, scan_tmp_.ensure(scan_tmp_words(n)), s);
-    GTH_CHECK(hipMemcpyAsync(host_misc, misc, sizeof(host_misc), hipMemcpyDeviceToHost, s));
-    GTH_CHECK(hipStreamSynchronize(s));
-    if (!(host_misc[1] & 1ull)) break;
-    if (words == 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
-    DBuf<uint64_t> &wb = words == 1 ? word1_ : word2_;
-    w[words] = wb.ensure(n);
+  for (int k = 1; k < p.words; ++k) {
     hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
-                       words, w[words], static_cast<uint64_t *>(nullptr),
-                       static_cast<uint32_t *>(nullptr), dg);
+                       k, w[k], static_cast<uint64_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                       dg);
     GTH_LAUNCH_CHECK();
-    for (int k = words; k >= 0; --k) {
+  }
+  for (int k = p.words - 1; k >= 0; --k) {
+    if (p.words > 1) {
       hipLaunchKernelGGL(gather_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[k], vals, n, keys);
       GTH_LAUNCH_CHECK();
-      radix_sort_u64(keys, vals, keys_alt_.ensure(n), vals_alt_.ensure(n), n, 3 * kLevels,
-                     sort_tmp_.ensure(radix_tmp_words(n)), s);
     }
-    ++words;
+    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, 3 * kLevels, sort_tmp_.get(),
+                   s);
   }
-  if (host_misc[0] > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "octree build: too many nodes");
-  const uint32_t nn = static_cast<uint32_t>(host_misc[0]);
-  n_nodes_ = nn;
+
+  // 4. leaf depths and node counts per sorted position; the node total goes
+  // to frame[kNodes]
+  uint8_t *lcp = lcp_.get();
+  uint8_t *ldep = ldep_.get();
+  uint32_t *starts = starts_.get();
+  uint32_t *offs = offs_.get();
+  hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
+                     p.words, vals, n, lcp);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
+                     kLevels * p.words, ldep, starts,
+                     reinterpret_cast<uint32_t *>(frame + frame::kUnresolved));
+  GTH_LAUNCH_CHECK();
+  exclusive_scan_u32(starts, offs, n, frame + frame::kNodes, scan_tmp_.get(), s);
 
   // 5. nodes in preorder, then stably by depth (level order)
-  uint64_t *ndep = ndep_.ensure(nn);
-  uint32_t *nbeg = nbeg_.ensure(nn);
+  uint64_t *ndep = ndep_.get();
+  uint32_t *nbeg = nbeg_.get();
   hipLaunchKernelGGL(emit_nodes, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, starts, offs, n,
-                     ndep, nbeg);
+                     frame, cap, ndep, nbeg);
   GTH_LAUNCH_CHECK();
-  radix_sort_u64(ndep, nbeg, ndep_alt_.ensure(nn), nbeg_alt_.ensure(nn), nn, 6,
-                 sort_tmp_.ensure(radix_tmp_words(nn)), s);
-  uint32_t *lstart = lstart_.ensure(kDepthLimit + 2);
-  hipLaunchKernelGGL(level_starts, dim3(blocks_for(nn)), dim3(kBlock), 0, s, ndep, nn, lstart);
+  radix_sort_u64_dev(ndep, nbeg, ndep_alt_.get(), nbeg_alt_.get(), frame + frame::kNodes, cap, 6,
+                     sort_tmp_.get(), s);
+  uint32_t *lstart = lstart_.get();
+  launch_fill_u32(lstart, kDepthLimit + 2, 0u, s);
+  hipLaunchKernelGGL(level_starts, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, frame, cap,
+                     lstart);
   GTH_LAUNCH_CHECK();
 
   // 6. links and geometry
-  double *geom = geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
-  int4 *lnk = lnk_.ensure(nn);
-  uint32_t *parent = parent_.ensure(nn);
-  uint32_t *nchild = nchild_.ensure(nn);
-  launch_fill_u32(nchild, nn, 0u, s);
-  hipLaunchKernelGGL(link_nodes, dim3(blocks_for(nn)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
-                     w[0], w[1], w[2], vals, root_.get(), nn, geom, lnk, parent, nchild);
+  double *geom = geom_.get();
+  int4 *lnk = lnk_.get();
+  uint32_t *parent = parent_.get();
+  uint32_t *nchild = nchild_.get();
+  launch_fill_u32(nchild, cap, 0u, s);
+  hipLaunchKernelGGL(link_nodes, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
+                     w[0], w[1], w[2], vals, root_.get(), frame, cap, geom, lnk, parent, nchild);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finish_links, dim3(blocks_for(nn)), dim3(kBlock), 0, s, parent, nchild, nbeg,
-                     nn, lnk);
+  hipLaunchKernelGGL(finish_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nchild, nbeg,
+                     frame, cap, lnk);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(node_ends, dim3(blocks_for(nn)), dim3(kBlock), 0, s, parent, nbeg, nn, n,
-                     lnk);
+  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, frame, cap,
+                     n, lnk);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(nn)), dim3(kBlock), 0, s, lnk, nn, vals);
+  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, vals);
   GTH_LAUNCH_CHECK();
-  double *tpos = tpos_.ensure(3 * static_cast<size_t>(n));
-  double *tmass = tmass_.ensure(n);
+  double *tpos = tpos_.get();
+  double *tmass = tmass_.get();
   hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
                      n, tpos, tmass);
   GTH_LAUNCH_CHECK();
 
-  // 7. moments, deepest level first
-  uint32_t hl[kDepthLimit + 2];
-  GTH_CHECK(hipMemcpyAsync(hl, lstart, sizeof(hl), hipMemcpyDeviceToHost, s));
-  uint64_t last_depth = 0;
-  GTH_CHECK(hipMemcpyAsync(&last_depth, ndep + (nn - 1), sizeof(uint64_t), hipMemcpyDeviceToHost,
-                           s));
-  GTH_CHECK(hipStreamSynchronize(s));
-  depth_ = static_cast<int>(last_depth);
-  for (int d = depth_; d >= 0; --d) {
-    const uint32_t a = hl[d], b = hl[d + 1];
-    if (b <= a || b > nn) fail(GT_HIP_ERR_DEVICE, "octree build: inconsistent levels");
-    hipLaunchKernelGGL(level_moments, dim3(blocks_for(b - a)), dim3(kBlock), 0, s, a, b, nn, lnk,
+  // 7. moments, deepest level first, down from the deepest level the plan
+  // allows (a deeper tree is reported by the depth in the frame)
+  const uint32_t mgrid = std::min<uint32_t>(blocks_for(cap), 1024u);
+  for (int d = p.depth_cap; d >= 0; --d) {
+    hipLaunchKernelGGL(level_moments, dim3(mgrid), dim3(kBlock), 0, s, d, lstart, frame, cap, lnk,
                        tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
   }

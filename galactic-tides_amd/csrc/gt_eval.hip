@@ -96,8 +96,10 @@ __device__ inline Acc exact_cell(Acc a, float px, float py, float pz, float4 cm,
 __global__ __launch_bounds__(kBlock) void eval_exact(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
     const uint32_t *__restrict__ cell_list, const uint32_t *__restrict__ body_list,
-    const gt_group *__restrict__ groups, uint32_t n_groups, float G, float eps2, uint32_t quad,
-    float4 *__restrict__ out) {
+    const gt_group *__restrict__ groups, const unsigned long long *__restrict__ frame,
+    uint32_t group_cap, float G, float eps2, uint32_t quad, float4 *__restrict__ out) {
+  if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
+  const uint32_t n_groups = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t gi = uniform_wave_id();
   if (gi >= n_groups) return;  // the whole wave leaves together
   const uint32_t lane = threadIdx.x % kWave;
@@ -299,13 +301,18 @@ template <bool QUAD, int WAVES>
 __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
     const uint32_t *__restrict__ cell_list, const uint32_t *__restrict__ body_list,
-    const gt_group *__restrict__ groups, uint32_t n_groups, float G, float eps2,
-    float4 *__restrict__ out) {
+    const gt_group *__restrict__ groups, const unsigned long long *__restrict__ frame,
+    uint32_t group_cap, float G, float eps2, float4 *__restrict__ out) {
   __shared__ float4 targets[kBlock / kWave][32];
+  if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
+  const uint32_t n_groups = bounded_count(frame + frame::kGroups, group_cap);
   // XCD-aware: each XCD evaluates a contiguous run of (spatially adjacent)
   // groups, whose lists gather the same cells, so its L2 holds them.
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t gi = __builtin_amdgcn_readfirstlane(blk * (kBlock / kWave) + threadIdx.x / kWave);
+  const uint32_t waves_per_block = kBlock / kWave;
+  const uint32_t blk =
+      xcd_block_used(blockIdx.x, (n_groups + waves_per_block - 1) / waves_per_block);
+  if (blk == kNone) return;
+  const uint32_t gi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
   if (gi >= n_groups) return;
   const uint32_t lane = threadIdx.x % kWave;
   float4 *tgt = targets[threadIdx.x / kWave];
@@ -335,16 +342,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
 }  // namespace
 
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
-                 const uint32_t *body_list, const gt_group *groups, uint32_t n_groups, float G,
-                 float eps2, uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s) {
-  if (n_groups == 0) return;
+                 const uint32_t *body_list, const gt_group *groups,
+                 const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
+                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s) {
+  if (group_cap == 0) return;
   const uint32_t waves_per_block = kBlock / kWave;
-  const uint32_t blocks = (n_groups + waves_per_block - 1) / waves_per_block;
+  const uint32_t blocks = (group_cap + waves_per_block - 1) / waves_per_block;
   const dim3 grid(exact ? blocks : xcd_grid(blocks)), block(kBlock);
-#define GTH_EVAL_ARGS bodies, cells, cell_list, body_list, groups, n_groups, G, eps2, out
+#define GTH_EVAL_ARGS bodies, cells, cell_list, body_list, groups, frame, group_cap, G, eps2, out
   if (exact)
     hipLaunchKernelGGL(eval_exact, grid, block, 0, s, bodies, cells, cell_list, body_list, groups,
-                       n_groups, G, eps2, quad, out);
+                       frame, group_cap, G, eps2, quad, out);
   else if (quad && waves == 3)
     hipLaunchKernelGGL((eval_fast<true, 3>), grid, block, 0, s, GTH_EVAL_ARGS);
   else if (quad)

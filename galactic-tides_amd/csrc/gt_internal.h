@@ -34,6 +34,10 @@ struct Error : std::runtime_error {
 
 // ---- device buffers -----------------------------------------------------------
 
+// Bumped by every device (re)allocation of a DBuf: a captured graph holds raw
+// pointers, so it is valid only while this has not changed since its capture.
+inline uint64_t alloc_generation = 0;
+
 // Device array that keeps its storage between calls and only grows
 // (geometrically), so steady-state steps allocate nothing.
 template <class T>
@@ -55,6 +59,7 @@ class DBuf {
       cap_ = 0;
       GTH_CHECK(hipMalloc(reinterpret_cast<void **>(&p_), c * sizeof(T)));
       cap_ = c;
+      ++alloc_generation;
     }
     return p_;
   }
@@ -113,6 +118,62 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
   return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
 
+// The same when only `used` of the launched blocks have work, a number known on
+// the device alone (the grid was sized from a host-side bound): the work is
+// still split evenly over the XCDs.  Returns kNone for blocks with no work.
+// Needs a grid of at least xcd_grid(used) blocks.
+__device__ inline uint32_t xcd_block_used(uint32_t b, uint32_t used) {
+  const uint32_t q = (used + kXcds - 1) / kXcds, i = b / kXcds;
+  const uint32_t logical = (b % kXcds) * q + i;
+  return i < q && logical < used ? logical : 0xffffffffu;
+}
+
+// ---- device-side counts of one evaluation -------------------------------------
+
+// The counts that later launches of an evaluation depend on (nodes, groups,
+// list lengths) stay in device memory, in a frame of 64-bit words, so an
+// evaluation is enqueued with no host round trip and can be captured in a
+// graph.  Grids are sized from host-side bounds (Plan); kernels read the
+// true count and idle past it.  A bound that proves too small sets a status
+// bit; kernels whose inputs would then be incomplete see a count of zero or
+// return at once, and the host runs the evaluation again with larger bounds.
+namespace frame {
+constexpr int kStatus = 0;      // kAbort* bits
+constexpr int kNodes = 1;       // nodes the tree needs (scan total)
+constexpr int kUnresolved = 2;  // bit 0: a leaf needs more octant-path words
+constexpr int kGroups = 3;      // groups (scan total)
+constexpr int kDepth = 4;       // level of the deepest node
+constexpr int kWalkError = 5;   // bit 0: malformed tree; bit 1: half != root half / 2^depth
+constexpr int kStats = 6;       // 8 words, see list_stats
+constexpr int kStepsDone = 14;  // leapfrog steps completed since the counter was cleared
+constexpr int kStepInteractions = 15;  // their cell + body interactions
+constexpr int kWords = 16;
+}  // namespace frame
+
+constexpr uint32_t kAbortNodes = 1u;   // more nodes than Plan::node_cap
+constexpr uint32_t kAbortPaths = 2u;   // more octant-path words needed than Plan::words
+constexpr uint32_t kAbortDepth = 4u;   // deeper than Plan::depth_cap
+constexpr uint32_t kAbortGroups = 8u;  // more groups than Plan::group_cap
+constexpr uint32_t kAbortLists = 16u;  // a group's lists do not fit the slots
+constexpr uint32_t kAbortWalk = 32u;   // malformed tree
+
+// Host-side bounds of one evaluation, fixed before it is enqueued.
+struct Plan {
+  uint32_t n = 0;          // particles
+  int words = 1;           // 21-level octant-path words computed per particle
+  uint32_t node_cap = 0;   // node arrays
+  int depth_cap = 60;      // deepest level whose moments are computed
+  uint32_t group_cap = 0;  // groups covered by the walk and evaluation grids
+  uint32_t cap_c = 0, cap_b = 0;  // per-group list slots (0: lists always overflow)
+};
+
+// A device count that must not exceed cap; past it (an overflow the status
+// reports) the count reads as zero, so dependent kernels do nothing.
+__device__ inline uint32_t bounded_count(const unsigned long long *c, uint32_t cap) {
+  const unsigned long long v = *c;
+  return v <= cap ? static_cast<uint32_t>(v) : 0u;
+}
+
 // ---- launchers (defined in the .hip sources) -------------------------------------
 
 // gt_scan.hip: exclusive prefix sum of n uint32 values into out (may alias in),
@@ -127,11 +188,18 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned lo
 size_t radix_tmp_words(size_t n);
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                     size_t n, int bits, unsigned long long *tmp, hipStream_t s);
+// The same for the first bounded_count(n_dev, cap) pairs of arrays of cap entries.
+void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                        const unsigned long long *n_dev, size_t cap, int bits,
+                        unsigned long long *tmp, hipStream_t s);
 
 // gt_eval.hip
+// Evaluates the first frame[kGroups] groups (at most group_cap) unless
+// frame[kStatus] is set.
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
-                 const uint32_t *body_list, const gt_group *groups, uint32_t n_groups, float G,
-                 float eps2, uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s);
+                 const uint32_t *body_list, const gt_group *groups,
+                 const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
+                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s);
 
 // gt_direct.hip
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
@@ -143,30 +211,41 @@ void launch_direct_f64_sample(const double *pos, const double *mass, uint64_t n,
 // gt_walk.hip
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
-void launch_link_up(const int4 *lnk, uint32_t n_nodes, uint32_t *parent, uint32_t *skip,
-                    uint32_t *depth, hipStream_t s);
+// Kernels over nodes take the node count as frame + kNodes (bounded by the
+// node capacity the grid was sized for), over groups as frame + kGroups.
+void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
+                    uint32_t *parent, uint32_t *skip, uint32_t *depth, hipStream_t s);
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       uint32_t n_nodes, double theta, WalkRec *recs, uint2 *span, Cell *cells,
-                       float4 *bodies, uint32_t *error, hipStream_t s);
-void launch_group_starts(const int4 *lnk, uint32_t n_nodes, uint32_t n, uint32_t gs, int pack,
-                         uint32_t *flags, hipStream_t s);
-void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n, uint32_t ng,
-                        uint32_t *starts, gt_group *groups, hipStream_t s);
+                       unsigned long long *frame, uint32_t node_cap, double theta, WalkRec *recs,
+                       uint2 *span, Cell *cells, float4 *bodies, hipStream_t s);
+void launch_group_starts(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
+                         uint32_t n, uint32_t gs, int pack, uint32_t *flags, hipStream_t s);
+void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
+                        const unsigned long long *frame, uint32_t group_cap, uint32_t *starts,
+                        gt_group *groups, hipStream_t s);
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
-                 gt_group *groups, uint32_t ng, uint32_t nn, const double *root, uint32_t cap_c,
-                 uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
-                 uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
-                 uint32_t *body_list, uint32_t *error, hipStream_t s);
+                 gt_group *groups, unsigned long long *frame, uint32_t group_cap,
+                 uint32_t node_cap, const double *root, uint32_t cap_c, uint32_t cap_b,
+                 const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
+                 uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s);
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
-                       const gt_group *groups, uint32_t ng, unsigned long long *stats,
+                       const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s);
+// Sets frame[kStatus] from the counts against the plan's bounds (list slots
+// are checked only when check_slots).
+void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_slots,
+                        hipStream_t s);
 
-// gt_step.hip
+// gt_step.hip: each does nothing once status (a frame's kStatus word) is set.
 void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
-                    double *pot, hipStream_t s);
+                    double *pot, const unsigned long long *status, hipStream_t s);
 void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,
-                       double dt, hipStream_t s);
-void launch_kick(double *vel, const double *acc, size_t n, double h, hipStream_t s);
+                       double dt, const unsigned long long *status, hipStream_t s);
+void launch_kick(double *vel, const double *acc, size_t n, double h,
+                 const unsigned long long *status, hipStream_t s);
+// Unless frame[kStatus] is set: frame[kStepsDone] += 1 and the step's
+// interactions are added to frame[kStepInteractions].
+void launch_count_step(unsigned long long *frame, hipStream_t s);
 
 }  // namespace gth

@@ -7,6 +7,7 @@
 // which keeps equal keys in input order.  Used for the octree's octant-path
 // keys, where stability is what reproduces the reference's stable counting
 // sort (src/octree.cpp:233-250).
+#include <algorithm>
 #include <utility>
 
 #include "gt_internal.h"
@@ -21,10 +22,20 @@ constexpr int kWave = 64;
 constexpr int kWaves = kBlock / kWave;
 constexpr int kDigits = 256;
 
+// Pairs to sort: n, or (n_dev set) the device count bounded by n.
+__device__ inline size_t sort_count(size_t n, const unsigned long long *n_dev) {
+  if (!n_dev) return n;
+  const unsigned long long v = *n_dev;
+  return v <= n ? static_cast<size_t>(v) : 0;
+}
+
 __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restrict__ keys,
-                                                         size_t n, int shift, uint32_t ntiles,
+                                                         size_t cap,
+                                                         const unsigned long long *n_dev,
+                                                         int shift, uint32_t ntiles,
                                                          uint32_t *__restrict__ counts) {
   __shared__ uint32_t h[kDigits];
+  const size_t n = sort_count(cap, n_dev);
   h[threadIdx.x] = 0;
   __syncthreads();
   const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
@@ -38,9 +49,11 @@ __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restr
 }
 
 __global__ __launch_bounds__(kBlock) void tile_scatter(
-    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, size_t n, int shift,
-    uint32_t ntiles, const uint32_t *__restrict__ offsets, uint64_t *__restrict__ keys_out,
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, size_t cap,
+    const unsigned long long *n_dev, int shift, uint32_t ntiles,
+    const uint32_t *__restrict__ offsets, uint64_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out) {
+  const size_t n = sort_count(cap, n_dev);
   __shared__ uint32_t run[kDigits];             // keys of each digit placed so far in this tile
   __shared__ uint32_t wave_cnt[kWaves][kDigits];  // per-round, per-wave digit counts
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
@@ -85,6 +98,49 @@ __global__ __launch_bounds__(kBlock) void tile_scatter(
   }
 }
 
+__global__ __launch_bounds__(kBlock) void copy_pairs(const uint64_t *__restrict__ kin,
+                                                     const uint32_t *__restrict__ vin, size_t cap,
+                                                     const unsigned long long *n_dev,
+                                                     uint64_t *__restrict__ kout,
+                                                     uint32_t *__restrict__ vout) {
+  const size_t n = sort_count(cap, n_dev);
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kBlock) {
+    kout[i] = kin[i];
+    vout[i] = vin[i];
+  }
+}
+
+void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                 size_t n, const unsigned long long *n_dev, int bits, unsigned long long *tmp,
+                 hipStream_t s) {
+  const size_t ntiles = (n + kTile - 1) / kTile;
+  const size_t ncounts = static_cast<size_t>(kDigits) * ntiles;
+  uint32_t *counts = reinterpret_cast<uint32_t *>(tmp);
+  unsigned long long *scan_tmp = tmp + (ncounts + 1) / 2;
+  unsigned long long *total = scan_tmp + scan_tmp_words(ncounts);
+  const int passes = (bits + 7) / 8;
+  uint64_t *kin = keys, *kout = keys_alt;
+  uint32_t *vin = vals, *vout = vals_alt;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(tile_histogram, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
+                       kin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts);
+    GTH_LAUNCH_CHECK();
+    exclusive_scan_u32(counts, counts, ncounts, total, scan_tmp, s);
+    hipLaunchKernelGGL(tile_scatter, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, kin,
+                       vin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts, kout, vout);
+    GTH_LAUNCH_CHECK();
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (kin != keys) {
+    const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((n + kBlock - 1) / kBlock, 2048));
+    hipLaunchKernelGGL(copy_pairs, dim3(grid), dim3(kBlock), 0, s, kin, vin, n, n_dev, keys, vals);
+    GTH_LAUNCH_CHECK();
+  }
+}
+
 }  // namespace
 
 size_t radix_tmp_words(size_t n) {
@@ -97,30 +153,14 @@ size_t radix_tmp_words(size_t n) {
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                     size_t n, int bits, unsigned long long *tmp, hipStream_t s) {
   if (n <= 1 || bits <= 0) return;
-  const size_t ntiles = (n + kTile - 1) / kTile;
-  const size_t ncounts = static_cast<size_t>(kDigits) * ntiles;
-  uint32_t *counts = reinterpret_cast<uint32_t *>(tmp);
-  unsigned long long *scan_tmp = tmp + (ncounts + 1) / 2;
-  unsigned long long *total = scan_tmp + scan_tmp_words(ncounts);
-  const int passes = (bits + 7) / 8;
-  uint64_t *kin = keys, *kout = keys_alt;
-  uint32_t *vin = vals, *vout = vals_alt;
-  for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(tile_histogram, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
-                       kin, n, shift, static_cast<uint32_t>(ntiles), counts);
-    GTH_LAUNCH_CHECK();
-    exclusive_scan_u32(counts, counts, ncounts, total, scan_tmp, s);
-    hipLaunchKernelGGL(tile_scatter, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, kin,
-                       vin, n, shift, static_cast<uint32_t>(ntiles), counts, kout, vout);
-    GTH_LAUNCH_CHECK();
-    std::swap(kin, kout);
-    std::swap(vin, vout);
-  }
-  if (kin != keys) {
-    GTH_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    GTH_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  }
+  sort_passes(keys, vals, keys_alt, vals_alt, n, nullptr, bits, tmp, s);
+}
+
+void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                        const unsigned long long *n_dev, size_t cap, int bits,
+                        unsigned long long *tmp, hipStream_t s) {
+  if (cap <= 1 || bits <= 0) return;
+  sort_passes(keys, vals, keys_alt, vals_alt, cap, n_dev, bits, tmp, s);
 }
 
 }  // namespace gth

@@ -49,8 +49,10 @@ __device__ inline uint64_t pack_walk_bits(uint32_t first_child, uint32_t skip, u
 }
 
 __global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ lnk,
-                                                      uint32_t n_nodes,
+                                                      const unsigned long long *__restrict__ frame,
+                                                      uint32_t node_cap,
                                                       uint32_t *__restrict__ parent) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= n_nodes) return;
   if (p == 0) parent[0] = kNone;
@@ -60,9 +62,11 @@ __global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ l
 
 __global__ __launch_bounds__(kBlock) void set_skip_depth(const int4 *__restrict__ lnk,
                                                          const uint32_t *__restrict__ parent,
-                                                         uint32_t n_nodes,
+                                                         const unsigned long long *__restrict__ frame,
+                                                         uint32_t node_cap,
                                                          uint32_t *__restrict__ skip,
                                                          uint32_t *__restrict__ depth) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
   if (x0 >= n_nodes) return;
   uint32_t d = 0;
@@ -90,8 +94,10 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const double *__restrict__ geom, const int4 *__restrict__ lnk,
     const uint32_t *__restrict__ skip, const uint32_t *__restrict__ depth,
     const double *__restrict__ tpos, const double *__restrict__ tmass, uint32_t n,
-    uint32_t n_nodes, double inv_theta, WalkRec *__restrict__ recs, uint2 *__restrict__ span,
-    Cell *__restrict__ cells, float4 *__restrict__ bodies, uint32_t *__restrict__ error) {
+    unsigned long long *__restrict__ frame, uint32_t node_cap, double inv_theta,
+    WalkRec *__restrict__ recs, uint2 *__restrict__ span, Cell *__restrict__ cells,
+    float4 *__restrict__ bodies) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   const double ox = geom[0], oy = geom[1], oz = geom[2];
   if (k < n) {
@@ -112,7 +118,8 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const int4 L = lnk[k];
     const uint32_t d = depth[k];
     // The packed record keeps half as root_half / 2^depth: check that it is.
-    if (d > 63 || ldexp(geom[3], -static_cast<int>(d)) != g[3]) atomicOr(error, 2u);
+    if (d > 63 || ldexp(geom[3], -static_cast<int>(d)) != g[3])
+      atomicOr(frame + frame::kWalkError, 2ull);
     WalkRec w;
     w.cx = g[0];
     w.cy = g[1];
@@ -131,14 +138,18 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
 template <int MODE>
 __global__ __launch_bounds__(kWalkBlock) void walk(
     const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
-    const double *__restrict__ tpos, gt_group *__restrict__ groups, uint32_t ng, uint32_t nn,
-    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b, const uint32_t *__restrict__ cell_off,
-    const uint32_t *__restrict__ body_off, uint32_t *__restrict__ cell_count,
-    uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
-    uint32_t *__restrict__ body_list, uint32_t *__restrict__ error, uint32_t nblocks) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const double *__restrict__ tpos, gt_group *__restrict__ groups,
+    unsigned long long *__restrict__ frame, uint32_t group_cap, uint32_t node_cap,
+    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b,
+    const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
+    uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
+    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t blk = xcd_block_used(blockIdx.x, (ng + kWalkBlock - 1) / kWalkBlock);
+  if (blk == kNone) return;
   const uint32_t g = blk * kWalkBlock + threadIdx.x;
-  if (blk >= nblocks || g >= ng) return;
+  if (g >= ng) return;
   const uint32_t gb = groups[g].begin, ge = groups[g].end;
   const double root_half = root[3];
   double lo[3], hi[3];
@@ -169,7 +180,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   uint32_t budget = nn;
   while (cur != kNone) {
     if (cur >= nn || budget-- == 0) {
-      atomicOr(error, 1u);
+      atomicOr(frame + frame::kWalkError, 1ull);
       break;
     }
     const WalkRec r = recs[cur];
@@ -259,16 +270,20 @@ constexpr uint32_t kAwake = 0xfffffffeu;
 template <int MODE, int LANES>
 __global__ __launch_bounds__(kBlock) void walk_coherent(
     const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
-    const double *__restrict__ tpos, gt_group *__restrict__ groups, uint32_t ng, uint32_t nn,
+    const double *__restrict__ tpos, gt_group *__restrict__ groups,
+    unsigned long long *__restrict__ frame, uint32_t group_cap, uint32_t node_cap,
     const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b,
     const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
-    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
-    uint32_t *__restrict__ error, uint32_t nblocks) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t waves = (ng + LANES - 1) / LANES;
+  const uint32_t blk = xcd_block_used(blockIdx.x, (waves + kBlock / 64 - 1) / (kBlock / 64));
+  if (blk == kNone) return;
   const uint32_t lane = threadIdx.x % 64;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * (kBlock / 64) + threadIdx.x / 64);
-  if (blk >= nblocks || wave * LANES >= ng) return;  // the whole wave leaves together
+  if (wave * LANES >= ng) return;  // the whole wave leaves together
   const uint32_t g = wave * LANES + lane;
   const bool mine = lane < static_cast<uint32_t>(LANES) && g < ng;
   const double root_half = root[3];
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
   uint32_t budget = nn;
   for (;;) {
     if (budget-- == 0) {
-      atomicOr(error, 1u);
+      atomicOr(frame + frame::kWalkError, 1ull);
       break;
     }
     const uint32_t fc = static_cast<uint32_t>(r.bits) & kRecIndexMask;
@@ -361,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     const uint32_t next = __builtin_amdgcn_readfirstlane(descend ? last : sk);
     if (next == kNone) break;
     if (next >= nn) {
-      atomicOr(error, 1u);
+      atomicOr(frame + frame::kWalkError, 1ull);
       break;
     }
     r = descend ? rc : rs;
@@ -382,8 +397,10 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
 #pragma clang fp contract(fast)
 
 __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ lnk,
-                                                       uint32_t n_nodes, uint32_t gs, int pack,
+                                                       const unsigned long long *__restrict__ frame,
+                                                       uint32_t node_cap, uint32_t gs, int pack,
                                                        uint32_t *__restrict__ flags) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const int4 L = lnk[x];
@@ -420,8 +437,10 @@ __global__ __launch_bounds__(kBlock) void compact_starts(const uint32_t *__restr
 }
 
 __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict__ starts,
-                                                      uint32_t ng, uint32_t n,
+                                                      const unsigned long long *__restrict__ frame,
+                                                      uint32_t group_cap, uint32_t n,
                                                       gt_group *__restrict__ groups) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   if (g >= ng) return;
   gt_group G;
@@ -437,8 +456,10 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
 __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict__ cell_count,
                                                      const uint32_t *__restrict__ body_count,
                                                      const gt_group *__restrict__ groups,
-                                                     uint32_t ng,
-                                                     unsigned long long *__restrict__ stats) {
+                                                     unsigned long long *__restrict__ frame,
+                                                     uint32_t group_cap) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  unsigned long long *stats = frame + frame::kStats;
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   unsigned long long ci = 0, bi = 0, ml = 0, mg = 0, mc = 0, mb = 0, sc = 0, sb = 0;
   if (g < ng) {
@@ -463,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict_
     sc += __shfl_xor(sc, off, 64);
     sb += __shfl_xor(sb, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
+  if ((threadIdx.x & 63) == 0 && g < ng) {  // lane 0 of a wave with groups
     atomicAdd(&stats[0], ci);
     atomicAdd(&stats[1], bi);
     atomicMax(&stats[2], ml);
@@ -473,6 +494,20 @@ __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict_
     atomicAdd(&stats[6], sc);
     atomicAdd(&stats[7], sb);
   }
+}
+
+// One thread: the status of an evaluation from its counts and the plan's bounds.
+__global__ void check_frame(unsigned long long *frame, uint32_t node_cap, uint32_t depth_cap,
+                            uint32_t group_cap, uint32_t cap_c, uint32_t cap_b, int check_slots) {
+  unsigned long long st = 0;
+  if (frame[frame::kNodes] > node_cap) st |= kAbortNodes;
+  if (frame[frame::kUnresolved] & 1ull) st |= kAbortPaths;
+  if (frame[frame::kDepth] > depth_cap) st |= kAbortDepth;
+  if (frame[frame::kGroups] > group_cap) st |= kAbortGroups;
+  if (check_slots && (frame[frame::kStats + 4] > cap_c || frame[frame::kStats + 5] > cap_b))
+    st |= kAbortLists;
+  if (frame[frame::kWalkError] & 1ull) st |= kAbortWalk;
+  frame[frame::kStatus] |= st;
 }
 
 __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
@@ -499,73 +534,70 @@ void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipS
   GTH_LAUNCH_CHECK();
 }
 
-void launch_link_up(const int4 *lnk, uint32_t n_nodes, uint32_t *parent, uint32_t *skip,
-                    uint32_t *depth, hipStream_t s) {
-  if (!n_nodes) return;
-  hipLaunchKernelGGL(set_parents, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, n_nodes,
-                     parent);
+void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
+                    uint32_t *parent, uint32_t *skip, uint32_t *depth, hipStream_t s) {
+  if (!node_cap) return;
+  hipLaunchKernelGGL(set_parents, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, frame,
+                     node_cap, parent);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(set_skip_depth, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, parent,
-                     n_nodes, skip, depth);
+  hipLaunchKernelGGL(set_skip_depth, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, parent,
+                     frame, node_cap, skip, depth);
   GTH_LAUNCH_CHECK();
 }
 
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       uint32_t n_nodes, double theta, WalkRec *recs, uint2 *span, Cell *cells,
-                       float4 *bodies, uint32_t *error, hipStream_t s) {
-  const uint32_t m = n > n_nodes ? n : n_nodes;
+                       unsigned long long *frame, uint32_t node_cap, double theta, WalkRec *recs,
+                       uint2 *span, Cell *cells, float4 *bodies, hipStream_t s) {
+  const uint32_t m = n > node_cap ? n : node_cap;
   if (!m) return;
   hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
-                     tpos, tmass, n, n_nodes, 1.0 / theta, recs, span, cells, bodies, error);
+                     tpos, tmass, n, frame, node_cap, 1.0 / theta, recs, span, cells, bodies);
   GTH_LAUNCH_CHECK();
 }
 
-void launch_group_starts(const int4 *lnk, uint32_t n_nodes, uint32_t n, uint32_t gs, int pack,
-                         uint32_t *flags, hipStream_t s) {
+void launch_group_starts(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
+                         uint32_t n, uint32_t gs, int pack, uint32_t *flags, hipStream_t s) {
   launch_fill_u32(flags, n, 0u, s);
-  if (!n_nodes) return;
-  hipLaunchKernelGGL(group_starts, dim3(blocks_for(n_nodes)), dim3(kBlock), 0, s, lnk, n_nodes, gs,
-                     pack, flags);
+  if (!node_cap) return;
+  hipLaunchKernelGGL(group_starts, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, frame,
+                     node_cap, gs, pack, flags);
   GTH_LAUNCH_CHECK();
 }
 
-void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n, uint32_t ng,
-                        uint32_t *starts, gt_group *groups, hipStream_t s) {
+void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
+                        const unsigned long long *frame, uint32_t group_cap, uint32_t *starts,
+                        gt_group *groups, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(compact_starts, dim3(blocks_for(n)), dim3(kBlock), 0, s, flags, offs, n,
                      starts);
   GTH_LAUNCH_CHECK();
-  if (!ng) return;
-  hipLaunchKernelGGL(make_groups, dim3(blocks_for(ng)), dim3(kBlock), 0, s, starts, ng, n, groups);
+  if (!group_cap) return;
+  hipLaunchKernelGGL(make_groups, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, starts, frame,
+                     group_cap, n, groups);
   GTH_LAUNCH_CHECK();
 }
 
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
-                 gt_group *groups, uint32_t ng, uint32_t nn, const double *root, uint32_t cap_c,
-                 uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
-                 uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
-                 uint32_t *body_list, uint32_t *error, hipStream_t s) {
-  if (!ng) return;
+                 gt_group *groups, unsigned long long *frame, uint32_t group_cap,
+                 uint32_t node_cap, const double *root, uint32_t cap_c, uint32_t cap_b,
+                 const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
+                 uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s) {
+  if (!group_cap) return;
+#define GTH_WALK_ARGS                                                                        \
+  recs, span, tpos, groups, frame, group_cap, node_cap, root, cap_c, cap_b, cell_off, body_off, \
+      cell_count, body_count, cell_list, body_list
   if (lanes == 1) {
-    const uint32_t nb = blocks_for(ng, kWalkBlock);
+    const dim3 grid(xcd_grid(blocks_for(group_cap, kWalkBlock))), block(kWalkBlock);
     if (mode == kWalkExact)
-      hipLaunchKernelGGL(walk<kWalkExact>, dim3(xcd_grid(nb)), dim3(kWalkBlock), 0, s, recs, span,
-                         tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off, cell_count,
-                         body_count, cell_list, body_list, error, nb);
+      hipLaunchKernelGGL(walk<kWalkExact>, grid, block, 0, s, GTH_WALK_ARGS);
     else
-      hipLaunchKernelGGL(walk<kWalkCapacity>, dim3(xcd_grid(nb)), dim3(kWalkBlock), 0, s, recs,
-                         span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off,
-                         cell_count, body_count, cell_list, body_list, error, nb);
+      hipLaunchKernelGGL(walk<kWalkCapacity>, grid, block, 0, s, GTH_WALK_ARGS);
     GTH_LAUNCH_CHECK();
     return;
   }
-  const uint32_t waves = static_cast<uint32_t>((ng + lanes - 1) / lanes);
-  const uint32_t nblocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-  const dim3 grid(xcd_grid(nblocks)), block(kBlock);
-#define GTH_WALK_ARGS \
-  recs, span, tpos, groups, ng, nn, root, cap_c, cap_b, cell_off, body_off, cell_count, \
-      body_count, cell_list, body_list, error, nblocks
+  const uint32_t waves = static_cast<uint32_t>((group_cap + lanes - 1) / lanes);
+  const dim3 grid(xcd_grid((waves + kBlock / 64 - 1) / (kBlock / 64))), block(kBlock);
   if (mode == kWalkExact) {
     if (lanes == 16)
       hipLaunchKernelGGL((walk_coherent<kWalkExact, 16>), grid, block, 0, s, GTH_WALK_ARGS);
@@ -586,12 +618,20 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, co
 }
 
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
-                       const gt_group *groups, uint32_t ng, unsigned long long *stats,
+                       const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s) {
-  launch_fill_u64(stats, 8, 0ull, s);
-  if (!ng) return;
-  hipLaunchKernelGGL(list_stats, dim3(blocks_for(ng)), dim3(kBlock), 0, s, cell_count, body_count,
-                     groups, ng, stats);
+  launch_fill_u64(frame + frame::kStats, 8, 0ull, s);
+  if (!group_cap) return;
+  hipLaunchKernelGGL(list_stats, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, cell_count,
+                     body_count, groups, frame, group_cap);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_slots,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(check_frame, dim3(1), dim3(1), 0, s, frame, plan.node_cap,
+                     static_cast<uint32_t>(plan.depth_cap), plan.group_cap, plan.cap_c, plan.cap_b,
+                     check_slots ? 1 : 0);
   GTH_LAUNCH_CHECK();
 }
 

@@ -44,6 +44,12 @@ class Timings(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("tree", "lists", "evaluate", "scatter", "total")]
 
 
+class PipelineStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("evaluations", "reruns", "list_rewrites",
+                                           "graph_captures", "graph_steps", "graph_stops",
+                                           "interactions")]
+
+
 _vp = C.c_void_p
 _P = C.POINTER
 
@@ -76,6 +82,8 @@ _state_set = _sig("gt_hip_state_set", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp])
 _state_get = _sig("gt_hip_state_get", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _prime = _sig("gt_hip_prime", C.c_int, [_vp, _P(GroupConfig)])
 _steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_double, C.c_int])
+_pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)])
+_debug_shrink_slots = _sig("gt_hip_debug_shrink_slots", C.c_int, [_vp, C.c_uint32])
 _direct_f64_sample = _sig("gt_hip_direct_f64_sample", C.c_int,
                           [_vp, C.c_uint64, _vp, _vp, C.c_double, C.c_double, C.c_uint64, _vp,
                            _vp])
@@ -207,6 +215,15 @@ class Context:
         t = Timings()
         self._check(_last_timings(self._h, C.byref(t)))
         return {k: float(getattr(t, k)) for k, _ in Timings._fields_}
+
+    def pipeline_stats(self):
+        st = PipelineStats()
+        self._check(_pipeline_stats(self._h, C.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in PipelineStats._fields_}
+
+    def debug_shrink_slots(self, shrink):
+        """Test hook: list slots `shrink` entries too short (0: normal)."""
+        self._check(_debug_shrink_slots(self._h, shrink))
 
     def get_tree(self):
         inf = self.info()

@@ -157,15 +157,43 @@ int gt_hip_state_get(gt_hip_context *ctx, double *pos, double *vel, double *acc,
 /* LeapfrogKDK::prime (integrator.h:104): forces for the resident state. */
 int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
 /* nsteps x LeapfrogKDK::step (src/integrator.cpp:5-16) on the resident state:
- * FP64 kick and drift kernels around each GPU force evaluation.  use_graph = 1
- * captures the step once as a hipGraph and replays it. */
+ * FP64 kick and drift kernels around each GPU force evaluation.  An
+ * evaluation is enqueued with no host round trip: sizes it depends on (nodes,
+ * groups, list slots) are bounded from the previous evaluation, and one that
+ * outgrows its bounds is run again with larger ones.  use_graph = 1 captures
+ * the step as HIP graphs (tree, lists, evaluation, kick; recaptured only when
+ * the bounds or settings change) and launches them nsteps times back to back,
+ * with one host synchronisation at the end; a step that outgrows the bounds
+ * stops the rest, and the call finishes them without graphs.  Results are
+ * bit-identical to use_graph = 0.  Timings are per step (means over up to 64
+ * steps). */
 int gt_hip_steps(gt_hip_context *ctx, const gt_group_config *cfg, int nsteps, double dt,
                  int use_graph);
+
+/* Counters of how evaluations ran on ctx since it was created (no counterpart
+ * in the reference; for monitoring and tests of the bound handling). */
+typedef struct {
+  uint64_t evaluations;    /* force evaluations completed */
+  uint64_t reruns;         /* evaluations run again with larger bounds */
+  uint64_t list_rewrites;  /* lists rewritten at exact offsets (slots too small) */
+  uint64_t graph_captures; /* step graphs captured */
+  uint64_t graph_steps;    /* leapfrog steps run from captured graphs */
+  uint64_t graph_stops;    /* graph steps that outgrew their bounds and were finished
+                              without graphs */
+  uint64_t interactions;   /* cell + body interactions of all completed evaluations */
+} gt_pipeline_stats;
+int gt_hip_pipeline_stats(const gt_hip_context *ctx, gt_pipeline_stats *stats);
+/* Test hook (no counterpart in the reference): while shrink > 0, evaluations
+ * on ctx get list slots `shrink` entries shorter than the previous
+ * evaluation's longest lists, so that the slot-overflow rewrite and the
+ * graph-stop recovery run on every evaluation; results must not change.
+ * 0 (the default) restores the normal bounds. */
+int gt_hip_debug_shrink_slots(gt_hip_context *ctx, uint32_t shrink);
 
 /* ---- measurement helpers ---- */
 
 /* FP64 direct summation for a sample of targets (apps/galactic_tides_bench.cpp:80-95):
- * acc is ns x 3.  Used to measure force errors on the GPU box. */
+ * acc is ns x 3: the reference solution force errors are measured against. */
 int gt_hip_direct_f64_sample(gt_hip_context *ctx, uint64_t n, const double *pos,
                              const double *mass, double G, double eps, uint64_t ns,
                              const uint64_t *sample, double *acc);
