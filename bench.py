@@ -51,20 +51,47 @@ def measured_traffic(kernel="eval_fast"):
     return None
 
 
-def cpu_baseline(pos, mass, cfg, budget_s=30.0):
+def cpu_baseline(pos, mass, cfg, offsets=None):
     """The reference's CPU group solver (packed lists, OpenMP on every host
-    core) timed on the same particles: one warm-up and best of up to three
-    compute() calls.  Falls back to the oracle's restatement if the reference
-    build is absent."""
+    core) timed on the same particles: one warm-up and best of 2 compute()
+    calls (for a batch: one solver per system, summed, best of 1).  Falls back
+    to the oracle's restatement if the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Checker, available
     kind = "reference" if available("ref") else "port"
     chk = Checker("ref" if kind == "reference" else "oracle")
     t0 = time.time()
-    best = chk.time_group(pos, mass, cfg["G"], cfg["theta"], cfg["eps"], True, cfg["group_size"],
-                          cfg["leaf_size"], True, repeat=2)
+    args = (cfg["G"], cfg["theta"], cfg["eps"], True, cfg["group_size"], cfg["leaf_size"], True)
+    if offsets is None:
+        best = chk.time_group(pos, mass, *args, repeat=2)
+    else:
+        best = None
+        for k in range(len(offsets) - 1):
+            b, e = int(offsets[k]), int(offsets[k + 1])
+            one = chk.time_group(pos[b:e], mass[b:e], *args, repeat=1)
+            best = one if best is None else {key: best[key] + one[key] for key in best}
     wall = time.time() - t0
     return kind, chk.max_threads(), best, wall
+
+
+def force_errors(ctx, pos, mass, acc, cfg, offsets=None, size=1000):
+    """|a - a64| / |a64| on a seeded sample (default_rng(1)) of `size` particles,
+    a64 by FP64 direct summation over the particle's own system."""
+    n = len(mass)
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(n, size=min(size, n), replace=False)).astype(np.uint64)
+    bounds = np.array([0, n], np.uint64) if offsets is None else np.asarray(offsets, np.uint64)
+    err = []
+    for k in range(len(bounds) - 1):
+        b, e = int(bounds[k]), int(bounds[k + 1])
+        mine = sample[(sample >= b) & (sample < e)]
+        if len(mine) == 0:
+            continue
+        ref = ctx.direct_f64_sample(pos[b:e], mass[b:e], mine - b, cfg["G"], cfg["eps"])
+        got = acc[mine.astype(np.int64)]
+        err.append(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1))
+    err = np.sort(np.concatenate(err))
+    return float(err[len(err) // 2]), float(err[len(err) * 99 // 100]), len(err)
 
 
 def main():
@@ -76,6 +103,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--exact", action="store_true", help="bit-exact evaluation mode")
     ap.add_argument("--eval-waves", type=int, default=0, help="tuning: 2 or 3 (0: default)")
+    ap.add_argument("--workload", default="encounter-2m", choices=["encounter-2m", "batch-128"],
+                    help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
+                         "independent galaxies of 2k-100k particles evaluated together")
     ap.add_argument("--eager", action="store_true",
                     help="launch each step's kernels directly instead of from captured graphs")
     args = ap.parse_args()
@@ -88,13 +118,20 @@ def main():
     cfg = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                pack=1, exact=args.exact, eval_waves=args.eval_waves)
     t0 = time.time()
-    pos, vel, mass, _ = encounter_2m(args.halo)
+    offsets = None
+    if args.workload == "batch-128":
+        import gt_ics
+        offsets, pos, vel, mass = gt_ics.batch_of_galaxies(halo=args.halo)
+    else:
+        pos, vel, mass, _ = encounter_2m(args.halo)
     n = len(mass)
     print(f"# ICs: {n} particles in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
 
     ctx = gt_hip.Context(0)
     print(f"# device: {ctx.device_name}", file=sys.stderr, flush=True)
     ctx.state_set(pos, vel, mass)
+    if offsets is not None:
+        ctx.state_set_systems(offsets)
     ctx.prime(**cfg)
     dt = 0.01
     graph = not args.eager
@@ -121,12 +158,7 @@ def main():
 
     # Force error against FP64 direct summation on a seeded sample of 1000.
     p_now, _, a_now, _ = ctx.state_get()
-    rng = np.random.default_rng(1)
-    sample = np.sort(rng.choice(n, size=1000, replace=False)).astype(np.uint64)
-    ref = ctx.direct_f64_sample(p_now, mass, sample, cfg["G"], cfg["eps"])
-    err = np.sort(np.linalg.norm(a_now[sample.astype(np.int64)] - ref, axis=1)
-                  / np.linalg.norm(ref, axis=1))
-    med, p99 = float(err[len(err) // 2]), float(err[len(err) * 99 // 100])
+    med, p99, n_err = force_errors(ctx, p_now, mass, a_now, cfg, offsets)
 
     # Roofline of the list-evaluation kernel (FP32 VALU bound): algorithmic
     # flops of the last evaluation / its kernel time measured with HIP events
@@ -138,20 +170,23 @@ def main():
     # vector ALUs; pairwise gravity has no matrix operation to put on MFMA).
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic(), "kernel": "eval_fast",
+                "traffic": measured_traffic() if offsets is None else None, "kernel": "eval_fast",
                 "kernel_ms": round(eval_ms, 4),
                 "flops_per_launch": flops, "compute": "fp32 vector ALU"}
 
     cpu = None
     if not args.no_cpu:
-        kind, cores, best, wall = cpu_baseline(pos, mass, cfg)
+        kind, cores, best, wall = cpu_baseline(pos, mass, cfg, offsets)
         cpu = {"value": round(best["interactions"] / best["total"], 1), "unit": "interactions/s",
                "cores": cores, "kind": kind,
                "ms_per_force_eval": round(best["total"] * 1e3, 1),
                "split_ms": {k: round(best[k] * 1e3, 1) for k in ("tree", "lists", "evaluate")},
-               "sample": ("the full 2M encounter at the initial positions: GroupTreeGravity on "
-                          "the cpu device, packed lists, one warm-up then best of 2 compute() "
-                          f"calls ({wall:.0f} s wall)")}
+               "sample": (("the full 2M encounter at the initial positions: GroupTreeGravity on "
+                           "the cpu device, packed lists, one warm-up then best of 2 compute() "
+                           if offsets is None else
+                           "all 128 galaxies at their initial positions: one GroupTreeGravity "
+                           "per galaxy on the cpu device, packed lists, one warm-up then one "
+                           "compute() each, summed ") + f"({wall:.0f} s wall)")}
 
     line = {
         "metric": "ms/force-eval & interactions/s at N=2M theta=0.75; p99 force err vs FP64 direct",
@@ -168,10 +203,12 @@ def main():
         "dtype": "fp32 (lists, evaluation); fp64 (tree, walk, kick-drift)",
         "data": "synthetic: Hernquist galaxies generated in-process (positions identical to "
                 "the reference ICs generator)",
-        "config": {"workload": "encounter-2M", "n": n, "theta": 0.75, "group_size": 32,
+        "config": {"workload": "encounter-2M" if offsets is None else "batch-128-galaxies",
+                   "systems": 1 if offsets is None else len(offsets) - 1,
+                   "n": n, "theta": 0.75, "group_size": 32,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
                    "exact": bool(args.exact), "graphs": graph, "parallelism": "single GPU"},
-        "force_error": {"median": med, "p99": p99, "sample": 1000},
+        "force_error": {"median": med, "p99": p99, "sample": n_err},
         "interactions_per_step": interactions,
         "roofline": roofline,
         "cpu_baseline": cpu,

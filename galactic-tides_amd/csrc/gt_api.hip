@@ -16,6 +16,7 @@ using namespace gth;
 // are derived from it.  Zero / -1 / false mean "not known".
 struct History {
   uint32_t n = 0;  // particles of the evaluation described
+  uint32_t systems = 1;
   int words = 1;
   uint32_t nodes = 0;
   int depth = -1;
@@ -34,7 +35,7 @@ struct GraphKey {
 };
 
 bool same_plan(const Plan &a, const Plan &b) {
-  return a.n == b.n && a.words == b.words && a.node_cap == b.node_cap &&
+  return a.n == b.n && a.systems == b.systems && a.words == b.words && a.node_cap == b.node_cap &&
          a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
          a.cap_b == b.cap_b;
 }
@@ -77,7 +78,13 @@ struct gt_hip_context {
   DBuf<Cell> cells;
   DBuf<float4> bodies, out;
   DBuf<uint32_t> parent, skip, depth, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
-      body_list;
+      body_list, group_root;
+  // systems of the current evaluation (a batch when more than one): count and
+  // particle offsets in device memory
+  uint32_t systems = 1;
+  const unsigned long long *sys_off = nullptr;
+  DBuf<unsigned long long> batch_off, state_off;
+  uint32_t state_systems = 1;
   DBuf<gt_group> groups;
   DBuf<unsigned long long> scan_tmp;
   uint32_t lists_n = 0, lists_nn = 0;
@@ -172,8 +179,8 @@ uint32_t slot_for(uint64_t most) {
 
 // Whether the bounds p still hold what the history says with some headroom
 // (then they are kept, so buffers and captured graphs stay as they are).
-bool plan_holds(const Plan &p, const History &h, uint32_t n) {
-  if (p.n != n || h.n != n || p.words < h.words || h.nodes == 0 || h.depth < 0 ||
+bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
+  if (p.n != n || h.n != n || p.systems != systems || h.systems != systems || p.words < h.words || h.nodes == 0 || h.depth < 0 ||
       h.groups == 0 || !h.lists_known)
     return false;
   if (static_cast<uint64_t>(h.nodes) + h.nodes / 16 > p.node_cap) return false;
@@ -185,18 +192,19 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n) {
 }
 
 // Whether the history describes every size of an evaluation of n particles.
-bool sizes_known(const History &h, uint32_t n) {
-  return h.n == n && h.nodes && h.depth >= 0 && h.groups && h.lists_known;
+bool sizes_known(const History &h, uint32_t n, uint32_t systems) {
+  return h.n == n && h.systems == systems && h.nodes && h.depth >= 0 && h.groups && h.lists_known;
 }
 
 // Bounds for an evaluation of n particles: the last ones while they hold,
 // else new ones from the history with headroom (kept in c->plan).
 Plan bounds_for(gt_hip_context *c, uint32_t n) {
   const History &h = c->hist;
-  if (plan_holds(c->plan, h, n)) return c->plan;
+  if (plan_holds(c->plan, h, n, c->systems)) return c->plan;
   Plan p;
   p.n = n;
-  const bool same = h.n == n;
+  p.systems = c->systems;
+  const bool same = h.n == n && h.systems == c->systems;
   p.words = same ? h.words : 1;
   p.node_cap = same && h.nodes ? grown(h.nodes, 1024, kMaxNodes)
                                : static_cast<uint32_t>(std::min<uint64_t>(n + 1024ull, kMaxNodes));
@@ -228,8 +236,9 @@ Plan plan_for(gt_hip_context *c, uint32_t n) {
 void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
   const unsigned long long *fw = c->fw;
   History &h = c->hist;
-  if (h.n != p.n) h = History{};
+  if (h.n != p.n || h.systems != p.systems) h = History{};
   h.n = p.n;
+  h.systems = p.systems;
   h.words = p.words;
   if (st & kAbortPaths) {
     if (p.words >= 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
@@ -281,6 +290,7 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   c->starts.ensure(n);
   c->scan_tmp.ensure(scan_tmp_words(n) + 8);
   c->groups.ensure(gc);
+  if (p.systems > 1) c->group_root.ensure(gc);
   c->ccount.ensure(gc);
   c->bcount.ensure(gc);
   c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_c, 1));
@@ -288,7 +298,8 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   c->out.ensure(n);
 }
 
-int walk_lanes(const gt_group_config *cfg) {
+int walk_lanes(const gt_group_config *cfg, const Plan &p) {
+  if (p.systems > 1) return 1;  // a coherent walk shares one tree
   return cfg->walk_lanes == 16 || cfg->walk_lanes == 32 || cfg->walk_lanes == 64 ? cfg->walk_lanes
                                                                                   : 1;
 }
@@ -300,27 +311,39 @@ void clear_counts(unsigned long long *frame, hipStream_t s) {
 
 // Group selection, walk into per-group slots, statistics and the status check,
 // for a tree in device memory whose node count is frame[kNodes].
+// tsid/roots/root_node: a batch's systems (null for one system).
 void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
-                   const double *geom, const int4 *lnk, const double *tpos, const double *tmass) {
+                   const double *geom, const int4 *lnk, const double *tpos, const double *tmass,
+                   const uint32_t *tsid = nullptr, const double *roots = nullptr,
+                   const uint32_t *root_node = nullptr) {
   hipStream_t s = c->stream;
   unsigned long long *frame = c->frame.get();
   reserve_all(c, p);
   const uint32_t n = p.n, cap = p.node_cap;
   launch_link_up(lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
   launch_prep_lists(geom, lnk, c->skip.get(), c->depth.get(), tpos, tmass, n, frame, cap,
-                    cfg->theta, c->recs.get(), c->span.get(), c->cells.get(), c->bodies.get(), s);
+                    cfg->theta, tsid, roots, c->recs.get(), c->span.get(), c->cells.get(),
+                    c->bodies.get(), s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  launch_group_starts(lnk, frame, cap, n, static_cast<uint32_t>(cfg->group_size), pack,
-                      c->flags.get(), s);
+  launch_group_starts(lnk, c->parent.get(), frame, cap, n, static_cast<uint32_t>(cfg->group_size),
+                      pack, c->flags.get(), s);
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
                      s);
-  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, c->starts.get(),
-                     c->groups.get(), s);
-  launch_walk(kWalkCapacity, walk_lanes(cfg), c->recs.get(), c->span.get(), tpos, c->groups.get(),
-              frame, p.group_cap, cap, geom, p.cap_c, p.cap_b, nullptr, nullptr, c->ccount.get(),
-              c->bcount.get(), c->cell_list.get(), c->body_list.get(), s);
+  uint32_t *groot = tsid ? c->group_root.get() : nullptr;
+  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, tsid, root_node,
+                     c->starts.get(), c->groups.get(), groot, s);
+  launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos,
+              c->groups.get(), frame, p.group_cap, cap, geom, groot, p.cap_c, p.cap_b, nullptr,
+              nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(), s);
   launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s);
   launch_check_frame(frame, p, true, s);
+}
+
+// Lists for the tree the GPU just built.
+void enqueue_tree_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+  const TreeBuilder &t = c->tree;
+  enqueue_lists(c, cfg, p, t.geom(), t.links(), t.tree_pos(), t.tree_mass(), t.tree_sid(),
+                t.root(), t.root_nodes());
 }
 
 // The lists again at exact (scanned) offsets, for an evaluation whose only
@@ -343,9 +366,10 @@ void lists_exact(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, c
   uint32_t *cl = c->cell_list.ensure(std::max<unsigned long long>(all_c, 1));
   uint32_t *bl = c->body_list.ensure(std::max<unsigned long long>(all_b, 1));
   launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
-  launch_walk(kWalkExact, walk_lanes(cfg), c->recs.get(), c->span.get(), tpos, c->groups.get(),
-              frame, p.group_cap, p.node_cap, geom, 0, 0, co, bo, c->ccount.get(), c->bcount.get(),
-              cl, bl, s);
+  uint32_t *groot = p.systems > 1 ? c->group_root.get() : nullptr;
+  launch_walk(kWalkExact, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos, c->groups.get(),
+              frame, p.group_cap, p.node_cap, geom, groot, 0, 0, co, bo, c->ccount.get(),
+              c->bcount.get(), cl, bl, s);
   launch_check_frame(frame, p, false, s);
 }
 
@@ -399,10 +423,10 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
-    c->tree.enqueue(pos, mass, p, cfg->leaf_size, frame, s);
+    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s);
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
-    enqueue_lists(c, cfg, p, geom, c->tree.links(), c->tree.tree_pos(), c->tree.tree_mass());
+    enqueue_tree_lists(c, cfg, p);
     GTH_CHECK(hipEventRecord(c->ev[2], s));
     enqueue_eval(c, cfg, p);
     GTH_CHECK(hipEventRecord(c->ev[3], s));
@@ -475,11 +499,10 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt,
                         frame + frame::kStatus, s);
       clear_counts(frame, s);
-      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, frame, s);
+      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s);
       break;
     case 1:
-      enqueue_lists(c, cfg, p, c->tree.geom(), c->tree.links(), c->tree.tree_pos(),
-                    c->tree.tree_mass());
+      enqueue_tree_lists(c, cfg, p);
       break;
     case 2:
       enqueue_eval(c, cfg, p);
@@ -506,7 +529,7 @@ GraphKey key_for(const gt_group_config *cfg, const Plan &p, double dt) {
   k.pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
   k.exact = cfg->exact != 0;
   k.eval_waves = cfg->eval_waves == 2 ? 2 : 3;
-  k.walk_lanes = walk_lanes(cfg);
+  k.walk_lanes = walk_lanes(cfg, p);
   k.generation = alloc_generation;
   return k;
 }
@@ -555,7 +578,7 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
   const uint32_t n = static_cast<uint32_t>(c->n_state);
   int done_total = 0;
   while (done_total < nsteps) {
-    if (!sizes_known(c->hist, n)) {  // sizes not known yet: one step without graphs
+    if (!sizes_known(c->hist, n, c->systems)) {  // sizes not known yet: one step without graphs
       eager_step(c, cfg, dt);
       ++done_total;
       continue;
@@ -772,6 +795,55 @@ int gt_hip_group_compute(gt_hip_context *c, const gt_group_config *cfg, uint64_t
     double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
     upload(dp, pos, 3 * n, s);
     upload(dm, mass, n, s);
+    c->systems = 1;
+    c->sys_off = nullptr;
+    force_eval(c, cfg, dp, dm, static_cast<uint32_t>(n), da, dq);
+    download(acc, da, 3 * n, s);
+    download(pot, dq, n, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+// Checks a batch's offsets (n_systems + 1 of them, from 0 to n, non-decreasing)
+// and puts them in device memory.
+void upload_systems(gt_hip_context *c, uint64_t n, uint64_t n_systems, const uint64_t *offsets,
+                    DBuf<unsigned long long> &dst) {
+  require(n_systems >= 1, "n_systems must be at least 1");
+  require(offsets != nullptr, "offsets is NULL");
+  if (n_systems > (1ull << 24)) fail(GT_HIP_ERR_LIMIT, "at most 2^24 systems in one batch");
+  require(offsets[0] == 0 && offsets[n_systems] == n, "offsets must run from 0 to n");
+  for (uint64_t k = 0; k < n_systems; ++k)
+    require(offsets[k] <= offsets[k + 1], "offsets must not decrease");
+  unsigned long long *d = dst.ensure(n_systems + 1);
+  upload(d, reinterpret_cast<const unsigned long long *>(offsets), n_systems + 1, c->stream);
+}
+
+int gt_hip_group_compute_batch(gt_hip_context *c, const gt_group_config *cfg, uint64_t n_systems,
+                               const uint64_t *offsets, const double *pos, const double *mass,
+                               double *acc, double *pot) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    check_config(cfg);
+    require(offsets != nullptr && n_systems >= 1, "offsets is NULL or n_systems is 0");
+    const uint64_t n = offsets[n_systems];
+    if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    upload_systems(c, n, n_systems, offsets, c->batch_off);
+    if (n == 0) {
+      GTH_CHECK(hipStreamSynchronize(s));
+      c->info = gt_lists_info{};
+      c->tm = gt_timings{};
+      return;
+    }
+    require(pos && mass && acc && pot, "array is NULL");
+    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+    double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
+    upload(dp, pos, 3 * n, s);
+    upload(dm, mass, n, s);
+    c->systems = static_cast<uint32_t>(n_systems);
+    c->sys_off = n_systems > 1 ? c->batch_off.get() : nullptr;
+    if (n_systems == 1) c->systems = 1;
     force_eval(c, cfg, dp, dm, static_cast<uint32_t>(n), da, dq);
     download(acc, da, 3 * n, s);
     download(pot, dq, n, s);
@@ -830,6 +902,8 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
       return;
     }
     unsigned long long *frame = c->frame.get();
+    c->systems = 1;
+    c->sys_off = nullptr;
     Plan p = plan_for(c, static_cast<uint32_t>(n));
     p.node_cap = static_cast<uint32_t>(n_nodes);
     for (int attempt = 0;; ++attempt) {
@@ -962,6 +1036,18 @@ int gt_hip_state_set(gt_hip_context *c, uint64_t n, const double *pos, const dou
     upload(c->mass.get(), mass, n, s);
     GTH_CHECK(hipStreamSynchronize(s));
     c->n_state = n;
+    c->state_systems = 1;
+    c->state_forces = false;
+  });
+}
+
+int gt_hip_state_set_systems(gt_hip_context *c, uint64_t n_systems, const uint64_t *offsets) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    GTH_CHECK(hipSetDevice(c->device));
+    upload_systems(c, c->n_state, n_systems, offsets, c->state_off);
+    GTH_CHECK(hipStreamSynchronize(c->stream));
+    c->state_systems = static_cast<uint32_t>(n_systems);
     c->state_forces = false;
   });
 }
@@ -984,6 +1070,8 @@ int gt_hip_prime(gt_hip_context *c, const gt_group_config *cfg) {
   return guarded(c, [&] {
     check_config(cfg);
     GTH_CHECK(hipSetDevice(c->device));
+    c->systems = c->state_systems;
+    c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
     if (c->n_state)
       force_eval(c, cfg, c->pos.get(), c->mass.get(), static_cast<uint32_t>(c->n_state),
                  c->acc.get(), c->pot.get());
@@ -1004,6 +1092,8 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     // Until the steps have finished the state is between steps: a failure
     // leaves it so, and the forces must be recomputed (gt_hip_prime).
     c->state_forces = false;
+    c->systems = c->state_systems;
+    c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
     if (use_graph)
       graph_steps(c, cfg, nsteps, dt);
     else

@@ -21,8 +21,10 @@ class TreeBuilder {
   // depth and the "more path words needed" flag go to the frame; the host
   // reads them after the evaluation and hands the node count back with
   // set_counts.  A tree with more nodes than p.node_cap is not built.
+  // Batches (p.systems > 1) take the system offsets (p.systems + 1 entries,
+  // device memory).
   void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
-               unsigned long long *frame, hipStream_t s);
+               const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s);
   void set_counts(uint32_t n_nodes, int depth) {
     n_nodes_ = n_nodes;
     depth_ = depth;
@@ -37,13 +39,18 @@ class TreeBuilder {
   const double *tree_pos() const { return tpos_.get(); }
   const double *tree_mass() const { return tmass_.get(); }
   const double *root() const { return root_.get(); }
+  // Batches only (null for one system): system of each tree position, and the
+  // root node of each system.
+  const uint32_t *tree_sid() const { return systems_ > 1 ? tsid_.get() : nullptr; }
+  const uint32_t *root_nodes() const { return systems_ > 1 ? root_node_.get() : nullptr; }
 
  private:
-  uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0;
+  uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0, systems_ = 1;
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, ndep_, ndep_alt_, word0_, word1_, word2_;
-  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, nchild_;
+  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, nchild_,
+      sid_, tsid_, root_node_;
   DBuf<uint8_t> dg_, lcp_, ldep_;
   DBuf<unsigned long long> sort_tmp_, scan_tmp_;
   DBuf<int4> lnk_;

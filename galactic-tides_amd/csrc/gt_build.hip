@@ -31,6 +31,12 @@
 //     in tree order (:175-201), inner nodes combine their children in order
 //     (:203-231), with no FMA contraction, so masses, centres of mass,
 //     quadrupoles and deltas are bit-identical to the reference's.
+//
+// A batch of independent systems (particles offsets[s] .. offsets[s + 1])
+// builds one forest: each system gets its own root cube, the system id is the
+// most significant sort key, and sorted neighbours of different systems share
+// no level, so every system's tree is exactly the one it would get alone.
+// All of it is enqueued with no host round trip (counts stay in the frame).
 #include "gt_build.h"
 
 namespace gth {
@@ -40,6 +46,15 @@ constexpr int kBlock = 256;
 constexpr int kLevels = 21;  // octant digits per 64-bit path
 constexpr int kDepthLimit = 60;
 constexpr uint8_t kNoDegenerate = 0xff;
+constexpr uint8_t kNoShare = 0xff;  // lcp of neighbours in different systems (no common root)
+
+// Shared levels as a signed number: -1 for neighbours of different systems.
+__device__ inline int shared(uint8_t v) { return v == kNoShare ? -1 : static_cast<int>(v); }
+
+// Depth of the first node that begins at sorted position p.
+__device__ inline int first_level(const uint8_t *lcp, uint32_t p) {
+  return p == 0 ? 0 : shared(lcp[p - 1]) + 1;
+}
 
 inline uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
@@ -117,6 +132,67 @@ __global__ __launch_bounds__(kBlock) void bbox_final(const double *__restrict__ 
   root[3] = half;
 }
 
+// Systems of a batch: particles offsets[s] .. offsets[s + 1] form system s.
+__global__ __launch_bounds__(kBlock) void system_ids(const unsigned long long *__restrict__ offsets,
+                                                     uint32_t n_sys, uint32_t n,
+                                                     uint32_t *__restrict__ sid) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = n_sys - 1;  // last system whose first particle is <= i
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (offsets[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  sid[i] = lo;
+}
+
+// One workgroup per system: its root cube, as bbox_final computes it for one.
+__global__ __launch_bounds__(kBlock) void bbox_systems(const double *__restrict__ pos,
+                                                       const unsigned long long *__restrict__ offsets,
+                                                       double *__restrict__ roots) {
+  __shared__ double r[6][kBlock];
+  const uint32_t s = blockIdx.x;
+  const uint32_t b = static_cast<uint32_t>(offsets[s]), e = static_cast<uint32_t>(offsets[s + 1]);
+  if (b >= e) {  // empty system: no tree
+    if (threadIdx.x < 4) roots[4 * s + threadIdx.x] = 0.0;
+    return;
+  }
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = pos[3 * static_cast<size_t>(b) + a];
+  for (uint32_t i = b + threadIdx.x; i < e; i += kBlock)
+    for (int a = 0; a < 3; ++a) {
+      const double v = pos[3 * static_cast<size_t>(i) + a];
+      lo[a] = v < lo[a] ? v : lo[a];
+      hi[a] = hi[a] < v ? v : hi[a];
+    }
+  for (int a = 0; a < 3; ++a) {
+    r[a][threadIdx.x] = lo[a];
+    r[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(w))
+      for (int a = 0; a < 3; ++a) {
+        const double l = r[a][threadIdx.x + w], h = r[3 + a][threadIdx.x + w];
+        if (l < r[a][threadIdx.x]) r[a][threadIdx.x] = l;
+        if (r[3 + a][threadIdx.x] < h) r[3 + a][threadIdx.x] = h;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double half = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double ext = 0.5 * (r[3 + a][0] - r[a][0]);
+    half = half < ext ? ext : half;
+  }
+  half = half * (1.0 + 1e-9) + 1e-300;
+  for (int a = 0; a < 3; ++a) roots[4 * s + a] = 0.5 * (r[a][0] + r[3 + a][0]);
+  roots[4 * s + 3] = half;
+}
+
 // ---- 2. octant paths ---------------------------------------------------------------
 
 __device__ inline bool degenerate(double cx, double cy, double cz, double half) {
@@ -124,11 +200,12 @@ __device__ inline bool degenerate(double cx, double cy, double cz, double half) 
 }
 
 // Digits of levels [21 word, 21 word + 21) of every particle's octant path
-// (the descent from the root replays all earlier levels), and the first depth
-// <= 21 (word + 1) at which the path's node is degenerate.  Word 0 also seeds
-// the sort: keys[i] = path, vals[i] = i.
+// (the descent from its system's root replays all earlier levels), and the
+// first depth <= 21 (word + 1) at which the path's node is degenerate.  Word 0
+// also seeds the sort: keys[i] = path, vals[i] = i.  sid may be null (one system).
 __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict__ pos, uint32_t n,
-                                                       const double *__restrict__ root, int word,
+                                                       const double *__restrict__ roots,
+                                                       const uint32_t *__restrict__ sid, int word,
                                                        uint64_t *__restrict__ wkey,
                                                        uint64_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals,
@@ -137,6 +214,7 @@ __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict_
   if (i >= n) return;
   const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
                z = pos[3 * static_cast<size_t>(i) + 2];
+  const double *root = roots + 4 * static_cast<size_t>(sid ? sid[i] : 0u);
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   uint64_t key = 0;
   uint8_t first_dg = kNoDegenerate;
@@ -167,6 +245,14 @@ __global__ __launch_bounds__(kBlock) void gather_keys(const uint64_t *__restrict
   if (p < n) keys[p] = wkey[vals[p]];
 }
 
+__global__ __launch_bounds__(kBlock) void gather_sid_keys(const uint32_t *__restrict__ sid,
+                                                          const uint32_t *__restrict__ vals,
+                                                          uint32_t n,
+                                                          uint64_t *__restrict__ keys) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p < n) keys[p] = sid[vals[p]];
+}
+
 // ---- 4. shared prefixes and leaf depths ---------------------------------------------
 
 // Levels shared by the paths of sorted neighbours p, p+1 (at most 21 words).
@@ -174,10 +260,15 @@ __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restri
                                                         const uint64_t *__restrict__ w1,
                                                         const uint64_t *__restrict__ w2, int words,
                                                         const uint32_t *__restrict__ vals,
+                                                        const uint32_t *__restrict__ sid,
                                                         uint32_t n, uint8_t *__restrict__ lcp) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p + 1 >= n) return;
   const uint32_t a = vals[p], b = vals[p + 1];
+  if (sid && sid[a] != sid[b]) {
+    lcp[p] = kNoShare;
+    return;
+  }
   const uint64_t *w[3] = {w0, w1, w2};
   int common = kLevels * words;
   for (int k = 0; k < words; ++k) {
@@ -207,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
     const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
     for (uint32_t j = jlo; j <= jhi; ++j) {
       int w = cap;
-      for (uint32_t k = j; k < j + leaf; ++k) w = min(w, static_cast<int>(lcp[k]));
+      for (uint32_t k = j; k < j + leaf; ++k) w = min(w, shared(lcp[k]));
       best = max(best, w);
     }
   }
@@ -220,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
   if (d > cap) atomicOr(unresolved, 1u);
   d = min(d, cap);
   ldep[p] = static_cast<uint8_t>(d);
-  const int first = p == 0 ? 0 : static_cast<int>(lcp[p - 1]) + 1;
+  const int first = first_level(lcp, p);
   starts[p] = d >= first ? static_cast<uint32_t>(d - first + 1) : 0u;
 }
 
@@ -236,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= n || frame[frame::kNodes] > node_cap) return;  // over capacity: no tree
   const uint32_t c = starts[p];
-  const uint32_t first = p == 0 ? 0u : static_cast<uint32_t>(lcp[p - 1]) + 1u;
+  const uint32_t first = static_cast<uint32_t>(first_level(lcp, p));
   for (uint32_t t = 0; t < c; ++t) {
     node_depth[offs[p] + t] = first + t;
     node_begin[offs[p] + t] = p;
@@ -266,16 +357,21 @@ __global__ __launch_bounds__(kBlock) void link_nodes(
     const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
     const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
     const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
-    const uint32_t *__restrict__ vals, const double *__restrict__ root,
-    const unsigned long long *__restrict__ frame, uint32_t node_cap, double *__restrict__ geom,
-    int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild) {
+    const uint32_t *__restrict__ vals, const double *__restrict__ roots,
+    const uint32_t *__restrict__ sid, const unsigned long long *__restrict__ frame,
+    uint32_t node_cap, double *__restrict__ geom, int4 *__restrict__ lnk,
+    uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild,
+    uint32_t *__restrict__ root_node) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t d = static_cast<uint32_t>(node_depth[x]);
   const uint32_t b = node_begin[x];
-  // Centre and half size: replay the octant path of the node's first body.
+  // Centre and half size: replay the octant path of the node's first body
+  // from its system's root.
   const uint32_t i = vals[b];
+  const uint32_t system = sid ? sid[i] : 0u;
+  const double *root = roots + 4 * static_cast<size_t>(system);
   const uint64_t *w[3] = {w0, w1, w2};
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   for (uint32_t l = 0; l < d; ++l) {
@@ -296,6 +392,7 @@ __global__ __launch_bounds__(kBlock) void link_nodes(
   lnk[x] = make_int4(-1, 0, static_cast<int>(b), 0);
   if (d == 0) {
     parent[x] = kNone;
+    if (root_node) root_node[system] = x;
     return;
   }
   // Parent: the last node of the level above beginning at or before b.
@@ -328,6 +425,7 @@ __global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restric
 // (for a last child) its parent's end; the root ends at n.
 __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__ parent,
                                                     const uint32_t *__restrict__ node_begin,
+                                                    const uint64_t *__restrict__ node_depth,
                                                     const unsigned long long *__restrict__ frame,
                                                     uint32_t node_cap, uint32_t n,
                                                     int4 *__restrict__ lnk) {
@@ -337,8 +435,9 @@ __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__
   uint32_t x = x0;
   for (;;) {
     const uint32_t p = parent[x];
-    if (p == kNone) {
-      lnk[x0].w = static_cast<int>(n);
+    if (p == kNone) {  // a root ends where the next system's root begins
+      const bool next_root = x + 1 < n_nodes && node_depth[x + 1] == 0;
+      lnk[x0].w = static_cast<int>(next_root ? node_begin[x + 1] : n);
       return;
     }
     const int4 P = lnk[p];
@@ -375,11 +474,14 @@ __global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ ln
 __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__restrict__ vals,
                                                             const double *__restrict__ pos,
                                                             const double *__restrict__ mass,
+                                                            const uint32_t *__restrict__ sid,
                                                             uint32_t n, double *__restrict__ tpos,
-                                                            double *__restrict__ tmass) {
+                                                            double *__restrict__ tmass,
+                                                            uint32_t *__restrict__ tsid) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
   const uint32_t i = vals[k];
+  if (sid) tsid[k] = sid[i];
   for (int a = 0; a < 3; ++a) tpos[3 * static_cast<size_t>(k) + a] = pos[3 * static_cast<size_t>(i) + a];
   tmass[k] = mass[i];
 }
@@ -491,8 +593,14 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
 void TreeBuilder::reserve(const Plan &p) {
   const size_t n = std::max<uint32_t>(p.n, 1), nn = std::max<uint32_t>(p.node_cap, 1);
   const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
+  const size_t ns = std::max<uint32_t>(p.systems, 1);
   part_.ensure(6 * static_cast<size_t>(nparts));
-  root_.ensure(4);
+  root_.ensure(4 * ns);
+  if (p.systems > 1) {
+    sid_.ensure(n);
+    tsid_.ensure(n);
+    root_node_.ensure(ns);
+  }
   keys_.ensure(n);
   vals_.ensure(n);
   keys_alt_.ensure(n);
@@ -521,10 +629,13 @@ void TreeBuilder::reserve(const Plan &p) {
 }
 
 void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
-                          unsigned long long *frame, hipStream_t s) {
+                          const unsigned long long *sys_off, unsigned long long *frame,
+                          hipStream_t s) {
   const uint32_t n = p.n;
   n_ = n;
   node_cap_ = p.node_cap;
+  systems_ = p.systems;
+  if (p.systems > 1 && !sys_off) fail(GT_HIP_ERR_ARGUMENT, "octree build: system offsets missing");
   if (n == 0) return;
   if (p.words < 1 || p.words > 3) fail(GT_HIP_ERR_ARGUMENT, "octree build: 1 to 3 path words");
   if (p.depth_cap < 0 || p.depth_cap > kDepthLimit)
@@ -533,13 +644,25 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   const uint32_t leaf = static_cast<uint32_t>(leaf_size < 1 ? 1 : leaf_size);
   const uint32_t cap = p.node_cap;
 
-  // 1. root cube
-  const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
-  double *part = part_.get();
-  hipLaunchKernelGGL(bbox_partial, dim3(nparts), dim3(kBlock), 0, s, pos, n, part);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, root_.get());
-  GTH_LAUNCH_CHECK();
+  // 1. root cube (of every system of a batch)
+  const bool batch = p.systems > 1;
+  uint32_t *sid = batch ? sid_.get() : nullptr;
+  uint32_t *root_node = batch ? root_node_.get() : nullptr;
+  if (batch) {
+    hipLaunchKernelGGL(system_ids, dim3(blocks_for(n)), dim3(kBlock), 0, s, sys_off, p.systems, n,
+                       sid);
+    GTH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bbox_systems, dim3(p.systems), dim3(kBlock), 0, s, pos, sys_off,
+                       root_.get());
+    GTH_LAUNCH_CHECK();
+  } else {
+    const uint32_t nparts = std::min<uint32_t>(blocks_for(n), 1024u);
+    double *part = part_.get();
+    hipLaunchKernelGGL(bbox_partial, dim3(nparts), dim3(kBlock), 0, s, pos, n, part);
+    GTH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, root_.get());
+    GTH_LAUNCH_CHECK();
+  }
 
   // 2-3. octant paths (p.words 21-level words of each) and their stable sort:
   // by the last word first, then by each earlier one.  One word is enough
@@ -550,13 +673,13 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint8_t *dg = dg_.get();
   uint64_t *w[3] = {word0_.get(), p.words > 1 ? word1_.get() : nullptr,
                     p.words > 2 ? word2_.get() : nullptr};
-  hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(), 0,
-                     w[0], keys, vals, dg);
+  hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
+                     sid, 0, w[0], keys, vals, dg);
   GTH_LAUNCH_CHECK();
   for (int k = 1; k < p.words; ++k) {
     hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
-                       k, w[k], static_cast<uint64_t *>(nullptr), static_cast<uint32_t *>(nullptr),
-                       dg);
+                       sid, k, w[k], static_cast<uint64_t *>(nullptr),
+                       static_cast<uint32_t *>(nullptr), dg);
     GTH_LAUNCH_CHECK();
   }
   for (int k = p.words - 1; k >= 0; --k) {
@@ -567,6 +690,14 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
     radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, 3 * kLevels, sort_tmp_.get(),
                    s);
   }
+  if (batch) {  // systems apart, most significant: one stable pass more
+    int bits = 1;
+    while (bits < 32 && (p.systems - 1) >> bits) ++bits;
+    hipLaunchKernelGGL(gather_sid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, sid, vals, n,
+                       keys);
+    GTH_LAUNCH_CHECK();
+    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, bits, sort_tmp_.get(), s);
+  }
 
   // 4. leaf depths and node counts per sorted position; the node total goes
   // to frame[kNodes]
@@ -575,7 +706,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint32_t *starts = starts_.get();
   uint32_t *offs = offs_.get();
   hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
-                     p.words, vals, n, lcp);
+                     p.words, vals, sid, n, lcp);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
                      kLevels * p.words, ldep, starts,
@@ -604,20 +735,21 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint32_t *nchild = nchild_.get();
   launch_fill_u32(nchild, cap, 0u, s);
   hipLaunchKernelGGL(link_nodes, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
-                     w[0], w[1], w[2], vals, root_.get(), frame, cap, geom, lnk, parent, nchild);
+                     w[0], w[1], w[2], vals, root_.get(), sid, frame, cap, geom, lnk, parent,
+                     nchild, root_node);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(finish_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nchild, nbeg,
                      frame, cap, lnk);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, frame, cap,
-                     n, lnk);
+  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep, frame,
+                     cap, n, lnk);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, vals);
   GTH_LAUNCH_CHECK();
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();
   hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
-                     n, tpos, tmass);
+                     sid, n, tpos, tmass, batch ? tsid_.get() : static_cast<uint32_t *>(nullptr));
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan

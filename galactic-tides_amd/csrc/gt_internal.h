@@ -160,6 +160,7 @@ constexpr uint32_t kAbortWalk = 32u;   // malformed tree
 // Host-side bounds of one evaluation, fixed before it is enqueued.
 struct Plan {
   uint32_t n = 0;          // particles
+  uint32_t systems = 1;    // independent systems (a batch), each with its own tree
   int words = 1;           // 21-level octant-path words computed per particle
   uint32_t node_cap = 0;   // node arrays
   int depth_cap = 60;      // deepest level whose moments are computed
@@ -215,20 +216,29 @@ void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipS
 // node capacity the grid was sized for), over groups as frame + kGroups.
 void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
                     uint32_t *parent, uint32_t *skip, uint32_t *depth, hipStream_t s);
+// Batches of independent systems pass tsid (system of each tree position),
+// roots (4 doubles per system: root centre, half) and root_node (root node
+// of each system); for one system they are null (the root is node 0).
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       unsigned long long *frame, uint32_t node_cap, double theta, WalkRec *recs,
-                       uint2 *span, Cell *cells, float4 *bodies, hipStream_t s);
-void launch_group_starts(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
-                         uint32_t n, uint32_t gs, int pack, uint32_t *flags, hipStream_t s);
+                       unsigned long long *frame, uint32_t node_cap, double theta,
+                       const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
+                       Cell *cells, float4 *bodies, hipStream_t s);
+void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
+                         uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
+                         hipStream_t s);
 void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
-                        const unsigned long long *frame, uint32_t group_cap, uint32_t *starts,
-                        gt_group *groups, hipStream_t s);
+                        const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
+                        const uint32_t *root_node, uint32_t *starts, gt_group *groups,
+                        uint32_t *group_root, hipStream_t s);
+// group_root: the node each group's walk starts from (null: node 0); geom
+// gives the root half sizes.  Coherent walks (lanes > 1) need a single tree.
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                 uint32_t node_cap, const double *root, uint32_t cap_c, uint32_t cap_b,
-                 const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
-                 uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s);
+                 uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
+                 uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
+                 uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
+                 uint32_t *body_list, hipStream_t s);
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s);

@@ -55,8 +55,7 @@ __global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ l
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= n_nodes) return;
-  if (p == 0) parent[0] = kNone;
-  const int4 L = lnk[p];
+  const int4 L = lnk[p];  // (roots keep the kNone the array was filled with)
   for (int c = 0; c < L.y; ++c) parent[L.x + c] = p;
 }
 
@@ -95,12 +94,16 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const uint32_t *__restrict__ skip, const uint32_t *__restrict__ depth,
     const double *__restrict__ tpos, const double *__restrict__ tmass, uint32_t n,
     unsigned long long *__restrict__ frame, uint32_t node_cap, double inv_theta,
+    const uint32_t *__restrict__ tsid, const double *__restrict__ roots,
     WalkRec *__restrict__ recs, uint2 *__restrict__ span, Cell *__restrict__ cells,
     float4 *__restrict__ bodies) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const double ox = geom[0], oy = geom[1], oz = geom[2];
+  // origin: the root centre of the body's (node's) system (tsid null: one
+  // system, whose root is node 0)
   if (k < n) {
+    const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[k]) : geom;
+    const double ox = o[0], oy = o[1], oz = o[2];
     bodies[k] = make_float4(static_cast<float>(tpos[3 * k] - ox),
                             static_cast<float>(tpos[3 * k + 1] - oy),
                             static_cast<float>(tpos[3 * k + 2] - oz),
@@ -108,6 +111,9 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
   }
   if (k < n_nodes) {
     const double *g = geom + static_cast<size_t>(kGeomStride) * k;
+    const int4 L = lnk[k];
+    const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[L.z]) : geom;
+    const double ox = o[0], oy = o[1], oz = o[2];
     Cell c;
     c.cm = make_float4(static_cast<float>(g[4] - ox), static_cast<float>(g[5] - oy),
                        static_cast<float>(g[6] - oz), static_cast<float>(g[7]));
@@ -115,10 +121,10 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
                        static_cast<float>(g[10]), static_cast<float>(g[11]));
     c.q1 = make_float4(static_cast<float>(g[12]), static_cast<float>(g[13]), 0.0f, 0.0f);
     cells[k] = c;
-    const int4 L = lnk[k];
     const uint32_t d = depth[k];
     // The packed record keeps half as root_half / 2^depth: check that it is.
-    if (d > 63 || ldexp(geom[3], -static_cast<int>(d)) != g[3])
+    const double root_half = tsid ? roots[4 * static_cast<size_t>(tsid[L.z]) + 3] : geom[3];
+    if (d > 63 || ldexp(root_half, -static_cast<int>(d)) != g[3])
       atomicOr(frame + frame::kWalkError, 2ull);
     WalkRec w;
     w.cx = g[0];
@@ -140,8 +146,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
     const double *__restrict__ tpos, gt_group *__restrict__ groups,
     unsigned long long *__restrict__ frame, uint32_t group_cap, uint32_t node_cap,
-    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b,
-    const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
+    const double *__restrict__ geom, const uint32_t *__restrict__ group_root, uint32_t cap_c,
+    uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
     uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
@@ -151,7 +157,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   const uint32_t g = blk * kWalkBlock + threadIdx.x;
   if (g >= ng) return;
   const uint32_t gb = groups[g].begin, ge = groups[g].end;
-  const double root_half = root[3];
+  // the walk starts at the root of the group's system (node 0 for one system)
+  const uint32_t start = group_root ? group_root[g] : 0u;
+  const double root_half = geom[static_cast<size_t>(kGeomStride) * (start < nn ? start : 0u) + 3];
   double lo[3], hi[3];
   for (int a = 0; a < 3; ++a) lo[a] = hi[a] = tpos[3 * static_cast<size_t>(gb) + a];
   for (uint32_t k = gb; k < ge; ++k)
@@ -174,7 +182,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   }
   uint32_t nc = 0, nb = 0;
   uint32_t cbuf[4] = {0, 0, 0, 0};  // pending cell ids, indexed by list position mod 4
-  uint32_t cur = 0;
+  uint32_t cur = start;
   // A depth-first walk visits each node at most once: more visits than nodes,
   // or an index past the end, means a malformed tree.  Stop and report it.
   uint32_t budget = nn;
@@ -272,10 +280,13 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
     const double *__restrict__ tpos, gt_group *__restrict__ groups,
     unsigned long long *__restrict__ frame, uint32_t group_cap, uint32_t node_cap,
-    const double *__restrict__ root, uint32_t cap_c, uint32_t cap_b,
-    const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
+    const double *__restrict__ root, const uint32_t *__restrict__ group_root, uint32_t cap_c,
+    uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
     uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
+  // One tree only: the shared traversal starts at node 0 (group_root unused;
+  // batches walk one lane per group).
+  (void)group_root;
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t waves = (ng + LANES - 1) / LANES;
@@ -397,6 +408,7 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
 #pragma clang fp contract(fast)
 
 __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ lnk,
+                                                       const uint32_t *__restrict__ parent,
                                                        const unsigned long long *__restrict__ frame,
                                                        uint32_t node_cap, uint32_t gs, int pack,
                                                        uint32_t *__restrict__ flags) {
@@ -405,7 +417,8 @@ __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ 
   if (x >= n_nodes) return;
   const int4 L = lnk[x];
   const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-  if (x != 0 && e - b <= gs) return;  // not reached by visit(): a small child of its parent
+  // not reached by visit(): a small child of its parent (roots always are)
+  if (parent[x] != kNone && e - b <= gs) return;
   if (e - b <= gs || L.y == 0) {      // chunk(begin, end)
     for (uint32_t s = b; s < e; s += gs) flags[s] = 1u;
     return;
@@ -439,7 +452,10 @@ __global__ __launch_bounds__(kBlock) void compact_starts(const uint32_t *__restr
 __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict__ starts,
                                                       const unsigned long long *__restrict__ frame,
                                                       uint32_t group_cap, uint32_t n,
-                                                      gt_group *__restrict__ groups) {
+                                                      const uint32_t *__restrict__ tsid,
+                                                      const uint32_t *__restrict__ root_node,
+                                                      gt_group *__restrict__ groups,
+                                                      uint32_t *__restrict__ group_root) {
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   if (g >= ng) return;
@@ -449,6 +465,7 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
   G.cell_begin = G.cell_end = G.body_begin = G.body_end = 0;
   G.pad0 = G.pad1 = 0;
   groups[g] = G;
+  if (tsid) group_root[g] = root_node[tsid[G.begin]];
 }
 
 // stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group,
@@ -537,6 +554,7 @@ void launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipS
 void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
                     uint32_t *parent, uint32_t *skip, uint32_t *depth, hipStream_t s) {
   if (!node_cap) return;
+  launch_fill_u32(parent, node_cap, kNone, s);
   hipLaunchKernelGGL(set_parents, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, frame,
                      node_cap, parent);
   GTH_LAUNCH_CHECK();
@@ -547,46 +565,52 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       unsigned long long *frame, uint32_t node_cap, double theta, WalkRec *recs,
-                       uint2 *span, Cell *cells, float4 *bodies, hipStream_t s) {
+                       unsigned long long *frame, uint32_t node_cap, double theta,
+                       const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
+                       Cell *cells, float4 *bodies, hipStream_t s) {
   const uint32_t m = n > node_cap ? n : node_cap;
   if (!m) return;
   hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
-                     tpos, tmass, n, frame, node_cap, 1.0 / theta, recs, span, cells, bodies);
+                     tpos, tmass, n, frame, node_cap, 1.0 / theta, tsid, roots, recs, span, cells,
+                     bodies);
   GTH_LAUNCH_CHECK();
 }
 
-void launch_group_starts(const int4 *lnk, const unsigned long long *frame, uint32_t node_cap,
-                         uint32_t n, uint32_t gs, int pack, uint32_t *flags, hipStream_t s) {
+void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
+                         uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
+                         hipStream_t s) {
   launch_fill_u32(flags, n, 0u, s);
   if (!node_cap) return;
-  hipLaunchKernelGGL(group_starts, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, frame,
-                     node_cap, gs, pack, flags);
+  hipLaunchKernelGGL(group_starts, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, parent,
+                     frame, node_cap, gs, pack, flags);
   GTH_LAUNCH_CHECK();
 }
 
 void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
-                        const unsigned long long *frame, uint32_t group_cap, uint32_t *starts,
-                        gt_group *groups, hipStream_t s) {
+                        const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
+                        const uint32_t *root_node, uint32_t *starts, gt_group *groups,
+                        uint32_t *group_root, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(compact_starts, dim3(blocks_for(n)), dim3(kBlock), 0, s, flags, offs, n,
                      starts);
   GTH_LAUNCH_CHECK();
   if (!group_cap) return;
   hipLaunchKernelGGL(make_groups, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, starts, frame,
-                     group_cap, n, groups);
+                     group_cap, n, tsid, root_node, groups, group_root);
   GTH_LAUNCH_CHECK();
 }
 
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                 uint32_t node_cap, const double *root, uint32_t cap_c, uint32_t cap_b,
+                 uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
+                 uint32_t cap_b,
                  const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
                  uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s) {
   if (!group_cap) return;
-#define GTH_WALK_ARGS                                                                        \
-  recs, span, tpos, groups, frame, group_cap, node_cap, root, cap_c, cap_b, cell_off, body_off, \
-      cell_count, body_count, cell_list, body_list
+  if (group_root && lanes != 1) fail(GT_HIP_ERR_ARGUMENT, "coherent walks need a single tree");
+#define GTH_WALK_ARGS                                                                   \
+  recs, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, cap_c, cap_b, \
+      cell_off, body_off, cell_count, body_count, cell_list, body_list
   if (lanes == 1) {
     const dim3 grid(xcd_grid(blocks_for(group_cap, kWalkBlock))), block(kWalkBlock);
     if (mode == kWalkExact)

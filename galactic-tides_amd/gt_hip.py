@@ -72,6 +72,9 @@ _tree = _sig("gt_hip_tree", C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp
                                       _vp, _vp, _P(Params), C.c_int, _vp])
 _group_compute = _sig("gt_hip_group_compute", C.c_int,
                       [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp])
+_group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
+                            [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
+_state_set_systems = _sig("gt_hip_state_set_systems", C.c_int, [_vp, C.c_uint64, _vp])
 _walk_tree = _sig("gt_hip_walk_tree", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, C.c_uint64,
                                                 _vp, _vp, _vp, _vp, _vp])
 _last_info = _sig("gt_hip_last_info", C.c_int, [_vp, _P(ListsInfo)])
@@ -194,6 +197,21 @@ class Context:
                                    _ptr(pot)))
         return acc, pot
 
+    def group_compute_batch(self, offsets, pos, mass, **cfg):
+        """Independent systems: system s is particles offsets[s]:offsets[s+1]."""
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        n = int(offsets[-1]) if len(offsets) else 0
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        if len(mass) != n:
+            raise ValueError(f"expected {n} masses, got {len(mass)}")
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        c = make_config(**cfg)
+        self._check(_group_compute_batch(self._h, C.byref(c), len(offsets) - 1, _ptr(offsets),
+                                         _ptr(pos), _ptr(mass), _ptr(acc), _ptr(pot)))
+        return acc, pot
+
     def walk_tree(self, geom, links, tree_pos, tree_mass, evaluate=True, **cfg):
         n = len(tree_mass)
         geom = _f64(geom)
@@ -253,6 +271,10 @@ class Context:
         self._n = n
         self._check(_state_set(self._h, n, _ptr(_f64(pos, (n, 3))), _ptr(_f64(vel, (n, 3))),
                                _ptr(_f64(mass))))
+
+    def state_set_systems(self, offsets):
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        self._check(_state_set_systems(self._h, len(offsets) - 1, _ptr(offsets)))
 
     def state_get(self):
         n = self._n

@@ -67,3 +67,29 @@ def encounter(m, g1_incl=0.0, g1_omega=0.0, g2_incl=60.0, g2_omega=30.0, rp=2.0,
     if got != n:
         raise ValueError("encounter: start separation below pericentre")
     return pos, vel, mass, typ
+
+
+def batch_of_galaxies(count=128, n_min=2000, n_max=100000, seed=5, halo="hernquist"):
+    """Independent galaxies for a batched evaluation: each galaxy's N is drawn
+    log-uniform from [n_min, n_max] (numpy default_rng(seed)), split 40/10/50
+    into disk/bulge/halo, and each scale length (disk_h, disk_z0, bulge_a and
+    the halo scale) is jittered by a factor uniform in [0.7, 1.3]; the other
+    parameters are the configurations' galaxy recipe.  Galaxy k's ICs use seed
+    1000 * seed + k.  Returns (offsets, pos, vel, mass): galaxy k is particles
+    offsets[k]:offsets[k + 1]."""
+    rng = np.random.default_rng(seed)
+    parts, sizes = [], []
+    for k in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        j = rng.uniform(0.7, 1.3, 4)
+        disk_n, bulge_n = int(0.4 * n), int(0.1 * n)
+        m = model(disk_n=disk_n, bulge_n=bulge_n, halo_n=n - disk_n - bulge_n, bulge_mass=0.333,
+                  bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0, halo_profile=halo,
+                  disk_h=1.0 * j[0], disk_z0=0.2 * j[1], bulge_a=0.2 * j[2], halo_a=4.0 * j[3],
+                  halo_core=1.0 * j[3], seed=1000 * seed + k)
+        pos, vel, mass, _ = galaxy(m)
+        parts.append((pos, vel, mass))
+        sizes.append(len(mass))
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    return (offsets, np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]),
+            np.concatenate([p[2] for p in parts]))

@@ -126,6 +126,16 @@ int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
  * back to particle order all run on the GPU.  pos/acc are n x 3 doubles. */
 int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                          const double *pos, const double *mass, double *acc, double *pot);
+/* Independent systems in one evaluation (no counterpart in the reference,
+ * which builds one GroupTreeGravity per system): system s is particles
+ * offsets[s] .. offsets[s + 1] (n_systems + 1 offsets from 0 to n, non-decreasing;
+ * empty systems allowed).  Each system gets exactly the tree, lists and forces
+ * gt_hip_group_compute gives it alone (its own root cube and FP32 origin; systems
+ * do not attract each other), but all of them are built, walked and evaluated by
+ * the same launches.  n = offsets[n_systems]. */
+int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
+                               uint64_t n_systems, const uint64_t *offsets, const double *pos,
+                               const double *mass, double *acc, double *pot);
 /* Walk and evaluate on a tree supplied by the caller (the reference's node
  * layout: geom n_nodes x 16 doubles = center xyz, half, com xyz, mass,
  * quad xx yy zz xy xz yz, delta, 0; links n_nodes x 4 int32 = first_child,
@@ -152,6 +162,11 @@ int gt_hip_get_lists(gt_hip_context *ctx, gt_group *groups, uint32_t *cell_list,
 /* Upload particles (n x 3 pos/vel, n mass) to device memory kept by ctx. */
 int gt_hip_state_set(gt_hip_context *ctx, uint64_t n, const double *pos, const double *vel,
                      const double *mass);
+/* The resident particles as a batch of independent systems (offsets as for
+ * gt_hip_group_compute_batch, ending at the n of gt_hip_state_set): later
+ * prime/steps evaluate each system on its own.  gt_hip_state_set resets to
+ * one system.  Forces must be primed again. */
+int gt_hip_state_set_systems(gt_hip_context *ctx, uint64_t n_systems, const uint64_t *offsets);
 /* Download; any pointer may be NULL. */
 int gt_hip_state_get(gt_hip_context *ctx, double *pos, double *vel, double *acc, double *pot);
 /* LeapfrogKDK::prime (integrator.h:104): forces for the resident state. */
