@@ -103,12 +103,29 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--exact", action="store_true", help="bit-exact evaluation mode")
     ap.add_argument("--eval-waves", type=int, default=0, help="tuning: 2 or 3 (0: default)")
+    ap.add_argument("--eval-order", type=int, default=0,
+                    help="tuning: 1 tree order, 2 longest first per XCD (0: default)")
+    ap.add_argument("--split-len", type=int, default=0,
+                    help="tuning: list entries per evaluation part (0: default, <0: never split)")
     ap.add_argument("--workload", default="encounter-2m", choices=["encounter-2m", "batch-128"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
                          "independent galaxies of 2k-100k particles evaluated together")
     ap.add_argument("--eager", action="store_true",
                     help="launch each step's kernels directly instead of from captured graphs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 3 times (the plain copy kernel "
+                         "that HBM traffic counters are calibrated against), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(64 << 20, device="cuda")  # 256 MiB of fp32
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 3 x 256 MiB", file=sys.stderr)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
               "is design-only in this build", file=sys.stderr)
@@ -116,7 +133,8 @@ def main():
 
     import gt_hip
     cfg = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
-               pack=1, exact=args.exact, eval_waves=args.eval_waves)
+               pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
+               eval_order=args.eval_order)
     t0 = time.time()
     offsets = None
     if args.workload == "batch-128":

@@ -23,6 +23,8 @@ struct History {
   uint32_t groups = 0;
   bool lists_known = false;
   uint64_t most_c = 0, most_b = 0;
+  bool items_known = false;
+  uint64_t items = 0, split_parts = 0;  // evaluation work items, parts of split groups
 };
 
 // Settings a captured step graph was made for.
@@ -37,7 +39,9 @@ struct GraphKey {
 bool same_plan(const Plan &a, const Plan &b) {
   return a.n == b.n && a.systems == b.systems && a.words == b.words && a.node_cap == b.node_cap &&
          a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
-         a.cap_b == b.cap_b;
+         a.cap_b == b.cap_b && a.split_len == b.split_len && a.item_cap == b.item_cap &&
+         a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
+         a.cost_buckets == b.cost_buckets;
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
@@ -87,6 +91,13 @@ struct gt_hip_context {
   uint32_t state_systems = 1;
   DBuf<gt_group> groups;
   DBuf<unsigned long long> scan_tmp;
+  // long lists in parts
+  DBuf<uint32_t> parts, split_count, item_off, split_off;
+  DBuf<uint2> items;
+  DBuf<float4> partial;
+  DBuf<uint64_t> item_keys, item_keys_alt;
+  DBuf<uint32_t> item_order, item_order_alt;
+  DBuf<unsigned long long> item_sort_tmp;
   uint32_t lists_n = 0, lists_nn = 0;
   const double *lists_geom = nullptr;  // geometry rows the last lists refer to
   // particle state (resident leapfrog) and inputs of single evaluations
@@ -188,6 +199,9 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
   if (p.group_cap < n && static_cast<uint64_t>(h.groups) + h.groups / 16 > p.group_cap) return false;
   if (h.most_c + h.most_c / 16 + 8 > p.cap_c || h.most_b + h.most_b / 16 + 8 > p.cap_b)
     return false;
+  if (p.split_len && (!h.items_known || h.items + h.items / 16 > p.item_cap ||
+                      h.split_parts + h.split_parts / 16 > p.split_cap))
+    return false;
   return true;
 }
 
@@ -198,9 +212,41 @@ bool sizes_known(const History &h, uint32_t n, uint32_t systems) {
 
 // Bounds for an evaluation of n particles: the last ones while they hold,
 // else new ones from the history with headroom (kept in c->plan).
-Plan bounds_for(gt_hip_context *c, uint32_t n) {
+constexpr uint32_t kDefaultSplitLen = 8192;
+// Quarter-octave cost classes, longest first: at 2M the evaluation takes
+// 2.45 ms against 2.79 in tree order and 2.46 fully sorted (with less L2
+// miss traffic than the full sort, since a class keeps tree order).
+constexpr int kDefaultEvalOrder = 4;
+
+int eval_order_for(const gt_group_config *cfg) {
+  return cfg->eval_order == 0 ? kDefaultEvalOrder : cfg->eval_order;
+}
+
+bool longest_first_for(const gt_group_config *cfg) {
+  return !cfg->exact && eval_order_for(cfg) >= 2;
+}
+
+// eval_order 3, 4: cost classes of half and quarter octaves
+int cost_buckets_for(const gt_group_config *cfg) {
+  const int o = eval_order_for(cfg);
+  return o == 3 ? 2 : o == 4 ? 4 : 0;
+}
+
+// List entries per evaluation part for these settings (0: no work items, the
+// groups are evaluated as they are; UINT32_MAX: items, but no list is split).
+uint32_t split_len_for(const gt_group_config *cfg) {
+  if (cfg->exact) return 0;
+  const uint32_t never = longest_first_for(cfg) ? 0xffffffffu : 0u;
+  if (cfg->split_len < 0) return never;
+  return cfg->split_len == 0 ? kDefaultSplitLen : static_cast<uint32_t>(cfg->split_len);
+}
+
+Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_first,
+                int buckets) {
   const History &h = c->hist;
-  if (plan_holds(c->plan, h, n, c->systems)) return c->plan;
+  if (c->plan.split_len == split_len && c->plan.longest_first == longest_first &&
+      c->plan.cost_buckets == buckets && plan_holds(c->plan, h, n, c->systems))
+    return c->plan;
   Plan p;
   p.n = n;
   p.systems = c->systems;
@@ -217,12 +263,22 @@ Plan bounds_for(gt_hip_context *c, uint32_t n) {
   if (static_cast<uint64_t>(p.group_cap) * p.cap_c > 0xffffffffull ||
       static_cast<uint64_t>(p.group_cap) * p.cap_b > 0xffffffffull)
     p.cap_c = p.cap_b = 0;
+  p.split_len = split_len;
+  p.longest_first = longest_first && split_len != 0;
+  p.cost_buckets = p.longest_first ? buckets : 0;
+  if (split_len) {
+    const bool known = same && h.items_known;
+    p.split_cap = known ? grown(h.split_parts, 64, 1u << 26) : p.group_cap / 8 + 64;
+    const uint64_t extra = known && h.items > h.groups ? h.items - h.groups : p.split_cap;
+    p.item_cap = static_cast<uint32_t>(
+        std::min<uint64_t>(p.group_cap + extra + extra / 4 + 64, 0xffffffffull));
+  }
   c->plan = p;
   return p;
 }
 
-Plan plan_for(gt_hip_context *c, uint32_t n) {
-  Plan p = bounds_for(c, n);
+Plan plan_for(gt_hip_context *c, uint32_t n, const gt_group_config *cfg) {
+  Plan p = bounds_for(c, n, split_len_for(cfg), longest_first_for(cfg), cost_buckets_for(cfg));
   const History &h = c->hist;
   if (c->debug_shrink && h.lists_known) {  // test hook: slots too short
     const uint64_t k = c->debug_shrink;
@@ -251,10 +307,17 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
     h.depth = static_cast<int>(fw[frame::kDepth]);
     h.groups = static_cast<uint32_t>(fw[frame::kGroups]);
   }
-  if ((st & ~static_cast<unsigned long long>(kAbortLists)) == 0) {
+  // List counts hold whenever the tree and groups were complete (slots or
+  // work items too small do not change them); item counts need whole lists.
+  if ((st & ~static_cast<unsigned long long>(kAbortLists | kAbortItems)) == 0) {
     h.lists_known = true;
     h.most_c = fw[frame::kStats + 4];
     h.most_b = fw[frame::kStats + 5];
+  }
+  if ((st & ~static_cast<unsigned long long>(kAbortItems)) == 0 && p.split_len) {
+    h.items_known = true;
+    h.items = fw[frame::kItems];
+    h.split_parts = fw[frame::kSplitParts];
   }
 }
 
@@ -296,6 +359,22 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_c, 1));
   c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_b, 1));
   c->out.ensure(n);
+  if (p.split_len) {
+    c->parts.ensure(gc);
+    c->split_count.ensure(gc);
+    c->item_off.ensure(gc);
+    c->split_off.ensure(gc);
+    c->items.ensure(std::max<uint32_t>(p.item_cap, 1));
+    c->partial.ensure(static_cast<size_t>(std::max<uint32_t>(p.split_cap, 1)) * kMaxGroup);
+  }
+  if (p.longest_first) {
+    const size_t ic = std::max<uint32_t>(p.item_cap, 1);
+    c->item_keys.ensure(ic);
+    c->item_keys_alt.ensure(ic);
+    c->item_order.ensure(ic);
+    c->item_order_alt.ensure(ic);
+    c->item_sort_tmp.ensure(radix_tmp_words(ic));
+  }
 }
 
 int walk_lanes(const gt_group_config *cfg, const Plan &p) {
@@ -306,7 +385,7 @@ int walk_lanes(const gt_group_config *cfg, const Plan &p) {
 
 // Clears the counts of the frame (not its status or step counter).
 void clear_counts(unsigned long long *frame, hipStream_t s) {
-  launch_fill_u64(frame + frame::kNodes, frame::kStats + 8 - frame::kNodes, 0ull, s);
+  launch_fill_u64(frame + frame::kNodes, frame::kCountsEnd - frame::kNodes, 0ull, s);
 }
 
 // Group selection, walk into per-group slots, statistics and the status check,
@@ -373,12 +452,59 @@ void lists_exact(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, c
   launch_check_frame(frame, p, false, s);
 }
 
+// The fast evaluation's work item arrays for bounds p.
+EvalSplit split_of(gt_hip_context *c, const Plan &p) {
+  EvalSplit sp;
+  sp.split_len = p.split_len;
+  sp.item_cap = p.item_cap;
+  sp.split_cap = p.split_cap;
+  sp.parts = c->parts.get();
+  sp.split_count = c->split_count.get();
+  sp.item_off = c->item_off.get();
+  sp.split_off = c->split_off.get();
+  sp.items = c->items.get();
+  sp.partial = c->partial.get();
+  if (p.longest_first) {
+    sp.longest_first = true;
+    sp.cost_buckets = p.cost_buckets;
+    sp.keys = c->item_keys.get();
+    sp.keys_alt = c->item_keys_alt.get();
+    sp.order = c->item_order.get();
+    sp.order_alt = c->item_order_alt.get();
+    sp.sort_tmp = c->item_sort_tmp.get();
+  }
+  return sp;
+}
+
+bool uses_items(const gt_group_config *cfg, const Plan &p) { return p.split_len && !cfg->exact; }
+
+// Work items of the fast evaluation (long lists in parts, each XCD's share
+// longest first): part of the lists phase.
+void enqueue_items(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+  if (!uses_items(cfg, p)) return;
+  reserve_all(c, p);
+  const EvalSplit sp = split_of(c, p);
+  launch_eval_items(c->groups.get(), c->frame.get(), p.group_cap, sp, c->scan_tmp.get(),
+                    c->stream);
+}
+
+// The list evaluation kernel alone.
 void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
   const float G = static_cast<float>(cfg->G);
   const float eps2 = static_cast<float>(cfg->eps * cfg->eps);
+  const bool items = uses_items(cfg, p);
+  const EvalSplit sp = items ? split_of(c, p) : EvalSplit{};
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
-              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), c->stream);
+              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), c->stream,
+              items ? &sp : nullptr);
+}
+
+// Results in tree order: split groups' parts added up.
+void enqueue_combine(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+  if (!uses_items(cfg, p)) return;
+  launch_combine_parts(c->groups.get(), c->frame.get(), p.group_cap, split_of(c, p), c->out.get(),
+                       c->stream);
 }
 
 // After a complete evaluation: sizes into the history and the statistics
@@ -418,7 +544,7 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
   unsigned long long *frame = c->frame.get();
   for (int attempt = 0;; ++attempt) {
     if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "force evaluation: bounds did not settle");
-    const Plan p = plan_for(c, n);
+    const Plan p = plan_for(c, n, cfg);
     const double *geom = c->tree.geom();
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     clear_counts(frame, s);
@@ -427,26 +553,38 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     enqueue_tree_lists(c, cfg, p);
-    GTH_CHECK(hipEventRecord(c->ev[2], s));
-    enqueue_eval(c, cfg, p);
-    GTH_CHECK(hipEventRecord(c->ev[3], s));
-    launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
-    GTH_CHECK(hipEventRecord(c->ev[4], s));
-    read_frame(c);
-    unsigned long long st = c->fw[frame::kStatus];
-    check_walk(c, st, GT_HIP_ERR_DEVICE);
-    if (st == kAbortLists) {
-      learn(c, p, st);
-      lists_exact(c, cfg, p, geom, c->tree.tree_pos());
+    // Without list slots yet (a first evaluation) the lists go to exact
+    // offsets below; evaluating now would only meet the overflow status.
+    const bool slots = p.cap_c != 0 && p.cap_b != 0;
+    if (slots) {
+      enqueue_items(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[2], s));
       enqueue_eval(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[3], s));
+      enqueue_combine(c, cfg, p);
+      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      GTH_CHECK(hipEventRecord(c->ev[4], s));
+    }
+    read_frame(c);
+    unsigned long long st = c->fw[frame::kStatus];
+    check_walk(c, st, GT_HIP_ERR_DEVICE);
+    if (!slots && st == 0) st = kAbortLists;  // lists complete but not evaluated: do both below
+    if ((st & ~static_cast<unsigned long long>(kAbortItems)) == kAbortLists) {
+      // slots too small (the item counts came from incomplete lists): rewrite
+      learn(c, p, st);
+      lists_exact(c, cfg, p, geom, c->tree.tree_pos());
+      enqueue_items(c, cfg, p);
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      enqueue_eval(c, cfg, p);
+      GTH_CHECK(hipEventRecord(c->ev[3], s));
+      enqueue_combine(c, cfg, p);
       launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
       GTH_CHECK(hipEventRecord(c->ev[4], s));
       read_frame(c);
       st = c->fw[frame::kStatus];
       check_walk(c, st, GT_HIP_ERR_DEVICE);
-      if (st) fail(GT_HIP_ERR_DEVICE, "group walk: exact lists incomplete");
+      if (st & ~static_cast<unsigned long long>(kAbortItems))
+        fail(GT_HIP_ERR_DEVICE, "group walk: exact lists incomplete");
     }
     if (st == 0) {
       finish(c, p, geom);
@@ -503,11 +641,13 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       break;
     case 1:
       enqueue_tree_lists(c, cfg, p);
+      enqueue_items(c, cfg, p);
       break;
     case 2:
       enqueue_eval(c, cfg, p);
       break;
     default:
+      enqueue_combine(c, cfg, p);
       launch_scatter(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
                      c->pot.get(), frame + frame::kStatus, s);
       launch_kick(c->vel.get(), c->acc.get(), n, h, frame + frame::kStatus, s);
@@ -583,7 +723,7 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
       ++done_total;
       continue;
     }
-    const Plan p = plan_for(c, n);
+    const Plan p = plan_for(c, n, cfg);
     prepare_step_graphs(c, cfg, p, dt);
     const int k = nsteps - done_total;
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
@@ -904,11 +1044,12 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
     unsigned long long *frame = c->frame.get();
     c->systems = 1;
     c->sys_off = nullptr;
-    Plan p = plan_for(c, static_cast<uint32_t>(n));
+    Plan p = plan_for(c, static_cast<uint32_t>(n), cfg);
     p.node_cap = static_cast<uint32_t>(n_nodes);
     for (int attempt = 0;; ++attempt) {
       if (attempt == 4) fail(GT_HIP_ERR_DEVICE, "group walk: bounds did not settle");
-      launch_fill_u64(frame, frame::kStats + 8, 0ull, s);
+      launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+      clear_counts(frame, s);
       launch_fill_u64(frame + frame::kNodes, 1, n_nodes, s);
       GTH_CHECK(hipEventRecord(c->ev[1], s));
       enqueue_lists(c, cfg, p, dg, dl, dp, dm);
@@ -931,14 +1072,28 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
       ++c->pstats.reruns;
     }
     finish(c, p, dg);
+    GTH_CHECK(hipEventRecord(c->ev[2], s));
+    if (out) {
+      for (int attempt = 0;; ++attempt) {
+        if (attempt == 4) fail(GT_HIP_ERR_DEVICE, "evaluation: bounds did not settle");
+        enqueue_items(c, cfg, p);
+        enqueue_eval(c, cfg, p);
+        enqueue_combine(c, cfg, p);
+        read_frame(c);
+        if (!(c->fw[frame::kStatus] & kAbortItems)) break;
+        // more work items than the bounds: room for what was counted
+        learn(c, p, kAbortItems);
+        p.split_cap = grown(c->fw[frame::kSplitParts], 64, 1u << 26);
+        p.item_cap = static_cast<uint32_t>(
+            std::min<uint64_t>(grown(c->fw[frame::kItems], 64, 0xffffffffull), 0xffffffffull));
+        launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+        ++c->pstats.reruns;
+      }
+      download(reinterpret_cast<float4 *>(out), c->out.get(), n, s);
+    }
     // The depth of a caller's tree is not known here: the next GPU build
     // starts from default bounds.
     c->hist.depth = -1;
-    GTH_CHECK(hipEventRecord(c->ev[2], s));
-    if (out) {
-      enqueue_eval(c, cfg, p);
-      download(reinterpret_cast<float4 *>(out), c->out.get(), n, s);
-    }
     GTH_CHECK(hipEventRecord(c->ev[3], s));
     GTH_CHECK(hipStreamSynchronize(s));
     c->tm.lists = elapsed_ms(c->ev[1], c->ev[2]);

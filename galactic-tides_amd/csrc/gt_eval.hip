@@ -10,7 +10,10 @@
 //    gt_group_accel (gravity_kernels.h:162-170) forms it, so results are bit-
 //    identical to the CPU evaluation.
 //
-//  * eval_fast: one wave64 per group, lane = list entry, targets wave-uniform.
+//  * eval_fast: one wave64 per work item (a group, or one part of a group
+//    whose lists are longer than the split length: the parts run on
+//    different waves and their per-target sums are added in part order by
+//    combine_parts), lane = list entry, targets wave-uniform.
 //    Each lane loads one list entry (coalesced index load plus one 48 B cell
 //    or 16 B body gather) and evaluates it against every target of the group,
 //    keeping per-target partial sums in registers; a cross-lane reduction
@@ -176,6 +179,26 @@ __device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, 
   return e;
 }
 
+// The list entries one wave evaluates: cells [cb, ce), bodies [bb, be).
+struct Range {
+  uint32_t cb, ce, bb, be;
+};
+
+// Part `part` of `parts` of a group's lists: the combined sequence (cells,
+// then bodies) cut into pieces of whole 64-entry chunks.
+__device__ inline Range part_range(const gt_group &g, uint32_t part, uint32_t parts) {
+  if (parts <= 1) return Range{g.cell_begin, g.cell_end, g.body_begin, g.body_end};
+  const uint32_t nc = g.cell_end - g.cell_begin, len = nc + (g.body_end - g.body_begin);
+  const uint32_t per = ((len + parts - 1) / parts + kWave - 1) / kWave * kWave;
+  const uint32_t lo = min(part * per, len), hi = min(lo + per, len);
+  Range r;
+  r.cb = g.cell_begin + min(lo, nc);
+  r.ce = g.cell_begin + min(hi, nc);
+  r.bb = g.body_begin + (lo > nc ? lo - nc : 0u);
+  r.be = g.body_begin + (hi > nc ? hi - nc : 0u);
+  return r;
+}
+
 // Keeps the compiler from hoisting the LDS target reads out of the chunk
 // loops (which would hold every target's coordinates in registers): the
 // offset is zero, but only known to be so at run time.
@@ -193,9 +216,9 @@ __device__ inline uint32_t opaque_zero() {
 template <int TB, bool QUAD>
 __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
                                   const uint32_t *__restrict__ cell_list,
-                                  const uint32_t *__restrict__ body_list, const gt_group &g,
+                                  const uint32_t *__restrict__ body_list, const Range &r,
                                   uint32_t tb, uint32_t nt, float G, float eps2, float4 *tgt,
-                                  uint32_t lane, float4 *__restrict__ out) {
+                                  uint32_t lane, float4 *__restrict__ dst) {
   if (lane < TB) tgt[lane] = bodies[tb + min(lane, nt - 1)];
   __builtin_amdgcn_wave_barrier();
   float ax[TB], ay[TB], az[TB], ap[TB];
@@ -203,7 +226,7 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
   for (int t = 0; t < TB; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
 
   {
-    const uint32_t cb = g.cell_begin, ce = g.cell_end;
+    const uint32_t cb = r.cb, ce = r.ce;
     uint32_t k = cb + lane;
     uint32_t idx = k < ce ? cell_list[k] : 0u;
     uint32_t idx_next = k + kWave < ce ? cell_list[k + kWave] : 0u;
@@ -248,7 +271,7 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
 
   {
     const float inf = __builtin_inff();
-    const uint32_t bb = g.body_begin, be = g.body_end;
+    const uint32_t bb = r.bb, be = r.be;
     uint32_t k = bb + lane;
     uint32_t j = k < be ? body_list[k] : kNone;
     uint32_t j_next = k + kWave < be ? body_list[k + kWave] : kNone;
@@ -291,38 +314,59 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
       rp = sp;
     }
   }
-  if (lane < nt) out[tb + lane] = make_float4(rx, ry, rz, rp);
+  if (lane < nt) dst[lane] = make_float4(rx, ry, rz, rp);
   __builtin_amdgcn_wave_barrier();  // targets of the next batch overwrite tgt
 }
 
 // WAVES: waves per SIMD the register budget is cut for (2: no spills; 3: a
 // few spilled registers for more latency hiding).  A tuning parameter.
+//
+// Work items: without `items`, item k is group k; with them, item k is part
+// (items[k].y >> 16) of (items[k].y & 0xffff) of group items[k].x, taken in
+// the order `order` gives (null: as they are).  A group
+// in one part writes its targets' results to out; a part of a split group
+// writes them to its slot of `partial` (kMaxGroup float4 each, slot
+// split_off[group] + part), which combine_parts adds up in part order.
 template <bool QUAD, int WAVES>
 __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
     const uint32_t *__restrict__ cell_list, const uint32_t *__restrict__ body_list,
     const gt_group *__restrict__ groups, const unsigned long long *__restrict__ frame,
-    uint32_t group_cap, float G, float eps2, float4 *__restrict__ out) {
+    uint32_t item_cap, const uint2 *__restrict__ items, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ split_off, float4 *__restrict__ partial, float G, float eps2,
+    float4 *__restrict__ out) {
   __shared__ float4 targets[kBlock / kWave][32];
   if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
-  const uint32_t n_groups = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t n_items =
+      bounded_count(frame + (items ? frame::kItems : frame::kGroups), item_cap);
   // XCD-aware: each XCD evaluates a contiguous run of (spatially adjacent)
   // groups, whose lists gather the same cells, so its L2 holds them.
   const uint32_t waves_per_block = kBlock / kWave;
   const uint32_t blk =
-      xcd_block_used(blockIdx.x, (n_groups + waves_per_block - 1) / waves_per_block);
+      xcd_block_used(blockIdx.x, (n_items + waves_per_block - 1) / waves_per_block);
   if (blk == kNone) return;
-  const uint32_t gi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
-  if (gi >= n_groups) return;
+  const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
+  if (wi >= n_items) return;
+  uint32_t gi = wi, part = 0, parts = 1;
+  if (items) {
+    const uint2 it = items[order ? order[wi] : wi];
+    gi = __builtin_amdgcn_readfirstlane(it.x);
+    part = __builtin_amdgcn_readfirstlane(it.y >> 16);
+    parts = __builtin_amdgcn_readfirstlane(it.y & 0xffffu);
+  }
   const uint32_t lane = threadIdx.x % kWave;
   float4 *tgt = targets[threadIdx.x / kWave];
   const gt_group g = groups[gi];
+  const Range r = part_range(g, part, parts);
+  float4 *dst0 = parts > 1 ? partial + static_cast<size_t>(split_off[gi] + part) * kMaxGroup
+                           : out + g.begin;
   const uint32_t ntot = g.end - g.begin;
   for (uint32_t t0 = 0; t0 < ntot; t0 += 32) {
     const uint32_t nt = min(32u, ntot - t0);
     const uint32_t tb = g.begin + t0;
+    float4 *dst = dst0 + t0;
 #define GTH_BATCH(TB) \
-  eval_batch<TB, QUAD>(bodies, cells, cell_list, body_list, g, tb, nt, G, eps2, tgt, lane, out)
+  eval_batch<TB, QUAD>(bodies, cells, cell_list, body_list, r, tb, nt, G, eps2, tgt, lane, dst)
     // Targets per batch rounded up to a multiple of 4: each path is
     // branch-free over its targets, and at most 3 targets are padding.
     switch ((nt + 3) / 4) {
@@ -339,17 +383,128 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   }
 }
 
+// ---- long lists in parts -----------------------------------------------------------
+
+// Parts of every group (lists longer than split_len entries are cut into
+// ceil(len / split_len) parts), and the parts of split groups only.  Entries
+// past the group count are zero, so the scans over group_cap entries add up.
+__global__ __launch_bounds__(kBlock) void count_parts(const gt_group *__restrict__ groups,
+                                                      const unsigned long long *__restrict__ frame,
+                                                      uint32_t group_cap, uint32_t split_len,
+                                                      uint32_t *__restrict__ parts,
+                                                      uint32_t *__restrict__ split_parts) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= group_cap) return;
+  uint32_t p = 0;
+  if (g < ng) {
+    const gt_group G = groups[g];
+    const uint32_t len = (G.cell_end - G.cell_begin) + (G.body_end - G.body_begin);
+    p = len > split_len ? min((len + split_len - 1) / split_len, 0xffffu) : 1u;
+  }
+  parts[g] = p;
+  split_parts[g] = p > 1 ? p : 0u;
+}
+
+// The work items, in group order (a split group's parts are adjacent, so they
+// run at the same time on one XCD); sets kAbortItems past the plan's bounds.
+__global__ __launch_bounds__(kBlock) void make_items(unsigned long long *__restrict__ frame,
+                                                     uint32_t group_cap, uint32_t item_cap,
+                                                     uint32_t split_cap,
+                                                     const uint32_t *__restrict__ parts,
+                                                     const uint32_t *__restrict__ item_off,
+                                                     uint2 *__restrict__ items) {
+  const bool fits = frame[frame::kItems] <= item_cap && frame[frame::kSplitParts] <= split_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !fits) atomicOr(frame + frame::kStatus, kAbortItems);
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (!fits || g >= ng) return;
+  const uint32_t p = parts[g], o = item_off[g];
+  for (uint32_t j = 0; j < p; ++j) items[o + j] = make_uint2(g, (j << 16) | p);
+}
+
+// Sort keys for evaluating each XCD's items longest first: the XCD chunk an
+// item's position falls in (most significant: items stay on their XCD, so
+// its L2 keeps serving the same region of the tree), then decreasing cost
+// (targets rounded up to the batch granularity times list entries).
+__global__ __launch_bounds__(kBlock) void item_keys(const unsigned long long *__restrict__ frame,
+                                                    uint32_t item_cap, int buckets,
+                                                    const uint2 *__restrict__ items,
+                                                    const gt_group *__restrict__ groups,
+                                                    uint64_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ order) {
+  // items were not written when they did not fit (make_items said so)
+  if (frame[frame::kStatus] & kAbortItems) return;
+  const uint32_t n_items = bounded_count(frame + frame::kItems, item_cap);
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n_items) return;
+  const uint32_t used = (n_items + kBlock / kWave - 1) / (kBlock / kWave);
+  const uint32_t q = (used + kXcds - 1) / kXcds;
+  const uint32_t xcd = (k / (kBlock / kWave)) / q;
+  const uint2 it = items[k];
+  const gt_group g = groups[it.x];
+  const Range r = part_range(g, it.y >> 16, it.y & 0xffffu);
+  const uint64_t cost = static_cast<uint64_t>((g.end - g.begin + 3) & ~3u) *
+                        ((r.ce - r.cb) + (r.be - r.bb));
+  // buckets > 0: cost classes of 1/buckets octave (items of a class keep
+  // their tree order, which keeps neighbouring groups together); else the
+  // cost itself
+  const uint64_t cq =
+      buckets > 0 ? static_cast<uint64_t>(buckets * log2f(static_cast<float>(cost) + 1.0f))
+                  : min(cost >> 4, 0xffffffull);
+  keys[k] = (static_cast<uint64_t>(xcd) << 24) | (0xffffffull - cq);
+  order[k] = k;
+}
+
+// Split groups: the parts' results added in part order (deterministic).
+__global__ __launch_bounds__(kBlock) void combine_parts(const gt_group *__restrict__ groups,
+                                                        const unsigned long long *__restrict__ frame,
+                                                        uint32_t group_cap,
+                                                        const uint32_t *__restrict__ parts,
+                                                        const uint32_t *__restrict__ split_off,
+                                                        const float4 *__restrict__ partial,
+                                                        float4 *__restrict__ out) {
+  if (frame[frame::kStatus]) return;
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ng) return;
+  const uint32_t p = parts[g];
+  if (p <= 1) return;
+  const gt_group G = groups[g];
+  const float4 *slot = partial + static_cast<size_t>(split_off[g]) * kMaxGroup;
+  for (uint32_t t = 0; t < G.end - G.begin; ++t) {
+    float4 a = slot[t];
+    for (uint32_t j = 1; j < p; ++j) {
+      const float4 b = slot[static_cast<size_t>(j) * kMaxGroup + t];
+      a.x += b.x;
+      a.y += b.y;
+      a.z += b.z;
+      a.w += b.w;
+    }
+    out[G.begin + t] = a;
+  }
+}
+
 }  // namespace
 
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
-                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s) {
+                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
+                 const EvalSplit *split) {
   if (group_cap == 0) return;
   const uint32_t waves_per_block = kBlock / kWave;
-  const uint32_t blocks = (group_cap + waves_per_block - 1) / waves_per_block;
+  const bool parts = split && !exact;
+  const uint32_t item_cap = parts ? split->item_cap : group_cap;
+  const uint32_t blocks = (item_cap + waves_per_block - 1) / waves_per_block;
   const dim3 grid(exact ? blocks : xcd_grid(blocks)), block(kBlock);
-#define GTH_EVAL_ARGS bodies, cells, cell_list, body_list, groups, frame, group_cap, G, eps2, out
+  const uint2 *items = parts ? split->items : nullptr;
+  const uint32_t *order = parts && split->longest_first ? split->order : nullptr;
+  const uint32_t *split_off = parts ? split->split_off : nullptr;
+  float4 *partial = parts ? split->partial : nullptr;
+#define GTH_EVAL_ARGS                                                                \
+  bodies, cells, cell_list, body_list, groups, frame, item_cap, items, order, split_off, \
+      partial, G, eps2, out
   if (exact)
     hipLaunchKernelGGL(eval_exact, grid, block, 0, s, bodies, cells, cell_list, body_list, groups,
                        frame, group_cap, G, eps2, quad, out);
@@ -363,6 +518,37 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
     hipLaunchKernelGGL((eval_fast<false, 2>), grid, block, 0, s, GTH_EVAL_ARGS);
 #undef GTH_EVAL_ARGS
   GTH_LAUNCH_CHECK();
+}
+
+void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
+                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s) {
+  if (group_cap == 0) return;
+  hipLaunchKernelGGL(combine_parts, dim3((group_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     groups, frame, group_cap, split.parts, split.split_off, split.partial, out);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
+                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s) {
+  if (group_cap == 0) return;
+  const dim3 grid((group_cap + kBlock - 1) / kBlock), block(kBlock);
+  hipLaunchKernelGGL(count_parts, grid, block, 0, s, groups, frame, group_cap, split.split_len,
+                     split.parts, split.split_count);
+  GTH_LAUNCH_CHECK();
+  exclusive_scan_u32(split.parts, split.item_off, group_cap, frame + frame::kItems, scan_tmp, s);
+  exclusive_scan_u32(split.split_count, split.split_off, group_cap, frame + frame::kSplitParts,
+                     scan_tmp, s);
+  hipLaunchKernelGGL(make_items, grid, block, 0, s, frame, group_cap, split.item_cap,
+                     split.split_cap, split.parts, split.item_off, split.items);
+  GTH_LAUNCH_CHECK();
+  if (split.longest_first && split.item_cap) {
+    hipLaunchKernelGGL(item_keys, dim3((split.item_cap + kBlock - 1) / kBlock), block, 0, s, frame,
+                       split.item_cap, split.cost_buckets, split.items, groups, split.keys,
+                       split.order);
+    GTH_LAUNCH_CHECK();
+    radix_sort_u64_dev(split.keys, split.order, split.keys_alt, split.order_alt,
+                       frame + frame::kItems, split.item_cap, 27, split.sort_tmp, s);
+  }
 }
 
 }  // namespace gth

@@ -145,9 +145,12 @@ constexpr int kGroups = 3;      // groups (scan total)
 constexpr int kDepth = 4;       // level of the deepest node
 constexpr int kWalkError = 5;   // bit 0: malformed tree; bit 1: half != root half / 2^depth
 constexpr int kStats = 6;       // 8 words, see list_stats
-constexpr int kStepsDone = 14;  // leapfrog steps completed since the counter was cleared
-constexpr int kStepInteractions = 15;  // their cell + body interactions
-constexpr int kWords = 16;
+constexpr int kItems = 14;      // evaluation work items (groups, long lists in parts)
+constexpr int kSplitParts = 15; // parts of the groups whose lists were split
+constexpr int kCountsEnd = 16;  // [kNodes, kCountsEnd): cleared before each evaluation
+constexpr int kStepsDone = 30;  // leapfrog steps completed since the counter was cleared
+constexpr int kStepInteractions = 31;  // their cell + body interactions
+constexpr int kWords = 32;
 }  // namespace frame
 
 constexpr uint32_t kAbortNodes = 1u;   // more nodes than Plan::node_cap
@@ -156,6 +159,7 @@ constexpr uint32_t kAbortDepth = 4u;   // deeper than Plan::depth_cap
 constexpr uint32_t kAbortGroups = 8u;  // more groups than Plan::group_cap
 constexpr uint32_t kAbortLists = 16u;  // a group's lists do not fit the slots
 constexpr uint32_t kAbortWalk = 32u;   // malformed tree
+constexpr uint32_t kAbortItems = 64u;  // more work items or split parts than the plan's bounds
 
 // Host-side bounds of one evaluation, fixed before it is enqueued.
 struct Plan {
@@ -166,6 +170,11 @@ struct Plan {
   int depth_cap = 60;      // deepest level whose moments are computed
   uint32_t group_cap = 0;  // groups covered by the walk and evaluation grids
   uint32_t cap_c = 0, cap_b = 0;  // per-group list slots (0: lists always overflow)
+  uint32_t split_len = 0;   // list entries per evaluation part (0: no work items)
+  uint32_t item_cap = 0;    // evaluation work items (split_len > 0)
+  uint32_t split_cap = 0;   // parts of split groups (split_len > 0)
+  bool longest_first = false;  // each XCD's items by decreasing cost
+  int cost_buckets = 0;        // > 0: by decreasing cost class (1/cost_buckets octave)
 };
 
 // A device count that must not exceed cap; past it (an overflow the status
@@ -195,12 +204,40 @@ void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint
                         unsigned long long *tmp, hipStream_t s);
 
 // gt_eval.hip
+// Long lists evaluated in parts (fast evaluation only): per group its part
+// count and its parts' first slot among split groups' parts; the work items;
+// one kMaxGroup float4 slot of partial sums per part of a split group.
+// With longest_first, each XCD evaluates its share of the items in order of
+// decreasing cost (a stable sort of item indices into `order`; keys and the
+// *_alt arrays hold item_cap entries, sort_tmp radix_tmp_words(item_cap)).
+struct EvalSplit {
+  uint32_t split_len = 0;  // list entries per part
+  uint32_t item_cap = 0, split_cap = 0;
+  uint32_t *parts = nullptr, *split_count = nullptr, *item_off = nullptr, *split_off = nullptr;
+  uint2 *items = nullptr;
+  float4 *partial = nullptr;
+  bool longest_first = false;
+  int cost_buckets = 0;  // > 0: order by cost classes of 1/cost_buckets octave
+  uint64_t *keys = nullptr, *keys_alt = nullptr;
+  uint32_t *order = nullptr, *order_alt = nullptr;
+  unsigned long long *sort_tmp = nullptr;
+};
+
+// Work items for the groups' lists (frame[kItems], frame[kSplitParts]);
+// sets kAbortItems past split.item_cap / split.split_cap.
+void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
+                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s);
 // Evaluates the first frame[kGroups] groups (at most group_cap) unless
-// frame[kStatus] is set.
+// frame[kStatus] is set; with split (fast evaluation), the work items that
+// launch_eval_items made (split groups' parts then need launch_combine_parts).
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
-                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s);
+                 uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
+                 const EvalSplit *split = nullptr);
+// Adds the parts of split groups into out, in part order.
+void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
+                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s);
 
 // gt_direct.hip
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,

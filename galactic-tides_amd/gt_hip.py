@@ -25,7 +25,8 @@ class GroupConfig(C.Structure):
     _fields_ = [("G", C.c_double), ("theta", C.c_double), ("eps", C.c_double),
                 ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
                 ("pack", C.c_int), ("exact", C.c_int), ("eval_waves", C.c_int),
-                ("walk_lanes", C.c_int), ("reserved", C.c_int * 5)]
+                ("walk_lanes", C.c_int), ("split_len", C.c_int), ("eval_order", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class Params(C.Structure):
@@ -108,11 +109,12 @@ def _f64(a, shape=None):
 
 
 def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
-                pack=-1, exact=False, eval_waves=0, walk_lanes=0):
+                pack=-1, exact=False, eval_waves=0, walk_lanes=0, split_len=0, eval_order=0):
     cfg = GroupConfig()
     cfg.G, cfg.theta, cfg.eps = G, theta, eps
     cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
     cfg.pack, cfg.exact, cfg.eval_waves, cfg.walk_lanes = pack, int(exact), eval_waves, walk_lanes
+    cfg.split_len, cfg.eval_order = split_len, eval_order
     return cfg
 
 

@@ -72,7 +72,15 @@ typedef struct {
                         for (2 or 3; 0 = library default, 3) */
   int walk_lanes;    /* tuning: 16, 32 or 64 groups share one wave-coherent walk;
                         0 or 1 (default): one independent walk per lane */
-  int reserved[5];   /* zero */
+  int split_len;     /* tuning: lists longer than this many entries are evaluated in
+                        parts by several waves (fast evaluation only; parts added in a
+                        fixed order); 0 = library default (8192), < 0 = never split */
+  int eval_order;    /* tuning (fast evaluation): the order in which each XCD takes its
+                        share of the groups: 1 = tree order, 2 = longest first, 3 / 4 =
+                        longest first by cost classes of half / quarter octaves (tree
+                        order within a class); 0 = library default (4).  Results do
+                        not depend on it. */
+  int reserved[3];   /* zero */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
