@@ -112,6 +112,9 @@ def main():
                          "independent galaxies of 2k-100k particles evaluated together")
     ap.add_argument("--eager", action="store_true",
                     help="launch each step's kernels directly instead of from captured graphs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tuning settings (otherwise the autotuner "
+                         "picks them on the initial particles before the timed steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 3 times (the plain copy kernel "
                          "that HBM traffic counters are calibrated against), then exit")
@@ -147,6 +150,14 @@ def main():
 
     ctx = gt_hip.Context(0)
     print(f"# device: {ctx.device_name}", file=sys.stderr, flush=True)
+    tuning = None
+    if not args.no_autotune and offsets is None and not args.exact:
+        # Only settings that leave the lists alone (group_size stays 32).
+        t0 = time.time()
+        tuning, tuned_ms = ctx.autotune(pos, mass, **cfg)
+        cfg.update({k: v for k, v in tuning.items() if k != "group_size"})
+        print(f"# autotune ({time.time() - t0:.1f} s): {tuning}, {tuned_ms:.2f} ms per evaluation",
+              file=sys.stderr, flush=True)
     ctx.state_set(pos, vel, mass)
     if offsets is not None:
         ctx.state_set_systems(offsets)
@@ -225,7 +236,8 @@ def main():
                    "systems": 1 if offsets is None else len(offsets) - 1,
                    "n": n, "theta": 0.75, "group_size": 32,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
-                   "exact": bool(args.exact), "graphs": graph, "parallelism": "single GPU"},
+                   "exact": bool(args.exact), "graphs": graph, "tuning": tuning,
+                   "parallelism": "single GPU"},
         "force_error": {"median": med, "p99": p99, "sample": n_err},
         "interactions_per_step": interactions,
         "roofline": roofline,

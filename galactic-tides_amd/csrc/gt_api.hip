@@ -991,6 +991,63 @@ int gt_hip_group_compute_batch(gt_hip_context *c, const gt_group_config *cfg, ui
   });
 }
 
+int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, const double *pos,
+                    const double *mass, int search_group_size, gt_group_config *best,
+                    double *best_ms) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    check_config(cfg);
+    require(best != nullptr, "best is NULL");
+    if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
+    *best = *cfg;
+    if (best_ms) *best_ms = 0.0;
+    if (n == 0) return;
+    require(pos && mass, "array is NULL");
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+    double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
+    upload(dp, pos, 3 * n, s);
+    upload(dm, mass, n, s);
+    c->systems = 1;
+    c->sys_off = nullptr;
+    // One warm-up (bounds settle) then the best of two timed evaluations.
+    auto time_of = [&](const gt_group_config &t) {
+      force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
+      double ms = 1e300;
+      for (int r = 0; r < 2; ++r) {
+        force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
+        ms = std::min(ms, c->tm.total);
+      }
+      return ms;
+    };
+    gt_group_config cur = *cfg;
+    double cur_ms = time_of(cur);
+    auto sweep = [&](int gt_group_config::*field, std::initializer_list<int> values) {
+      const int start = cur.*field;
+      for (int v : values) {
+        if (v == start) continue;
+        gt_group_config t = cur;
+        t.*field = v;
+        const double ms = time_of(t);
+        if (ms < 0.99 * cur_ms) {  // a change must beat run-to-run noise
+          cur = t;
+          cur_ms = ms;
+        }
+      }
+    };
+    if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
+    sweep(&gt_group_config::walk_lanes, {1, 16, 32, 64});
+    if (!cfg->exact) {
+      sweep(&gt_group_config::eval_waves, {2, 3});
+      sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
+      sweep(&gt_group_config::split_len, {2048, 8192, -1});
+    }
+    *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+  });
+}
+
 int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
                      uint64_t n_nodes, const double *geom, const int32_t *links,
                      const double *tree_pos, const double *tree_mass, gt_float4 *out) {

@@ -76,6 +76,8 @@ _group_compute = _sig("gt_hip_group_compute", C.c_int,
 _group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
                             [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
 _state_set_systems = _sig("gt_hip_state_set_systems", C.c_int, [_vp, C.c_uint64, _vp])
+_autotune = _sig("gt_hip_autotune", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, C.c_int,
+                                              _P(GroupConfig), _P(C.c_double)])
 _walk_tree = _sig("gt_hip_walk_tree", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, C.c_uint64,
                                                 _vp, _vp, _vp, _vp, _vp])
 _last_info = _sig("gt_hip_last_info", C.c_int, [_vp, _P(ListsInfo)])
@@ -213,6 +215,19 @@ class Context:
         self._check(_group_compute_batch(self._h, C.byref(c), len(offsets) - 1, _ptr(offsets),
                                          _ptr(pos), _ptr(mass), _ptr(acc), _ptr(pot)))
         return acc, pot
+
+    def autotune(self, pos, mass, search_group_size=False, **cfg):
+        """Fastest tuning settings for these particles: (settings dict, ms per evaluation)."""
+        n = len(mass)
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        c = make_config(**cfg)
+        best = GroupConfig()
+        ms = C.c_double()
+        self._check(_autotune(self._h, C.byref(c), n, _ptr(pos), _ptr(mass),
+                              int(search_group_size), C.byref(best), C.byref(ms)))
+        keys = ("group_size", "eval_waves", "walk_lanes", "split_len", "eval_order")
+        return {k: int(getattr(best, k)) for k in keys}, ms.value
 
     def walk_tree(self, geom, links, tree_pos, tree_mass, evaluate=True, **cfg):
         n = len(tree_mass)

@@ -144,6 +144,21 @@ int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64
 int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
                                uint64_t n_systems, const uint64_t *offsets, const double *pos,
                                const double *mass, double *acc, double *pot);
+/* Autotuner (no counterpart in the reference): times force evaluations of the
+ * given particles on the GPU (HIP events, best of 2 after a warm-up) while it
+ * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {1, 16,
+ * 32, 64}, eval_order {1, 2, 3, 4}, split_len {2048, 8192, never}, and group_size
+ * {32, 64} when search_group_size is 1), keeping each change that is at least 1%
+ * faster,
+ * and writes the fastest settings into *best (the other fields copied from
+ * *cfg) and their time per evaluation in ms into *best_ms (may be NULL).
+ * The tuning settings do not change the lists (results are identical or, for
+ * split lists, within the fast evaluation's tolerance); group_size is the
+ * reference's own parameter and changes the lists, so it is only searched on
+ * request.  Exact evaluation (cfg->exact) has nothing to tune but walk_lanes. */
+int gt_hip_autotune(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
+                    const double *pos, const double *mass, int search_group_size,
+                    gt_group_config *best, double *best_ms);
 /* Walk and evaluate on a tree supplied by the caller (the reference's node
  * layout: geom n_nodes x 16 doubles = center xyz, half, com xyz, mass,
  * quad xx yy zz xy xz yz, delta, 0; links n_nodes x 4 int32 = first_child,

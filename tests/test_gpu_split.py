@@ -72,3 +72,20 @@ def test_longest_first_order_changes_nothing(hip, oracle, split):
     a1, p1 = hip.group_compute(pos, mass, eval_order=1, **cfg)
     a2, p2 = hip.group_compute(pos, mass, eval_order=2, **cfg)
     assert np.array_equal(bits(a1), bits(a2)) and np.array_equal(bits(p1), bits(p2))
+
+
+def test_autotune_picks_valid_settings(hip, oracle):
+    pos, mass = oracle.test_system(1, 30000, 55)
+    cfg = dict(theta=0.6, eps=0.01, group_size=32)
+    best, ms = hip.autotune(pos, mass, **cfg)
+    print("autotune", best, ms)
+    assert ms > 0 and best["group_size"] == 32
+    assert best["eval_waves"] in (0, 2, 3) and best["walk_lanes"] in (0, 1, 16, 32, 64)
+    assert best["eval_order"] in (0, 1, 2, 3, 4) and best["split_len"] in (0, 2048, 8192, -1)
+    a0, _ = hip.group_compute(pos, mass, **cfg)
+    a1, _ = hip.group_compute(pos, mass, **{**cfg, **best})
+    e = rel_err(a1, a0)
+    assert np.median(e) < 1e-6 and e.max() < 1e-4
+    # with group sizes searched too
+    best, ms = hip.autotune(pos, mass, search_group_size=True, **cfg)
+    assert best["group_size"] in (32, 64)
