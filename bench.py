@@ -112,6 +112,8 @@ def main():
                          "independent galaxies of 2k-100k particles evaluated together")
     ap.add_argument("--eager", action="store_true",
                     help="launch each step's kernels directly instead of from captured graphs")
+    ap.add_argument("--hilo", action="store_true",
+                    help="hi/lo FP32 positions relative to group centres (precision mode)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tuning settings (otherwise the autotuner "
                          "picks them on the initial particles before the timed steps)")
@@ -137,7 +139,7 @@ def main():
     import gt_hip
     cfg = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
-               eval_order=args.eval_order)
+               eval_order=args.eval_order, hilo=args.hilo)
     t0 = time.time()
     offsets = None
     if args.workload == "batch-128":
@@ -236,7 +238,8 @@ def main():
                    "systems": 1 if offsets is None else len(offsets) - 1,
                    "n": n, "theta": 0.75, "group_size": 32,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
-                   "exact": bool(args.exact), "graphs": graph, "tuning": tuning,
+                   "exact": bool(args.exact), "hilo": bool(args.hilo), "graphs": graph,
+                   "tuning": tuning,
                    "parallelism": "single GPU"},
         "force_error": {"median": med, "p99": p99, "sample": n_err},
         "interactions_per_step": interactions,

@@ -32,7 +32,7 @@ struct GraphKey {
   Plan plan;
   double G = 0, theta = 0, eps = 0, dt = 0;
   int quadrupole = 0, group_size = 0, leaf_size = 0, pack = 0, exact = 0, eval_waves = 0,
-      walk_lanes = 0;
+      walk_lanes = 0, hilo = 0;
   uint64_t generation = 0;
 };
 
@@ -41,14 +41,14 @@ bool same_plan(const Plan &a, const Plan &b) {
          a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
          a.cap_b == b.cap_b && a.split_len == b.split_len && a.item_cap == b.item_cap &&
          a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
-         a.cost_buckets == b.cost_buckets;
+         a.cost_buckets == b.cost_buckets && a.hilo == b.hilo;
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
   return same_plan(a.plan, b.plan) && a.G == b.G && a.theta == b.theta && a.eps == b.eps &&
          a.dt == b.dt && a.quadrupole == b.quadrupole && a.group_size == b.group_size &&
          a.leaf_size == b.leaf_size && a.pack == b.pack && a.exact == b.exact &&
-         a.eval_waves == b.eval_waves && a.walk_lanes == b.walk_lanes &&
+         a.eval_waves == b.eval_waves && a.walk_lanes == b.walk_lanes && a.hilo == b.hilo &&
          a.generation == b.generation;
 }
 
@@ -95,6 +95,8 @@ struct gt_hip_context {
   DBuf<uint32_t> parts, split_count, item_off, split_off;
   DBuf<uint2> items;
   DBuf<float4> partial;
+  // hi/lo precision mode
+  DBuf<float4> bodies_lo, cells_lo, group_hilo;
   DBuf<uint64_t> item_keys, item_keys_alt;
   DBuf<uint32_t> item_order, item_order_alt;
   DBuf<unsigned long long> item_sort_tmp;
@@ -212,6 +214,8 @@ bool sizes_known(const History &h, uint32_t n, uint32_t systems) {
 
 // Bounds for an evaluation of n particles: the last ones while they hold,
 // else new ones from the history with headroom (kept in c->plan).
+bool uses_hilo(const gt_group_config *cfg) { return cfg->hilo != 0 && !cfg->exact; }
+
 constexpr uint32_t kDefaultSplitLen = 8192;
 // Quarter-octave cost classes, longest first: at 2M the evaluation takes
 // 2.45 ms against 2.79 in tree order and 2.46 fully sorted (with less L2
@@ -279,6 +283,7 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
 
 Plan plan_for(gt_hip_context *c, uint32_t n, const gt_group_config *cfg) {
   Plan p = bounds_for(c, n, split_len_for(cfg), longest_first_for(cfg), cost_buckets_for(cfg));
+  p.hilo = uses_hilo(cfg);
   const History &h = c->hist;
   if (c->debug_shrink && h.lists_known) {  // test hook: slots too short
     const uint64_t k = c->debug_shrink;
@@ -367,6 +372,11 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
     c->items.ensure(std::max<uint32_t>(p.item_cap, 1));
     c->partial.ensure(static_cast<size_t>(std::max<uint32_t>(p.split_cap, 1)) * kMaxGroup);
   }
+  if (p.hilo) {
+    c->bodies_lo.ensure(n);
+    c->cells_lo.ensure(nn);
+    c->group_hilo.ensure(2 * gc);
+  }
   if (p.longest_first) {
     const size_t ic = std::max<uint32_t>(p.item_cap, 1);
     c->item_keys.ensure(ic);
@@ -398,11 +408,13 @@ void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
   hipStream_t s = c->stream;
   unsigned long long *frame = c->frame.get();
   reserve_all(c, p);
+  const bool hilo = p.hilo;
   const uint32_t n = p.n, cap = p.node_cap;
   launch_link_up(lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
   launch_prep_lists(geom, lnk, c->skip.get(), c->depth.get(), tpos, tmass, n, frame, cap,
                     cfg->theta, tsid, roots, c->recs.get(), c->span.get(), c->cells.get(),
-                    c->bodies.get(), s);
+                    c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
+                    hilo ? c->cells_lo.get() : nullptr, s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
   launch_group_starts(lnk, c->parent.get(), frame, cap, n, static_cast<uint32_t>(cfg->group_size),
                       pack, c->flags.get(), s);
@@ -411,6 +423,9 @@ void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
   uint32_t *groot = tsid ? c->group_root.get() : nullptr;
   launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, tsid, root_node,
                      c->starts.get(), c->groups.get(), groot, s);
+  if (hilo)
+    launch_group_centres(c->groups.get(), frame, p.group_cap, tpos, tsid, roots, geom,
+                         c->group_hilo.get(), s);
   launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos,
               c->groups.get(), frame, p.group_cap, cap, geom, groot, p.cap_c, p.cap_b, nullptr,
               nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(), s);
@@ -494,10 +509,16 @@ void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) 
   const float eps2 = static_cast<float>(cfg->eps * cfg->eps);
   const bool items = uses_items(cfg, p);
   const EvalSplit sp = items ? split_of(c, p) : EvalSplit{};
+  EvalHiLo hl;
+  if (p.hilo) {
+    hl.bodies_lo = c->bodies_lo.get();
+    hl.cells_lo = c->cells_lo.get();
+    hl.group_hilo = c->group_hilo.get();
+  }
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
               cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), c->stream,
-              items ? &sp : nullptr);
+              items ? &sp : nullptr, p.hilo ? &hl : nullptr);
 }
 
 // Results in tree order: split groups' parts added up.
@@ -670,6 +691,7 @@ GraphKey key_for(const gt_group_config *cfg, const Plan &p, double dt) {
   k.exact = cfg->exact != 0;
   k.eval_waves = cfg->eval_waves == 2 ? 2 : 3;
   k.walk_lanes = walk_lanes(cfg, p);
+  k.hilo = uses_hilo(cfg);
   k.generation = alloc_generation;
   return k;
 }

@@ -156,14 +156,38 @@ struct Entry {
   float q0, q1, q2, q3, q4, q5;
 };
 
-template <bool QUAD>
-__device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, bool valid, float G) {
+// Hi/lo positions (precision mode): every position p relative to its system's
+// origin is hi + lo (two FP32, hi = fl(p), lo = fl(p - hi)); the wave's group
+// centre likewise is (gx, gy, gz) + (lx, ly, lz).  Positions are then taken
+// relative to the group centre, (hi - g_hi) + (lo - g_lo), which keeps the
+// differences of nearby positions exact to FP32 rounding at their own scale.
+struct HiLo {
+  const float4 *__restrict__ bodies_lo;
+  const float4 *__restrict__ cells_lo;
+  float gx, gy, gz, lx, ly, lz;
+};
+
+template <bool HILO>
+__device__ inline void relative(float &x, float &y, float &z, const float4 *__restrict__ lo_arr,
+                                uint32_t idx, const HiLo &h) {
+  if (HILO) {
+    const float4 lo = lo_arr[idx];
+    x = (x - h.gx) + (lo.x - h.lx);
+    y = (y - h.gy) + (lo.y - h.ly);
+    z = (z - h.gz) + (lo.z - h.lz);
+  }
+}
+
+template <bool QUAD, bool HILO>
+__device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, bool valid, float G,
+                                  const HiLo &h) {
   Entry e = {kFar, kFar, kFar, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (valid) {
     const float4 cm = cells[idx].cm;
     e.x = cm.x;
     e.y = cm.y;
     e.z = cm.z;
+    relative<HILO>(e.x, e.y, e.z, h.cells_lo, idx, h);
     e.ngm = -G * cm.w;
     if (QUAD) {
       const float4 a = cells[idx].q0;
@@ -177,6 +201,15 @@ __device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, 
     }
   }
   return e;
+}
+
+// Body j of a list (kNone: a lane past the end, far away and massless).
+template <bool HILO>
+__device__ inline float4 load_body(const float4 *__restrict__ bodies, uint32_t j, const HiLo &h) {
+  if (j == kNone) return make_float4(kFar, kFar, kFar, 0.0f);
+  float4 b = bodies[j];
+  relative<HILO>(b.x, b.y, b.z, h.bodies_lo, j, h);
+  return b;
 }
 
 // The list entries one wave evaluates: cells [cb, ce), bodies [bb, be).
@@ -213,13 +246,18 @@ __device__ inline uint32_t opaque_zero() {
 // list entry base + k.  Targets past nt repeat the last one and are not stored.
 // The list gathers are software-pipelined: the index of chunk c+2 and the
 // entry of chunk c+1 are in flight while chunk c is evaluated.
-template <int TB, bool QUAD>
+template <int TB, bool QUAD, bool HILO>
 __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
                                   const uint32_t *__restrict__ cell_list,
                                   const uint32_t *__restrict__ body_list, const Range &r,
                                   uint32_t tb, uint32_t nt, float G, float eps2, float4 *tgt,
-                                  uint32_t lane, float4 *__restrict__ dst) {
-  if (lane < TB) tgt[lane] = bodies[tb + min(lane, nt - 1)];
+                                  uint32_t lane, float4 *__restrict__ dst, const HiLo &h) {
+  if (lane < TB) {
+    const uint32_t t = tb + min(lane, nt - 1);
+    float4 p = bodies[t];
+    relative<HILO>(p.x, p.y, p.z, h.bodies_lo, t, h);
+    tgt[lane] = p;
+  }
   __builtin_amdgcn_wave_barrier();
   float ax[TB], ay[TB], az[TB], ap[TB];
 #pragma unroll
@@ -230,10 +268,10 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
     uint32_t k = cb + lane;
     uint32_t idx = k < ce ? cell_list[k] : 0u;
     uint32_t idx_next = k + kWave < ce ? cell_list[k + kWave] : 0u;
-    Entry e = load_cell<QUAD>(cells, idx, k < ce, G);
+    Entry e = load_cell<QUAD, HILO>(cells, idx, k < ce, G, h);
     for (uint32_t base = cb; base < ce; base += kWave) {
       const uint32_t kn = base + kWave + lane;
-      const Entry en = load_cell<QUAD>(cells, idx_next, kn < ce, G);
+      const Entry en = load_cell<QUAD, HILO>(cells, idx_next, kn < ce, G, h);
       const uint32_t k2 = kn + kWave;
       idx_next = k2 < ce ? cell_list[k2] : 0u;
       const uint32_t z = opaque_zero();
@@ -275,9 +313,9 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
     uint32_t k = bb + lane;
     uint32_t j = k < be ? body_list[k] : kNone;
     uint32_t j_next = k + kWave < be ? body_list[k + kWave] : kNone;
-    float4 b = j != kNone ? bodies[j] : make_float4(kFar, kFar, kFar, 0.0f);
+    float4 b = load_body<HILO>(bodies, j, h);
     for (uint32_t base = bb; base < be; base += kWave) {
-      const float4 bn = j_next != kNone ? bodies[j_next] : make_float4(kFar, kFar, kFar, 0.0f);
+      const float4 bn = load_body<HILO>(bodies, j_next, h);
       const uint32_t k2 = base + 2 * kWave + lane;
       const uint32_t j2 = k2 < be ? body_list[k2] : kNone;
       const float ngm = -G * b.w;
@@ -327,14 +365,15 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
 // in one part writes its targets' results to out; a part of a split group
 // writes them to its slot of `partial` (kMaxGroup float4 each, slot
 // split_off[group] + part), which combine_parts adds up in part order.
-template <bool QUAD, int WAVES>
+template <bool QUAD, int WAVES, bool HILO>
 __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
     const uint32_t *__restrict__ cell_list, const uint32_t *__restrict__ body_list,
     const gt_group *__restrict__ groups, const unsigned long long *__restrict__ frame,
     uint32_t item_cap, const uint2 *__restrict__ items, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ split_off, float4 *__restrict__ partial, float G, float eps2,
-    float4 *__restrict__ out) {
+    const float4 *__restrict__ bodies_lo, const float4 *__restrict__ cells_lo,
+    const float4 *__restrict__ group_hilo, float4 *__restrict__ out) {
   __shared__ float4 targets[kBlock / kWave][32];
   if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
   const uint32_t n_items =
@@ -358,6 +397,17 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   float4 *tgt = targets[threadIdx.x / kWave];
   const gt_group g = groups[gi];
   const Range r = part_range(g, part, parts);
+  HiLo h = {bodies_lo, cells_lo, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (HILO) {
+    const float4 hi = group_hilo[2 * static_cast<size_t>(gi)];
+    const float4 lo = group_hilo[2 * static_cast<size_t>(gi) + 1];
+    h.gx = hi.x;
+    h.gy = hi.y;
+    h.gz = hi.z;
+    h.lx = lo.x;
+    h.ly = lo.y;
+    h.lz = lo.z;
+  }
   float4 *dst0 = parts > 1 ? partial + static_cast<size_t>(split_off[gi] + part) * kMaxGroup
                            : out + g.begin;
   const uint32_t ntot = g.end - g.begin;
@@ -365,8 +415,9 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const uint32_t nt = min(32u, ntot - t0);
     const uint32_t tb = g.begin + t0;
     float4 *dst = dst0 + t0;
-#define GTH_BATCH(TB) \
-  eval_batch<TB, QUAD>(bodies, cells, cell_list, body_list, r, tb, nt, G, eps2, tgt, lane, dst)
+#define GTH_BATCH(TB)                                                                      \
+  eval_batch<TB, QUAD, HILO>(bodies, cells, cell_list, body_list, r, tb, nt, G, eps2, tgt, lane, \
+                             dst, h)
     // Targets per batch rounded up to a multiple of 4: each path is
     // branch-free over its targets, and at most 3 targets are padding.
     switch ((nt + 3) / 4) {
@@ -456,6 +507,40 @@ __global__ __launch_bounds__(kBlock) void item_keys(const unsigned long long *__
   order[k] = k;
 }
 
+// Hi/lo group centres: the centre of each group's FP64 bounding box (as the
+// walk computes it) relative to its system's origin, split into hi + lo
+// FP32 (group_hilo[2g], group_hilo[2g + 1]).
+__global__ __launch_bounds__(kBlock) void group_centres(const gt_group *__restrict__ groups,
+                                                        const unsigned long long *__restrict__ frame,
+                                                        uint32_t group_cap,
+                                                        const double *__restrict__ tpos,
+                                                        const uint32_t *__restrict__ tsid,
+                                                        const double *__restrict__ roots,
+                                                        const double *__restrict__ geom,
+                                                        float4 *__restrict__ group_hilo) {
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ng) return;
+  const gt_group G = groups[g];
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = tpos[3 * static_cast<size_t>(G.begin) + a];
+  for (uint32_t k = G.begin; k < G.end; ++k)
+    for (int a = 0; a < 3; ++a) {
+      const double v = tpos[3 * static_cast<size_t>(k) + a];
+      lo[a] = v < lo[a] ? v : lo[a];
+      hi[a] = hi[a] < v ? v : hi[a];
+    }
+  const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[G.begin]) : geom;
+  float h[3], l[3];
+  for (int a = 0; a < 3; ++a) {
+    const double c = 0.5 * (lo[a] + hi[a]) - o[a];
+    h[a] = static_cast<float>(c);
+    l[a] = static_cast<float>(c - static_cast<double>(h[a]));
+  }
+  group_hilo[2 * static_cast<size_t>(g)] = make_float4(h[0], h[1], h[2], 0.0f);
+  group_hilo[2 * static_cast<size_t>(g) + 1] = make_float4(l[0], l[1], l[2], 0.0f);
+}
+
 // Split groups: the parts' results added in part order (deterministic).
 __global__ __launch_bounds__(kBlock) void combine_parts(const gt_group *__restrict__ groups,
                                                         const unsigned long long *__restrict__ frame,
@@ -491,7 +576,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
-                 const EvalSplit *split) {
+                 const EvalSplit *split, const EvalHiLo *hilo) {
   if (group_cap == 0) return;
   const uint32_t waves_per_block = kBlock / kWave;
   const bool parts = split && !exact;
@@ -502,20 +587,30 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
   const uint32_t *order = parts && split->longest_first ? split->order : nullptr;
   const uint32_t *split_off = parts ? split->split_off : nullptr;
   float4 *partial = parts ? split->partial : nullptr;
+  const bool hl = hilo != nullptr && !exact;
+  const float4 *blo = hl ? hilo->bodies_lo : nullptr, *clo = hl ? hilo->cells_lo : nullptr,
+               *ghl = hl ? hilo->group_hilo : nullptr;
 #define GTH_EVAL_ARGS                                                                \
   bodies, cells, cell_list, body_list, groups, frame, item_cap, items, order, split_off, \
-      partial, G, eps2, out
-  if (exact)
+      partial, G, eps2, blo, clo, ghl, out
+#define GTH_EVAL(Q, W)                                                                    \
+  if (hl)                                                                                \
+    hipLaunchKernelGGL((eval_fast<Q, W, true>), grid, block, 0, s, GTH_EVAL_ARGS);  \
+  else                                                                                   \
+    hipLaunchKernelGGL((eval_fast<Q, W, false>), grid, block, 0, s, GTH_EVAL_ARGS)
+  if (exact) {
     hipLaunchKernelGGL(eval_exact, grid, block, 0, s, bodies, cells, cell_list, body_list, groups,
                        frame, group_cap, G, eps2, quad, out);
-  else if (quad && waves == 3)
-    hipLaunchKernelGGL((eval_fast<true, 3>), grid, block, 0, s, GTH_EVAL_ARGS);
-  else if (quad)
-    hipLaunchKernelGGL((eval_fast<true, 2>), grid, block, 0, s, GTH_EVAL_ARGS);
-  else if (waves == 3)
-    hipLaunchKernelGGL((eval_fast<false, 3>), grid, block, 0, s, GTH_EVAL_ARGS);
-  else
-    hipLaunchKernelGGL((eval_fast<false, 2>), grid, block, 0, s, GTH_EVAL_ARGS);
+  } else if (quad && waves == 3) {
+    GTH_EVAL(true, 3);
+  } else if (quad) {
+    GTH_EVAL(true, 2);
+  } else if (waves == 3) {
+    GTH_EVAL(false, 3);
+  } else {
+    GTH_EVAL(false, 2);
+  }
+#undef GTH_EVAL
 #undef GTH_EVAL_ARGS
   GTH_LAUNCH_CHECK();
 }
@@ -525,6 +620,16 @@ void launch_combine_parts(const gt_group *groups, const unsigned long long *fram
   if (group_cap == 0) return;
   hipLaunchKernelGGL(combine_parts, dim3((group_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      groups, frame, group_cap, split.parts, split.split_off, split.partial, out);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_group_centres(const gt_group *groups, const unsigned long long *frame,
+                          uint32_t group_cap, const double *tpos, const uint32_t *tsid,
+                          const double *roots, const double *geom, float4 *group_hilo,
+                          hipStream_t s) {
+  if (group_cap == 0) return;
+  hipLaunchKernelGGL(group_centres, dim3((group_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     groups, frame, group_cap, tpos, tsid, roots, geom, group_hilo);
   GTH_LAUNCH_CHECK();
 }
 

@@ -175,6 +175,7 @@ struct Plan {
   uint32_t split_cap = 0;   // parts of split groups (split_len > 0)
   bool longest_first = false;  // each XCD's items by decreasing cost
   int cost_buckets = 0;        // > 0: by decreasing cost class (1/cost_buckets octave)
+  bool hilo = false;           // hi/lo positions relative to group centres
 };
 
 // A device count that must not exceed cap; past it (an overflow the status
@@ -230,11 +231,20 @@ void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32
 // Evaluates the first frame[kGroups] groups (at most group_cap) unless
 // frame[kStatus] is set; with split (fast evaluation), the work items that
 // launch_eval_items made (split groups' parts then need launch_combine_parts).
+// Hi/lo precision mode: lo parts of the bodies' and cells' FP32 positions
+// (prep_lists), and each group's centre as hi, lo float4 pairs.
+struct EvalHiLo {
+  const float4 *bodies_lo = nullptr, *cells_lo = nullptr, *group_hilo = nullptr;
+};
+void launch_group_centres(const gt_group *groups, const unsigned long long *frame,
+                          uint32_t group_cap, const double *tpos, const uint32_t *tsid,
+                          const double *roots, const double *geom, float4 *group_hilo,
+                          hipStream_t s);
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
-                 const EvalSplit *split = nullptr);
+                 const EvalSplit *split = nullptr, const EvalHiLo *hilo = nullptr);
 // Adds the parts of split groups into out, in part order.
 void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
                           uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s);
@@ -256,11 +266,13 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 // Batches of independent systems pass tsid (system of each tree position),
 // roots (4 doubles per system: root centre, half) and root_node (root node
 // of each system); for one system they are null (the root is node 0).
+// bodies_lo / cells_lo (may be null): the lo parts of the FP32 positions.
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
                        unsigned long long *frame, uint32_t node_cap, double theta,
                        const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
-                       Cell *cells, float4 *bodies, hipStream_t s);
+                       Cell *cells, float4 *bodies, float4 *bodies_lo, float4 *cells_lo,
+                       hipStream_t s);
 void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
                          uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
                          hipStream_t s);

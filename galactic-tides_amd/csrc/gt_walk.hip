@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     unsigned long long *__restrict__ frame, uint32_t node_cap, double inv_theta,
     const uint32_t *__restrict__ tsid, const double *__restrict__ roots,
     WalkRec *__restrict__ recs, uint2 *__restrict__ span, Cell *__restrict__ cells,
-    float4 *__restrict__ bodies) {
+    float4 *__restrict__ bodies, float4 *__restrict__ bodies_lo, float4 *__restrict__ cells_lo) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   // origin: the root centre of the body's (node's) system (tsid null: one
@@ -104,10 +104,13 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
   if (k < n) {
     const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[k]) : geom;
     const double ox = o[0], oy = o[1], oz = o[2];
-    bodies[k] = make_float4(static_cast<float>(tpos[3 * k] - ox),
-                            static_cast<float>(tpos[3 * k + 1] - oy),
-                            static_cast<float>(tpos[3 * k + 2] - oz),
-                            static_cast<float>(tmass[k]));
+    const double px = tpos[3 * k] - ox, py = tpos[3 * k + 1] - oy, pz = tpos[3 * k + 2] - oz;
+    const float hx = static_cast<float>(px), hy = static_cast<float>(py),
+                hz = static_cast<float>(pz);
+    bodies[k] = make_float4(hx, hy, hz, static_cast<float>(tmass[k]));
+    if (bodies_lo)
+      bodies_lo[k] = make_float4(static_cast<float>(px - hx), static_cast<float>(py - hy),
+                                 static_cast<float>(pz - hz), 0.0f);
   }
   if (k < n_nodes) {
     const double *g = geom + static_cast<size_t>(kGeomStride) * k;
@@ -115,8 +118,12 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[L.z]) : geom;
     const double ox = o[0], oy = o[1], oz = o[2];
     Cell c;
-    c.cm = make_float4(static_cast<float>(g[4] - ox), static_cast<float>(g[5] - oy),
-                       static_cast<float>(g[6] - oz), static_cast<float>(g[7]));
+    const double mx = g[4] - ox, my = g[5] - oy, mz = g[6] - oz;
+    c.cm = make_float4(static_cast<float>(mx), static_cast<float>(my), static_cast<float>(mz),
+                       static_cast<float>(g[7]));
+    if (cells_lo)
+      cells_lo[k] = make_float4(static_cast<float>(mx - c.cm.x), static_cast<float>(my - c.cm.y),
+                                static_cast<float>(mz - c.cm.z), 0.0f);
     c.q0 = make_float4(static_cast<float>(g[8]), static_cast<float>(g[9]),
                        static_cast<float>(g[10]), static_cast<float>(g[11]));
     c.q1 = make_float4(static_cast<float>(g[12]), static_cast<float>(g[13]), 0.0f, 0.0f);
@@ -567,12 +574,13 @@ void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
                        unsigned long long *frame, uint32_t node_cap, double theta,
                        const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
-                       Cell *cells, float4 *bodies, hipStream_t s) {
+                       Cell *cells, float4 *bodies, float4 *bodies_lo, float4 *cells_lo,
+                       hipStream_t s) {
   const uint32_t m = n > node_cap ? n : node_cap;
   if (!m) return;
   hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
                      tpos, tmass, n, frame, node_cap, 1.0 / theta, tsid, roots, recs, span, cells,
-                     bodies);
+                     bodies, bodies_lo, cells_lo);
   GTH_LAUNCH_CHECK();
 }
 

@@ -26,7 +26,7 @@ class GroupConfig(C.Structure):
                 ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
                 ("pack", C.c_int), ("exact", C.c_int), ("eval_waves", C.c_int),
                 ("walk_lanes", C.c_int), ("split_len", C.c_int), ("eval_order", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("hilo", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class Params(C.Structure):
@@ -111,12 +111,13 @@ def _f64(a, shape=None):
 
 
 def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
-                pack=-1, exact=False, eval_waves=0, walk_lanes=0, split_len=0, eval_order=0):
+                pack=-1, exact=False, eval_waves=0, walk_lanes=0, split_len=0, eval_order=0,
+                hilo=False):
     cfg = GroupConfig()
     cfg.G, cfg.theta, cfg.eps = G, theta, eps
     cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
     cfg.pack, cfg.exact, cfg.eval_waves, cfg.walk_lanes = pack, int(exact), eval_waves, walk_lanes
-    cfg.split_len, cfg.eval_order = split_len, eval_order
+    cfg.split_len, cfg.eval_order, cfg.hilo = split_len, eval_order, int(hilo)
     return cfg
 
 

@@ -80,7 +80,12 @@ typedef struct {
                         longest first by cost classes of half / quarter octaves (tree
                         order within a class); 0 = library default (4).  Results do
                         not depend on it. */
-  int reserved[3];   /* zero */
+  int hilo;          /* 1: fast evaluation with every position as an FP32 hi + lo pair
+                        taken relative to its group's centre (bounding-box centre in
+                        FP64), instead of FP32 relative to the root centre as the
+                        reference does; same lists, more precise differences of nearby
+                        positions.  Ignored by exact evaluation. */
+  int reserved[2];   /* zero */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
