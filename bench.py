@@ -38,6 +38,15 @@ def encounter_2m(halo="hernquist"):
     return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
 
 
+def galaxy_4m(halo="hernquist"):
+    import gt_ics
+    m = gt_ics.model(disk_n=1_600_000, bulge_n=400_000, halo_n=2_000_000, bulge_mass=0.333,
+                     bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
+                     halo_profile=halo, halo_a=4.0, seed=6)
+    pos, vel, mass, _ = gt_ics.galaxy(m)
+    return pos, vel, mass
+
+
 def measured_traffic(kernel="eval_fast"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
@@ -107,9 +116,15 @@ def main():
                     help="tuning: 1 tree order, 2 longest first per XCD (0: default)")
     ap.add_argument("--split-len", type=int, default=0,
                     help="tuning: list entries per evaluation part (0: default, <0: never split)")
-    ap.add_argument("--workload", default="encounter-2m", choices=["encounter-2m", "batch-128"],
+    ap.add_argument("--workload", default="encounter-2m",
+                    choices=["encounter-2m", "batch-128", "galaxy-4m"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
-                         "independent galaxies of 2k-100k particles evaluated together")
+                         "independent galaxies of 2k-100k particles evaluated together; "
+                         "galaxy-4m: one 4M-particle galaxy (seed 6)")
+    ap.add_argument("--theta", type=float, default=0.75)
+    ap.add_argument("--predict", type=int, default=0,
+                    help="list slots: 0 uniform, 1 learned predictor (trained on the first "
+                         "50 evaluations), 2 two passes (count, fill)")
     ap.add_argument("--eager", action="store_true",
                     help="launch each step's kernels directly instead of from captured graphs")
     ap.add_argument("--hilo", action="store_true",
@@ -137,14 +152,16 @@ def main():
         sys.exit(2)
 
     import gt_hip
-    cfg = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
+    cfg = dict(G=1.0, theta=args.theta, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
-               eval_order=args.eval_order, hilo=args.hilo)
+               eval_order=args.eval_order, hilo=args.hilo, predict=args.predict)
     t0 = time.time()
     offsets = None
     if args.workload == "batch-128":
         import gt_ics
         offsets, pos, vel, mass = gt_ics.batch_of_galaxies(halo=args.halo)
+    elif args.workload == "galaxy-4m":
+        pos, vel, mass = galaxy_4m(args.halo)
     else:
         pos, vel, mass, _ = encounter_2m(args.halo)
     n = len(mass)
@@ -153,7 +170,7 @@ def main():
     ctx = gt_hip.Context(0)
     print(f"# device: {ctx.device_name}", file=sys.stderr, flush=True)
     tuning = None
-    if not args.no_autotune and offsets is None and not args.exact:
+    if not args.no_autotune and args.workload == "encounter-2m" and not args.exact:
         # Only settings that leave the lists alone (group_size stays 32).
         t0 = time.time()
         tuning, tuned_ms = ctx.autotune(pos, mass, **cfg)
@@ -201,7 +218,8 @@ def main():
     # vector ALUs; pairwise gravity has no matrix operation to put on MFMA).
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic() if offsets is None else None, "kernel": "eval_fast",
+                "traffic": measured_traffic() if args.workload == "encounter-2m" else None,
+                "kernel": "eval_fast",
                 "kernel_ms": round(eval_ms, 4),
                 "flops_per_launch": flops, "compute": "fp32 vector ALU"}
 
@@ -234,14 +252,16 @@ def main():
         "dtype": "fp32 (lists, evaluation); fp64 (tree, walk, kick-drift)",
         "data": "synthetic: Hernquist galaxies generated in-process (positions identical to "
                 "the reference ICs generator)",
-        "config": {"workload": "encounter-2M" if offsets is None else "batch-128-galaxies",
+        "config": {"workload": {"encounter-2m": "encounter-2M", "batch-128": "batch-128-galaxies",
+                                "galaxy-4m": "galaxy-4M"}[args.workload],
                    "systems": 1 if offsets is None else len(offsets) - 1,
-                   "n": n, "theta": 0.75, "group_size": 32,
+                   "n": n, "theta": args.theta, "group_size": 32, "predict": args.predict,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
                    "exact": bool(args.exact), "hilo": bool(args.hilo), "graphs": graph,
                    "tuning": tuning,
                    "parallelism": "single GPU"},
         "force_error": {"median": med, "p99": p99, "sample": n_err},
+        "predictor": ctx.predictor_stats() if args.predict == 1 else None,
         "interactions_per_step": interactions,
         "roofline": roofline,
         "cpu_baseline": cpu,

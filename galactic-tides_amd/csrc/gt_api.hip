@@ -3,12 +3,15 @@
 // evaluation on the GPU, and the device-resident leapfrog.
 #include <algorithm>
 #include <cstring>
+#include <cmath>
 #include <new>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "gt_build.h"
 #include "gt_internal.h"
+#include "gt_predict.h"
 
 using namespace gth;
 
@@ -25,7 +28,30 @@ struct History {
   uint64_t most_c = 0, most_b = 0;
   bool items_known = false;
   uint64_t items = 0, split_parts = 0;  // evaluation work items, parts of split groups
+  uint64_t all_c = 0, all_b = 0;        // list entries (lists_known)
+  bool pred_known = false;
+  uint64_t pred_c = 0, pred_b = 0;      // predicted slot totals
 };
+
+// The learned list-length predictor of a context.
+struct Predictor {
+  uint32_t collect = 50;     // evaluations to gather samples from
+  uint32_t collected = 0;    // evaluations sampled so far
+  uint32_t samples = 0;      // samples in the buffer (at most kSampleCap)
+  uint32_t next = 0;         // where the next sample goes
+  bool trained = false;
+  uint64_t iterations = 0;
+  double loss = 0.0;
+  float margin = 0.0f;        // log-space headroom (its copy in device memory is used)
+  float margin_floor = 0.0f;  // the margin right after training
+  uint64_t predicted = 0, overflows = 0;
+  double err = 0.0;
+  uint64_t slot_entries = 0, list_entries = 0;
+};
+constexpr uint32_t kSampleCap = 1u << 18;      // samples kept
+constexpr uint32_t kSamplesPerEval = 4096;     // taken from each evaluation
+constexpr uint32_t kTrainIterations = 1500;
+constexpr uint32_t kTrainBlocks = 32;          // x 64 samples per minibatch
 
 // Settings a captured step graph was made for.
 struct GraphKey {
@@ -41,7 +67,8 @@ bool same_plan(const Plan &a, const Plan &b) {
          a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
          a.cap_b == b.cap_b && a.split_len == b.split_len && a.item_cap == b.item_cap &&
          a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
-         a.cost_buckets == b.cost_buckets && a.hilo == b.hilo;
+         a.cost_buckets == b.cost_buckets && a.hilo == b.hilo && a.predicted == b.predicted &&
+         a.slot_total_c == b.slot_total_c && a.slot_total_b == b.slot_total_b;
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
@@ -97,6 +124,11 @@ struct gt_hip_context {
   DBuf<float4> partial;
   // hi/lo precision mode
   DBuf<float4> bodies_lo, cells_lo, group_hilo;
+  // learned list lengths
+  Predictor pred;
+  DBuf<float> pred_params, pred_m, pred_v, pred_stats, pred_samples, pred_grad, pred_loss, feat,
+      pred_out;
+  DBuf<uint32_t> slot_c, slot_b, slot_off_c, slot_off_b;
   DBuf<uint64_t> item_keys, item_keys_alt;
   DBuf<uint32_t> item_order, item_order_alt;
   DBuf<unsigned long long> item_sort_tmp;
@@ -204,6 +236,9 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
   if (p.split_len && (!h.items_known || h.items + h.items / 16 > p.item_cap ||
                       h.split_parts + h.split_parts / 16 > p.split_cap))
     return false;
+  if (p.predicted && (!h.pred_known || h.pred_c + h.pred_c / 16 > p.slot_total_c ||
+                      h.pred_b + h.pred_b / 16 > p.slot_total_b))
+    return false;
   return true;
 }
 
@@ -246,10 +281,11 @@ uint32_t split_len_for(const gt_group_config *cfg) {
 }
 
 Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_first,
-                int buckets) {
+                int buckets, bool predicted) {
   const History &h = c->hist;
   if (c->plan.split_len == split_len && c->plan.longest_first == longest_first &&
-      c->plan.cost_buckets == buckets && plan_holds(c->plan, h, n, c->systems))
+      c->plan.cost_buckets == buckets && c->plan.predicted == predicted &&
+      plan_holds(c->plan, h, n, c->systems))
     return c->plan;
   Plan p;
   p.n = n;
@@ -267,6 +303,13 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
   if (static_cast<uint64_t>(p.group_cap) * p.cap_c > 0xffffffffull ||
       static_cast<uint64_t>(p.group_cap) * p.cap_b > 0xffffffffull)
     p.cap_c = p.cap_b = 0;
+  p.predicted = predicted;
+  if (predicted) {  // the list buffers: the predicted totals (or the true ones) with headroom
+    const bool known = same && h.pred_known;
+    const uint64_t lim = 0xffffffffull;  // list offsets are 32 bit
+    p.slot_total_c = std::min(known ? h.pred_c + h.pred_c / 4 + 4096 : 2 * h.all_c + 4096, lim);
+    p.slot_total_b = std::min(known ? h.pred_b + h.pred_b / 4 + 4096 : 2 * h.all_b + 4096, lim);
+  }
   p.split_len = split_len;
   p.longest_first = longest_first && split_len != 0;
   p.cost_buckets = p.longest_first ? buckets : 0;
@@ -282,8 +325,11 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
 }
 
 Plan plan_for(gt_hip_context *c, uint32_t n, const gt_group_config *cfg) {
-  Plan p = bounds_for(c, n, split_len_for(cfg), longest_first_for(cfg), cost_buckets_for(cfg));
+  const bool predicted = cfg->predict == 1 && c->pred.trained && c->hist.lists_known;
+  Plan p = bounds_for(c, n, split_len_for(cfg), longest_first_for(cfg), cost_buckets_for(cfg),
+                      predicted);
   p.hilo = uses_hilo(cfg);
+  if (cfg->predict == 2) p.cap_c = p.cap_b = 0;  // two passes: count, then fill
   const History &h = c->hist;
   if (c->debug_shrink && h.lists_known) {  // test hook: slots too short
     const uint64_t k = c->debug_shrink;
@@ -318,6 +364,13 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
     h.lists_known = true;
     h.most_c = fw[frame::kStats + 4];
     h.most_b = fw[frame::kStats + 5];
+    h.all_c = fw[frame::kStats + 6];
+    h.all_b = fw[frame::kStats + 7];
+  }
+  if (p.predicted && !(st & (kAbortNodes | kAbortGroups | kAbortPaths | kAbortDepth))) {
+    h.pred_known = true;
+    h.pred_c = fw[frame::kPredCells];
+    h.pred_b = fw[frame::kPredBodies];
   }
   if ((st & ~static_cast<unsigned long long>(kAbortItems)) == 0 && p.split_len) {
     h.items_known = true;
@@ -361,8 +414,19 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   if (p.systems > 1) c->group_root.ensure(gc);
   c->ccount.ensure(gc);
   c->bcount.ensure(gc);
-  c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_c, 1));
-  c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_b, 1));
+  if (p.predicted) {
+    c->cell_list.ensure(std::max<uint64_t>(p.slot_total_c, 1));
+    c->body_list.ensure(std::max<uint64_t>(p.slot_total_b, 1));
+    c->feat.ensure(static_cast<size_t>(kPredFeatures) * gc);
+    c->pred_out.ensure(static_cast<size_t>(kPredOutputs) * gc);
+    c->slot_c.ensure(gc);
+    c->slot_b.ensure(gc);
+    c->slot_off_c.ensure(gc);
+    c->slot_off_b.ensure(gc);
+  } else {
+    c->cell_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_c, 1));
+    c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_b, 1));
+  }
   c->out.ensure(n);
   if (p.split_len) {
     c->parts.ensure(gc);
@@ -426,9 +490,30 @@ void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
   if (hilo)
     launch_group_centres(c->groups.get(), frame, p.group_cap, tpos, tsid, roots, geom,
                          c->group_hilo.get(), s);
-  launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos,
-              c->groups.get(), frame, p.group_cap, cap, geom, groot, p.cap_c, p.cap_b, nullptr,
-              nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(), s);
+  if (p.predicted) {
+    // slots from the learned lengths, packed at scanned offsets
+    launch_group_features(c->groups.get(), frame, p.group_cap, cap, tpos, lnk, geom, groot,
+                          c->feat.get(), s);
+    launch_predict_slots(c->feat.get(), frame, p.group_cap, c->pred_params.get(),
+                         c->pred_stats.get(), c->slot_c.get(), c->slot_b.get(), c->pred_out.get(),
+                         s);
+    exclusive_scan_u32(c->slot_c.get(), c->slot_off_c.get(), p.group_cap,
+                       frame + frame::kPredCells, c->scan_tmp.get(), s);
+    exclusive_scan_u32(c->slot_b.get(), c->slot_off_b.get(), p.group_cap,
+                       frame + frame::kPredBodies, c->scan_tmp.get(), s);
+    launch_check_slot_totals(frame, p.slot_total_c, p.slot_total_b, s);
+    launch_walk(kWalkCapacity, 1, c->recs.get(), c->span.get(), tpos, c->groups.get(), frame,
+                p.group_cap, cap, geom, groot, 0, 0, c->slot_off_c.get(), c->slot_off_b.get(),
+                c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(), s,
+                c->slot_c.get(), c->slot_b.get());
+    launch_prediction_error(frame, p.group_cap, c->pred_out.get(), c->ccount.get(),
+                            c->bcount.get(), frame + frame::kPredError, s);
+  } else {
+    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos,
+                c->groups.get(), frame, p.group_cap, cap, geom, groot, p.cap_c, p.cap_b, nullptr,
+                nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(),
+                s);
+  }
   launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s);
   launch_check_frame(frame, p, true, s);
 }
@@ -554,6 +639,102 @@ void finish(gt_hip_context *c, const Plan &p, const double *geom, bool count = t
   }
 }
 
+// ---- the learned list lengths --------------------------------------------------------
+
+// The slot headroom, in the predictor's device statistics (read by the
+// slot kernel, so captured graphs follow changes).
+void set_margin(gt_hip_context *c, float margin) {
+  c->pred.margin = margin;  // (the copy reads the member, which outlives the call)
+  upload(c->pred_stats.get() + kPredStats, &c->pred.margin, 1, c->stream);
+}
+
+// Fits the MLP to the gathered samples (Adam, kTrainIterations minibatches of
+// kTrainBlocks x 64 samples, learning rate decaying linearly to a tenth) and
+// sets the slot headroom from the final error.
+void train_predictor(gt_hip_context *c) {
+  Predictor &P = c->pred;
+  hipStream_t s = c->stream;
+  float *params = c->pred_params.ensure(kPredParams), *m = c->pred_m.ensure(kPredParams),
+        *v = c->pred_v.ensure(kPredParams), *stats = c->pred_stats.ensure(kPredStats + 1);
+  float *grad = c->pred_grad.ensure(static_cast<size_t>(kTrainBlocks) * kPredParams);
+  float *loss = c->pred_loss.ensure(kTrainBlocks);
+  // He-uniform initial weights, zero biases, from a fixed seed
+  std::vector<float> init(kPredParams, 0.0f);
+  std::mt19937 rng(12345u);
+  auto layer = [&](int off, int rows, int cols) {
+    const float a = std::sqrt(6.0f / static_cast<float>(cols));
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int k = 0; k < rows * cols; ++k) init[off + k] = u(rng);
+  };
+  constexpr int F = kPredFeatures, H = kPredHidden, O = kPredOutputs;
+  layer(0, H, F);
+  layer(H * F + H, H, H);
+  layer(H * F + H + H * H + H, O, H);
+  upload(params, init.data(), kPredParams, s);
+  launch_fill_u32(reinterpret_cast<uint32_t *>(m), kPredParams, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t *>(v), kPredParams, 0u, s);
+  launch_sample_stats(c->pred_samples.get(), P.samples, stats, s);
+  for (uint32_t it = 0; it < kTrainIterations; ++it) {
+    const float lr = 3e-3f * (1.0f - 0.9f * static_cast<float>(it) / kTrainIterations);
+    launch_train_iteration(c->pred_samples.get(), P.samples, stats, params, m, v, grad, loss,
+                           kTrainBlocks, it, lr, s);
+  }
+  unsigned int *rmax = reinterpret_cast<unsigned int *>(c->misc.ensure(4) + 3);
+  launch_sample_residual_max(c->pred_samples.get(), P.samples, params, stats, rmax, s);
+  std::vector<float> lh(kTrainBlocks);
+  unsigned int rbits = 0;
+  download(lh.data(), loss, kTrainBlocks, s);
+  download(&rbits, rmax, 1, s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  double sum = 0.0;
+  for (float x : lh) sum += x;
+  P.loss = sum / (static_cast<double>(kTrainBlocks) * 64 * O);
+  // Headroom: the largest residual over the samples (the samples are a subset
+  // of the groups, and lists grow between steps: a little more on top).
+  float rmaxf = 0.0f;
+  std::memcpy(&rmaxf, &rbits, sizeof rmaxf);
+  P.margin_floor = std::min(rmaxf + 0.1f, 3.0f);
+  set_margin(c, P.margin_floor);
+  P.iterations = kTrainIterations;
+  P.trained = true;
+}
+
+// After a complete evaluation with cfg->predict = 1: the predictor's
+// statistics, and (until it is trained) samples of this evaluation's groups.
+void predictor_after_eval(gt_hip_context *c, const Plan &p, bool overflowed, const double *geom,
+                          const int4 *lnk, const double *tpos) {
+  Predictor &P = c->pred;
+  const unsigned long long *fw = c->fw;
+  const uint64_t ng = fw[frame::kGroups];
+  if (p.predicted) {
+    ++P.predicted;
+    if (overflowed) ++P.overflows;
+    // widen the headroom after an overflow, let it settle back slowly
+    const float m = overflowed ? std::min(P.margin + 0.1f, 3.0f)
+                               : std::max(P.margin_floor, P.margin - 0.005f);
+    if (m != P.margin) set_margin(c, m);
+    P.err = ng ? static_cast<double>(fw[frame::kPredError]) * 1e-6 / static_cast<double>(ng) : 0.0;
+    P.slot_entries = fw[frame::kPredCells] + fw[frame::kPredBodies];
+    P.list_entries = fw[frame::kStats + 6] + fw[frame::kStats + 7];
+  }
+  if (P.trained || P.collected >= P.collect || ng == 0) return;
+  hipStream_t s = c->stream;
+  const uint32_t gc = std::max<uint32_t>(p.group_cap, 1);
+  float *feat = c->feat.ensure(static_cast<size_t>(kPredFeatures) * gc);
+  float *samples = c->pred_samples.ensure(static_cast<size_t>(kPredSampleFloats) * kSampleCap);
+  const uint32_t *groot = p.systems > 1 ? c->group_root.get() : nullptr;
+  launch_group_features(c->groups.get(), c->frame.get(), p.group_cap, p.node_cap, tpos, lnk, geom,
+                        groot, feat, s);
+  const uint32_t stride = static_cast<uint32_t>(std::max<uint64_t>(1, ng / kSamplesPerEval));
+  const uint32_t count = static_cast<uint32_t>((ng + stride - 1) / stride);
+  launch_collect_samples(c->frame.get(), p.group_cap, stride, feat, c->ccount.get(),
+                         c->bcount.get(), P.next, kSampleCap, count, samples, s);
+  P.next = (P.next + count) % kSampleCap;
+  P.samples = std::min<uint32_t>(P.samples + count, kSampleCap);
+  ++P.collected;
+  if (P.collected >= P.collect) train_predictor(c);
+}
+
 // One force evaluation of particles in device memory: build, lists, evaluate,
 // scatter, enqueued with no host round trip, then one synchronisation to read
 // the status.  Slots too small: the lists are rewritten at exact offsets and
@@ -590,7 +771,8 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     unsigned long long st = c->fw[frame::kStatus];
     check_walk(c, st, GT_HIP_ERR_DEVICE);
     if (!slots && st == 0) st = kAbortLists;  // lists complete but not evaluated: do both below
-    if ((st & ~static_cast<unsigned long long>(kAbortItems)) == kAbortLists) {
+    const bool overflowed = (st & ~static_cast<unsigned long long>(kAbortItems)) == kAbortLists;
+    if (overflowed) {
       // slots too small (the item counts came from incomplete lists): rewrite
       learn(c, p, st);
       lists_exact(c, cfg, p, geom, c->tree.tree_pos());
@@ -610,6 +792,9 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     if (st == 0) {
       finish(c, p, geom);
       c->tree.set_counts(static_cast<uint32_t>(c->info.n_nodes), static_cast<int>(c->info.depth));
+      if (cfg->predict == 1)
+        predictor_after_eval(c, p, overflowed && slots, geom, c->tree.links(),
+                             c->tree.tree_pos());
       break;
     }
     learn(c, p, st);
@@ -740,7 +925,9 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
   const uint32_t n = static_cast<uint32_t>(c->n_state);
   int done_total = 0;
   while (done_total < nsteps) {
-    if (!sizes_known(c->hist, n, c->systems)) {  // sizes not known yet: one step without graphs
+    const bool collecting = cfg->predict == 1 && !c->pred.trained;
+    if (collecting || !sizes_known(c->hist, n, c->systems)) {
+      // sizes not known yet, or the predictor still learning: a step without graphs
       eager_step(c, cfg, dt);
       ++done_total;
       continue;
@@ -784,12 +971,24 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     if (st == 0) {
       // counts of the last step
       finish(c, p, c->tree.geom(), false);
+      if (p.predicted) {
+        Predictor &P = c->pred;
+        P.predicted += done;
+        const uint64_t ng = c->fw[frame::kGroups];
+        P.err = ng ? static_cast<double>(c->fw[frame::kPredError]) * 1e-6 / ng : 0.0;
+        P.slot_entries = c->fw[frame::kPredCells] + c->fw[frame::kPredBodies];
+        P.list_entries = c->fw[frame::kStats + 6] + c->fw[frame::kStats + 7];
+      }
       c->tree.set_counts(static_cast<uint32_t>(c->info.n_nodes), static_cast<int>(c->info.depth));
       continue;
     }
     check_walk(c, st, GT_HIP_ERR_DEVICE);
     // Step done_total ran its kick and drift only: its forces and second kick here.
     ++c->pstats.graph_stops;
+    if (p.predicted && (st & kAbortLists)) {
+      ++c->pred.overflows;
+      set_margin(c, std::min(c->pred.margin + 0.1f, 3.0f));
+    }
     learn(c, p, st);
     force_eval(c, cfg, c->pos.get(), c->mass.get(), n, c->acc.get(), c->pot.get());
     launch_kick(c->vel.get(), c->acc.get(), n, 0.5 * dt, frame + frame::kStatus, s);
@@ -1335,6 +1534,30 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     GTH_CHECK(hipStreamSynchronize(c->stream));
     c->state_forces = true;
   });
+}
+
+int gt_hip_predictor_reset(gt_hip_context *c, int collect) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  c->pred = Predictor{};
+  if (collect > 0) c->pred.collect = static_cast<uint32_t>(collect);
+  c->hist.pred_known = false;
+  return GT_HIP_OK;
+}
+
+int gt_hip_predictor_stats(const gt_hip_context *c, gt_predictor_stats *st) {
+  if (!c || !st) return GT_HIP_ERR_ARGUMENT;
+  const Predictor &P = c->pred;
+  st->samples = P.samples;
+  st->trained = P.trained ? 1 : 0;
+  st->train_iterations = P.iterations;
+  st->train_loss = P.loss;
+  st->margin = P.margin;
+  st->predicted = P.predicted;
+  st->overflows = P.overflows;
+  st->mean_abs_log_error = P.err;
+  st->slot_entries = P.slot_entries;
+  st->list_entries = P.list_entries;
+  return GT_HIP_OK;
 }
 
 int gt_hip_debug_shrink_slots(gt_hip_context *c, uint32_t shrink) {

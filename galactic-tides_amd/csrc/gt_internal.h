@@ -147,7 +147,11 @@ constexpr int kWalkError = 5;   // bit 0: malformed tree; bit 1: half != root ha
 constexpr int kStats = 6;       // 8 words, see list_stats
 constexpr int kItems = 14;      // evaluation work items (groups, long lists in parts)
 constexpr int kSplitParts = 15; // parts of the groups whose lists were split
-constexpr int kCountsEnd = 16;  // [kNodes, kCountsEnd): cleared before each evaluation
+constexpr int kSlotOverflow = 16;  // a group's lists outgrew their slots (the walk sets it)
+constexpr int kPredCells = 17;  // sum of the predicted cell-list slots
+constexpr int kPredBodies = 18; // sum of the predicted body-list slots
+constexpr int kPredError = 19;  // 1e6 x sum over groups of |log(1 + length) - prediction|
+constexpr int kCountsEnd = 20;  // [kNodes, kCountsEnd): cleared before each evaluation
 constexpr int kStepsDone = 30;  // leapfrog steps completed since the counter was cleared
 constexpr int kStepInteractions = 31;  // their cell + body interactions
 constexpr int kWords = 32;
@@ -160,6 +164,7 @@ constexpr uint32_t kAbortGroups = 8u;  // more groups than Plan::group_cap
 constexpr uint32_t kAbortLists = 16u;  // a group's lists do not fit the slots
 constexpr uint32_t kAbortWalk = 32u;   // malformed tree
 constexpr uint32_t kAbortItems = 64u;  // more work items or split parts than the plan's bounds
+constexpr uint32_t kAbortSlots = 128u; // predicted slots add up to more than the list buffers
 
 // Host-side bounds of one evaluation, fixed before it is enqueued.
 struct Plan {
@@ -176,6 +181,10 @@ struct Plan {
   bool longest_first = false;  // each XCD's items by decreasing cost
   int cost_buckets = 0;        // > 0: by decreasing cost class (1/cost_buckets octave)
   bool hilo = false;           // hi/lo positions relative to group centres
+  // Predicted per-group slots (instead of cap_c / cap_b for every group): the
+  // list buffers' sizes, which the predicted slots must add up to at most.
+  bool predicted = false;
+  uint64_t slot_total_c = 0, slot_total_b = 0;
 };
 
 // A device count that must not exceed cap; past it (an overflow the status
@@ -282,12 +291,17 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
                         uint32_t *group_root, hipStream_t s);
 // group_root: the node each group's walk starts from (null: node 0); geom
 // gives the root half sizes.  Coherent walks (lanes > 1) need a single tree.
+// Capacity mode: per-group slots of cap_c / cap_b entries at g * cap, or (with
+// slot lengths given) slot_c[g] / slot_b[g] entries at cell_off[g] /
+// body_off[g]; a group whose lists do not fit sets frame[kSlotOverflow].
+// Any status set before the walk (kAbortSlots) stops it.
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                  uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
                  uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
                  uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
-                 uint32_t *body_list, hipStream_t s);
+                 uint32_t *body_list, hipStream_t s, const uint32_t *slot_c = nullptr,
+                 const uint32_t *slot_b = nullptr);
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s);

@@ -156,7 +156,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     const double *__restrict__ geom, const uint32_t *__restrict__ group_root, uint32_t cap_c,
     uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
-    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
+    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
+    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b) {
+  if (frame[frame::kStatus]) return;  // the slots do not fit the buffers
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t blk = xcd_block_used(blockIdx.x, (ng + kWalkBlock - 1) / kWalkBlock);
@@ -183,6 +185,11 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   if (MODE == kWalkExact) {
     cpos = cell_off[g];
     bpos = body_off[g];
+  } else if (slot_c) {  // predicted slots, packed at scanned offsets
+    cpos = cell_off[g];
+    bpos = body_off[g];
+    cap_c = slot_c[g];
+    cap_b = slot_b[g];
   } else {
     cpos = g * cap_c;
     bpos = g * cap_b;
@@ -260,6 +267,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     if (first < cpos) first = cpos;
     for (uint32_t q = first; q < end; ++q) cell_list[q] = cbuf[q & 3u];
   }
+  if (MODE == kWalkCapacity && (nc > cap_c || nb > cap_b))
+    atomicOr(frame + frame::kSlotOverflow, 1ull);
   cell_count[g] = nc;
   body_count[g] = nb;
   gt_group G = groups[g];
@@ -290,10 +299,14 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     const double *__restrict__ root, const uint32_t *__restrict__ group_root, uint32_t cap_c,
     uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
-    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list) {
-  // One tree only: the shared traversal starts at node 0 (group_root unused;
-  // batches walk one lane per group).
+    uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
+    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b) {
+  // One tree and uniform slots only: the shared traversal starts at node 0
+  // (group_root unused; batches and predicted slots walk one lane per group).
   (void)group_root;
+  (void)slot_c;
+  (void)slot_b;
+  if (frame[frame::kStatus]) return;
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t waves = (ng + LANES - 1) / LANES;
@@ -401,6 +414,8 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     cur = next;
   }
   if (mine) {
+    if (MODE == kWalkCapacity && (nc > cap_c || nb > cap_b))
+      atomicOr(frame + frame::kSlotOverflow, 1ull);
     cell_count[g] = nc;
     body_count[g] = nb;
     gt_group G = groups[g];
@@ -522,14 +537,13 @@ __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict_
 
 // One thread: the status of an evaluation from its counts and the plan's bounds.
 __global__ void check_frame(unsigned long long *frame, uint32_t node_cap, uint32_t depth_cap,
-                            uint32_t group_cap, uint32_t cap_c, uint32_t cap_b, int check_slots) {
+                            uint32_t group_cap, int check_slots) {
   unsigned long long st = 0;
   if (frame[frame::kNodes] > node_cap) st |= kAbortNodes;
   if (frame[frame::kUnresolved] & 1ull) st |= kAbortPaths;
   if (frame[frame::kDepth] > depth_cap) st |= kAbortDepth;
   if (frame[frame::kGroups] > group_cap) st |= kAbortGroups;
-  if (check_slots && (frame[frame::kStats + 4] > cap_c || frame[frame::kStats + 5] > cap_b))
-    st |= kAbortLists;
+  if (check_slots && frame[frame::kSlotOverflow]) st |= kAbortLists;
   if (frame[frame::kWalkError] & 1ull) st |= kAbortWalk;
   frame[frame::kStatus] |= st;
 }
@@ -613,12 +627,14 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, co
                  uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
                  uint32_t cap_b,
                  const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
-                 uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s) {
+                 uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s,
+                 const uint32_t *slot_c, const uint32_t *slot_b) {
   if (!group_cap) return;
-  if (group_root && lanes != 1) fail(GT_HIP_ERR_ARGUMENT, "coherent walks need a single tree");
+  if ((group_root || slot_c) && lanes != 1)
+    fail(GT_HIP_ERR_ARGUMENT, "coherent walks need a single tree and uniform slots");
 #define GTH_WALK_ARGS                                                                   \
   recs, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, cap_c, cap_b, \
-      cell_off, body_off, cell_count, body_count, cell_list, body_list
+      cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c, slot_b
   if (lanes == 1) {
     const dim3 grid(xcd_grid(blocks_for(group_cap, kWalkBlock))), block(kWalkBlock);
     if (mode == kWalkExact)
@@ -662,8 +678,7 @@ void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
 void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_slots,
                         hipStream_t s) {
   hipLaunchKernelGGL(check_frame, dim3(1), dim3(1), 0, s, frame, plan.node_cap,
-                     static_cast<uint32_t>(plan.depth_cap), plan.group_cap, plan.cap_c, plan.cap_b,
-                     check_slots ? 1 : 0);
+                     static_cast<uint32_t>(plan.depth_cap), plan.group_cap, check_slots ? 1 : 0);
   GTH_LAUNCH_CHECK();
 }
 

@@ -26,7 +26,7 @@ class GroupConfig(C.Structure):
                 ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
                 ("pack", C.c_int), ("exact", C.c_int), ("eval_waves", C.c_int),
                 ("walk_lanes", C.c_int), ("split_len", C.c_int), ("eval_order", C.c_int),
-                ("hilo", C.c_int), ("reserved", C.c_int * 2)]
+                ("hilo", C.c_int), ("predict", C.c_int), ("reserved", C.c_int * 1)]
 
 
 class Params(C.Structure):
@@ -43,6 +43,13 @@ class ListsInfo(C.Structure):
 
 class Timings(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("tree", "lists", "evaluate", "scatter", "total")]
+
+
+class PredictorStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("trained", C.c_uint64), ("train_iterations", C.c_uint64),
+                ("train_loss", C.c_double), ("margin", C.c_double), ("predicted", C.c_uint64),
+                ("overflows", C.c_uint64), ("mean_abs_log_error", C.c_double),
+                ("slot_entries", C.c_uint64), ("list_entries", C.c_uint64)]
 
 
 class PipelineStats(C.Structure):
@@ -89,6 +96,8 @@ _state_get = _sig("gt_hip_state_get", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _prime = _sig("gt_hip_prime", C.c_int, [_vp, _P(GroupConfig)])
 _steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_double, C.c_int])
 _pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)])
+_predictor_reset = _sig("gt_hip_predictor_reset", C.c_int, [_vp, C.c_int])
+_predictor_stats = _sig("gt_hip_predictor_stats", C.c_int, [_vp, _P(PredictorStats)])
 _debug_shrink_slots = _sig("gt_hip_debug_shrink_slots", C.c_int, [_vp, C.c_uint32])
 _direct_f64_sample = _sig("gt_hip_direct_f64_sample", C.c_int,
                           [_vp, C.c_uint64, _vp, _vp, C.c_double, C.c_double, C.c_uint64, _vp,
@@ -112,12 +121,13 @@ def _f64(a, shape=None):
 
 def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                 pack=-1, exact=False, eval_waves=0, walk_lanes=0, split_len=0, eval_order=0,
-                hilo=False):
+                hilo=False, predict=0):
     cfg = GroupConfig()
     cfg.G, cfg.theta, cfg.eps = G, theta, eps
     cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
     cfg.pack, cfg.exact, cfg.eval_waves, cfg.walk_lanes = pack, int(exact), eval_waves, walk_lanes
     cfg.split_len, cfg.eval_order, cfg.hilo = split_len, eval_order, int(hilo)
+    cfg.predict = int(predict)
     return cfg
 
 
@@ -256,6 +266,15 @@ class Context:
         st = PipelineStats()
         self._check(_pipeline_stats(self._h, C.byref(st)))
         return {k: int(getattr(st, k)) for k, _ in PipelineStats._fields_}
+
+    def predictor_reset(self, collect=0):
+        self._check(_predictor_reset(self._h, collect))
+
+    def predictor_stats(self):
+        st = PredictorStats()
+        self._check(_predictor_stats(self._h, C.byref(st)))
+        return {k: (float(getattr(st, k)) if t is C.c_double else int(getattr(st, k)))
+                for k, t in PredictorStats._fields_}
 
     def debug_shrink_slots(self, shrink):
         """Test hook: list slots `shrink` entries too short (0: normal)."""

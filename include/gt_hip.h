@@ -85,7 +85,12 @@ typedef struct {
                         FP64), instead of FP32 relative to the root centre as the
                         reference does; same lists, more precise differences of nearby
                         positions.  Ignored by exact evaluation. */
-  int reserved[2];   /* zero */
+  int predict;       /* list slots of the one-pass walk: 0 = uniform (the previous
+                        evaluation's longest lists + 25%), 1 = per group from the learned
+                        predictor (an MLP trained on the first evaluations' lists; uniform
+                        until it is trained), 2 = no slots: a count pass and a fill pass
+                        every evaluation (the plain two-pass solver, for comparison) */
+  int reserved[1];   /* zero */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
@@ -226,6 +231,27 @@ typedef struct {
   uint64_t interactions;   /* cell + body interactions of all completed evaluations */
 } gt_pipeline_stats;
 int gt_hip_pipeline_stats(const gt_hip_context *ctx, gt_pipeline_stats *stats);
+/* The learned list-length predictor (no counterpart in the reference): with
+ * cfg->predict = 1, the first `collect` evaluations (default 50) gather samples
+ * (the features of a spread of groups and their true list lengths), then an FP32
+ * MLP 8 -> 64 -> 64 -> 2 is trained with Adam on the GPU and sizes every group's
+ * slots from then on.  gt_hip_predictor_reset forgets the model and sets
+ * `collect` (0: default). */
+typedef struct {
+  uint64_t samples;          /* training samples gathered */
+  uint64_t trained;          /* 1 once the model is in use */
+  uint64_t train_iterations; /* Adam iterations run */
+  double train_loss;         /* final mean squared error, standardised targets */
+  double margin;             /* log-space headroom added to predictions */
+  uint64_t predicted;        /* evaluations with predicted slots */
+  uint64_t overflows;        /* ... of which some group outgrew its slots (exact rewrite) */
+  double mean_abs_log_error; /* last predicted evaluation: mean |log(1+n) - prediction| */
+  uint64_t slot_entries;     /* last predicted evaluation: list entries reserved */
+  uint64_t list_entries;     /* ... and used */
+} gt_predictor_stats;
+int gt_hip_predictor_reset(gt_hip_context *ctx, int collect);
+int gt_hip_predictor_stats(const gt_hip_context *ctx, gt_predictor_stats *stats);
+
 /* Test hook (no counterpart in the reference): while shrink > 0, evaluations
  * on ctx get list slots `shrink` entries shorter than the previous
  * evaluation's longest lists, so that the slot-overflow rewrite and the
