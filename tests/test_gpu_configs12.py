@@ -1,0 +1,68 @@
+"""Configurations 1 and 2 at their real sizes (Hernquist halo, as configured):
+
+ 1. the 50k galaxy (seed 1): one grouped evaluation (theta 0.75, group size 32)
+    and 10 leapfrog steps at dt 0.01, forces checked against FP64 direct
+    summation on all 50,000 particles;
+ 2. the 200k galaxy (seed 2): grouped evaluation and the FP32 direct kernel,
+    both against FP64 direct on a seeded uniform sample of 1000 particles
+    (numpy default_rng(1)).
+Error bounds: the reference's own accuracy tests (test_metal.cpp:117, median
+below 3e-3 theta^2; test_octree.cpp:66-89) and the grouped solver's measured
+p99 at this opening angle."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def galaxy(disk_n, bulge_n, halo_n, seed):
+    import gt_ics
+    m = gt_ics.model(disk_n=disk_n, bulge_n=bulge_n, halo_n=halo_n, bulge_mass=0.333,
+                     bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
+                     halo_profile="hernquist", halo_a=4.0, seed=seed)
+    pos, vel, mass, _ = gt_ics.galaxy(m)
+    return pos, vel, mass
+
+
+def test_config1_50k_against_fp64_direct_on_all(hip):
+    pos, vel, mass = galaxy(20000, 5000, 25000, 1)
+    n = len(mass)
+    cfg = dict(theta=0.75, eps=0.025, group_size=32, pack=1)
+    every = np.arange(n, dtype=np.uint64)
+    acc, _ = hip.group_compute(pos, mass, **cfg)
+    e = np.sort(rel_err(acc, hip.direct_f64_sample(pos, mass, every, 1.0, 0.025)))
+    med, p99 = e[n // 2], e[n * 99 // 100]
+    print("initial: median", med, "p99", p99)
+    assert med < 3e-3 * 0.75 ** 2 and p99 < 2e-2
+    hip.state_set(pos, vel, mass)
+    hip.prime(**cfg)
+    hip.steps(10, 0.01, use_graph=True, **cfg)
+    p, _, a, _ = hip.state_get()
+    e = np.sort(rel_err(a, hip.direct_f64_sample(p, mass, every, 1.0, 0.025)))
+    print("after 10 steps: median", e[n // 2], "p99", e[n * 99 // 100])
+    assert e[n // 2] < 3e-3 * 0.75 ** 2 and e[n * 99 // 100] < 2e-2
+
+
+def test_config2_200k_group_and_direct(hip):
+    pos, _, mass = galaxy(80000, 20000, 100000, 2)
+    n = len(mass)
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(n, size=1000, replace=False)).astype(np.uint64)
+    ref = hip.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
+    acc, _ = hip.group_compute(pos, mass, theta=0.75, eps=0.025, group_size=32, pack=1)
+    t_group = hip.timings()["total"]
+    e_group = np.sort(rel_err(acc[sample.astype(np.int64)], ref))
+    # the FP32 direct kernel, positions relative to the bounding-box centre as
+    # DirectGravity packs them (src/gravity_backends.cpp:15-44)
+    centre = 0.5 * (pos.min(0) + pos.max(0))
+    bodies = np.concatenate([(pos - centre).astype(np.float32),
+                             mass.astype(np.float32)[:, None]], axis=1)
+    out = hip.direct(bodies, G=1.0, eps=0.025)
+    t_direct = hip.last_gpu_seconds * 1e3
+    e_direct = np.sort(rel_err(out[sample.astype(np.int64), :3].astype(np.float64), ref))
+    print(f"group: {t_group:.2f} ms, median {e_group[500]:.2e} p99 {e_group[990]:.2e}; "
+          f"direct: {t_direct:.1f} ms, median {e_direct[500]:.2e} p99 {e_direct[990]:.2e}")
+    assert e_group[500] < 3e-3 * 0.75 ** 2 and e_group[990] < 2e-2
+    assert e_direct[500] < 1e-5 and e_direct[990] < 1e-4
