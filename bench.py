@@ -112,6 +112,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--exact", action="store_true", help="bit-exact evaluation mode")
     ap.add_argument("--eval-waves", type=int, default=0, help="tuning: 2 or 3 (0: default)")
+    ap.add_argument("--overlap", type=int, default=0,
+                    help="tuning: walk ranges evaluated alongside the walk (1 off, 0 default)")
+    ap.add_argument("--walk-lanes", type=int, default=0,
+                    help="tuning: 1 per-lane walk fetching ahead, 2 plain, 16/32/64 coherent")
     ap.add_argument("--eval-order", type=int, default=0,
                     help="tuning: 1 tree order, 2 longest first per XCD (0: default)")
     ap.add_argument("--split-len", type=int, default=0,
@@ -154,7 +158,8 @@ def main():
     import gt_hip
     cfg = dict(G=1.0, theta=args.theta, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
-               eval_order=args.eval_order, hilo=args.hilo, predict=args.predict)
+               eval_order=args.eval_order, hilo=args.hilo, predict=args.predict,
+               walk_lanes=args.walk_lanes, overlap=args.overlap)
     t0 = time.time()
     offsets = None
     if args.workload == "batch-128":

@@ -68,7 +68,8 @@ bool same_plan(const Plan &a, const Plan &b) {
          a.cap_b == b.cap_b && a.split_len == b.split_len && a.item_cap == b.item_cap &&
          a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
          a.cost_buckets == b.cost_buckets && a.hilo == b.hilo && a.predicted == b.predicted &&
-         a.slot_total_c == b.slot_total_c && a.slot_total_b == b.slot_total_b;
+         a.slot_total_c == b.slot_total_c && a.slot_total_b == b.slot_total_b &&
+         a.chunks == b.chunks;
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
@@ -82,10 +83,16 @@ bool same_key(const GraphKey &a, const GraphKey &b) {
 constexpr int kTimedSteps = 64;  // graph steps whose phases are timed with events
 constexpr int kPhases = 4;       // tree, lists, evaluation, scatter + kick
 
+constexpr int kMaxChunks = 8;  // overlap: walk chunks whose evaluation runs alongside
+
 struct gt_hip_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // evaluation of walked chunks, alongside the walk
   hipEvent_t ev[6] = {};
+  hipEvent_t walk_done[kMaxChunks] = {}, eval_done = nullptr;
+  hipEvent_t eval_t0[kMaxChunks] = {}, eval_t1[kMaxChunks] = {};
+  uint32_t overlapped = 0;  // chunks of the last evaluation run alongside its walk (0: none)
   std::string err;
   double gpu_seconds = 0.0;
   gt_lists_info info{};
@@ -141,6 +148,12 @@ struct gt_hip_context {
   bool state_forces = false;
   // captured step graphs
   hipGraphExec_t step_exec[kPhases] = {};
+  // overlapped steps: the lists phase and the evaluation as graphs per walk
+  // range, launched on the two streams with the events between them
+  uint32_t ov_chunks = 0;
+  hipGraphExec_t ov_prep = nullptr, ov_finish = nullptr;
+  hipGraphExec_t ov_walk[kMaxChunks] = {}, ov_items[kMaxChunks] = {}, ov_eval[kMaxChunks] = {},
+                 ov_comb[kMaxChunks] = {};
   bool step_valid = false;
   GraphKey step_key;
   std::vector<hipEvent_t> step_ev;  // kTimedSteps x (kPhases + 1)
@@ -324,11 +337,27 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
   return p;
 }
 
+// One range by default: the 2M encounter's 87k group walks fill the SIMDs
+// about 1.3 times, so each range's walk lasts about as long as the whole walk
+// and ranges add up (6.1 ms per step at 1 range, 6.9 at 2, 8.7 at 4).
+constexpr uint32_t kDefaultChunks = 1;
+
+// Walk chunks whose evaluation runs alongside the walk (1: none).  Exact
+// evaluation and coherent walks take all groups at once.
+uint32_t chunks_for(const gt_group_config *cfg, const Plan &p) {
+  if (cfg->exact) return 1;
+  const int l = cfg->walk_lanes;
+  if ((l == 16 || l == 32 || l == 64) && p.systems <= 1 && !p.predicted) return 1;
+  const int k = cfg->overlap == 0 ? static_cast<int>(kDefaultChunks) : cfg->overlap;
+  return static_cast<uint32_t>(std::min(std::max(k, 1), kMaxChunks));
+}
+
 Plan plan_for(gt_hip_context *c, uint32_t n, const gt_group_config *cfg) {
   const bool predicted = cfg->predict == 1 && c->pred.trained && c->hist.lists_known;
   Plan p = bounds_for(c, n, split_len_for(cfg), longest_first_for(cfg), cost_buckets_for(cfg),
                       predicted);
   p.hilo = uses_hilo(cfg);
+  p.chunks = chunks_for(cfg, p);
   if (cfg->predict == 2) p.cap_c = p.cap_b = 0;  // two passes: count, then fill
   const History &h = c->hist;
   if (c->debug_shrink && h.lists_known) {  // test hook: slots too short
@@ -373,9 +402,13 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
     h.pred_b = fw[frame::kPredBodies];
   }
   if ((st & ~static_cast<unsigned long long>(kAbortItems)) == 0 && p.split_len) {
+    // (with the walk in chunks the frame holds the last chunk's counts: after
+    // an overflow the bounds double instead)
+    const bool over = st & kAbortItems;
     h.items_known = true;
-    h.items = fw[frame::kItems];
-    h.split_parts = fw[frame::kSplitParts];
+    h.items = over ? std::max<uint64_t>(2 * p.item_cap, fw[frame::kItems]) : fw[frame::kItems];
+    h.split_parts = over ? std::max<uint64_t>(2 * p.split_cap, fw[frame::kSplitParts])
+                         : fw[frame::kSplitParts];
   }
 }
 
@@ -451,10 +484,13 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   }
 }
 
+// 2: one walk per lane (the default), 1: the same fetching the next records
+// ahead, 16/32/64: coherent walks.
 int walk_lanes(const gt_group_config *cfg, const Plan &p) {
-  if (p.systems > 1) return 1;  // a coherent walk shares one tree
-  return cfg->walk_lanes == 16 || cfg->walk_lanes == 32 || cfg->walk_lanes == 64 ? cfg->walk_lanes
-                                                                                  : 1;
+  const int per_lane = cfg->walk_lanes == 1 ? 1 : 2;
+  if (p.systems > 1 || p.predicted) return per_lane;  // a coherent walk shares one tree
+  const int l = cfg->walk_lanes;
+  return l == 16 || l == 32 || l == 64 ? l : per_lane;
 }
 
 // Clears the counts of the frame (not its status or step counter).
@@ -464,36 +500,45 @@ void clear_counts(unsigned long long *frame, hipStream_t s) {
 
 // Group selection, walk into per-group slots, statistics and the status check,
 // for a tree in device memory whose node count is frame[kNodes].
-// tsid/roots/root_node: a batch's systems (null for one system).
-void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
-                   const double *geom, const int4 *lnk, const double *tpos, const double *tmass,
-                   const uint32_t *tsid = nullptr, const double *roots = nullptr,
-                   const uint32_t *root_node = nullptr) {
-  hipStream_t s = c->stream;
+// A tree the lists are built for (tsid/roots/root_node: a batch's systems,
+// null for one system).
+struct ListsInput {
+  const double *geom = nullptr;
+  const int4 *lnk = nullptr;
+  const double *tpos = nullptr, *tmass = nullptr;
+  const uint32_t *tsid = nullptr;
+  const double *roots = nullptr;
+  const uint32_t *root_node = nullptr;
+};
+
+// Everything of the lists before the walk: links, records, groups, and the
+// predicted slots.
+void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                        const ListsInput &in, hipStream_t s) {
   unsigned long long *frame = c->frame.get();
   reserve_all(c, p);
   const bool hilo = p.hilo;
   const uint32_t n = p.n, cap = p.node_cap;
-  launch_link_up(lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
-  launch_prep_lists(geom, lnk, c->skip.get(), c->depth.get(), tpos, tmass, n, frame, cap,
-                    cfg->theta, tsid, roots, c->recs.get(), c->span.get(), c->cells.get(),
-                    c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
+  launch_link_up(in.lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
+  launch_prep_lists(in.geom, in.lnk, c->skip.get(), c->depth.get(), in.tpos, in.tmass, n, frame,
+                    cap, cfg->theta, in.tsid, in.roots, c->recs.get(), c->span.get(),
+                    c->cells.get(), c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
                     hilo ? c->cells_lo.get() : nullptr, s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  launch_group_starts(lnk, c->parent.get(), frame, cap, n, static_cast<uint32_t>(cfg->group_size),
-                      pack, c->flags.get(), s);
+  launch_group_starts(in.lnk, c->parent.get(), frame, cap, n,
+                      static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s);
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
                      s);
-  uint32_t *groot = tsid ? c->group_root.get() : nullptr;
-  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, tsid, root_node,
-                     c->starts.get(), c->groups.get(), groot, s);
+  uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
+  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, in.tsid,
+                     in.root_node, c->starts.get(), c->groups.get(), groot, s);
   if (hilo)
-    launch_group_centres(c->groups.get(), frame, p.group_cap, tpos, tsid, roots, geom,
+    launch_group_centres(c->groups.get(), frame, p.group_cap, in.tpos, in.tsid, in.roots, in.geom,
                          c->group_hilo.get(), s);
   if (p.predicted) {
     // slots from the learned lengths, packed at scanned offsets
-    launch_group_features(c->groups.get(), frame, p.group_cap, cap, tpos, lnk, geom, groot,
-                          c->feat.get(), s);
+    launch_group_features(c->groups.get(), frame, p.group_cap, cap, in.tpos, in.lnk, in.geom,
+                          groot, c->feat.get(), s);
     launch_predict_slots(c->feat.get(), frame, p.group_cap, c->pred_params.get(),
                          c->pred_stats.get(), c->slot_c.get(), c->slot_b.get(), c->pred_out.get(),
                          s);
@@ -502,27 +547,70 @@ void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
     exclusive_scan_u32(c->slot_b.get(), c->slot_off_b.get(), p.group_cap,
                        frame + frame::kPredBodies, c->scan_tmp.get(), s);
     launch_check_slot_totals(frame, p.slot_total_c, p.slot_total_b, s);
-    launch_walk(kWalkCapacity, 1, c->recs.get(), c->span.get(), tpos, c->groups.get(), frame,
-                p.group_cap, cap, geom, groot, 0, 0, c->slot_off_c.get(), c->slot_off_b.get(),
-                c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(), s,
-                c->slot_c.get(), c->slot_b.get());
+  }
+}
+
+// The walk of range k of p.chunks ranges of groups.
+void enqueue_walk_range(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                        const ListsInput &in, uint32_t k, hipStream_t s) {
+  unsigned long long *frame = c->frame.get();
+  const uint32_t chunks = std::max<uint32_t>(p.chunks, 1);
+  const uint32_t span = (p.group_cap + chunks - 1) / chunks;
+  uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
+  if (p.predicted)
+    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), in.tpos,
+                c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, 0, 0,
+                c->slot_off_c.get(), c->slot_off_b.get(), c->ccount.get(), c->bcount.get(),
+                c->cell_list.get(), c->body_list.get(), s, c->slot_c.get(), c->slot_b.get(),
+                k * span, span);
+  else
+    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), in.tpos,
+                c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, p.cap_c, p.cap_b,
+                nullptr, nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(),
+                c->body_list.get(), s, nullptr, nullptr, k * span, span);
+}
+
+// After the walk: the predictor's error, the statistics and the status check.
+void enqueue_lists_finish(gt_hip_context *c, const Plan &p, hipStream_t s) {
+  unsigned long long *frame = c->frame.get();
+  if (p.predicted)
     launch_prediction_error(frame, p.group_cap, c->pred_out.get(), c->ccount.get(),
                             c->bcount.get(), frame + frame::kPredError, s);
-  } else {
-    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos,
-                c->groups.get(), frame, p.group_cap, cap, geom, groot, p.cap_c, p.cap_b, nullptr,
-                nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(), c->body_list.get(),
-                s);
-  }
   launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s);
   launch_check_frame(frame, p, true, s);
 }
 
+// The whole lists phase on the main stream; with p.chunks > 1 the walk runs
+// in ranges and walk_done[k] marks the end of range k.
+void enqueue_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                   const ListsInput &in) {
+  hipStream_t s = c->stream;
+  enqueue_lists_prep(c, cfg, p, in, s);
+  const uint32_t chunks = std::max<uint32_t>(p.chunks, 1);
+  for (uint32_t k = 0; k < chunks; ++k) {
+    enqueue_walk_range(c, cfg, p, in, k, s);
+    if (chunks > 1) GTH_CHECK(hipEventRecord(c->walk_done[k], s));
+  }
+  enqueue_lists_finish(c, p, s);
+}
+
+// The tree the GPU just built, as lists input.
+ListsInput tree_input(const gt_hip_context *c) {
+  const TreeBuilder &t = c->tree;
+  ListsInput in;
+  in.geom = t.geom();
+  in.lnk = t.links();
+  in.tpos = t.tree_pos();
+  in.tmass = t.tree_mass();
+  in.tsid = t.tree_sid();
+  in.roots = t.root();
+  in.root_node = t.root_nodes();
+  return in;
+}
+
 // Lists for the tree the GPU just built.
 void enqueue_tree_lists(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
-  const TreeBuilder &t = c->tree;
-  enqueue_lists(c, cfg, p, t.geom(), t.links(), t.tree_pos(), t.tree_mass(), t.tree_sid(),
-                t.root(), t.root_nodes());
+  enqueue_lists(c, cfg, p, tree_input(c));
 }
 
 // The lists again at exact (scanned) offsets, for an evaluation whose only
@@ -580,16 +668,18 @@ bool uses_items(const gt_group_config *cfg, const Plan &p) { return p.split_len 
 
 // Work items of the fast evaluation (long lists in parts, each XCD's share
 // longest first): part of the lists phase.
-void enqueue_items(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+void enqueue_items(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                   hipStream_t s = nullptr, uint32_t g0 = 0, uint32_t span = 0) {
   if (!uses_items(cfg, p)) return;
   reserve_all(c, p);
   const EvalSplit sp = split_of(c, p);
   launch_eval_items(c->groups.get(), c->frame.get(), p.group_cap, sp, c->scan_tmp.get(),
-                    c->stream);
+                    s ? s : c->stream, g0, span);
 }
 
-// The list evaluation kernel alone.
-void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+// The list evaluation kernel alone (groups g0 .. g0 + span; span 0: all).
+void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                  hipStream_t s = nullptr, uint32_t g0 = 0, uint32_t span = 0) {
   const float G = static_cast<float>(cfg->G);
   const float eps2 = static_cast<float>(cfg->eps * cfg->eps);
   const bool items = uses_items(cfg, p);
@@ -602,15 +692,43 @@ void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) 
   }
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
-              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), c->stream,
-              items ? &sp : nullptr, p.hilo ? &hl : nullptr);
+              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), s ? s : c->stream,
+              items ? &sp : nullptr, p.hilo ? &hl : nullptr, g0, span);
 }
 
 // Results in tree order: split groups' parts added up.
-void enqueue_combine(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+void enqueue_combine(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
+                     hipStream_t s = nullptr, uint32_t g0 = 0, uint32_t span = 0) {
   if (!uses_items(cfg, p)) return;
   launch_combine_parts(c->groups.get(), c->frame.get(), p.group_cap, split_of(c, p), c->out.get(),
-                       c->stream);
+                       s ? s : c->stream, g0, span);
+}
+
+// The evaluation of the walk's p.chunks ranges on the second stream, each as
+// soon as its range is walked (so it runs alongside the walk of the next
+// range); eval_done marks the end.  eval_t0/eval_t1 time each range's
+// evaluation kernel.  The lists a range's evaluation reads stay inside their
+// slots even when they overflowed (the walk cuts the ranges), and an
+// evaluation whose lists turn out incomplete is discarded by the status.
+void enqueue_overlapped_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+  hipStream_t b = c->stream2;
+  const uint32_t chunks = p.chunks, span = (p.group_cap + chunks - 1) / chunks;
+  for (uint32_t k = 0; k < chunks; ++k) {
+    GTH_CHECK(hipStreamWaitEvent(b, c->walk_done[k], 0));
+    enqueue_items(c, cfg, p, b, k * span, span);
+    GTH_CHECK(hipEventRecord(c->eval_t0[k], b));
+    enqueue_eval(c, cfg, p, b, k * span, span);
+    GTH_CHECK(hipEventRecord(c->eval_t1[k], b));
+    enqueue_combine(c, cfg, p, b, k * span, span);
+  }
+  GTH_CHECK(hipEventRecord(c->eval_done, b));
+  GTH_CHECK(hipStreamWaitEvent(c->stream, c->eval_done, 0));
+}
+
+double overlapped_eval_ms(const gt_hip_context *c) {
+  double t = 0.0;
+  for (uint32_t k = 0; k < c->overlapped; ++k) t += elapsed_ms(c->eval_t0[k], c->eval_t1[k]);
+  return t;
 }
 
 // After a complete evaluation: sizes into the history and the statistics
@@ -757,8 +875,16 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     enqueue_tree_lists(c, cfg, p);
     // Without list slots yet (a first evaluation) the lists go to exact
     // offsets below; evaluating now would only meet the overflow status.
-    const bool slots = p.cap_c != 0 && p.cap_b != 0;
-    if (slots) {
+    const bool slots = p.predicted || (p.cap_c != 0 && p.cap_b != 0);
+    c->overlapped = 0;
+    if (slots && p.chunks > 1) {  // evaluation alongside the walk
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      enqueue_overlapped_eval(c, cfg, p);
+      c->overlapped = p.chunks;
+      GTH_CHECK(hipEventRecord(c->ev[3], s));
+      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      GTH_CHECK(hipEventRecord(c->ev[4], s));
+    } else if (slots) {
       enqueue_items(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[2], s));
       enqueue_eval(c, cfg, p);
@@ -776,6 +902,7 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
       // slots too small (the item counts came from incomplete lists): rewrite
       learn(c, p, st);
       lists_exact(c, cfg, p, geom, c->tree.tree_pos());
+      c->overlapped = 0;
       enqueue_items(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[2], s));
       enqueue_eval(c, cfg, p);
@@ -802,7 +929,9 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
   }
   c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
   c->tm.lists = elapsed_ms(c->ev[1], c->ev[2]);
-  c->tm.evaluate = elapsed_ms(c->ev[2], c->ev[3]);
+  // overlapped: the evaluation kernels' own times (they ran alongside the
+  // walk, which `lists` covers), and the wait for the last of them
+  c->tm.evaluate = c->overlapped ? overlapped_eval_ms(c) : elapsed_ms(c->ev[2], c->ev[3]);
   c->tm.scatter = elapsed_ms(c->ev[3], c->ev[4]);
   c->tm.total = elapsed_ms(c->ev[0], c->ev[4]);
   c->gpu_seconds = c->tm.total * 1e-3;
@@ -820,13 +949,40 @@ void eager_step(gt_hip_context *c, const gt_group_config *cfg, double dt) {
   launch_kick(c->vel.get(), c->acc.get(), n, h, status, c->stream);
 }
 
+void destroy_graph(hipGraphExec_t &e) {
+  if (e) (void)hipGraphExecDestroy(e);
+  e = nullptr;
+}
+
 void destroy_step_graphs(gt_hip_context *c) {
-  for (auto &e : c->step_exec)
-    if (e) {
-      (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+  for (auto &e : c->step_exec) destroy_graph(e);
+  destroy_graph(c->ov_prep);
+  destroy_graph(c->ov_finish);
+  for (hipGraphExec_t *arr : {c->ov_walk, c->ov_items, c->ov_eval, c->ov_comb})
+    for (uint32_t k = 0; k < kMaxChunks; ++k) destroy_graph(arr[k]);
+  c->ov_chunks = 0;
   c->step_valid = false;
+}
+
+// What f enqueues on stream st, as an instantiated graph.
+template <class F>
+hipGraphExec_t capture_graph(hipStream_t st, F &&f) {
+  GTH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    f();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  GTH_CHECK(hipStreamEndCapture(st, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  GTH_CHECK(r);
+  return e;
 }
 
 // The kernels of one leapfrog step in phase `phase` (all of them read the
@@ -846,6 +1002,7 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s);
       break;
     case 1:
+      // (overlapped steps launch this phase and the next from graphs per range)
       enqueue_tree_lists(c, cfg, p);
       enqueue_items(c, cfg, p);
       break;
@@ -853,7 +1010,7 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       enqueue_eval(c, cfg, p);
       break;
     default:
-      enqueue_combine(c, cfg, p);
+      if (p.chunks <= 1) enqueue_combine(c, cfg, p);
       launch_scatter(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
                      c->pot.get(), frame + frame::kStatus, s);
       launch_kick(c->vel.get(), c->acc.get(), n, h, frame + frame::kStatus, s);
@@ -888,22 +1045,27 @@ void prepare_step_graphs(gt_hip_context *c, const gt_group_config *cfg, const Pl
   const GraphKey key = key_for(cfg, p, dt);
   if (c->step_valid && same_key(c->step_key, key)) return;
   destroy_step_graphs(c);
-  hipStream_t s = c->stream;
+  hipStream_t s = c->stream, b = c->stream2;
+  const bool overlap = p.chunks > 1;
   for (int phase = 0; phase < kPhases; ++phase) {
-    GTH_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      enqueue_step_phase(c, cfg, p, dt, phase);
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
+    if (overlap && (phase == 1 || phase == 2)) continue;
+    c->step_exec[phase] = capture_graph(s, [&] { enqueue_step_phase(c, cfg, p, dt, phase); });
+  }
+  if (overlap) {
+    const ListsInput in = tree_input(c);
+    const uint32_t chunks = p.chunks, span = (p.group_cap + chunks - 1) / chunks;
+    const bool items = uses_items(cfg, p);
+    c->ov_prep = capture_graph(s, [&] { enqueue_lists_prep(c, cfg, p, in, s); });
+    c->ov_finish = capture_graph(s, [&] { enqueue_lists_finish(c, p, s); });
+    for (uint32_t k = 0; k < chunks; ++k) {
+      c->ov_walk[k] = capture_graph(s, [&] { enqueue_walk_range(c, cfg, p, in, k, s); });
+      if (items) {
+        c->ov_items[k] = capture_graph(b, [&] { enqueue_items(c, cfg, p, b, k * span, span); });
+        c->ov_comb[k] = capture_graph(b, [&] { enqueue_combine(c, cfg, p, b, k * span, span); });
+      }
+      c->ov_eval[k] = capture_graph(b, [&] { enqueue_eval(c, cfg, p, b, k * span, span); });
     }
-    hipGraph_t g = nullptr;
-    GTH_CHECK(hipStreamEndCapture(s, &g));
-    const hipError_t e = hipGraphInstantiate(&c->step_exec[phase], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    GTH_CHECK(e);
+    c->ov_chunks = chunks;
   }
   if (alloc_generation != key.generation)
     fail(GT_HIP_ERR_DEVICE, "step graph: a buffer was allocated during capture");
@@ -914,6 +1076,31 @@ void prepare_step_graphs(gt_hip_context *c, const gt_group_config *cfg, const Pl
     c->step_ev.assign(static_cast<size_t>(kTimedSteps) * (kPhases + 1), nullptr);
     for (auto &e : c->step_ev) GTH_CHECK(hipEventCreate(&e));
   }
+}
+
+void launch_graph(hipGraphExec_t e, hipStream_t st) {
+  if (e) GTH_CHECK(hipGraphLaunch(e, st));
+}
+
+// The lists phase and the evaluation of an overlapped step from their graphs:
+// each walk range's evaluation on the second stream as soon as the range is
+// walked, with the same events as the launches without graphs.
+void launch_overlapped_lists_eval(gt_hip_context *c) {
+  hipStream_t s = c->stream, b = c->stream2;
+  launch_graph(c->ov_prep, s);
+  for (uint32_t k = 0; k < c->ov_chunks; ++k) {
+    launch_graph(c->ov_walk[k], s);
+    GTH_CHECK(hipEventRecord(c->walk_done[k], s));
+    GTH_CHECK(hipStreamWaitEvent(b, c->walk_done[k], 0));
+    launch_graph(c->ov_items[k], b);
+    GTH_CHECK(hipEventRecord(c->eval_t0[k], b));
+    launch_graph(c->ov_eval[k], b);
+    GTH_CHECK(hipEventRecord(c->eval_t1[k], b));
+    launch_graph(c->ov_comb[k], b);
+  }
+  launch_graph(c->ov_finish, s);
+  GTH_CHECK(hipEventRecord(c->eval_done, b));
+  GTH_CHECK(hipStreamWaitEvent(s, c->eval_done, 0));
 }
 
 // nsteps leapfrog steps from captured graphs, launched back to back; a step
@@ -942,7 +1129,8 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
                                        : nullptr;
       if (ev) GTH_CHECK(hipEventRecord(ev[0], s));
       for (int phase = 0; phase < kPhases; ++phase) {
-        GTH_CHECK(hipGraphLaunch(c->step_exec[phase], s));
+        if (phase == 1 && c->ov_chunks) launch_overlapped_lists_eval(c);
+        if (c->step_exec[phase]) GTH_CHECK(hipGraphLaunch(c->step_exec[phase], s));
         if (ev) GTH_CHECK(hipEventRecord(ev[phase + 1], s));
       }
     }
@@ -962,7 +1150,9 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
       }
       c->tm.tree = t[0] / timed;
       c->tm.lists = t[1] / timed;
-      c->tm.evaluate = t[2] / timed;
+      // overlapped: the last step's evaluation kernels
+      c->overlapped = p.chunks > 1 ? p.chunks : 0;
+      c->tm.evaluate = c->overlapped ? overlapped_eval_ms(c) : t[2] / timed;
       c->tm.scatter = t[3] / timed;
       c->tm.total = total / timed;
       c->gpu_seconds = c->tm.total * 1e-3;
@@ -1018,7 +1208,12 @@ int gt_hip_create(gt_hip_context **out, int device) {
     c->device = device;
     GTH_CHECK(hipGetDeviceProperties(&c->prop, device));
     GTH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    GTH_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     for (auto &e : c->ev) GTH_CHECK(hipEventCreate(&e));
+    for (auto &e : c->eval_t0) GTH_CHECK(hipEventCreate(&e));
+    for (auto &e : c->eval_t1) GTH_CHECK(hipEventCreate(&e));
+    for (auto &e : c->walk_done) GTH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    GTH_CHECK(hipEventCreateWithFlags(&c->eval_done, hipEventDisableTiming));
     launch_fill_u64(c->frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
     launch_fill_u64(c->raw_frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
     GTH_CHECK(hipStreamSynchronize(c->stream));
@@ -1045,8 +1240,14 @@ void gt_hip_destroy(gt_hip_context *c) {
   destroy_step_graphs(c);
   for (auto &e : c->step_ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto *arr : {c->eval_t0, c->eval_t1, c->walk_done})
+    for (int k = 0; k < kMaxChunks; ++k)
+      if (arr[k]) (void)hipEventDestroy(arr[k]);
+  if (c->eval_done) (void)hipEventDestroy(c->eval_done);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1258,7 +1459,7 @@ int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, c
       }
     };
     if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
-    sweep(&gt_group_config::walk_lanes, {1, 16, 32, 64});
+    sweep(&gt_group_config::walk_lanes, {1, 2, 16, 32, 64});
     if (!cfg->exact) {
       sweep(&gt_group_config::eval_waves, {2, 3});
       sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
@@ -1330,7 +1531,12 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
       clear_counts(frame, s);
       launch_fill_u64(frame + frame::kNodes, 1, n_nodes, s);
       GTH_CHECK(hipEventRecord(c->ev[1], s));
-      enqueue_lists(c, cfg, p, dg, dl, dp, dm);
+      ListsInput in;
+      in.geom = dg;
+      in.lnk = dl;
+      in.tpos = dp;
+      in.tmass = dm;
+      enqueue_lists(c, cfg, p, in);
       read_frame(c);
       unsigned long long st = c->fw[frame::kStatus];
       check_walk(c, st, GT_HIP_ERR_ARGUMENT);

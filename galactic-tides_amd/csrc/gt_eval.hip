@@ -373,11 +373,18 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     uint32_t item_cap, const uint2 *__restrict__ items, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ split_off, float4 *__restrict__ partial, float G, float eps2,
     const float4 *__restrict__ bodies_lo, const float4 *__restrict__ cells_lo,
-    const float4 *__restrict__ group_hilo, float4 *__restrict__ out) {
+    const float4 *__restrict__ group_hilo, float4 *__restrict__ out, uint32_t group_cap,
+    uint32_t g_begin, uint32_t g_span) {
   __shared__ float4 targets[kBlock / kWave][32];
   if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
-  const uint32_t n_items =
-      bounded_count(frame + (items ? frame::kItems : frame::kGroups), item_cap);
+  // items: this range's work items; else groups g_begin .. g_begin + g_span
+  uint32_t n_items;
+  if (items) {
+    n_items = bounded_count(frame + frame::kItems, item_cap);
+  } else {
+    const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+    n_items = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  }
   // XCD-aware: each XCD evaluates a contiguous run of (spatially adjacent)
   // groups, whose lists gather the same cells, so its L2 holds them.
   const uint32_t waves_per_block = kBlock / kWave;
@@ -386,7 +393,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   if (blk == kNone) return;
   const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
   if (wi >= n_items) return;
-  uint32_t gi = wi, part = 0, parts = 1;
+  uint32_t gi = g_begin + wi, part = 0, parts = 1;
   if (items) {
     const uint2 it = items[order ? order[wi] : wi];
     gi = __builtin_amdgcn_readfirstlane(it.x);
@@ -408,7 +415,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     h.ly = lo.y;
     h.lz = lo.z;
   }
-  float4 *dst0 = parts > 1 ? partial + static_cast<size_t>(split_off[gi] + part) * kMaxGroup
+  float4 *dst0 = parts > 1 ? partial + static_cast<size_t>(split_off[gi - g_begin] + part) *
+                                          kMaxGroup
                            : out + g.begin;
   const uint32_t ntot = g.end - g.begin;
   for (uint32_t t0 = 0; t0 < ntot; t0 += 32) {
@@ -441,26 +449,29 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
 // past the group count are zero, so the scans over group_cap entries add up.
 __global__ __launch_bounds__(kBlock) void count_parts(const gt_group *__restrict__ groups,
                                                       const unsigned long long *__restrict__ frame,
-                                                      uint32_t group_cap, uint32_t split_len,
+                                                      uint32_t group_cap, uint32_t g_begin,
+                                                      uint32_t g_span, uint32_t split_len,
                                                       uint32_t *__restrict__ parts,
                                                       uint32_t *__restrict__ split_parts) {
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  if (g >= group_cap) return;
+  const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;  // index within the range
+  if (i >= g_span) return;
   uint32_t p = 0;
-  if (g < ng) {
-    const gt_group G = groups[g];
+  if (i < count) {
+    const gt_group G = groups[g_begin + i];
     const uint32_t len = (G.cell_end - G.cell_begin) + (G.body_end - G.body_begin);
     p = len > split_len ? min((len + split_len - 1) / split_len, 0xffffu) : 1u;
   }
-  parts[g] = p;
-  split_parts[g] = p > 1 ? p : 0u;
+  parts[i] = p;
+  split_parts[i] = p > 1 ? p : 0u;
 }
 
 // The work items, in group order (a split group's parts are adjacent, so they
 // run at the same time on one XCD); sets kAbortItems past the plan's bounds.
 __global__ __launch_bounds__(kBlock) void make_items(unsigned long long *__restrict__ frame,
-                                                     uint32_t group_cap, uint32_t item_cap,
+                                                     uint32_t group_cap, uint32_t g_begin,
+                                                     uint32_t g_span, uint32_t item_cap,
                                                      uint32_t split_cap,
                                                      const uint32_t *__restrict__ parts,
                                                      const uint32_t *__restrict__ item_off,
@@ -468,10 +479,11 @@ __global__ __launch_bounds__(kBlock) void make_items(unsigned long long *__restr
   const bool fits = frame[frame::kItems] <= item_cap && frame[frame::kSplitParts] <= split_cap;
   if (blockIdx.x == 0 && threadIdx.x == 0 && !fits) atomicOr(frame + frame::kStatus, kAbortItems);
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  if (!fits || g >= ng) return;
-  const uint32_t p = parts[g], o = item_off[g];
-  for (uint32_t j = 0; j < p; ++j) items[o + j] = make_uint2(g, (j << 16) | p);
+  const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (!fits || i >= count) return;
+  const uint32_t p = parts[i], o = item_off[i];
+  for (uint32_t j = 0; j < p; ++j) items[o + j] = make_uint2(g_begin + i, (j << 16) | p);
 }
 
 // Sort keys for evaluating each XCD's items longest first: the XCD chunk an
@@ -544,19 +556,21 @@ __global__ __launch_bounds__(kBlock) void group_centres(const gt_group *__restri
 // Split groups: the parts' results added in part order (deterministic).
 __global__ __launch_bounds__(kBlock) void combine_parts(const gt_group *__restrict__ groups,
                                                         const unsigned long long *__restrict__ frame,
-                                                        uint32_t group_cap,
+                                                        uint32_t group_cap, uint32_t g_begin,
+                                                        uint32_t g_span,
                                                         const uint32_t *__restrict__ parts,
                                                         const uint32_t *__restrict__ split_off,
                                                         const float4 *__restrict__ partial,
                                                         float4 *__restrict__ out) {
   if (frame[frame::kStatus]) return;
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  if (g >= ng) return;
-  const uint32_t p = parts[g];
+  const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t p = parts[i];
   if (p <= 1) return;
-  const gt_group G = groups[g];
-  const float4 *slot = partial + static_cast<size_t>(split_off[g]) * kMaxGroup;
+  const gt_group G = groups[g_begin + i];
+  const float4 *slot = partial + static_cast<size_t>(split_off[i]) * kMaxGroup;
   for (uint32_t t = 0; t < G.end - G.begin; ++t) {
     float4 a = slot[t];
     for (uint32_t j = 1; j < p; ++j) {
@@ -576,11 +590,15 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
-                 const EvalSplit *split, const EvalHiLo *hilo) {
+                 const EvalSplit *split, const EvalHiLo *hilo, uint32_t g_begin,
+                 uint32_t g_span) {
   if (group_cap == 0) return;
+  if (g_span == 0 || g_span > group_cap) g_span = group_cap;
+  if (exact && (g_begin != 0 || g_span != group_cap))
+    fail(GT_HIP_ERR_ARGUMENT, "exact evaluation takes all groups at once");
   const uint32_t waves_per_block = kBlock / kWave;
   const bool parts = split && !exact;
-  const uint32_t item_cap = parts ? split->item_cap : group_cap;
+  const uint32_t item_cap = parts ? split->item_cap : g_span;
   const uint32_t blocks = (item_cap + waves_per_block - 1) / waves_per_block;
   const dim3 grid(exact ? blocks : xcd_grid(blocks)), block(kBlock);
   const uint2 *items = parts ? split->items : nullptr;
@@ -592,7 +610,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                *ghl = hl ? hilo->group_hilo : nullptr;
 #define GTH_EVAL_ARGS                                                                \
   bodies, cells, cell_list, body_list, groups, frame, item_cap, items, order, split_off, \
-      partial, G, eps2, blo, clo, ghl, out
+      partial, G, eps2, blo, clo, ghl, out, group_cap, g_begin, g_span
 #define GTH_EVAL(Q, W)                                                                    \
   if (hl)                                                                                \
     hipLaunchKernelGGL((eval_fast<Q, W, true>), grid, block, 0, s, GTH_EVAL_ARGS);  \
@@ -616,10 +634,13 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
 }
 
 void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
-                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s) {
+                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s,
+                          uint32_t g_begin, uint32_t g_span) {
   if (group_cap == 0) return;
-  hipLaunchKernelGGL(combine_parts, dim3((group_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     groups, frame, group_cap, split.parts, split.split_off, split.partial, out);
+  if (g_span == 0 || g_span > group_cap) g_span = group_cap;
+  hipLaunchKernelGGL(combine_parts, dim3((g_span + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     groups, frame, group_cap, g_begin, g_span, split.parts, split.split_off,
+                     split.partial, out);
   GTH_LAUNCH_CHECK();
 }
 
@@ -634,17 +655,19 @@ void launch_group_centres(const gt_group *groups, const unsigned long long *fram
 }
 
 void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s) {
+                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s,
+                       uint32_t g_begin, uint32_t g_span) {
   if (group_cap == 0) return;
-  const dim3 grid((group_cap + kBlock - 1) / kBlock), block(kBlock);
-  hipLaunchKernelGGL(count_parts, grid, block, 0, s, groups, frame, group_cap, split.split_len,
-                     split.parts, split.split_count);
+  if (g_span == 0 || g_span > group_cap) g_span = group_cap;
+  const dim3 grid((g_span + kBlock - 1) / kBlock), block(kBlock);
+  hipLaunchKernelGGL(count_parts, grid, block, 0, s, groups, frame, group_cap, g_begin, g_span,
+                     split.split_len, split.parts, split.split_count);
   GTH_LAUNCH_CHECK();
-  exclusive_scan_u32(split.parts, split.item_off, group_cap, frame + frame::kItems, scan_tmp, s);
-  exclusive_scan_u32(split.split_count, split.split_off, group_cap, frame + frame::kSplitParts,
+  exclusive_scan_u32(split.parts, split.item_off, g_span, frame + frame::kItems, scan_tmp, s);
+  exclusive_scan_u32(split.split_count, split.split_off, g_span, frame + frame::kSplitParts,
                      scan_tmp, s);
-  hipLaunchKernelGGL(make_items, grid, block, 0, s, frame, group_cap, split.item_cap,
-                     split.split_cap, split.parts, split.item_off, split.items);
+  hipLaunchKernelGGL(make_items, grid, block, 0, s, frame, group_cap, g_begin, g_span,
+                     split.item_cap, split.split_cap, split.parts, split.item_off, split.items);
   GTH_LAUNCH_CHECK();
   if (split.longest_first && split.item_cap) {
     hipLaunchKernelGGL(item_keys, dim3((split.item_cap + kBlock - 1) / kBlock), block, 0, s, frame,

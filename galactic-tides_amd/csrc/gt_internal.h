@@ -185,6 +185,7 @@ struct Plan {
   // list buffers' sizes, which the predicted slots must add up to at most.
   bool predicted = false;
   uint64_t slot_total_c = 0, slot_total_b = 0;
+  uint32_t chunks = 1;  // walk ranges whose evaluation runs alongside the walk
 };
 
 // A device count that must not exceed cap; past it (an overflow the status
@@ -235,8 +236,11 @@ struct EvalSplit {
 
 // Work items for the groups' lists (frame[kItems], frame[kSplitParts]);
 // sets kAbortItems past split.item_cap / split.split_cap.
+// (Groups g_begin .. g_begin + g_span, g_span 0: all; the per-group arrays of
+// split are indexed from g_begin.)
 void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s);
+                       const EvalSplit &split, unsigned long long *scan_tmp, hipStream_t s,
+                       uint32_t g_begin = 0, uint32_t g_span = 0);
 // Evaluates the first frame[kGroups] groups (at most group_cap) unless
 // frame[kStatus] is set; with split (fast evaluation), the work items that
 // launch_eval_items made (split groups' parts then need launch_combine_parts).
@@ -253,10 +257,12 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
-                 const EvalSplit *split = nullptr, const EvalHiLo *hilo = nullptr);
+                 const EvalSplit *split = nullptr, const EvalHiLo *hilo = nullptr,
+                 uint32_t g_begin = 0, uint32_t g_span = 0);
 // Adds the parts of split groups into out, in part order.
 void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
-                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s);
+                          uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s,
+                          uint32_t g_begin = 0, uint32_t g_span = 0);
 
 // gt_direct.hip
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
@@ -293,15 +299,17 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
 // gives the root half sizes.  Coherent walks (lanes > 1) need a single tree.
 // Capacity mode: per-group slots of cap_c / cap_b entries at g * cap, or (with
 // slot lengths given) slot_c[g] / slot_b[g] entries at cell_off[g] /
-// body_off[g]; a group whose lists do not fit sets frame[kSlotOverflow].
-// Any status set before the walk (kAbortSlots) stops it.
+// body_off[g]; a group whose lists do not fit sets frame[kSlotOverflow] and
+// its recorded ranges are cut at its slots.  Any status set before the walk
+// (kAbortSlots) stops it.  Groups g_begin .. g_begin + g_span only (g_span 0:
+// all); coherent walks (lanes > 2) take all groups.
 void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                  uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
                  uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
                  uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
                  uint32_t *body_list, hipStream_t s, const uint32_t *slot_c = nullptr,
-                 const uint32_t *slot_b = nullptr);
+                 const uint32_t *slot_b = nullptr, uint32_t g_begin = 0, uint32_t g_span = 0);
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s);

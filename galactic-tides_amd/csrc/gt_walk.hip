@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
   }
 }
 
-template <int MODE>
+template <int MODE, bool PREFETCH>
 __global__ __launch_bounds__(kWalkBlock) void walk(
     const WalkRec *__restrict__ recs, const uint2 *__restrict__ span,
     const double *__restrict__ tpos, gt_group *__restrict__ groups,
@@ -157,14 +157,17 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
     uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
-    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b) {
+    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b, uint32_t g_begin,
+    uint32_t g_span) {
   if (frame[frame::kStatus]) return;  // the slots do not fit the buffers
+  // groups g_begin .. g_begin + g_span of the first frame[kGroups]
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t blk = xcd_block_used(blockIdx.x, (ng + kWalkBlock - 1) / kWalkBlock);
+  const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  const uint32_t blk = xcd_block_used(blockIdx.x, (count + kWalkBlock - 1) / kWalkBlock);
   if (blk == kNone) return;
-  const uint32_t g = blk * kWalkBlock + threadIdx.x;
-  if (g >= ng) return;
+  const uint32_t g = g_begin + blk * kWalkBlock + threadIdx.x;
+  if (g >= g_begin + count) return;
   const uint32_t gb = groups[g].begin, ge = groups[g].end;
   // the walk starts at the root of the group's system (node 0 for one system)
   const uint32_t start = group_root ? group_root[g] : 0u;
@@ -200,18 +203,44 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   // A depth-first walk visits each node at most once: more visits than nodes,
   // or an index past the end, means a malformed tree.  Stop and report it.
   uint32_t budget = nn;
+  if (cur >= nn) {
+    atomicOr(frame + frame::kWalkError, 1ull);
+    cur = kNone;
+  }
+  // The node's record and body range are loaded together (the range does not
+  // depend on the record).  With PREFETCH, the records of both possible next
+  // nodes (last child, skip node) are requested as soon as the current record
+  // is in, so they are in flight while the FP64 test runs.
+  WalkRec r{};
+  uint2 sp{};
+  if (cur != kNone) {
+    r = recs[cur];
+    sp = span[cur];
+  }
   while (cur != kNone) {
-    if (cur >= nn || budget-- == 0) {
+    if (budget-- == 0) {
       atomicOr(frame + frame::kWalkError, 1ull);
       break;
     }
-    const WalkRec r = recs[cur];
     const uint32_t fc = static_cast<uint32_t>(r.bits) & kRecIndexMask;
     uint32_t sk = static_cast<uint32_t>(r.bits >> 26) & kRecIndexMask;
     sk = sk == kRecIndexMask ? kNone : sk;
     const uint32_t nch = static_cast<uint32_t>(r.bits >> 52) & 15u;
     const int depth = static_cast<int>((r.bits >> 56) & 63u);
     const bool small = (r.bits >> 62) & 1u;
+    const uint32_t down = nch ? fc + nch - 1 : kNone;  // the last child
+    WalkRec r_down{}, r_skip{};
+    uint2 sp_down{}, sp_skip{};
+    if (PREFETCH) {
+      if (down < nn) {
+        r_down = recs[down];
+        sp_down = span[down];
+      }
+      if (sk < nn) {
+        r_skip = recs[sk];
+        sp_skip = span[sk];
+      }
+    }
     const double half = ldexp(root_half, -depth);
     bool overlap = true;
     double d2 = 0.0;
@@ -231,6 +260,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
       if (d > 0) d2 += d * d;
     }
     const bool accept = !overlap && d2 > r.crit * r.crit;
+    uint32_t next;
     if (accept && !small) {
       // Cell ids go out four at a time (one 16-byte store) once the slot is
       // 16-byte aligned: fewer partial cache-line writes.
@@ -246,18 +276,31 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
         }
       }
       ++nc;
-      cur = sk;
+      next = sk;
     } else if (accept || nch == 0) {
-      const uint2 sp = span[cur];
       for (uint32_t j = sp.x; j < sp.y; ++j) {
         const uint32_t at = nb + (j - sp.x);
         if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = j;
       }
       nb += sp.y - sp.x;
-      cur = sk;
+      next = sk;
     } else {
-      cur = fc + nch - 1;
+      next = down;
     }
+    if (next == kNone) break;
+    if (next >= nn) {
+      atomicOr(frame + frame::kWalkError, 1ull);
+      break;
+    }
+    if (PREFETCH) {
+      const bool d = next == down;
+      r = d ? r_down : r_skip;
+      sp = d ? sp_down : sp_skip;
+    } else {
+      r = recs[next];
+      sp = span[next];
+    }
+    cur = next;
   }
   // the ids still pending (the last partial group of four)
   {
@@ -271,11 +314,14 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     atomicOr(frame + frame::kSlotOverflow, 1ull);
   cell_count[g] = nc;
   body_count[g] = nb;
+  // The ranges a group's evaluation reads stay inside its slots even when the
+  // lists outgrew them (an evaluation may run before the overflow is seen;
+  // its results are then discarded).
   gt_group G = groups[g];
   G.cell_begin = cpos;
-  G.cell_end = cpos + nc;
+  G.cell_end = cpos + (MODE == kWalkExact ? nc : min(nc, cap_c));
   G.body_begin = bpos;
-  G.body_end = bpos + nb;
+  G.body_end = bpos + (MODE == kWalkExact ? nb : min(nb, cap_b));
   groups[g] = G;
 }
 
@@ -300,7 +346,10 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     uint32_t cap_b, const uint32_t *__restrict__ cell_off, const uint32_t *__restrict__ body_off,
     uint32_t *__restrict__ cell_count, uint32_t *__restrict__ body_count,
     uint32_t *__restrict__ cell_list, uint32_t *__restrict__ body_list,
-    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b) {
+    const uint32_t *__restrict__ slot_c, const uint32_t *__restrict__ slot_b, uint32_t g_begin,
+    uint32_t g_span) {
+  (void)g_begin;  // all groups at once
+  (void)g_span;
   // One tree and uniform slots only: the shared traversal starts at node 0
   // (group_root unused; batches and predicted slots walk one lane per group).
   (void)group_root;
@@ -420,9 +469,9 @@ __global__ __launch_bounds__(kBlock) void walk_coherent(
     body_count[g] = nb;
     gt_group G = groups[g];
     G.cell_begin = cpos;
-    G.cell_end = cpos + nc;
+    G.cell_end = cpos + (MODE == kWalkExact ? nc : min(nc, cap_c));
     G.body_begin = bpos;
-    G.body_end = bpos + nb;
+    G.body_end = bpos + (MODE == kWalkExact ? nb : min(nb, cap_b));
     groups[g] = G;
   }
 }
@@ -545,7 +594,7 @@ __global__ void check_frame(unsigned long long *frame, uint32_t node_cap, uint32
   if (frame[frame::kGroups] > group_cap) st |= kAbortGroups;
   if (check_slots && frame[frame::kSlotOverflow]) st |= kAbortLists;
   if (frame[frame::kWalkError] & 1ull) st |= kAbortWalk;
-  frame[frame::kStatus] |= st;
+  if (st) atomicOr(frame + frame::kStatus, st);  // (another stream may set bits too)
 }
 
 __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
@@ -628,19 +677,28 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, co
                  uint32_t cap_b,
                  const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
                  uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s,
-                 const uint32_t *slot_c, const uint32_t *slot_b) {
+                 const uint32_t *slot_c, const uint32_t *slot_b, uint32_t g_begin,
+                 uint32_t g_span) {
   if (!group_cap) return;
-  if ((group_root || slot_c) && lanes != 1)
+  if (g_span == 0 || g_span > group_cap) g_span = group_cap;
+  if (lanes > 2 && (g_begin != 0 || g_span != group_cap))
+    fail(GT_HIP_ERR_ARGUMENT, "coherent walks take all groups at once");
+  if ((group_root || slot_c) && lanes > 2)
     fail(GT_HIP_ERR_ARGUMENT, "coherent walks need a single tree and uniform slots");
 #define GTH_WALK_ARGS                                                                   \
   recs, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, cap_c, cap_b, \
-      cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c, slot_b
-  if (lanes == 1) {
-    const dim3 grid(xcd_grid(blocks_for(group_cap, kWalkBlock))), block(kWalkBlock);
-    if (mode == kWalkExact)
-      hipLaunchKernelGGL(walk<kWalkExact>, grid, block, 0, s, GTH_WALK_ARGS);
+      cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c, slot_b, g_begin, \
+      g_span
+  if (lanes == 1 || lanes == 2) {  // one walk per lane: 1 prefetching, 2 plain
+    const dim3 grid(xcd_grid(blocks_for(g_span, kWalkBlock))), block(kWalkBlock);
+    if (mode == kWalkExact && lanes == 1)
+      hipLaunchKernelGGL((walk<kWalkExact, true>), grid, block, 0, s, GTH_WALK_ARGS);
+    else if (mode == kWalkExact)
+      hipLaunchKernelGGL((walk<kWalkExact, false>), grid, block, 0, s, GTH_WALK_ARGS);
+    else if (lanes == 1)
+      hipLaunchKernelGGL((walk<kWalkCapacity, true>), grid, block, 0, s, GTH_WALK_ARGS);
     else
-      hipLaunchKernelGGL(walk<kWalkCapacity>, grid, block, 0, s, GTH_WALK_ARGS);
+      hipLaunchKernelGGL((walk<kWalkCapacity, false>), grid, block, 0, s, GTH_WALK_ARGS);
     GTH_LAUNCH_CHECK();
     return;
   }

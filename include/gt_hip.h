@@ -71,7 +71,8 @@ typedef struct {
   int eval_waves;    /* tuning: waves per SIMD the list-evaluation kernel is built
                         for (2 or 3; 0 = library default, 3) */
   int walk_lanes;    /* tuning: 16, 32 or 64 groups share one wave-coherent walk;
-                        0 or 1 (default): one independent walk per lane */
+                        0 or 2 (default): one independent walk per lane; 1: the same,
+                        fetching both possible next nodes' records ahead */
   int split_len;     /* tuning: lists longer than this many entries are evaluated in
                         parts by several waves (fast evaluation only; parts added in a
                         fixed order); 0 = library default (8192), < 0 = never split */
@@ -90,7 +91,11 @@ typedef struct {
                         predictor (an MLP trained on the first evaluations' lists; uniform
                         until it is trained), 2 = no slots: a count pass and a fill pass
                         every evaluation (the plain two-pass solver, for comparison) */
-  int reserved[1];   /* zero */
+  int overlap;       /* tuning (fast evaluation): the walk runs in this many ranges of
+                        groups and each range is evaluated on a second stream while the
+                        next is walked; 1 = no overlap, 0 = library default (1), at most
+                        8.  Pays only where the walk has many more groups than the GPU
+                        has SIMDs: a range's walk lasts as long as its longest group's */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
