@@ -112,6 +112,7 @@ struct gt_hip_context {
   DBuf<int4> up_lnk;
   // lists of the last walk
   DBuf<WalkRec> recs;
+  DBuf<WalkRec32> images;
   DBuf<uint2> span;
   DBuf<Cell> cells;
   DBuf<float4> bodies, out;
@@ -436,6 +437,7 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   c->skip.ensure(nn);
   c->depth.ensure(nn);
   c->recs.ensure(nn);
+  c->images.ensure(nn);
   c->span.ensure(nn);
   c->cells.ensure(nn);
   c->bodies.ensure(n);
@@ -484,13 +486,20 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   }
 }
 
-// 2: one walk per lane (the default), 1: the same fetching the next records
-// ahead, 16/32/64: coherent walks.
+// 3: one walk per lane over child sets (the default), 2: one walk per lane
+// one record per step, 1: the same fetching the next records ahead,
+// 16/32/64: coherent walks.
 int walk_lanes(const gt_group_config *cfg, const Plan &p) {
-  const int per_lane = cfg->walk_lanes == 1 ? 1 : 2;
-  if (p.systems > 1 || p.predicted) return per_lane;  // a coherent walk shares one tree
   const int l = cfg->walk_lanes;
+  const int per_lane = l == 1 || l == 2 ? l : 3;
+  if (p.systems > 1 || p.predicted) return per_lane;  // a coherent walk shares one tree
   return l == 16 || l == 32 || l == 64 ? l : per_lane;
+}
+
+// Stack entries per lane of the walk over child sets: one per level below the
+// root (the tree is at most depth_cap deep, or the tree phase stops first).
+uint32_t levels_for(const Plan &p) {
+  return static_cast<uint32_t>(std::min(std::max(p.depth_cap, 1), 63) + 1);
 }
 
 // Clears the counts of the frame (not its status or step counter).
@@ -521,7 +530,8 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
   const uint32_t n = p.n, cap = p.node_cap;
   launch_link_up(in.lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
   launch_prep_lists(in.geom, in.lnk, c->skip.get(), c->depth.get(), in.tpos, in.tmass, n, frame,
-                    cap, cfg->theta, in.tsid, in.roots, c->recs.get(), c->span.get(),
+                    cap, cfg->theta, in.tsid, in.roots, c->recs.get(), c->images.get(),
+                    c->span.get(),
                     c->cells.get(), c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
                     hilo ? c->cells_lo.get() : nullptr, s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
@@ -558,13 +568,15 @@ void enqueue_walk_range(gt_hip_context *c, const gt_group_config *cfg, const Pla
   const uint32_t span = (p.group_cap + chunks - 1) / chunks;
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
   if (p.predicted)
-    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), in.tpos,
+    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->images.get(),
+                levels_for(p), c->span.get(), in.tpos,
                 c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, 0, 0,
                 c->slot_off_c.get(), c->slot_off_b.get(), c->ccount.get(), c->bcount.get(),
                 c->cell_list.get(), c->body_list.get(), s, c->slot_c.get(), c->slot_b.get(),
                 k * span, span);
   else
-    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->span.get(), in.tpos,
+    launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->images.get(),
+                levels_for(p), c->span.get(), in.tpos,
                 c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, p.cap_c, p.cap_b,
                 nullptr, nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(),
                 c->body_list.get(), s, nullptr, nullptr, k * span, span);
@@ -634,7 +646,8 @@ void lists_exact(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, c
   uint32_t *bl = c->body_list.ensure(std::max<unsigned long long>(all_b, 1));
   launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
   uint32_t *groot = p.systems > 1 ? c->group_root.get() : nullptr;
-  launch_walk(kWalkExact, walk_lanes(cfg, p), c->recs.get(), c->span.get(), tpos, c->groups.get(),
+  launch_walk(kWalkExact, walk_lanes(cfg, p), c->recs.get(), c->images.get(), levels_for(p),
+              c->span.get(), tpos, c->groups.get(),
               frame, p.group_cap, p.node_cap, geom, groot, 0, 0, co, bo, c->ccount.get(),
               c->bcount.get(), cl, bl, s);
   launch_check_frame(frame, p, false, s);
@@ -748,6 +761,7 @@ void finish(gt_hip_context *c, const Plan &p, const double *geom, bool count = t
   c->info.max_group = fw[frame::kStats + 3];
   c->info.n_cell_list = fw[frame::kStats + 6];
   c->info.n_body_list = fw[frame::kStats + 7];
+  c->pstats.last_exact_tests = fw[frame::kExactTests];
   c->lists_n = p.n;
   c->lists_nn = nn;
   c->lists_geom = geom;
@@ -1459,7 +1473,7 @@ int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, c
       }
     };
     if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
-    sweep(&gt_group_config::walk_lanes, {1, 2, 16, 32, 64});
+    sweep(&gt_group_config::walk_lanes, {2, 16, 32, 64});
     if (!cfg->exact) {
       sweep(&gt_group_config::eval_waves, {2, 3});
       sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
@@ -1497,11 +1511,17 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
     require(n_nodes == 0 || (links[2] == 0 && static_cast<uint64_t>(links[3]) == n),
             "tree: the root must hold every body");
     std::vector<uint8_t> seen(n_nodes, 0);
+    // depths (children come after their parent, so one pass in node order)
+    std::vector<uint8_t> node_depth(n_nodes, 0);
+    int tree_depth = 0;
     for (uint64_t k = 0; k < n_nodes; ++k)
       for (int32_t ch = 0; ch < links[4 * k + 1]; ++ch) {
         const uint64_t x = static_cast<uint64_t>(links[4 * k]) + ch;
         require(seen[x] == 0, "tree: a node has two parents");
         seen[x] = 1;
+        require(node_depth[k] < 63, "tree: deeper than 63 levels");
+        node_depth[x] = static_cast<uint8_t>(node_depth[k] + 1);
+        tree_depth = std::max<int>(tree_depth, node_depth[x]);
       }
     GTH_CHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -1525,6 +1545,7 @@ int gt_hip_walk_tree(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
     c->sys_off = nullptr;
     Plan p = plan_for(c, static_cast<uint32_t>(n), cfg);
     p.node_cap = static_cast<uint32_t>(n_nodes);
+    p.depth_cap = std::max(p.depth_cap, tree_depth);  // the walk's stack has a level each
     for (int attempt = 0;; ++attempt) {
       if (attempt == 4) fail(GT_HIP_ERR_DEVICE, "group walk: bounds did not settle");
       launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "gt_hip.h"
+#include "gt_walk_rec.h"
 
 namespace gth {
 
@@ -151,7 +152,8 @@ constexpr int kSlotOverflow = 16;  // a group's lists outgrew their slots (the w
 constexpr int kPredCells = 17;  // sum of the predicted cell-list slots
 constexpr int kPredBodies = 18; // sum of the predicted body-list slots
 constexpr int kPredError = 19;  // 1e6 x sum over groups of |log(1 + length) - prediction|
-constexpr int kCountsEnd = 20;  // [kNodes, kCountsEnd): cleared before each evaluation
+constexpr int kExactTests = 20; // opening tests the FP32 filter left to the FP64 test
+constexpr int kCountsEnd = 21;  // [kNodes, kCountsEnd): cleared before each evaluation
 constexpr int kStepsDone = 30;  // leapfrog steps completed since the counter was cleared
 constexpr int kStepInteractions = 31;  // their cell + body interactions
 constexpr int kWords = 32;
@@ -285,9 +287,9 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
                        unsigned long long *frame, uint32_t node_cap, double theta,
-                       const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
-                       Cell *cells, float4 *bodies, float4 *bodies_lo, float4 *cells_lo,
-                       hipStream_t s);
+                       const uint32_t *tsid, const double *roots, WalkRec *recs,
+                       WalkRec32 *images, uint2 *span, Cell *cells, float4 *bodies,
+                       float4 *bodies_lo, float4 *cells_lo, hipStream_t s);
 void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
                          uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
                          hipStream_t s);
@@ -303,7 +305,11 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
 // its recorded ranges are cut at its slots.  Any status set before the walk
 // (kAbortSlots) stops it.  Groups g_begin .. g_begin + g_span only (g_span 0:
 // all); coherent walks (lanes > 2) take all groups.
-void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
+// lanes: 3 the walk over child sets (images, levels_cap levels of stack per
+// lane; the default), 2 one record per step, 1 the same fetching the next
+// records ahead, 16/32/64 coherent walks.
+void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *images,
+                 uint32_t levels_cap, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                  uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
                  uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,

@@ -95,8 +95,9 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const double *__restrict__ tpos, const double *__restrict__ tmass, uint32_t n,
     unsigned long long *__restrict__ frame, uint32_t node_cap, double inv_theta,
     const uint32_t *__restrict__ tsid, const double *__restrict__ roots,
-    WalkRec *__restrict__ recs, uint2 *__restrict__ span, Cell *__restrict__ cells,
-    float4 *__restrict__ bodies, float4 *__restrict__ bodies_lo, float4 *__restrict__ cells_lo) {
+    WalkRec *__restrict__ recs, WalkRec32 *__restrict__ images, uint2 *__restrict__ span,
+    Cell *__restrict__ cells, float4 *__restrict__ bodies, float4 *__restrict__ bodies_lo,
+    float4 *__restrict__ cells_lo) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   // origin: the root centre of the body's (node's) system (tsid null: one
@@ -141,9 +142,13 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     w.my = g[5];
     w.mz = g[6];
     w.crit = 2.0 * g[3] * inv_theta + g[14];
+    const bool small = static_cast<uint32_t>(L.w - L.z) <= kBodiesIfAtMost;
     w.bits = pack_walk_bits(static_cast<uint32_t>(L.x), skip[k], static_cast<uint32_t>(L.y), d,
-                            static_cast<uint32_t>(L.w - L.z) <= kBodiesIfAtMost);
+                            small);
     recs[k] = w;
+    images[k] = make_rec32(w.cx, w.cy, w.cz, w.mx, w.my, w.mz, w.crit, ox, oy, oz, root_half,
+                           image_meta(static_cast<uint32_t>(L.x), static_cast<uint32_t>(L.y)),
+                           small);
     span[k] = make_uint2(static_cast<uint32_t>(L.z), static_cast<uint32_t>(L.w));
   }
 }
@@ -310,6 +315,196 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
     if (first < cpos) first = cpos;
     for (uint32_t q = first; q < end; ++q) cell_list[q] = cbuf[q & 3u];
   }
+  if (MODE == kWalkCapacity && (nc > cap_c || nb > cap_b))
+    atomicOr(frame + frame::kSlotOverflow, 1ull);
+  cell_count[g] = nc;
+  body_count[g] = nb;
+  // The ranges a group's evaluation reads stay inside its slots even when the
+  // lists outgrew them (an evaluation may run before the overflow is seen;
+  // its results are then discarded).
+  gt_group G = groups[g];
+  G.cell_begin = cpos;
+  G.cell_end = cpos + (MODE == kWalkExact ? nc : min(nc, cap_c));
+  G.body_begin = bpos;
+  G.body_end = bpos + (MODE == kWalkExact ? nb : min(nb, cap_b));
+  groups[g] = G;
+}
+
+// Walk over child sets, eight lanes per group.  The nodes of one set (a
+// node's children, first to last) are tested together, one per lane of the
+// group's octet: their 32-byte images come in one round of independent loads,
+// and the FP64 record is read only for a node whose image cannot decide.  The
+// set is then taken in the reference's order, last child first: the nodes
+// after the last one that opens (in child order) go to the lists at once, each
+// lane writing its own entries at positions counted across the octet, and
+// the node that opens makes its children the next set; the nodes before it
+// wait as one entry (first node, how many, depth) on the octet's stack in
+// LDS, one entry per tree level, and are loaded and tested again when the
+// walk returns to them.  Decisions and order are those of the reference's
+// stack walk, so the lists are identical.  A round of loads serves a whole
+// set instead of one node, and eight lanes per group give the GPU eight times
+// the waves of a walk with one lane per group to hide memory latency with.
+constexpr int kCellCode = 0, kBodiesCode = 1, kOpenCode = 2;
+constexpr uint32_t kOctet = 8;
+constexpr uint32_t kOctets = kWalkBlock / kOctet;  // groups per block (one wave)
+
+template <int MODE>
+__global__ __launch_bounds__(kWalkBlock) void walk_sets(
+    const WalkRec *__restrict__ recs, const WalkRec32 *__restrict__ images, uint32_t levels_cap,
+    const uint2 *__restrict__ span, const double *__restrict__ tpos,
+    gt_group *__restrict__ groups, unsigned long long *__restrict__ frame, uint32_t group_cap,
+    uint32_t node_cap, const double *__restrict__ geom, const uint32_t *__restrict__ group_root,
+    uint32_t cap_c, uint32_t cap_b, const uint32_t *__restrict__ cell_off,
+    const uint32_t *__restrict__ body_off, uint32_t *__restrict__ cell_count,
+    uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
+    uint32_t *__restrict__ body_list, const uint32_t *__restrict__ slot_c,
+    const uint32_t *__restrict__ slot_b, uint32_t g_begin, uint32_t g_span) {
+  extern __shared__ uint2 levels[];  // [kOctets][levels_cap]
+  if (frame[frame::kStatus]) return;  // the slots do not fit the buffers
+  // groups g_begin .. g_begin + g_span of the first frame[kGroups]
+  const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
+  const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
+  const uint32_t blk = xcd_block_used(blockIdx.x, (count + kOctets - 1) / kOctets);
+  if (blk == kNone) return;
+  const uint32_t k = threadIdx.x % kOctet, oct = threadIdx.x / kOctet;
+  const uint32_t g = g_begin + blk * kOctets + oct;
+  const bool valid = g < g_begin + count;  // (the same for the octet's eight lanes)
+  uint2 *stack = levels + oct * levels_cap;
+  uint32_t gb = 0, ge = 0, start = 0;
+  if (valid) {
+    gb = groups[g].begin;
+    ge = groups[g].end;
+    // the walk starts at the root of the group's system (node 0 for one system)
+    start = group_root ? group_root[g] : 0u;
+  }
+  const double *origin = geom + static_cast<size_t>(kGeomStride) * (start < nn ? start : 0u);
+  const double root_half = origin[3];
+  // group box: the lanes take every eighth body; min and max do not depend
+  // on the order they are taken in
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t j = gb + k; j < ge; j += kOctet)
+    for (int a = 0; a < 3; ++a) {
+      const double v = tpos[3 * static_cast<size_t>(j) + a];
+      lo[a] = (v < lo[a]) ? v : lo[a];  // std::min / std::max semantics
+      hi[a] = (hi[a] < v) ? v : hi[a];
+    }
+  for (int off = 4; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      const double l = __shfl_xor(lo[a], off, kOctet), h = __shfl_xor(hi[a], off, kOctet);
+      lo[a] = (l < lo[a]) ? l : lo[a];
+      hi[a] = (hi[a] < h) ? h : hi[a];
+    }
+  const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]),
+               gcz = 0.5 * (lo[2] + hi[2]);
+  const double ghx = 0.5 * (hi[0] - lo[0]), ghy = 0.5 * (hi[1] - lo[1]),
+               ghz = 0.5 * (hi[2] - lo[2]);
+  const GroupImage gim =
+      group_image(origin[0], origin[1], origin[2], root_half, gcx, gcy, gcz, ghx, ghy, ghz);
+  const float root_half32 = static_cast<float>(root_half);
+  uint32_t cpos = 0, bpos = 0;
+  if (valid) {
+    if (MODE == kWalkExact) {
+      cpos = cell_off[g];
+      bpos = body_off[g];
+    } else if (slot_c) {  // predicted slots, packed at scanned offsets
+      cpos = cell_off[g];
+      bpos = body_off[g];
+      cap_c = slot_c[g];
+      cap_b = slot_b[g];
+    } else {
+      cpos = g * cap_c;
+      bpos = g * cap_b;
+    }
+  }
+  // the current set: nodes fc .. fc + n - 1 at depth d (first: the root alone)
+  uint32_t fc = start, n = 1, d = 0, top = 0, nc = 0, nb = 0, exact_tests = 0;
+  bool done = !valid;
+  bool failed = valid && start >= nn;
+  done = done || failed;
+  // Each round descends one level or returns to a waiting entry: a well-formed
+  // tree takes fewer rounds than twice its node count.
+  uint32_t budget = 2 * nn + 2;
+  while (__any(!done)) {
+    if (done) continue;
+    if (budget-- == 0 || n > kOctet || fc >= nn || n > nn - fc || d > 63) {
+      failed = done = true;
+      continue;
+    }
+    const bool act = k < n;
+    WalkRec32 im{};
+    uint2 bs{};
+    int code = -1;
+    if (act) {
+      im = images[fc + k];
+      bs = span[fc + k];
+      int dec = gim.usable ? accept_filter(im, half_image(root_half32, static_cast<int>(d)), gim)
+                           : kUndecided;
+      if (dec == kUndecided) {
+        const WalkRec x = recs[fc + k];
+        dec = accept_exact(x.cx, x.cy, x.cz, x.mx, x.my, x.mz,
+                           ldexp(root_half, -static_cast<int>(d)), x.crit, gcx, gcy, gcz, ghx,
+                           ghy, ghz);
+        ++exact_tests;
+      }
+      const bool small = signbit(im.crit), leaf = (im.meta & kMetaLeaf) != 0u;
+      code = (dec == 1 && !small) ? kCellCode : (dec == 1 || leaf) ? kBodiesCode : kOpenCode;
+    }
+    const uint32_t shift = threadIdx.x & ~(kOctet - 1);
+    const uint32_t open_bits = static_cast<uint32_t>(__ballot(code == kOpenCode) >> shift) & 0xffu;
+    const uint32_t cell_bits = static_cast<uint32_t>(__ballot(code == kCellCode) >> shift) & 0xffu;
+    // the last node (in child order) that opens; the nodes after it go out now
+    const int open = open_bits ? 31 - __clz(static_cast<int>(open_bits)) : -1;
+    const uint32_t emit = (0xffu << (open + 1)) & 0xffu;
+    const bool mine = act && ((emit >> k) & 1u);
+    const uint32_t after_me = emit & (0xfeu << k);  // emitted before this node
+    if (mine && code == kCellCode) {
+      const uint32_t at = nc + __popc(cell_bits & after_me);
+      if (MODE == kWalkExact || at < cap_c) cell_list[cpos + at] = fc + k;
+    }
+    nc += __popc(cell_bits & emit);
+    // bodies: each lane's range, after those of the lanes above it
+    const uint32_t cnt = (mine && code == kBodiesCode) ? bs.y - bs.x : 0u;
+    uint32_t suffix = cnt;  // this lane's and the higher lanes' counts
+    for (uint32_t off = 1; off < kOctet; off <<= 1) {
+      const uint32_t u = __shfl_down(suffix, off, kOctet);
+      if (k + off < kOctet) suffix += u;
+    }
+    const uint32_t total = __shfl(suffix, 0, kOctet);
+    const uint32_t first = nb + (suffix - cnt);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t at = first + j;
+      if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = bs.x + j;
+    }
+    nb += total;
+    // the next set
+    const uint32_t open_meta = __shfl(im.meta, open < 0 ? 0 : open, kOctet);
+    if (open >= 0) {
+      if (open > 0) {  // the nodes before it wait
+        if (top >= levels_cap) {
+          failed = done = true;
+          continue;
+        }
+        stack[top] = make_uint2(fc | (static_cast<uint32_t>(open) << 26), d);
+        ++top;
+      }
+      fc = meta_first_child(open_meta);
+      n = meta_children(open_meta);
+      ++d;
+    } else if (top > 0) {
+      --top;
+      const uint2 e = stack[top];
+      fc = e.x & kMetaIndexMask;
+      n = e.x >> 26;
+      d = e.y;
+    } else {
+      done = true;
+    }
+  }
+  if (exact_tests)
+    atomicAdd(frame + frame::kExactTests, static_cast<unsigned long long>(exact_tests));
+  if (!valid || k != 0) return;
+  if (failed) atomicOr(frame + frame::kWalkError, 1ull);
   if (MODE == kWalkCapacity && (nc > cap_c || nb > cap_b))
     atomicOr(frame + frame::kSlotOverflow, 1ull);
   cell_count[g] = nc;
@@ -636,14 +831,14 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
                        unsigned long long *frame, uint32_t node_cap, double theta,
-                       const uint32_t *tsid, const double *roots, WalkRec *recs, uint2 *span,
-                       Cell *cells, float4 *bodies, float4 *bodies_lo, float4 *cells_lo,
-                       hipStream_t s) {
+                       const uint32_t *tsid, const double *roots, WalkRec *recs,
+                       WalkRec32 *images, uint2 *span, Cell *cells, float4 *bodies,
+                       float4 *bodies_lo, float4 *cells_lo, hipStream_t s) {
   const uint32_t m = n > node_cap ? n : node_cap;
   if (!m) return;
   hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
-                     tpos, tmass, n, frame, node_cap, 1.0 / theta, tsid, roots, recs, span, cells,
-                     bodies, bodies_lo, cells_lo);
+                     tpos, tmass, n, frame, node_cap, 1.0 / theta, tsid, roots, recs, images,
+                     span, cells, bodies, bodies_lo, cells_lo);
   GTH_LAUNCH_CHECK();
 }
 
@@ -671,7 +866,8 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
   GTH_LAUNCH_CHECK();
 }
 
-void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, const double *tpos,
+void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *images,
+                 uint32_t levels_cap, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                  uint32_t node_cap, const double *geom, const uint32_t *group_root, uint32_t cap_c,
                  uint32_t cap_b,
@@ -681,14 +877,31 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const uint2 *span, co
                  uint32_t g_span) {
   if (!group_cap) return;
   if (g_span == 0 || g_span > group_cap) g_span = group_cap;
-  if (lanes > 2 && (g_begin != 0 || g_span != group_cap))
+  const bool coherent = lanes >= 16;
+  if (coherent && (g_begin != 0 || g_span != group_cap))
     fail(GT_HIP_ERR_ARGUMENT, "coherent walks take all groups at once");
-  if ((group_root || slot_c) && lanes > 2)
+  if ((group_root || slot_c) && coherent)
     fail(GT_HIP_ERR_ARGUMENT, "coherent walks need a single tree and uniform slots");
 #define GTH_WALK_ARGS                                                                   \
   recs, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, cap_c, cap_b, \
       cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c, slot_b, g_begin, \
       g_span
+  if (lanes == 3) {  // eight lanes per group, over child sets
+    if (levels_cap == 0 || levels_cap > 64) fail(GT_HIP_ERR_ARGUMENT, "walk: levels_cap out of range");
+    const dim3 grid(xcd_grid(blocks_for(g_span, kOctets))), block(kWalkBlock);
+    const size_t lds = static_cast<size_t>(levels_cap) * kOctets * sizeof(uint2);
+#define GTH_SETS_ARGS                                                                        \
+  recs, images, levels_cap, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, \
+      cap_c, cap_b, cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c,  \
+      slot_b, g_begin, g_span
+    if (mode == kWalkExact)
+      hipLaunchKernelGGL((walk_sets<kWalkExact>), grid, block, lds, s, GTH_SETS_ARGS);
+    else
+      hipLaunchKernelGGL((walk_sets<kWalkCapacity>), grid, block, lds, s, GTH_SETS_ARGS);
+#undef GTH_SETS_ARGS
+    GTH_LAUNCH_CHECK();
+    return;
+  }
   if (lanes == 1 || lanes == 2) {  // one walk per lane: 1 prefetching, 2 plain
     const dim3 grid(xcd_grid(blocks_for(g_span, kWalkBlock))), block(kWalkBlock);
     if (mode == kWalkExact && lanes == 1)

@@ -55,7 +55,7 @@ class PredictorStats(C.Structure):
 class PipelineStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("evaluations", "reruns", "list_rewrites",
                                            "graph_captures", "graph_steps", "graph_stops",
-                                           "interactions")]
+                                           "interactions", "last_exact_tests")]
 
 
 _vp = C.c_void_p

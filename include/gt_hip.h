@@ -70,9 +70,12 @@ typedef struct {
                         forces bit-identical to evaluate_group_lists_cpu */
   int eval_waves;    /* tuning: waves per SIMD the list-evaluation kernel is built
                         for (2 or 3; 0 = library default, 3) */
-  int walk_lanes;    /* tuning: 16, 32 or 64 groups share one wave-coherent walk;
-                        0 or 2 (default): one independent walk per lane; 1: the same,
-                        fetching both possible next nodes' records ahead */
+  int walk_lanes;    /* tuning: 0 or 3 (default): one walk per lane that tests a node's
+                        children together (one round of loads per child set, FP32 node
+                        images with an exact FP64 fallback); 2: one walk per lane, one
+                        node record per step; 1: the same, fetching both possible next
+                        nodes' records ahead; 16, 32 or 64: that many groups share one
+                        wave-coherent walk.  Every choice gives the same lists */
   int split_len;     /* tuning: lists longer than this many entries are evaluated in
                         parts by several waves (fast evaluation only; parts added in a
                         fixed order); 0 = library default (8192), < 0 = never split */
@@ -161,7 +164,7 @@ int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
                                const double *mass, double *acc, double *pot);
 /* Autotuner (no counterpart in the reference): times force evaluations of the
  * given particles on the GPU (HIP events, best of 2 after a warm-up) while it
- * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {1, 16,
+ * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {2, 16,
  * 32, 64}, eval_order {1, 2, 3, 4}, split_len {2048, 8192, never}, and group_size
  * {32, 64} when search_group_size is 1), keeping each change that is at least 1%
  * faster,
@@ -234,6 +237,8 @@ typedef struct {
   uint64_t graph_stops;    /* graph steps that outgrew their bounds and were finished
                               without graphs */
   uint64_t interactions;   /* cell + body interactions of all completed evaluations */
+  uint64_t last_exact_tests; /* opening tests of the last evaluation's walk that the FP32
+                                filter left to the FP64 test */
 } gt_pipeline_stats;
 int gt_hip_pipeline_stats(const gt_hip_context *ctx, gt_pipeline_stats *stats);
 /* The learned list-length predictor (no counterpart in the reference): with

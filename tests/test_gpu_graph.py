@@ -35,7 +35,7 @@ def run(hip, pos, vel, mass, plan, use_graph, between=None, **cfg):
     return hip.state_get()
 
 
-@pytest.mark.parametrize("exact,lanes", [(True, 1), (False, 1), (False, 32)])
+@pytest.mark.parametrize("exact,lanes", [(True, 0), (False, 0), (True, 1), (False, 2), (False, 32)])
 def test_graph_steps_equal_eager_steps(hip, oracle, exact, lanes):
     pos, mass = oracle.test_system(0, 20000, 21)
     rng = np.random.default_rng(4)
