@@ -501,7 +501,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
       done = true;
     }
   }
-  if (exact_tests)
+  // (the whole wave is here: one atomic for its count)
+  for (int off = 32; off > 0; off >>= 1) exact_tests += __shfl_xor(exact_tests, off, 64);
+  if (threadIdx.x == 0 && exact_tests)
     atomicAdd(frame + frame::kExactTests, static_cast<unsigned long long>(exact_tests));
   if (!valid || k != 0) return;
   if (failed) atomicOr(frame + frame::kWalkError, 1ull);
@@ -735,47 +737,51 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
 }
 
 // stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group,
-// [4] most cells of a group, [5] most bodies of a group, [6] all cells, [7] all bodies
+// [4] most cells of a group, [5] most bodies of a group, [6] all cells, [7] all bodies.
+// A few hundred blocks stride over the groups and each adds its totals with one
+// set of atomics (one set per wave made the atomics the kernel's cost).
+constexpr uint32_t kStatsBlocks = 256;
+
 __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict__ cell_count,
                                                      const uint32_t *__restrict__ body_count,
                                                      const gt_group *__restrict__ groups,
                                                      unsigned long long *__restrict__ frame,
                                                      uint32_t group_cap) {
+  __shared__ unsigned long long part[kBlock / 64][8];
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   unsigned long long *stats = frame + frame::kStats;
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  unsigned long long ci = 0, bi = 0, ml = 0, mg = 0, mc = 0, mb = 0, sc = 0, sb = 0;
-  if (g < ng) {
+  unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // ci, bi, ml, mg, mc, mb, sc, sb
+  for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < ng; g += gridDim.x * kBlock) {
     const unsigned long long t = groups[g].end - groups[g].begin, c = cell_count[g],
                              b = body_count[g];
-    ci = t * c;
-    bi = t * b - t;
-    ml = c + b;
-    mg = t;
-    mc = c;
-    mb = b;
-    sc = c;
-    sb = b;
+    v[0] += t * c;
+    v[1] += t * b - t;
+    v[2] = max(v[2], c + b);
+    v[3] = max(v[3], t);
+    v[4] = max(v[4], c);
+    v[5] = max(v[5], b);
+    v[6] += c;
+    v[7] += b;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    ci += __shfl_xor(ci, off, 64);
-    bi += __shfl_xor(bi, off, 64);
-    ml = max(ml, __shfl_xor(ml, off, 64));
-    mg = max(mg, __shfl_xor(mg, off, 64));
-    mc = max(mc, __shfl_xor(mc, off, 64));
-    mb = max(mb, __shfl_xor(mb, off, 64));
-    sc += __shfl_xor(sc, off, 64);
-    sb += __shfl_xor(sb, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0 && g < ng) {  // lane 0 of a wave with groups
-    atomicAdd(&stats[0], ci);
-    atomicAdd(&stats[1], bi);
-    atomicMax(&stats[2], ml);
-    atomicMax(&stats[3], mg);
-    atomicMax(&stats[4], mc);
-    atomicMax(&stats[5], mb);
-    atomicAdd(&stats[6], sc);
-    atomicAdd(&stats[7], sb);
+  for (int off = 32; off > 0; off >>= 1)
+    for (int i = 0; i < 8; ++i) {
+      const unsigned long long o = __shfl_xor(v[i], off, 64);
+      v[i] = (i >= 2 && i <= 5) ? max(v[i], o) : v[i] + o;
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < 8; ++i) part[threadIdx.x / 64][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int i = threadIdx.x;
+    unsigned long long a = part[0][i];
+    for (int w = 1; w < kBlock / 64; ++w)
+      a = (i >= 2 && i <= 5) ? max(a, part[w][i]) : a + part[w][i];
+    if (a != 0) {
+      if (i >= 2 && i <= 5)
+        atomicMax(&stats[i], a);
+      else
+        atomicAdd(&stats[i], a);
+    }
   }
 }
 
@@ -941,8 +947,9 @@ void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        hipStream_t s) {
   launch_fill_u64(frame + frame::kStats, 8, 0ull, s);
   if (!group_cap) return;
-  hipLaunchKernelGGL(list_stats, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, cell_count,
-                     body_count, groups, frame, group_cap);
+  const uint32_t blocks = std::min<uint32_t>(blocks_for(group_cap), kStatsBlocks);
+  hipLaunchKernelGGL(list_stats, dim3(blocks), dim3(kBlock), 0, s, cell_count, body_count, groups,
+                     frame, group_cap);
   GTH_LAUNCH_CHECK();
 }
 
