@@ -31,6 +31,8 @@ struct History {
   uint64_t all_c = 0, all_b = 0;        // list entries (lists_known)
   bool pred_known = false;
   uint64_t pred_c = 0, pred_b = 0;      // predicted slot totals
+  bool levels_known = false;
+  uint32_t level_nodes[frame::kLevels] = {};  // nodes per tree level
 };
 
 // The learned list-length predictor of a context.
@@ -69,7 +71,8 @@ bool same_plan(const Plan &a, const Plan &b) {
          a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
          a.cost_buckets == b.cost_buckets && a.hilo == b.hilo && a.predicted == b.predicted &&
          a.slot_total_c == b.slot_total_c && a.slot_total_b == b.slot_total_b &&
-         a.chunks == b.chunks;
+         a.chunks == b.chunks && a.levels_sized == b.levels_sized &&
+         std::equal(a.level_blocks, a.level_blocks + frame::kLevels, b.level_blocks);
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
@@ -244,6 +247,12 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
     return false;
   if (static_cast<uint64_t>(h.nodes) + h.nodes / 16 > p.node_cap) return false;
   if (h.depth + 1 > p.depth_cap && p.depth_cap < 60) return false;
+  // moment grids: sized once the levels are known, then kept while no level
+  // needs more than two passes of its grid
+  if (h.levels_known && !p.levels_sized) return false;
+  if (p.levels_sized)
+    for (int d = 0; d < frame::kLevels; ++d)
+      if (h.level_nodes[d] > 2ull * 256 * p.level_blocks[d]) return false;
   if (p.group_cap < n && static_cast<uint64_t>(h.groups) + h.groups / 16 > p.group_cap) return false;
   if (h.most_c + h.most_c / 16 + 8 > p.cap_c || h.most_b + h.most_b / 16 + 8 > p.cap_b)
     return false;
@@ -309,6 +318,16 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
   p.node_cap = same && h.nodes ? grown(h.nodes, 1024, kMaxNodes)
                                : static_cast<uint32_t>(std::min<uint64_t>(n + 1024ull, kMaxNodes));
   p.depth_cap = same && h.depth >= 0 ? std::min(60, h.depth + 3) : 60;
+  if (same && h.levels_known) {
+    p.levels_sized = true;
+    for (int d = 0; d < frame::kLevels; ++d) {
+      // a quarter of headroom, rounded up to a power of two
+      const uint64_t need = (h.level_nodes[d] + h.level_nodes[d] / 4 + 255) / 256;
+      uint32_t b = 1;
+      while (b < need && b < kMomentBlocks) b <<= 1;
+      p.level_blocks[d] = static_cast<uint16_t>(b);
+    }
+  }
   p.group_cap = same && h.groups ? grown(h.groups, 64, n) : n;
   if (h.lists_known) {
     p.cap_c = slot_for(h.most_c);
@@ -387,6 +406,11 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
   if (!(st & kAbortNodes)) {
     h.depth = static_cast<int>(fw[frame::kDepth]);
     h.groups = static_cast<uint32_t>(fw[frame::kGroups]);
+  }
+  if (!(st & (kAbortNodes | kAbortPaths | kAbortDepth))) {
+    h.levels_known = true;
+    for (int d = 0; d < frame::kLevels; ++d)
+      h.level_nodes[d] = static_cast<uint32_t>(fw[frame::kLevelNodes + d]);
   }
   // List counts hold whenever the tree and groups were complete (slots or
   // work items too small do not change them); item counts need whole lists.

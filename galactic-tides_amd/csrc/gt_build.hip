@@ -336,6 +336,16 @@ __global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__
 
 // lstart[d] = first node of level d (levels past the deepest stay 0), and the
 // depth of the tree into the frame.
+// frame[kLevelNodes + d] = nodes on level d (the host sizes the next moments
+// launches from them).  One workgroup of 64.
+__global__ void level_counts(const uint32_t *__restrict__ lstart,
+                             unsigned long long *__restrict__ frame) {
+  for (int d = threadIdx.x; d < frame::kLevels; d += blockDim.x) {
+    const uint32_t a = lstart[d], b = lstart[d + 1];
+    frame[frame::kLevelNodes + d] = b > a ? b - a : 0u;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restrict__ node_depth,
                                                        unsigned long long *__restrict__ frame,
                                                        uint32_t node_cap,
@@ -727,6 +737,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   hipLaunchKernelGGL(level_starts, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, frame, cap,
                      lstart);
   GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(level_counts, dim3(1), dim3(64), 0, s, lstart, frame);
+  GTH_LAUNCH_CHECK();
 
   // 6. links and geometry
   double *geom = geom_.get();
@@ -753,11 +765,16 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan
-  // allows (a deeper tree is reported by the depth in the frame)
-  const uint32_t mgrid = std::min<uint32_t>(blocks_for(cap), 1024u);
+  // allows (a deeper tree is reported by the depth in the frame).  Each level's
+  // grid is sized from the previous tree's level (the launches stride, so any
+  // grid is correct): most levels are small, and a full-size grid of idle
+  // workgroups per level made the moments launch-bound.
+  const uint32_t mgrid = std::min<uint32_t>(blocks_for(cap), kMomentBlocks);
   for (int d = p.depth_cap; d >= 0; --d) {
-    hipLaunchKernelGGL(level_moments, dim3(mgrid), dim3(kBlock), 0, s, d, lstart, frame, cap, lnk,
-                       tpos, tmass, geom);
+    const uint32_t blocks =
+        p.levels_sized && d < frame::kLevels ? std::min<uint32_t>(p.level_blocks[d], mgrid) : mgrid;
+    hipLaunchKernelGGL(level_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0, s, d,
+                       lstart, frame, cap, lnk, tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
   }
 }

@@ -156,7 +156,9 @@ constexpr int kExactTests = 20; // opening tests the FP32 filter left to the FP6
 constexpr int kCountsEnd = 21;  // [kNodes, kCountsEnd): cleared before each evaluation
 constexpr int kStepsDone = 30;  // leapfrog steps completed since the counter was cleared
 constexpr int kStepInteractions = 31;  // their cell + body interactions
-constexpr int kWords = 32;
+constexpr int kLevelNodes = 32;  // kLevels words: nodes on each tree level 0 .. kLevels - 1
+constexpr int kLevels = 61;
+constexpr int kWords = kLevelNodes + kLevels;
 }  // namespace frame
 
 constexpr uint32_t kAbortNodes = 1u;   // more nodes than Plan::node_cap
@@ -188,7 +190,12 @@ struct Plan {
   bool predicted = false;
   uint64_t slot_total_c = 0, slot_total_b = 0;
   uint32_t chunks = 1;  // walk ranges whose evaluation runs alongside the walk
+  // Workgroups of each level's moments launch, sized from the previous tree's
+  // level populations (levels_sized; else kMomentBlocks for every level).
+  bool levels_sized = false;
+  uint16_t level_blocks[frame::kLevels] = {};
 };
+constexpr uint16_t kMomentBlocks = 1024;
 
 // A device count that must not exceed cap; past it (an overflow the status
 // reports) the count reads as zero, so dependent kernels do nothing.

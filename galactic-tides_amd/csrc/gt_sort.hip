@@ -48,29 +48,65 @@ __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restr
   counts[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// The tile's keys are ranked (stably) and placed in LDS in digit order first,
+// then written out in that order: consecutive threads write consecutive
+// addresses of a digit's run instead of scattering one key per digit bucket.
 __global__ __launch_bounds__(kBlock) void tile_scatter(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, size_t cap,
     const unsigned long long *n_dev, int shift, uint32_t ntiles,
     const uint32_t *__restrict__ offsets, uint64_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out) {
   const size_t n = sort_count(cap, n_dev);
-  __shared__ uint32_t run[kDigits];             // keys of each digit placed so far in this tile
+  __shared__ uint64_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
+  __shared__ uint32_t run[kDigits];       // next tile-local slot of each digit
+  __shared__ uint32_t out_base[kDigits];  // global position of tile-local slot 0 of each digit
   __shared__ uint32_t wave_cnt[kWaves][kDigits];  // per-round, per-wave digit counts
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  run[threadIdx.x] = offsets[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x];
   const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
+  const uint32_t in_tile = base < n ? static_cast<uint32_t>(min(n - base, static_cast<size_t>(kTile))) : 0u;
+  // 1. the tile's digit counts, then their exclusive sum: each digit's first
+  // tile-local slot
+  run[threadIdx.x] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = k * kBlock + threadIdx.x;
+    if (i < in_tile) atomicAdd(&run[(keys[base + i] >> shift) & 0xff], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {  // one wave: four digits per lane
+    const uint32_t a0 = run[4 * lane], a1 = run[4 * lane + 1], a2 = run[4 * lane + 2],
+                   a3 = run[4 * lane + 3];
+    const uint32_t sum = a0 + a1 + a2 + a3;
+    uint32_t incl = sum;
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += u;
+    }
+    const uint32_t ex = incl - sum;
+    run[4 * lane] = ex;
+    run[4 * lane + 1] = ex + a0;
+    run[4 * lane + 2] = ex + a0 + a1;
+    run[4 * lane + 3] = ex + a0 + a1 + a2;
+  }
+  __syncthreads();
+  out_base[threadIdx.x] =
+      offsets[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] - run[threadIdx.x];
+  // 2. stable ranks: rounds of 256 consecutive keys, lanes of equal digit
+  // matched with wave ballots, waves in order
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   for (int round = 0; round < kItems; ++round) {
     for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
     __syncthreads();
-    const size_t i = base + round * kBlock + threadIdx.x;
-    const bool valid = i < n;
+    const uint32_t i = round * kBlock + threadIdx.x;
+    const bool valid = i < in_tile;
     uint64_t key = 0;
     uint32_t val = 0;
     uint32_t d = 0;
     if (valid) {
-      key = keys[i];
-      val = vals[i];
+      key = keys[base + i];
+      val = vals[base + i];
       d = static_cast<uint32_t>((key >> shift) & 0xff);
     }
     // Lanes of this wave holding the same digit.
@@ -87,14 +123,25 @@ __global__ __launch_bounds__(kBlock) void tile_scatter(
     if (valid) {
       uint32_t pos = run[d] + rank_in_wave;
       for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
-      keys_out[pos] = key;
-      vals_out[pos] = val;
+      skey[pos] = key;
+      sval[pos] = val;
     }
     __syncthreads();
     uint32_t add = 0;
     for (int w = 0; w < kWaves; ++w) add += wave_cnt[w][threadIdx.x];
     run[threadIdx.x] += add;
     __syncthreads();
+  }
+  // 3. out in digit order
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = k * kBlock + threadIdx.x;
+    if (i < in_tile) {
+      const uint64_t key = skey[i];
+      const uint32_t pos = out_base[(key >> shift) & 0xff] + i;
+      keys_out[pos] = key;
+      vals_out[pos] = sval[i];
+    }
   }
 }
 

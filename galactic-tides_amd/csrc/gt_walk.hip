@@ -425,12 +425,11 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
   // Each round descends one level or returns to a waiting entry: a well-formed
   // tree takes fewer rounds than twice its node count.
   uint32_t budget = 2 * nn + 2;
-  while (__any(!done)) {
-    if (done) continue;
-    if (budget-- == 0 || n > kOctet || fc >= nn || n > nn - fc || d > 63) {
+  for (;;) {
+    if (!done && (budget-- == 0 || n > kOctet || fc >= nn || n > nn - fc || d > 63))
       failed = done = true;
-      continue;
-    }
+    if (!__any(!done)) break;
+    if (done) continue;
     const bool act = k < n;
     WalkRec32 im{};
     uint2 bs{};
@@ -483,10 +482,10 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
       if (open > 0) {  // the nodes before it wait
         if (top >= levels_cap) {
           failed = done = true;
-          continue;
+        } else {
+          stack[top] = make_uint2(fc | (static_cast<uint32_t>(open) << 26), d);
+          ++top;
         }
-        stack[top] = make_uint2(fc | (static_cast<uint32_t>(open) << 26), d);
-        ++top;
       }
       fc = meta_first_child(open_meta);
       n = meta_children(open_meta);
