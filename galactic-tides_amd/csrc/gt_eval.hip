@@ -511,11 +511,16 @@ __global__ __launch_bounds__(kBlock) void item_keys(const unsigned long long *__
                         ((r.ce - r.cb) + (r.be - r.bb));
   // buckets > 0: cost classes of 1/buckets octave (items of a class keep
   // their tree order, which keeps neighbouring groups together); else the
-  // cost itself
-  const uint64_t cq =
-      buckets > 0 ? static_cast<uint64_t>(buckets * log2f(static_cast<float>(cost) + 1.0f))
-                  : min(cost >> 4, 0xffffffull);
-  keys[k] = (static_cast<uint64_t>(xcd) << 24) | (0xffffffull - cq);
+  // cost itself.  Classes fit 8 bits (costs stay below 2^38: 64 targets x
+  // 2^32 entries), so the keys of classes are 11 bits: two sort passes.
+  if (buckets > 0) {
+    const uint64_t cq = min(
+        static_cast<uint64_t>(buckets * log2f(static_cast<float>(cost) + 1.0f)), 0xffull);
+    keys[k] = (static_cast<uint64_t>(xcd) << 8) | (0xffull - cq);
+  } else {
+    const uint64_t cq = min(cost >> 4, 0xffffffull);
+    keys[k] = (static_cast<uint64_t>(xcd) << 24) | (0xffffffull - cq);
+  }
   order[k] = k;
 }
 
@@ -675,7 +680,8 @@ void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32
                        split.order);
     GTH_LAUNCH_CHECK();
     radix_sort_u64_dev(split.keys, split.order, split.keys_alt, split.order_alt,
-                       frame + frame::kItems, split.item_cap, 27, split.sort_tmp, s);
+                       frame + frame::kItems, split.item_cap, split.cost_buckets > 0 ? 11 : 27,
+                       split.sort_tmp, s);
   }
 }
 
