@@ -1404,6 +1404,91 @@ int gt_hip_group_compute(gt_hip_context *c, const gt_group_config *cfg, uint64_t
   });
 }
 
+// The tree phase's sizes into the history (not the lists': a tree solve has
+// none, and the lists history of the same particles stays valid).
+void learn_tree(gt_hip_context *c, const Plan &p) {
+  History &h = c->hist;
+  if (h.n != p.n || h.systems != p.systems) h = History{};
+  h.n = p.n;
+  h.systems = p.systems;
+  h.words = p.words;
+  h.nodes = static_cast<uint32_t>(c->fw[frame::kNodes]);
+  h.depth = static_cast<int>(c->fw[frame::kDepth]);
+  h.levels_known = true;
+  for (int d = 0; d < frame::kLevels; ++d)
+    h.level_nodes[d] = static_cast<uint32_t>(c->fw[frame::kLevelNodes + d]);
+}
+
+int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
+                        const double *pos, const double *mass, double *acc, double *pot,
+                        uint64_t *stats) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    check_config(cfg);
+    if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
+    c->tm = gt_timings{};
+    if (n == 0) {
+      if (stats) stats[0] = stats[1] = 0;
+      return;
+    }
+    require(pos && mass && acc && pot, "array is NULL");
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+    double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
+    unsigned long long *st_dev = c->misc.ensure(4);
+    upload(dp, pos, 3 * n, s);
+    upload(dm, mass, n, s);
+    c->systems = 1;
+    c->sys_off = nullptr;
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    unsigned long long *frame = c->frame.get();
+    Plan p;
+    for (int attempt = 0;; ++attempt) {
+      if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "tree solver: bounds did not settle");
+      p = plan_for(c, n32, cfg);
+      launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+      clear_counts(frame, s);
+      GTH_CHECK(hipEventRecord(c->ev[0], s));
+      c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
+      GTH_CHECK(hipEventRecord(c->ev[1], s));
+      read_frame(c);
+      const unsigned long long *fw = c->fw;
+      unsigned long long st = 0;
+      if (fw[frame::kNodes] > p.node_cap) st |= kAbortNodes;
+      if (fw[frame::kUnresolved] & 1ull) st |= kAbortPaths;
+      if (fw[frame::kDepth] > static_cast<unsigned long long>(p.depth_cap)) st |= kAbortDepth;
+      if (st == 0) break;
+      learn(c, p, st);
+      ++c->pstats.reruns;
+    }
+    learn_tree(c, p);
+    const uint32_t cap = p.node_cap;
+    launch_link_up(c->tree.links(), frame, cap, c->parent.ensure(cap), c->skip.ensure(cap),
+                   c->depth.ensure(cap), s);
+    launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
+                        c->tree.tree_mass(), c->tree.order(), n32, frame, cap, cfg->G, cfg->eps,
+                        cfg->theta, cfg->quadrupole != 0, da, dq, st_dev, s);
+    GTH_CHECK(hipEventRecord(c->ev[2], s));
+    download(acc, da, 3 * n, s);
+    download(pot, dq, n, s);
+    unsigned long long counts[2] = {0, 0};
+    download(counts, st_dev, 2, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+    if (stats) {
+      stats[0] = counts[0];
+      stats[1] = counts[1];
+    }
+    c->tree.set_counts(static_cast<uint32_t>(c->fw[frame::kNodes]),
+                       static_cast<int>(c->fw[frame::kDepth]));
+    c->lists_n = 0;  // (the last lists referred to the tree just replaced)
+    c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
+    c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
+    c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);
+    c->gpu_seconds = c->tm.total * 1e-3;
+  });
+}
+
 // Checks a batch's offsets (n_systems + 1 of them, from 0 to n, non-decreasing)
 // and puts them in device memory.
 void upload_systems(gt_hip_context *c, uint64_t n, uint64_t n_systems, const uint64_t *offsets,

@@ -331,6 +331,15 @@ void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
 void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_slots,
                         hipStream_t s);
 
+// gt_treegrav.hip: the per-particle FP64 tree solver over a built tree (one
+// system, root node 0): acc (N x 3) and pot in original order, stats[0..1] =
+// cell and particle interactions.
+void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *skip,
+                         const double *tpos, const double *tmass, const uint32_t *order,
+                         uint32_t n, const unsigned long long *frame, uint32_t node_cap, double G,
+                         double eps, double theta, int quadrupole, double *acc, double *pot,
+                         unsigned long long *stats, hipStream_t s);
+
 // gt_step.hip: each does nothing once status (a frame's kStatus word) is set.
 void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
                     double *pot, const unsigned long long *status, hipStream_t s);

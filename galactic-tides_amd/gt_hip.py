@@ -80,6 +80,8 @@ _tree = _sig("gt_hip_tree", C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp
                                       _vp, _vp, _P(Params), C.c_int, _vp])
 _group_compute = _sig("gt_hip_group_compute", C.c_int,
                       [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp])
+_tree_gravity = _sig("gt_hip_tree_gravity", C.c_int,
+                     [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
 _group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
                             [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
 _state_set_systems = _sig("gt_hip_state_set_systems", C.c_int, [_vp, C.c_uint64, _vp])
@@ -211,6 +213,19 @@ class Context:
         self._check(_group_compute(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
                                    _ptr(pot)))
         return acc, pot
+
+    def tree_gravity(self, pos, mass, **cfg):
+        """The per-particle FP64 tree solver: (acc, pot, [cells, particles])."""
+        n = len(mass)
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        st = np.zeros(2, np.uint64)
+        c = make_config(**cfg)
+        self._check(_tree_gravity(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
+                                  _ptr(pot), _ptr(st)))
+        return acc, pot, st
 
     def group_compute_batch(self, offsets, pos, mass, **cfg):
         """Independent systems: system s is particles offsets[s]:offsets[s+1]."""

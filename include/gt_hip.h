@@ -152,6 +152,16 @@ int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
  * back to particle order all run on the GPU.  pos/acc are n x 3 doubles. */
 int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                          const double *pos, const double *mass, double *acc, double *pot);
+/* TreeGravity::compute / compute_gravity_tree (include/galactic_tides/integrator.h:20-34,
+ * src/gravity.cpp:7-26, walk Octree::accel_pot src/octree.cpp:276-344), which the
+ * reference runs on the CPU only ("gravity = tree"): the per-particle FP64 tree
+ * walk, here on the GPU over the GPU-built tree, with the reference's arithmetic
+ * op for op, so acc/pot equal the reference's bit for bit.  Uses cfg->G, theta,
+ * eps, quadrupole and leaf_size.  stats (may be NULL): [0] cell interactions,
+ * [1] particle interactions (WalkStats, include/galactic_tides/octree.h:22-29). */
+int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
+                        const double *pos, const double *mass, double *acc, double *pot,
+                        uint64_t *stats);
 /* Independent systems in one evaluation (no counterpart in the reference,
  * which builds one GroupTreeGravity per system): system s is particles
  * offsets[s] .. offsets[s + 1] (n_systems + 1 offsets from 0 to n, non-decreasing;
