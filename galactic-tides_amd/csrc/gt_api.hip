@@ -147,6 +147,7 @@ struct gt_hip_context {
   const double *lists_geom = nullptr;  // geometry rows the last lists refer to
   // particle state (resident leapfrog) and inputs of single evaluations
   DBuf<double> pos, vel, mass, acc, pot;
+  DBuf<double> diag_partial, diag_total;  // diagnostics sums
   DBuf<double> in_pos, in_mass, in_acc, in_pot;
   uint64_t n_state = 0;
   bool state_forces = false;
@@ -1831,6 +1832,44 @@ int gt_hip_state_get(gt_hip_context *c, double *pos, double *vel, double *acc, d
     if (acc) download(acc, c->acc.get(), 3 * n, s);
     if (pot) download(pot, c->pot.get(), n, s);
     GTH_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int gt_hip_state_diagnostics(gt_hip_context *c, double time, uint64_t max_out,
+                             gt_diagnostics *out) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    const uint32_t systems = c->state_systems;
+    require(out != nullptr, "out is NULL");
+    require(max_out >= systems, "out holds fewer entries than the state has systems");
+    require(c->n_state == 0 || c->state_forces,
+            "potentials are not current: call gt_hip_prime first");
+    for (uint32_t k = 0; k < systems; ++k) {
+      out[k] = gt_diagnostics{};
+      out[k].time = time;
+    }
+    if (c->n_state == 0) return;
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    double *partial = c->diag_partial.ensure(diag_partial_words(systems));
+    double *total = c->diag_total.ensure(14 * static_cast<size_t>(systems));
+    launch_diagnostics(c->pos.get(), c->vel.get(), c->mass.get(), c->pot.get(),
+                       systems > 1 ? c->state_off.get() : nullptr, systems, c->n_state, partial,
+                       total, s);
+    std::vector<double> t(14 * static_cast<size_t>(systems));
+    download(t.data(), total, t.size(), s);
+    GTH_CHECK(hipStreamSynchronize(s));
+    for (uint32_t k = 0; k < systems; ++k) {
+      const double *v = t.data() + 14 * static_cast<size_t>(k);
+      gt_diagnostics &d = out[k];
+      d.kinetic = v[0];
+      d.potential = v[1];
+      for (int a = 0; a < 3; ++a) {
+        d.momentum[a] = v[2 + a];
+        d.angular_momentum[a] = v[5 + a];
+        d.centre_of_mass[a] = v[11] > 0 ? v[8 + a] / v[11] : 0.0;
+      }
+    }
   });
 }
 

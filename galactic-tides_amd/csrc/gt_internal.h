@@ -350,5 +350,12 @@ void launch_kick(double *vel, const double *acc, size_t n, double h,
 // Unless frame[kStatus] is set: frame[kStepsDone] += 1 and the step's
 // interactions are added to frame[kStepInteractions].
 void launch_count_step(unsigned long long *frame, hipStream_t s);
+// Per system (offsets: systems + 1 particle offsets, or null for one system
+// of n): 14 sums each into total (K, W, P, L, sum m x, M, 2 unused); partial
+// holds diag_partial_words(systems) doubles.
+size_t diag_partial_words(uint32_t systems);
+void launch_diagnostics(const double *pos, const double *vel, const double *mass,
+                        const double *pot, const unsigned long long *offsets, uint32_t systems,
+                        uint64_t n, double *partial, double *total, hipStream_t s);
 
 }  // namespace gth

@@ -54,6 +54,68 @@ __global__ __launch_bounds__(kBlock) void kick(double *__restrict__ vel,
 
 #pragma clang fp contract(fast)
 
+// Diagnostics (reference: compute_diagnostics, src/integrator.cpp:18-33): per
+// particle 0.5 m |v|^2, 0.5 m phi, m v, m (x cross v), m x and m, summed over
+// each system.  Block (j, s) sums the j-th of kDiagChunks equal pieces of
+// system s (threads stride the piece, then a fixed-shape tree over the
+// block); diag_total adds the pieces in order.  The sums are deterministic,
+// though not in the reference's sequential order.
+constexpr int kDiagChunks = 256;
+constexpr int kDiagTerms = 14;  // K, W, P[3], L[3], M x[3], M, (2 unused)
+
+__global__ __launch_bounds__(kBlock) void diag_partial(
+    const double *__restrict__ pos, const double *__restrict__ vel,
+    const double *__restrict__ mass, const double *__restrict__ pot,
+    const unsigned long long *__restrict__ offsets, uint64_t n, double *__restrict__ partial) {
+  __shared__ double red[kBlock];
+  const uint32_t s = blockIdx.y, j = blockIdx.x;
+  const uint64_t a = offsets ? offsets[s] : 0, b = offsets ? offsets[s + 1] : n;
+  const uint64_t len = b > a ? b - a : 0;
+  const uint64_t lo = a + len * j / kDiagChunks, hi = a + len * (j + 1) / kDiagChunks;
+  double t[kDiagTerms];
+  for (int q = 0; q < kDiagTerms; ++q) t[q] = 0.0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double m = mass[i];
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+    t[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    t[1] += 0.5 * m * pot[i];
+    t[2] += m * vx;
+    t[3] += m * vy;
+    t[4] += m * vz;
+    t[5] += m * (y * vz - z * vy);
+    t[6] += m * (z * vx - x * vz);
+    t[7] += m * (x * vy - y * vx);
+    t[8] += m * x;
+    t[9] += m * y;
+    t[10] += m * z;
+    t[11] += m;
+  }
+  double *out = partial + (static_cast<size_t>(s) * kDiagChunks + j) * kDiagTerms;
+  for (int q = 0; q < kDiagTerms; ++q) {
+    red[threadIdx.x] = t[q];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void diag_total(const double *__restrict__ partial, uint32_t systems,
+                           double *__restrict__ total) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= systems) return;
+  for (int q = 0; q < kDiagTerms; ++q) {
+    double v = 0.0;
+    for (int j = 0; j < kDiagChunks; ++j)
+      v += partial[(static_cast<size_t>(s) * kDiagChunks + j) * kDiagTerms + q];
+    total[static_cast<size_t>(s) * kDiagTerms + q] = v;
+  }
+}
+
 __global__ void count_step(unsigned long long *frame) {
   if (frame[frame::kStatus] != 0) return;
   frame[frame::kStepsDone] += 1;
@@ -83,6 +145,22 @@ void launch_kick(double *vel, const double *acc, size_t n, double h,
   if (!n) return;
   hipLaunchKernelGGL(kick, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, vel, acc, 3 * n, h,
                      status);
+  GTH_LAUNCH_CHECK();
+}
+
+size_t diag_partial_words(uint32_t systems) {
+  return static_cast<size_t>(systems) * kDiagChunks * kDiagTerms;
+}
+
+void launch_diagnostics(const double *pos, const double *vel, const double *mass,
+                        const double *pot, const unsigned long long *offsets, uint32_t systems,
+                        uint64_t n, double *partial, double *total, hipStream_t s) {
+  if (!systems) return;
+  hipLaunchKernelGGL(diag_partial, dim3(kDiagChunks, systems), dim3(kBlock), 0, s, pos, vel, mass,
+                     pot, offsets, n, partial);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(diag_total, dim3((systems + 63) / 64), dim3(64), 0, s, partial, systems,
+                     total);
   GTH_LAUNCH_CHECK();
 }
 

@@ -80,6 +80,14 @@ _tree = _sig("gt_hip_tree", C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp
                                       _vp, _vp, _P(Params), C.c_int, _vp])
 _group_compute = _sig("gt_hip_group_compute", C.c_int,
                       [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp])
+class Diagnostics(C.Structure):
+    _fields_ = [("time", C.c_double), ("kinetic", C.c_double), ("potential", C.c_double),
+                ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
+                ("centre_of_mass", C.c_double * 3)]
+
+
+_state_diagnostics = _sig("gt_hip_state_diagnostics", C.c_int,
+                          [_vp, C.c_double, C.c_uint64, _vp])
 _tree_gravity = _sig("gt_hip_tree_gravity", C.c_int,
                      [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
 _group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
@@ -213,6 +221,14 @@ class Context:
         self._check(_group_compute(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
                                    _ptr(pot)))
         return acc, pot
+
+    def state_diagnostics(self, time=0.0, systems=1):
+        """compute_diagnostics of the resident state: one dict per system."""
+        out = (Diagnostics * max(systems, 1))()
+        self._check(_state_diagnostics(self._h, time, max(systems, 1), C.cast(out, _vp)))
+        return [{"time": d.time, "kinetic": d.kinetic, "potential": d.potential,
+                 "momentum": list(d.momentum), "angular_momentum": list(d.angular_momentum),
+                 "centre_of_mass": list(d.centre_of_mass)} for d in out]
 
     def tree_gravity(self, pos, mass, **cfg):
         """The per-particle FP64 tree solver: (acc, pot, [cells, particles])."""

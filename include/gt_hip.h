@@ -220,6 +220,25 @@ int gt_hip_state_set(gt_hip_context *ctx, uint64_t n, const double *pos, const d
 int gt_hip_state_set_systems(gt_hip_context *ctx, uint64_t n_systems, const uint64_t *offsets);
 /* Download; any pointer may be NULL. */
 int gt_hip_state_get(gt_hip_context *ctx, double *pos, double *vel, double *acc, double *pot);
+
+/* Diagnostics (include/galactic_tides/integrator.h:49-58): total energy is
+ * kinetic + potential, the virial ratio -2 kinetic / potential. */
+typedef struct {
+  double time;
+  double kinetic;              /* sum 0.5 m |v|^2 */
+  double potential;            /* sum 0.5 m phi */
+  double momentum[3];          /* sum m v */
+  double angular_momentum[3];  /* sum m (x cross v) */
+  double centre_of_mass[3];    /* sum m x / sum m (0 when the mass is 0) */
+} gt_diagnostics;
+/* compute_diagnostics (src/integrator.cpp:18-33) on the resident state, one
+ * entry per system (max_out must be at least the number of systems; the run
+ * loop of apps/galactic_tides_run.cpp:65-112 logs these every few steps).
+ * Uses the potentials, so forces must be current (after gt_hip_prime).  Sums
+ * run in a fixed parallel order: deterministic, equal to the reference's
+ * sequential sums to FP64 rounding. */
+int gt_hip_state_diagnostics(gt_hip_context *ctx, double time, uint64_t max_out,
+                             gt_diagnostics *out);
 /* LeapfrogKDK::prime (integrator.h:104): forces for the resident state. */
 int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
 /* nsteps x LeapfrogKDK::step (src/integrator.cpp:5-16) on the resident state:
