@@ -511,12 +511,12 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   }
 }
 
-// 3: one walk per lane over child sets (the default), 2: one walk per lane
-// one record per step, 1: the same fetching the next records ahead,
-// 16/32/64: coherent walks.
+// 3: eight lanes per group over child sets (the default), 4: four lanes per
+// group, 2: one walk per lane one record per step, 1: the same fetching the
+// next records ahead, 16/32/64: coherent walks.
 int walk_lanes(const gt_group_config *cfg, const Plan &p) {
   const int l = cfg->walk_lanes;
-  const int per_lane = l == 1 || l == 2 ? l : 3;
+  const int per_lane = l == 1 || l == 2 || l == 4 ? l : 3;
   if (p.systems > 1 || p.predicted) return per_lane;  // a coherent walk shares one tree
   return l == 16 || l == 32 || l == 64 ? l : per_lane;
 }
@@ -1583,7 +1583,7 @@ int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, c
       }
     };
     if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
-    sweep(&gt_group_config::walk_lanes, {2, 16, 32, 64});
+    sweep(&gt_group_config::walk_lanes, {2, 4, 16, 32, 64});
     if (!cfg->exact) {
       sweep(&gt_group_config::eval_waves, {2, 3});
       sweep(&gt_group_config::eval_order, {1, 2, 3, 4});

@@ -312,9 +312,10 @@ void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
 // its recorded ranges are cut at its slots.  Any status set before the walk
 // (kAbortSlots) stops it.  Groups g_begin .. g_begin + g_span only (g_span 0:
 // all); coherent walks (lanes > 2) take all groups.
-// lanes: 3 the walk over child sets (images, levels_cap levels of stack per
-// lane; the default), 2 one record per step, 1 the same fetching the next
-// records ahead, 16/32/64 coherent walks.
+// lanes: 3 (8 lanes per group) or 4 (4 lanes per group) the walk over child
+// sets (images, levels_cap levels of stack per group; 3 is the default), 2 one
+// record per step, 1 the same fetching the next records ahead, 16/32/64
+// coherent walks.
 void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *images,
                  uint32_t levels_cap, const uint2 *span, const double *tpos,
                  gt_group *groups, unsigned long long *frame, uint32_t group_cap,

@@ -345,10 +345,13 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
 // set instead of one node, and eight lanes per group give the GPU eight times
 // the waves of a walk with one lane per group to hide memory latency with.
 constexpr int kCellCode = 0, kBodiesCode = 1, kOpenCode = 2;
-constexpr uint32_t kOctet = 8;
-constexpr uint32_t kOctets = kWalkBlock / kOctet;  // groups per block (one wave)
+constexpr uint32_t kSetMax = 8;  // children of a node at most
 
-template <int MODE>
+// GL lanes per group (8 or 4): lane k of a group tests children k, k + GL, ...
+// of the current set.  With 4, a wave serves 16 groups and each lane tests two
+// children; the bookkeeping, which every lane of a group repeats, costs half
+// as much per group.
+template <int MODE, int GL>
 __global__ __launch_bounds__(kWalkBlock) void walk_sets(
     const WalkRec *__restrict__ recs, const WalkRec32 *__restrict__ images, uint32_t levels_cap,
     const uint2 *__restrict__ span, const double *__restrict__ tpos,
@@ -359,18 +362,21 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
     uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
     uint32_t *__restrict__ body_list, const uint32_t *__restrict__ slot_c,
     const uint32_t *__restrict__ slot_b, uint32_t g_begin, uint32_t g_span) {
-  extern __shared__ uint2 levels[];  // [kOctets][levels_cap]
+  constexpr uint32_t kPer = kSetMax / GL;            // children per lane
+  constexpr uint32_t kGroups = kWalkBlock / GL;      // groups per block (one wave)
+  constexpr uint32_t kLaneMask = (1u << GL) - 1u;
+  extern __shared__ uint2 levels[];  // [kGroups][levels_cap]
   if (frame[frame::kStatus]) return;  // the slots do not fit the buffers
   // groups g_begin .. g_begin + g_span of the first frame[kGroups]
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t count = ng > g_begin ? min(ng - g_begin, g_span) : 0u;
-  const uint32_t blk = xcd_block_used(blockIdx.x, (count + kOctets - 1) / kOctets);
+  const uint32_t blk = xcd_block_used(blockIdx.x, (count + kGroups - 1) / kGroups);
   if (blk == kNone) return;
-  const uint32_t k = threadIdx.x % kOctet, oct = threadIdx.x / kOctet;
-  const uint32_t g = g_begin + blk * kOctets + oct;
-  const bool valid = g < g_begin + count;  // (the same for the octet's eight lanes)
-  uint2 *stack = levels + oct * levels_cap;
+  const uint32_t k = threadIdx.x % GL, grp = threadIdx.x / GL;
+  const uint32_t g = g_begin + blk * kGroups + grp;
+  const bool valid = g < g_begin + count;  // (the same for the group's lanes)
+  uint2 *stack = levels + grp * levels_cap;
   uint32_t gb = 0, ge = 0, start = 0;
   if (valid) {
     gb = groups[g].begin;
@@ -380,18 +386,18 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
   }
   const double *origin = geom + static_cast<size_t>(kGeomStride) * (start < nn ? start : 0u);
   const double root_half = origin[3];
-  // group box: the lanes take every eighth body; min and max do not depend
-  // on the order they are taken in
+  // group box: the lanes take every GL-th body; min and max do not depend on
+  // the order they are taken in
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t j = gb + k; j < ge; j += kOctet)
+  for (uint32_t j = gb + k; j < ge; j += GL)
     for (int a = 0; a < 3; ++a) {
       const double v = tpos[3 * static_cast<size_t>(j) + a];
       lo[a] = (v < lo[a]) ? v : lo[a];  // std::min / std::max semantics
       hi[a] = (hi[a] < v) ? v : hi[a];
     }
-  for (int off = 4; off > 0; off >>= 1)
+  for (int off = GL / 2; off > 0; off >>= 1)
     for (int a = 0; a < 3; ++a) {
-      const double l = __shfl_xor(lo[a], off, kOctet), h = __shfl_xor(hi[a], off, kOctet);
+      const double l = __shfl_xor(lo[a], off, GL), h = __shfl_xor(hi[a], off, GL);
       lo[a] = (l < lo[a]) ? l : lo[a];
       hi[a] = (hi[a] < h) ? h : hi[a];
     }
@@ -425,59 +431,88 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
   // Each round descends one level or returns to a waiting entry: a well-formed
   // tree takes fewer rounds than twice its node count.
   uint32_t budget = 2 * nn + 2;
+  const uint32_t shift = threadIdx.x & ~(GL - 1u);
   for (;;) {
-    if (!done && (budget-- == 0 || n > kOctet || fc >= nn || n > nn - fc || d > 63))
+    if (!done && (budget-- == 0 || n > kSetMax || fc >= nn || n > nn - fc || d > 63))
       failed = done = true;
     if (!__any(!done)) break;
     if (done) continue;
-    const bool act = k < n;
-    WalkRec32 im{};
-    uint2 bs{};
-    int code = -1;
-    if (act) {
-      im = images[fc + k];
-      bs = span[fc + k];
-      int dec = gim.usable ? accept_filter(im, half_image(root_half32, static_cast<int>(d)), gim)
-                           : kUndecided;
-      if (dec == kUndecided) {
-        const WalkRec x = recs[fc + k];
-        dec = accept_exact(x.cx, x.cy, x.cz, x.mx, x.my, x.mz,
-                           ldexp(root_half, -static_cast<int>(d)), x.crit, gcx, gcy, gcz, ghx,
-                           ghy, ghz);
-        ++exact_tests;
+    WalkRec32 im[kPer];
+    uint2 bs[kPer];
+    int code[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t c = k + GL * r;
+      im[r] = WalkRec32{};
+      bs[r] = uint2{};
+      if (c < n) {
+        im[r] = images[fc + c];
+        bs[r] = span[fc + c];
       }
-      const bool small = signbit(im.crit), leaf = (im.meta & kMetaLeaf) != 0u;
-      code = (dec == 1 && !small) ? kCellCode : (dec == 1 || leaf) ? kBodiesCode : kOpenCode;
     }
-    const uint32_t shift = threadIdx.x & ~(kOctet - 1);
-    const uint32_t open_bits = static_cast<uint32_t>(__ballot(code == kOpenCode) >> shift) & 0xffu;
-    const uint32_t cell_bits = static_cast<uint32_t>(__ballot(code == kCellCode) >> shift) & 0xffu;
+    uint32_t open_bits = 0, cell_bits = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t c = k + GL * r;
+      code[r] = -1;
+      if (c < n) {
+        int dec = gim.usable
+                      ? accept_filter(im[r], half_image(root_half32, static_cast<int>(d)), gim)
+                      : kUndecided;
+        if (dec == kUndecided) {
+          const WalkRec x = recs[fc + c];
+          dec = accept_exact(x.cx, x.cy, x.cz, x.mx, x.my, x.mz,
+                             ldexp(root_half, -static_cast<int>(d)), x.crit, gcx, gcy, gcz, ghx,
+                             ghy, ghz);
+          ++exact_tests;
+        }
+        const bool small = signbit(im[r].crit), leaf = (im[r].meta & kMetaLeaf) != 0u;
+        code[r] = (dec == 1 && !small) ? kCellCode : (dec == 1 || leaf) ? kBodiesCode : kOpenCode;
+      }
+      open_bits |= (static_cast<uint32_t>(__ballot(code[r] == kOpenCode) >> shift) & kLaneMask)
+                   << (GL * r);
+      cell_bits |= (static_cast<uint32_t>(__ballot(code[r] == kCellCode) >> shift) & kLaneMask)
+                   << (GL * r);
+    }
     // the last node (in child order) that opens; the nodes after it go out now
     const int open = open_bits ? 31 - __clz(static_cast<int>(open_bits)) : -1;
     const uint32_t emit = (0xffu << (open + 1)) & 0xffu;
-    const bool mine = act && ((emit >> k) & 1u);
-    const uint32_t after_me = emit & (0xfeu << k);  // emitted before this node
-    if (mine && code == kCellCode) {
-      const uint32_t at = nc + __popc(cell_bits & after_me);
-      if (MODE == kWalkExact || at < cap_c) cell_list[cpos + at] = fc + k;
+    // bodies: per slot, this lane's and the higher lanes' counts
+    uint32_t cnt[kPer], suffix[kPer], slot_total[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t c = k + GL * r;
+      const bool mine = c < n && ((emit >> c) & 1u);
+      cnt[r] = (mine && code[r] == kBodiesCode) ? bs[r].y - bs[r].x : 0u;
+      suffix[r] = cnt[r];
+      for (uint32_t off = 1; off < GL; off <<= 1) {
+        const uint32_t u = __shfl_down(suffix[r], off, GL);
+        if (k + off < GL) suffix[r] += u;
+      }
+      slot_total[r] = __shfl(suffix[r], 0, GL);
+      if (mine && code[r] == kCellCode) {
+        const uint32_t at = nc + __popc(cell_bits & emit & (0xfeu << c));
+        if (MODE == kWalkExact || at < cap_c) cell_list[cpos + at] = fc + c;
+      }
+    }
+    uint32_t higher_slots = 0;  // bodies of the slots above
+#pragma unroll
+    for (int r = static_cast<int>(kPer) - 1; r >= 0; --r) {
+      const uint32_t first = nb + higher_slots + (suffix[r] - cnt[r]);
+      for (uint32_t j = 0; j < cnt[r]; ++j) {
+        const uint32_t at = first + j;
+        if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = bs[r].x + j;
+      }
+      higher_slots += slot_total[r];
     }
     nc += __popc(cell_bits & emit);
-    // bodies: each lane's range, after those of the lanes above it
-    const uint32_t cnt = (mine && code == kBodiesCode) ? bs.y - bs.x : 0u;
-    uint32_t suffix = cnt;  // this lane's and the higher lanes' counts
-    for (uint32_t off = 1; off < kOctet; off <<= 1) {
-      const uint32_t u = __shfl_down(suffix, off, kOctet);
-      if (k + off < kOctet) suffix += u;
-    }
-    const uint32_t total = __shfl(suffix, 0, kOctet);
-    const uint32_t first = nb + (suffix - cnt);
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t at = first + j;
-      if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = bs.x + j;
-    }
-    nb += total;
+    nb += higher_slots;
     // the next set
-    const uint32_t open_meta = __shfl(im.meta, open < 0 ? 0 : open, kOctet);
+    const uint32_t open_slot = open < 0 ? 0u : static_cast<uint32_t>(open) / GL;
+    uint32_t meta_sel = im[0].meta;
+#pragma unroll
+    for (uint32_t r = 1; r < kPer; ++r) meta_sel = open_slot == r ? im[r].meta : meta_sel;
+    const uint32_t open_meta = __shfl(meta_sel, open < 0 ? 0 : open % GL, GL);
     if (open >= 0) {
       if (open > 0) {  // the nodes before it wait
         if (top >= levels_cap) {
@@ -891,18 +926,23 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
   recs, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, cap_c, cap_b, \
       cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c, slot_b, g_begin, \
       g_span
-  if (lanes == 3) {  // eight lanes per group, over child sets
+  if (lanes == 3 || lanes == 4) {  // 8 (lanes 3) or 4 (lanes 4) lanes per group, over child sets
     if (levels_cap == 0 || levels_cap > 64) fail(GT_HIP_ERR_ARGUMENT, "walk: levels_cap out of range");
-    const dim3 grid(xcd_grid(blocks_for(g_span, kOctets))), block(kWalkBlock);
-    const size_t lds = static_cast<size_t>(levels_cap) * kOctets * sizeof(uint2);
+    const uint32_t gl = lanes == 3 ? 8u : 4u, per_block = kWalkBlock / gl;
+    const dim3 grid(xcd_grid(blocks_for(g_span, per_block))), block(kWalkBlock);
+    const size_t lds = static_cast<size_t>(levels_cap) * per_block * sizeof(uint2);
 #define GTH_SETS_ARGS                                                                        \
   recs, images, levels_cap, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, \
       cap_c, cap_b, cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c,  \
       slot_b, g_begin, g_span
-    if (mode == kWalkExact)
-      hipLaunchKernelGGL((walk_sets<kWalkExact>), grid, block, lds, s, GTH_SETS_ARGS);
+    if (mode == kWalkExact && gl == 8)
+      hipLaunchKernelGGL((walk_sets<kWalkExact, 8>), grid, block, lds, s, GTH_SETS_ARGS);
+    else if (mode == kWalkExact)
+      hipLaunchKernelGGL((walk_sets<kWalkExact, 4>), grid, block, lds, s, GTH_SETS_ARGS);
+    else if (gl == 8)
+      hipLaunchKernelGGL((walk_sets<kWalkCapacity, 8>), grid, block, lds, s, GTH_SETS_ARGS);
     else
-      hipLaunchKernelGGL((walk_sets<kWalkCapacity>), grid, block, lds, s, GTH_SETS_ARGS);
+      hipLaunchKernelGGL((walk_sets<kWalkCapacity, 4>), grid, block, lds, s, GTH_SETS_ARGS);
 #undef GTH_SETS_ARGS
     GTH_LAUNCH_CHECK();
     return;

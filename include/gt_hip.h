@@ -70,11 +70,12 @@ typedef struct {
                         forces bit-identical to evaluate_group_lists_cpu */
   int eval_waves;    /* tuning: waves per SIMD the list-evaluation kernel is built
                         for (2 or 3; 0 = library default, 3) */
-  int walk_lanes;    /* tuning: 0 or 3 (default): one walk per lane that tests a node's
+  int walk_lanes;    /* tuning: 0 or 3 (default): eight lanes per group test a node's
                         children together (one round of loads per child set, FP32 node
-                        images with an exact FP64 fallback); 2: one walk per lane, one
-                        node record per step; 1: the same, fetching both possible next
-                        nodes' records ahead; 16, 32 or 64: that many groups share one
+                        images with an exact FP64 fallback); 4: the same with four lanes
+                        per group, two children each; 2: one walk per lane, one node
+                        record per step; 1: the same, fetching both possible next nodes'
+                        records ahead; 16, 32 or 64: that many groups share one
                         wave-coherent walk.  Every choice gives the same lists */
   int split_len;     /* tuning: lists longer than this many entries are evaluated in
                         parts by several waves (fast evaluation only; parts added in a
@@ -174,8 +175,8 @@ int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
                                const double *mass, double *acc, double *pot);
 /* Autotuner (no counterpart in the reference): times force evaluations of the
  * given particles on the GPU (HIP events, best of 2 after a warm-up) while it
- * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {2, 16,
- * 32, 64}, eval_order {1, 2, 3, 4}, split_len {2048, 8192, never}, and group_size
+ * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {2, 4,
+ * 16, 32, 64}, eval_order {1, 2, 3, 4}, split_len {2048, 8192, never}, and group_size
  * {32, 64} when search_group_size is 1), keeping each change that is at least 1%
  * faster,
  * and writes the fastest settings into *best (the other fields copied from

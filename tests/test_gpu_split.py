@@ -80,7 +80,7 @@ def test_autotune_picks_valid_settings(hip, oracle):
     best, ms = hip.autotune(pos, mass, **cfg)
     print("autotune", best, ms)
     assert ms > 0 and best["group_size"] == 32
-    assert best["eval_waves"] in (0, 2, 3) and best["walk_lanes"] in (0, 2, 16, 32, 64)
+    assert best["eval_waves"] in (0, 2, 3) and best["walk_lanes"] in (0, 2, 4, 16, 32, 64)
     assert best["eval_order"] in (0, 1, 2, 3, 4) and best["split_len"] in (0, 2048, 8192, -1)
     a0, _ = hip.group_compute(pos, mass, **cfg)
     a1, _ = hip.group_compute(pos, mass, **{**cfg, **best})

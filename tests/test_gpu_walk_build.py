@@ -114,7 +114,7 @@ def test_group_size_limit(hip, oracle):
         hip.group_compute(pos, mass, group_size=65)
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 2, 3, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 3, 4, 16, 32, 64])
 @pytest.mark.parametrize("gs,pack,theta", [(32, 1, 0.75), (7, 0, 0.5), (64, 1, 1.0)])
 def test_walk_variants_give_identical_lists(hip, oracle, lanes, gs, pack, theta):
     # one independent walk per lane, or 16/32/64 groups sharing one traversal
