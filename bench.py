@@ -203,7 +203,7 @@ def main():
     tm = ctx.timings()  # per-step means over the timed steps (HIP events)
     interactions = (st1["interactions"] - st0["interactions"]) / args.steps
     value = interactions / (ms_per_step * 1e-3)
-    pipeline = {k: st1[k] - st0[k] for k in st1}
+    pipeline = {k: st1[k] if k.startswith("last_") else st1[k] - st0[k] for k in st1}
     print(f"# step {ms_per_step:.2f} ms, force eval (GPU events) {tm['total']:.2f} ms: "
           f"tree {tm['tree']:.2f} lists {tm['lists']:.2f} evaluate {tm['evaluate']:.2f} "
           f"scatter+kick {tm['scatter']:.2f}; {inf['n_groups']} groups, {inf['n_nodes']} nodes, "

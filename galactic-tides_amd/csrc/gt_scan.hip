@@ -1,6 +1,10 @@
-// Exclusive prefix sums, three passes: per-tile totals, a scan of the tile
-// totals in one workgroup, then each tile scanned with its offset.  Sums are
-// formed in 64 bits so callers can detect results that no longer fit 32 bits.
+// Exclusive prefix sums: per-tile totals, then each tile scanned with its
+// offset.  Up to kSummedTiles tiles each workgroup adds its offset up from the
+// totals of the tiles before it (a few loads per thread, from L2) and the last
+// tile writes the grand total: two passes, one for a single tile.  Beyond
+// that a single workgroup scans the tile totals in a pass of its own, so the
+// offset sums do not grow with the square of the tile count.  Sums are formed
+// in 64 bits so callers can detect results that no longer fit 32 bits.
 #include "gt_internal.h"
 
 namespace gth {
@@ -10,6 +14,7 @@ constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;
 constexpr int kWave = 64;
+constexpr size_t kSummedTiles = 1024;
 
 // Inclusive scan of one value per thread across the workgroup.
 __device__ inline unsigned long long block_inclusive_scan(unsigned long long v,
@@ -61,9 +66,25 @@ __global__ __launch_bounds__(kBlock) void scan_totals(unsigned long long *__rest
   if (threadIdx.x == 0) *total = carry;
 }
 
+// kSum: tot holds the tiles' totals and the workgroup sums those before it
+// (the last tile writes the grand total); otherwise tot holds the offsets.
+template <bool kSum>
 __global__ __launch_bounds__(kBlock) void tile_scan(const uint32_t *in, uint32_t *out, size_t n,
-                                                    const unsigned long long *__restrict__ tot) {
+                                                    const unsigned long long *__restrict__ tot,
+                                                    unsigned long long *__restrict__ total) {
   __shared__ unsigned long long wt[kBlock / kWave];
+  __shared__ unsigned long long offset_s;
+  unsigned long long offset;
+  if constexpr (kSum) {
+    unsigned long long part = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) part += tot[t];
+    part = block_inclusive_scan(part, wt);
+    if (threadIdx.x == kBlock - 1) offset_s = part;
+    __syncthreads();
+    offset = offset_s;
+  } else {
+    offset = tot[blockIdx.x];
+  }
   const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
   uint32_t v[kItems];
   unsigned long long s = 0;
@@ -73,12 +94,13 @@ __global__ __launch_bounds__(kBlock) void tile_scan(const uint32_t *in, uint32_t
     s += v[k];
   }
   const unsigned long long inc = block_inclusive_scan(s, wt);
-  unsigned long long run = tot[blockIdx.x] + inc - s;
+  unsigned long long run = offset + inc - s;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (base + k < n) out[base + k] = static_cast<uint32_t>(run);
     run += v[k];
   }
+  if (kSum && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total = offset + inc;
 }
 
 __global__ void zero_total(unsigned long long *total) { *total = 0ull; }
@@ -95,13 +117,21 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned lo
     GTH_LAUNCH_CHECK();
     return;
   }
-  hipLaunchKernelGGL(tile_totals, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in, n,
-                     tmp);
-  GTH_LAUNCH_CHECK();
+  if (ntiles > 1) {
+    hipLaunchKernelGGL(tile_totals, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in,
+                       n, tmp);
+    GTH_LAUNCH_CHECK();
+  }
+  if (ntiles <= kSummedTiles) {
+    hipLaunchKernelGGL(tile_scan<true>, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
+                       in, out, n, tmp, total);
+    GTH_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL(scan_totals, dim3(1), dim3(kBlock), 0, s, tmp, ntiles, total);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(tile_scan, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in, out,
-                     n, tmp);
+  hipLaunchKernelGGL(tile_scan<false>, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in,
+                     out, n, tmp, total);
   GTH_LAUNCH_CHECK();
 }
 
