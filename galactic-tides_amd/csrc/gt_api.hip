@@ -169,6 +169,11 @@ struct gt_hip_context {
   DBuf<gt_group> raw_groups;
   DBuf<uint64_t> sample_idx;
   DBuf<double> sample_acc;
+  // neighbour queries
+  DBuf<double> nb_centres, nb_radii, nb_dist2;
+  DBuf<uint32_t> nb_counts, nb_offs, nb_perm, nb_perm_alt;
+  DBuf<uint64_t> nb_keys, nb_keys_alt;
+  DBuf<unsigned long long> nb_index, nb_tmp, nb_sort_tmp;
 };
 
 namespace {
@@ -1225,6 +1230,59 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
   }
 }
 
+// The tree phase's sizes into the history (not the lists': a tree solve has
+// none, and the lists history of the same particles stays valid).
+void learn_tree(gt_hip_context *c, const Plan &p) {
+  History &h = c->hist;
+  if (h.n != p.n || h.systems != p.systems) h = History{};
+  h.n = p.n;
+  h.systems = p.systems;
+  h.words = p.words;
+  h.nodes = static_cast<uint32_t>(c->fw[frame::kNodes]);
+  h.depth = static_cast<int>(c->fw[frame::kDepth]);
+  h.levels_known = true;
+  for (int d = 0; d < frame::kLevels; ++d)
+    h.level_nodes[d] = static_cast<uint32_t>(c->fw[frame::kLevelNodes + d]);
+}
+
+// The octree of n particles (one system) from device arrays, with its parent
+// and skip links: the tree alone, sized again until it fits its bounds.  The
+// last lists no longer refer to it.
+Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n,
+                       const double *dp, const double *dm) {
+  hipStream_t s = c->stream;
+  c->systems = 1;
+  c->sys_off = nullptr;
+  unsigned long long *frame = c->frame.get();
+  Plan p;
+  for (int attempt = 0;; ++attempt) {
+    if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "tree build: bounds did not settle");
+    p = plan_for(c, n, cfg);
+    launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
+    clear_counts(frame, s);
+    GTH_CHECK(hipEventRecord(c->ev[0], s));
+    c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
+    GTH_CHECK(hipEventRecord(c->ev[1], s));
+    read_frame(c);
+    const unsigned long long *fw = c->fw;
+    unsigned long long st = 0;
+    if (fw[frame::kNodes] > p.node_cap) st |= kAbortNodes;
+    if (fw[frame::kUnresolved] & 1ull) st |= kAbortPaths;
+    if (fw[frame::kDepth] > static_cast<unsigned long long>(p.depth_cap)) st |= kAbortDepth;
+    if (st == 0) break;
+    learn(c, p, st);
+    ++c->pstats.reruns;
+  }
+  learn_tree(c, p);
+  const uint32_t cap = p.node_cap;
+  launch_link_up(c->tree.links(), frame, cap, c->parent.ensure(cap), c->skip.ensure(cap),
+                 c->depth.ensure(cap), s);
+  c->tree.set_counts(static_cast<uint32_t>(c->fw[frame::kNodes]),
+                     static_cast<int>(c->fw[frame::kDepth]));
+  c->lists_n = 0;  // (the last lists referred to the tree just replaced)
+  return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1405,21 +1463,6 @@ int gt_hip_group_compute(gt_hip_context *c, const gt_group_config *cfg, uint64_t
   });
 }
 
-// The tree phase's sizes into the history (not the lists': a tree solve has
-// none, and the lists history of the same particles stays valid).
-void learn_tree(gt_hip_context *c, const Plan &p) {
-  History &h = c->hist;
-  if (h.n != p.n || h.systems != p.systems) h = History{};
-  h.n = p.n;
-  h.systems = p.systems;
-  h.words = p.words;
-  h.nodes = static_cast<uint32_t>(c->fw[frame::kNodes]);
-  h.depth = static_cast<int>(c->fw[frame::kDepth]);
-  h.levels_known = true;
-  for (int d = 0; d < frame::kLevels; ++d)
-    h.level_nodes[d] = static_cast<uint32_t>(c->fw[frame::kLevelNodes + d]);
-}
-
 int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
                         const double *pos, const double *mass, double *acc, double *pot,
                         uint64_t *stats) {
@@ -1440,33 +1483,10 @@ int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t 
     unsigned long long *st_dev = c->misc.ensure(4);
     upload(dp, pos, 3 * n, s);
     upload(dm, mass, n, s);
-    c->systems = 1;
-    c->sys_off = nullptr;
     const uint32_t n32 = static_cast<uint32_t>(n);
     unsigned long long *frame = c->frame.get();
-    Plan p;
-    for (int attempt = 0;; ++attempt) {
-      if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "tree solver: bounds did not settle");
-      p = plan_for(c, n32, cfg);
-      launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
-      clear_counts(frame, s);
-      GTH_CHECK(hipEventRecord(c->ev[0], s));
-      c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
-      GTH_CHECK(hipEventRecord(c->ev[1], s));
-      read_frame(c);
-      const unsigned long long *fw = c->fw;
-      unsigned long long st = 0;
-      if (fw[frame::kNodes] > p.node_cap) st |= kAbortNodes;
-      if (fw[frame::kUnresolved] & 1ull) st |= kAbortPaths;
-      if (fw[frame::kDepth] > static_cast<unsigned long long>(p.depth_cap)) st |= kAbortDepth;
-      if (st == 0) break;
-      learn(c, p, st);
-      ++c->pstats.reruns;
-    }
-    learn_tree(c, p);
+    const Plan p = build_single_tree(c, cfg, n32, dp, dm);
     const uint32_t cap = p.node_cap;
-    launch_link_up(c->tree.links(), frame, cap, c->parent.ensure(cap), c->skip.ensure(cap),
-                   c->depth.ensure(cap), s);
     launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
                         c->tree.tree_mass(), c->tree.order(), n32, frame, cap, cfg->G, cfg->eps,
                         cfg->theta, cfg->quadrupole != 0, da, dq, st_dev, s);
@@ -1480,9 +1500,98 @@ int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t 
       stats[0] = counts[0];
       stats[1] = counts[1];
     }
-    c->tree.set_counts(static_cast<uint32_t>(c->fw[frame::kNodes]),
-                       static_cast<int>(c->fw[frame::kDepth]));
-    c->lists_n = 0;  // (the last lists referred to the tree just replaced)
+    c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
+    c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
+    c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);
+    c->gpu_seconds = c->tm.total * 1e-3;
+  });
+}
+
+int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double *pos,
+                      const double *mass, uint64_t nq, const double *centres,
+                      const double *radii, uint64_t n_radii, uint64_t max_out,
+                      uint64_t *offsets, uint64_t *index, double *dist2, uint64_t *total) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    require(leaf_size >= 1, "leaf_size must be at least 1");
+    if (n > 0xffffffffull || nq > 0xffffffffull)
+      fail(GT_HIP_ERR_LIMIT, "too many particles or queries for the 32 bit GPU indices");
+    require(offsets != nullptr, "offsets is NULL");
+    require(n == 0 || pos != nullptr, "pos is NULL");
+    require(nq == 0 || (centres != nullptr && radii != nullptr), "centres or radii is NULL");
+    require(nq == 0 || n_radii == 1 || n_radii == nq, "n_radii must be 1 or nq");
+    c->tm = gt_timings{};
+    if (nq == 0 || n == 0) {  // no queries, or an empty tree: nothing found
+      std::fill(offsets, offsets + nq + 1, 0ull);
+      if (total) *total = 0;
+      return;
+    }
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const uint32_t n32 = static_cast<uint32_t>(n), nq32 = static_cast<uint32_t>(nq);
+    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+    upload(dp, pos, 3 * n, s);
+    if (mass) {
+      upload(dm, mass, n, s);
+    } else {  // the tree's shape does not depend on the masses: unit masses
+      launch_fill_u64(reinterpret_cast<unsigned long long *>(dm), n, 0x3ff0000000000000ull, s);
+    }
+    gt_group_config cfg{};
+    cfg.G = 1.0;
+    cfg.theta = 1.0;
+    cfg.group_size = 32;
+    cfg.leaf_size = leaf_size;
+    const Plan p = build_single_tree(c, &cfg, n32, dp, dm);
+    double *dc = c->nb_centres.ensure(3 * nq);
+    double *dr = c->nb_radii.ensure(n_radii);
+    upload(dc, centres, 3 * nq, s);
+    upload(dr, radii, n_radii, s);
+    uint32_t *counts = c->nb_counts.ensure(nq), *offs = c->nb_offs.ensure(nq);
+    unsigned long long *tmp = c->nb_tmp.ensure(scan_tmp_words(nq));
+    unsigned long long *dev = c->misc.ensure(4);  // [0] error bits, [1] total
+    unsigned long long *frame = c->frame.get();
+    const uint32_t stride = n_radii == 1 ? 0u : 1u;
+    // lanes take the queries in Morton order of their centres
+    uint32_t *perm = c->nb_perm.ensure(nq);
+    uint64_t *keys = c->nb_keys.ensure(nq);
+    launch_query_keys(c->tree.geom(), frame, dc, nq32, keys, perm, s);
+    radix_sort_u64(keys, perm, c->nb_keys_alt.ensure(nq), c->nb_perm_alt.ensure(nq), nq,
+                   kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s);
+    launch_fill_u64(dev, 2, 0ull, s);
+    launch_within(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
+                  c->tree.order(), n32, frame, p.node_cap, dc, dr, stride, nq32, perm, counts,
+                  nullptr, nullptr, nullptr, dev, s);
+    exclusive_scan_u32(counts, offs, nq, dev + 1, tmp, s);
+    unsigned long long h[2] = {0, 0};
+    download(h, dev, 2, s);
+    std::vector<uint32_t> ho(nq);
+    download(ho.data(), offs, nq, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+    if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: malformed tree");
+    if (h[1] > 0xffffffffull)
+      fail(GT_HIP_ERR_LIMIT, "more than 2^32 - 1 neighbours in one call: split the queries");
+    for (uint64_t q = 0; q < nq; ++q) offsets[q] = ho[q];
+    offsets[nq] = h[1];
+    if (total) *total = h[1];
+    if ((index || dist2) && h[1] > max_out)
+      fail(GT_HIP_ERR_LIMIT, "max_out too small: the queries have " + std::to_string(h[1]) +
+                                 " neighbours (offsets and total are written)");
+    if (h[1] && (index || dist2)) {
+      unsigned long long *di = c->nb_index.ensure(h[1]);
+      double *dd = c->nb_dist2.ensure(h[1]);
+      launch_within(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
+                    c->tree.order(), n32, frame, p.node_cap, dc, dr, stride, nq32, perm,
+                    counts, offs, di, dd, dev, s);
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      if (index) download(reinterpret_cast<unsigned long long *>(index), di, h[1], s);
+      if (dist2) download(dist2, dd, h[1], s);
+      download(h, dev, 1, s);
+      GTH_CHECK(hipStreamSynchronize(s));
+      if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: the count and fill walks differ");
+    } else {
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      GTH_CHECK(hipStreamSynchronize(s));
+    }
     c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
     c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
     c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);

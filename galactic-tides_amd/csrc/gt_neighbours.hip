@@ -1,0 +1,182 @@
+// Fixed-radius neighbour queries on the GPU octree (reference:
+// Octree::for_each_within, include/galactic_tides/octree.h:103-127).
+//
+// One lane per query.  Each lane walks the tree in the reference's LIFO order
+// (children last to first) without a stack, through the skip links (`skip` is
+// the node the LIFO walk pops after a subtree), and takes a leaf's particles
+// in tree order: the order in which the reference's callback receives them.
+// The box test and the squared distances are the reference's FP64 expressions
+// in its operation order with no contraction, so the neighbour lists and the
+// distances equal the reference's bit for bit.  Two passes over the same walk:
+// the counts, then, after an exclusive scan of the counts, the lists.
+//
+// Lanes take the queries in the order of a 30-bit Morton key of their centres
+// in the root cube (query_keys + a radix sort), so the lanes of a wave walk
+// nearby parts of the tree; results stay in the caller's query order.
+#include "gt_internal.h"
+
+namespace gth {
+namespace {
+
+constexpr int kBlock = 256;
+
+#pragma clang fp contract(off)
+
+// Squared distance from c to the node's cube, as the reference sums it
+// (axes in order, only the axes where c is outside the slab).
+__device__ inline double box_d2(const double *g, double cx, double cy, double cz) {
+  const double half = g[3];
+  double d2 = 0.0;
+  const double dx = fabs(cx - g[0]) - half;
+  if (dx > 0) d2 += dx * dx;
+  const double dy = fabs(cy - g[1]) - half;
+  if (dy > 0) d2 += dy * dy;
+  const double dz = fabs(cz - g[2]) - half;
+  if (dz > 0) d2 += dz * dz;
+  return d2;
+}
+
+// kFill false: counts[q] = neighbours of query q.  kFill true: query q's
+// neighbours into index / dist2 from offs[q], at most counts[q] of them.
+// Queries are taken in the order `perm` gives (null: as they are).
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void within(
+    const double *__restrict__ geom, const int4 *__restrict__ lnk,
+    const uint32_t *__restrict__ skip, const double *__restrict__ tpos,
+    const uint32_t *__restrict__ order, uint32_t n, const unsigned long long *__restrict__ frame,
+    uint32_t node_cap, const double *__restrict__ centres, const double *__restrict__ radii,
+    uint32_t radius_stride, uint32_t nq, const uint32_t *__restrict__ perm,
+    uint32_t *__restrict__ counts, const uint32_t *__restrict__ offs,
+    unsigned long long *__restrict__ index, double *__restrict__ dist2,
+    unsigned long long *__restrict__ error) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nq) return;
+  const uint32_t q = perm ? perm[k] : k;
+  if (q >= nq) {
+    atomicOr(error, 2ull);
+    return;
+  }
+  const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const double cx = centres[3 * static_cast<size_t>(q)];
+  const double cy = centres[3 * static_cast<size_t>(q) + 1];
+  const double cz = centres[3 * static_cast<size_t>(q) + 2];
+  const double r = radii[static_cast<size_t>(q) * radius_stride];
+  const double r2 = r * r;
+  uint32_t found = 0, at = 0, end = 0;
+  if (kFill) {
+    at = offs[q];
+    end = at + counts[q];
+  }
+  uint32_t cur = nn > 0 ? 0u : kNone, budget = nn;  // a walk visits each node once at most
+  while (cur != kNone) {
+    if (cur >= nn || budget-- == 0) {
+      atomicOr(error, 1ull);  // malformed tree (never for a built one)
+      break;
+    }
+    const double *g = geom + static_cast<size_t>(kGeomStride) * cur;
+    if (box_d2(g, cx, cy, cz) > r2) {
+      cur = skip[cur];
+      continue;
+    }
+    const int4 L = lnk[cur];
+    if (L.y == 0) {
+      const uint32_t b = static_cast<uint32_t>(L.z), e = min(static_cast<uint32_t>(L.w), n);
+      for (uint32_t j = b; j < e; ++j) {
+        const double dx = tpos[3 * static_cast<size_t>(j)] - cx;
+        const double dy = tpos[3 * static_cast<size_t>(j) + 1] - cy;
+        const double dz = tpos[3 * static_cast<size_t>(j) + 2] - cz;
+        const double dd = dx * dx + dy * dy + dz * dz;
+        if (dd <= r2) {
+          if (kFill) {
+            if (at < end) {
+              index[at] = order[j];
+              dist2[at] = dd;
+            }
+            ++at;
+          } else {
+            ++found;
+          }
+        }
+      }
+      cur = skip[cur];
+    } else {
+      cur = static_cast<uint32_t>(L.x + L.y - 1);  // the last child first
+    }
+  }
+  if (kFill) {
+    if (at != end) atomicOr(error, 2ull);  // the passes disagree (never)
+  } else {
+    counts[q] = found;
+  }
+}
+
+#pragma clang fp contract(fast)
+
+// Spreads the low 10 bits of v to every third bit.
+__device__ inline uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x3ffu;
+  x = (x | (x << 16)) & 0x30000ffull;
+  x = (x | (x << 8)) & 0x300f00full;
+  x = (x | (x << 4)) & 0x30c30c3ull;
+  x = (x | (x << 2)) & 0x9249249ull;
+  return x;
+}
+
+// 10 bits of a coordinate's place in [lo, lo + 2 half] (clamped; NaN gives 0).
+__device__ inline uint32_t cell10(double x, double lo, double inv_side) {
+  const double t = (x - lo) * inv_side * 1024.0;
+  return t > 0.0 ? static_cast<uint32_t>(fmin(t, 1023.0)) : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void query_keys(const double *__restrict__ geom,
+                                                     const unsigned long long *__restrict__ frame,
+                                                     const double *__restrict__ centres,
+                                                     uint32_t nq, uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ vals) {
+  const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= nq) return;
+  uint64_t key = 0;
+  if (frame[frame::kNodes] > 0) {
+    const double half = geom[3];
+    const double inv = half > 0.0 ? 0.5 / half : 0.0;
+    const uint32_t x = cell10(centres[3 * static_cast<size_t>(q)], geom[0] - half, inv);
+    const uint32_t y = cell10(centres[3 * static_cast<size_t>(q) + 1], geom[1] - half, inv);
+    const uint32_t z = cell10(centres[3 * static_cast<size_t>(q) + 2], geom[2] - half, inv);
+    key = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+  }
+  keys[q] = key;
+  vals[q] = q;
+}
+
+}  // namespace
+
+void launch_query_keys(const double *geom, const unsigned long long *frame,
+                       const double *centres, uint32_t nq, uint64_t *keys, uint32_t *vals,
+                       hipStream_t s) {
+  if (!nq) return;
+  hipLaunchKernelGGL(query_keys, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, s, geom,
+                     frame, centres, nq, keys, vals);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_within(const double *geom, const int4 *lnk, const uint32_t *skip, const double *tpos,
+                   const uint32_t *order, uint32_t n, const unsigned long long *frame,
+                   uint32_t node_cap, const double *centres, const double *radii,
+                   uint32_t radius_stride, uint32_t nq, const uint32_t *perm, uint32_t *counts,
+                   const uint32_t *offs, unsigned long long *index, double *dist2,
+                   unsigned long long *error, hipStream_t s) {
+  if (!nq) return;
+  const dim3 grid((nq + kBlock - 1) / kBlock);
+  if (offs) {
+    hipLaunchKernelGGL(within<true>, grid, dim3(kBlock), 0, s, geom, lnk, skip, tpos, order, n,
+                       frame, node_cap, centres, radii, radius_stride, nq, perm, counts, offs,
+                       index, dist2, error);
+  } else {
+    hipLaunchKernelGGL(within<false>, grid, dim3(kBlock), 0, s, geom, lnk, skip, tpos, order, n,
+                       frame, node_cap, centres, radii, radius_stride, nq, perm, counts, offs,
+                       index, dist2, error);
+  }
+  GTH_LAUNCH_CHECK();
+}
+
+}  // namespace gth

@@ -90,6 +90,9 @@ _state_diagnostics = _sig("gt_hip_state_diagnostics", C.c_int,
                           [_vp, C.c_double, C.c_uint64, _vp])
 _tree_gravity = _sig("gt_hip_tree_gravity", C.c_int,
                      [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
+_neighbours = _sig("gt_hip_neighbours", C.c_int,
+                   [_vp, C.c_int, C.c_uint64, _vp, _vp, C.c_uint64, _vp, _vp, C.c_uint64,
+                    C.c_uint64, _vp, _vp, _vp, _vp])
 _group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
                             [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
 _state_set_systems = _sig("gt_hip_state_set_systems", C.c_int, [_vp, C.c_uint64, _vp])
@@ -242,6 +245,41 @@ class Context:
         self._check(_tree_gravity(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
                                   _ptr(pot), _ptr(st)))
         return acc, pot, st
+
+    def neighbours(self, pos, centres, radii, mass=None, leaf_size=8, max_out=None):
+        """Octree::for_each_within for each query (centre, radius; one radius for
+        all or one per query) on the tree of pos: CSR (offsets, index, dist2) in
+        the order the reference's callback receives the neighbours.  max_out
+        (default: as many as there are, found by a counting call first) bounds
+        the lists; when they need more, OverflowError names the total."""
+        n = len(pos)
+        pos = _f64(pos, (n, 3))
+        if mass is not None:
+            mass = _f64(mass)
+            if len(mass) != n:
+                raise ValueError(f"expected {n} masses, got {len(mass)}")
+        centres = np.ascontiguousarray(centres, np.float64).reshape(-1, 3)
+        nq = len(centres)
+        radii = np.ascontiguousarray(np.atleast_1d(radii), np.float64)
+        offsets = np.zeros(nq + 1, np.uint64)
+        total = C.c_uint64(0)
+        mp = _ptr(mass) if mass is not None else None
+        if max_out is None:  # count first, then fill
+            self._check(_neighbours(self._h, leaf_size, n, _ptr(pos), mp, nq, _ptr(centres),
+                                    _ptr(radii), len(radii), 0, _ptr(offsets), None, None,
+                                    C.byref(total)))
+            max_out = total.value
+        index = np.empty(max(max_out, 1), np.uint64)
+        dist2 = np.empty(max(max_out, 1))
+        rc = _neighbours(self._h, leaf_size, n, _ptr(pos), mp, nq, _ptr(centres), _ptr(radii),
+                         len(radii), max_out, _ptr(offsets), _ptr(index), _ptr(dist2),
+                         C.byref(total))
+        if rc == ERR_LIMIT and total.value > max_out:
+            raise OverflowError(f"{total.value} neighbours, room for {max_out}: "
+                                + _last_error(self._h).decode())
+        self._check(rc)
+        t = total.value
+        return offsets, index[:t], dist2[:t]
 
     def group_compute_batch(self, offsets, pos, mass, **cfg):
         """Independent systems: system s is particles offsets[s]:offsets[s+1]."""

@@ -5,8 +5,10 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "gt_hip.h"
 
@@ -76,6 +78,36 @@ class Context {
     gt_timings t{};
     check(gt_hip_last_timings(c_, &t));
     return t;
+  }
+  // Octree::build + Octree::for_each_within (include/galactic_tides/octree.h:73-75)
+  // for a batch of queries: CSR lists in the reference's callback order.
+  struct Neighbours {
+    std::vector<uint64_t> offsets, index;
+    std::vector<double> dist2;
+  };
+  Neighbours neighbours(int leaf_size, size_t n, const double *pos, const double *mass,
+                        size_t nq, const double *centres, const double *radii,
+                        size_t n_radii) {
+    Neighbours r;
+    r.offsets.resize(nq + 1);
+    uint64_t total = 0;
+    check(gt_hip_neighbours(c_, leaf_size, n, pos, mass, nq, centres, radii, n_radii, 0,
+                            r.offsets.data(), nullptr, nullptr, &total));
+    r.index.resize(total);
+    r.dist2.resize(total);
+    check(gt_hip_neighbours(c_, leaf_size, n, pos, mass, nq, centres, radii, n_radii, total,
+                            r.offsets.data(), r.index.data(), r.dist2.data(), &total));
+    return r;
+  }
+  // f(query, index, distance_squared) for every neighbour, queries in order and
+  // each query's neighbours in the order Octree::for_each_within calls f.
+  template <class F>
+  void for_each_within(int leaf_size, size_t n, const double *pos, const double *mass,
+                       size_t nq, const double *centres, const double *radii, size_t n_radii,
+                       F &&f) {
+    const Neighbours r = neighbours(leaf_size, n, pos, mass, nq, centres, radii, n_radii);
+    for (size_t q = 0; q < nq; ++q)
+      for (uint64_t k = r.offsets[q]; k < r.offsets[q + 1]; ++k) f(q, r.index[k], r.dist2[k]);
   }
   // resident leapfrog
   void state_set(size_t n, const double *pos, const double *vel, const double *mass) {

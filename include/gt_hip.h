@@ -163,6 +163,23 @@ int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64
 int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                         const double *pos, const double *mass, double *acc, double *pot,
                         uint64_t *stats);
+/* Octree::build (src/octree.cpp:169-262) of the n particles with leaf_size, then
+ * Octree::for_each_within (include/galactic_tides/octree.h:73-75,103-127) for nq
+ * queries at once, on the GPU.  Query q has centre centres[3q .. 3q+2] and radius
+ * radii[q] (n_radii = nq) or radii[0] (n_radii = 1).  Its neighbours are every
+ * particle within the radius, inclusive, in the order the reference's callback
+ * receives them (its LIFO tree walk; tree order within a leaf), as CSR arrays:
+ * entries offsets[q] .. offsets[q + 1] - 1 (nq + 1 offsets, from 0) of index
+ * (original particle index) and dist2 (squared distance, the reference's FP64
+ * expression, bit for bit).  mass may be NULL (the tree's shape does not depend
+ * on it).  offsets and *total (may be NULL) are always written.  index and dist2
+ * (each may be NULL; both NULL: the call only counts) are written when the total
+ * fits max_out; otherwise the call returns GT_HIP_ERR_LIMIT and can be made again
+ * with room for *total entries.  At most 2^32 - 1 neighbours in all per call. */
+int gt_hip_neighbours(gt_hip_context *ctx, int leaf_size, uint64_t n, const double *pos,
+                      const double *mass, uint64_t nq, const double *centres,
+                      const double *radii, uint64_t n_radii, uint64_t max_out,
+                      uint64_t *offsets, uint64_t *index, double *dist2, uint64_t *total);
 /* Independent systems in one evaluation (no counterpart in the reference,
  * which builds one GroupTreeGravity per system): system s is particles
  * offsets[s] .. offsets[s + 1] (n_systems + 1 offsets from 0 to n, non-decreasing;

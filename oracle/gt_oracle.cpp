@@ -851,6 +851,65 @@ void gto_test_system(int kind, uint64_t n, uint64_t seed, double *pos, double *m
   }
 }
 
+uint64_t gto_within(const gto_run *r, uint64_t nq, const double *centres, const double *radii,
+                    uint64_t n_radii, uint64_t *offsets, uint64_t cap, uint64_t *index,
+                    double *dist2) {
+  const Tree &t = r->tree;
+  const long nl = static_cast<long>(nq);
+  std::vector<uint64_t> count(nq, 0);
+  // The two passes run the same walk: counting, then writing from the offsets.
+  auto walk = [&](uint64_t q, uint64_t *out_i, double *out_d) {
+    uint64_t found = 0;
+    if (t.nodes.empty()) return found;
+    const double *c = centres + 3 * q;
+    const double rad = radii[n_radii == 1 ? 0 : q];
+    const double r2 = rad * rad;
+    std::vector<uint32_t> stack{0};  // LIFO: the last child pushed is taken first
+    while (!stack.empty()) {
+      const Node &nd = t.nodes[stack.back()];
+      stack.pop_back();
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double d = std::fabs(c[k] - nd.c[k]) - nd.half;
+        if (d > 0) d2 += d * d;
+      }
+      if (d2 > r2) continue;
+      if (nd.n_children == 0) {
+        for (uint32_t j = nd.begin; j < nd.end; ++j) {
+          const double *p = &t.tpos[3 * static_cast<size_t>(j)];
+          const double dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+          const double dd = dx * dx + dy * dy + dz * dz;
+          if (dd <= r2) {
+            if (out_i) {
+              out_i[found] = t.idx[j];
+              out_d[found] = dd;
+            }
+            ++found;
+          }
+        }
+      } else {
+        for (int k = 0; k < nd.n_children; ++k)
+          stack.push_back(static_cast<uint32_t>(nd.first_child + k));
+      }
+    }
+    return found;
+  };
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long q = 0; q < nl; ++q) count[q] = walk(static_cast<uint64_t>(q), nullptr, nullptr);
+  uint64_t total = 0;
+  for (uint64_t q = 0; q < nq; ++q) {
+    offsets[q] = total;
+    total += count[q];
+  }
+  offsets[nq] = total;
+  if (index && dist2 && total <= cap) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long q = 0; q < nl; ++q)
+      walk(static_cast<uint64_t>(q), index + offsets[q], dist2 + offsets[q]);
+  }
+  return total;
+}
+
 int gto_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

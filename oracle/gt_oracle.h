@@ -76,6 +76,15 @@ int gto_kdk_group(gto_run *r, uint64_t n, double *pos, double *vel, const double
 int gto_time_group(gto_run *r, uint64_t n, const double *pos, const double *mass, double G,
                    double theta, double eps, int quadrupole, int group_size, int leaf_size,
                    int pack, int repeat, double *best4);
+/* Octree::for_each_within (include/galactic_tides/octree.h:103-127) on the tree
+ * of the last gto_build, for nq queries: query q has centre centres[3q..3q+2] and
+ * radius radii[q] (n_radii = nq) or radii[0] (n_radii = 1).  offsets (nq + 1)
+ * gets the CSR offsets; the neighbours (original index, squared distance) in
+ * the order the reference's callback receives them go to index / dist2 when
+ * both are non-NULL and the total fits cap.  Returns the total. */
+uint64_t gto_within(const gto_run *r, uint64_t nq, const double *centres, const double *radii,
+                    uint64_t n_radii, uint64_t *offsets, uint64_t cap, uint64_t *index,
+                    double *dist2);
 int gto_max_threads(void);
 /* Seeded test systems of the reference's tests (std::mt19937_64 +
  * uniform_real_distribution<double>):

@@ -47,6 +47,7 @@ _SIGS = {
     "kdk_group": (_i, [_vp, _u64t, _dp, _dp, _dp, _d, _d, _d, _i, _i, _i, _i, _i, _d, _dp, _dp]),
     "time_group": (_i, [_vp, _u64t, _dp, _dp, _d, _d, _d, _i, _i, _i, _i, _i, _dp]),
     "max_threads": (_i, []),
+    "within": (_u64t, [_vp, _u64t, _dp, _dp, _u64t, _u64, _u64t, _vp, _vp]),
 }
 _ORACLE_ONLY = {"test_system": (None, [_i, _u64t, _u64t, _dp, _dp])}
 _REF_ONLY = {
@@ -216,6 +217,21 @@ class Checker:
             raise RuntimeError("timing failed")
         return dict(total=best[0], tree=best[1], lists=best[2], evaluate=best[3],
                     interactions=best[4] + best[5])
+
+    def within(self, centres, radii):
+        """Octree::for_each_within for each query on the last build's tree: CSR
+        offsets (nq + 1), original indices and squared distances, in the order
+        the reference's callback receives them."""
+        centres = np.ascontiguousarray(centres, np.float64).reshape(-1, 3)
+        radii = np.ascontiguousarray(np.atleast_1d(radii), np.float64)
+        nq = len(centres)
+        offsets = np.zeros(nq + 1, np.uint64)
+        total = self._within(self.h, nq, centres, radii, len(radii), offsets, 0, None, None)
+        index = np.zeros(max(total, 1), np.uint64)
+        dist2 = np.zeros(max(total, 1))
+        self._within(self.h, nq, centres, radii, len(radii), offsets, total,
+                     index.ctypes.data, dist2.ctypes.data)
+        return offsets, index[:total], dist2[:total]
 
     def max_threads(self):
         return int(self._max_threads())

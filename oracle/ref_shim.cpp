@@ -337,6 +337,33 @@ int gtref_time_tree(uint64_t n, const double *pos, const double *mass, double G,
   return 0;
 }
 
+uint64_t gtref_within(const gtref_run *r, uint64_t nq, const double *centres,
+                      const double *radii, uint64_t n_radii, uint64_t *offsets, uint64_t cap,
+                      uint64_t *index, double *dist2) {
+  std::vector<std::vector<std::pair<uint64_t, double>>> found(nq);
+  for (uint64_t q = 0; q < nq; ++q) {
+    const Vec3 c(centres[3 * q], centres[3 * q + 1], centres[3 * q + 2]);
+    r->tree.for_each_within(c, radii[n_radii == 1 ? 0 : q],
+                            [&](std::size_t i, const Vec3 &, double dd) {
+                              found[q].emplace_back(i, dd);
+                            });
+  }
+  uint64_t total = 0;
+  for (uint64_t q = 0; q < nq; ++q) {
+    offsets[q] = total;
+    total += found[q].size();
+  }
+  offsets[nq] = total;
+  if (index && dist2 && total <= cap) {
+    for (uint64_t q = 0; q < nq; ++q)
+      for (std::size_t k = 0; k < found[q].size(); ++k) {
+        index[offsets[q] + k] = found[q][k].first;
+        dist2[offsets[q] + k] = found[q][k].second;
+      }
+  }
+  return total;
+}
+
 int gtref_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

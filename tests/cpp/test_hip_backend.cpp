@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <set>
 #include <vector>
 
 #include "hip_backend.h"
@@ -130,6 +131,33 @@ int main() {
       CHECK(same.max < 1e-3);
       CHECK(acc.median < 3e-3 * theta * theta);
     }
+  }
+  // neighbour search (test_octree.cpp:116-128): the same sets as a scan of
+  // every particle, for 50 queries at particles with radii 0.05 .. 0.5
+  {
+    std::mt19937_64 rng(7);
+    std::uniform_int_distribution<size_t> pick(0, n - 1);
+    std::vector<double> centres, radii;
+    for (int trial = 0; trial < 50; ++trial) {
+      const size_t i = pick(rng);
+      centres.insert(centres.end(), {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+      radii.push_back(0.05 * (1 + trial % 10));
+    }
+    std::vector<std::set<size_t>> got(50);
+    ctx.for_each_within(8, n, pos.data(), mass.data(), 50, centres.data(), radii.data(), 50,
+                        [&](size_t q, uint64_t i, double) { got[q].insert(i); });
+    int same = 0;
+    for (int q = 0; q < 50; ++q) {
+      std::set<size_t> want;
+      for (size_t i = 0; i < n; ++i) {
+        const double dx = pos[3 * i] - centres[3 * q], dy = pos[3 * i + 1] - centres[3 * q + 1],
+                     dz = pos[3 * i + 2] - centres[3 * q + 2];
+        if (dx * dx + dy * dy + dz * dz <= radii[q] * radii[q]) want.insert(i);
+      }
+      same += got[q] == want;
+    }
+    std::printf("hip neighbour search: %d of 50 queries match the scan\n", same);
+    CHECK(same == 50);
   }
   // groups wider than a wave are refused (test_metal.cpp:121-130)
   {

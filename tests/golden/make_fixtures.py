@@ -6,12 +6,15 @@ produced by the reference's own code (Octree::build, build_group_lists,
 evaluate_group_lists_cpu, LeapfrogKDK with GroupTreeGravity, make_hernquist_galaxy).
 The tests compare the CPU oracle and the GPU library against these files.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [--only neighbours]
 
 Outputs:
-  tiny.npz       full inputs and outputs for three small systems
-  configs.json   SHA-256 digests and statistics for the 50k and 200k galaxies,
-                 a small encounter, and 10 leapfrog steps on the 50k galaxy
+  tiny.npz        full inputs and outputs for three small systems
+  configs.json    SHA-256 digests and statistics for the 50k and 200k galaxies,
+                  a small encounter, and 10 leapfrog steps on the 50k galaxy
+  neighbours.npz  Octree::for_each_within results (CSR: offsets, original
+                  indices, squared distances, in callback order) on the 3000-particle
+                  system of test_gpu_kernels.cpp, leaf sizes 1 and 8
 """
 import hashlib
 import json
@@ -85,8 +88,37 @@ def summary(ref, pos, mass, theta=0.75, gs=32, pack=True, eps=0.025, leaf=8):
     }
 
 
+def neighbour_queries(pos):
+    """Query centres and radii: particles as centres with the radii of the
+    reference's neighbour test (0.05 (1 + k mod 10), tests/test_octree.cpp:116-128),
+    seeded random centres, a zero radius and one radius that takes in every
+    particle."""
+    rng = np.random.default_rng(7)
+    pick = rng.integers(0, len(pos), 40)
+    centres = [pos[pick], rng.normal(size=(10, 3)) * 0.5, pos[[5]], np.zeros((1, 3))]
+    radii = [0.05 * (1 + np.arange(40) % 10), np.full(10, 0.3), [0.0], [1e3]]
+    return np.concatenate(centres), np.concatenate(radii).astype(np.float64)
+
+
+def write_neighbours(ref):
+    oracle = Checker("oracle")
+    pos, mass = oracle.test_system(0, 3000, 11)
+    centres, radii = neighbour_queries(pos)
+    out = {"pos": pos, "mass": mass, "centres": centres, "radii": radii}
+    for leaf in (1, 8):
+        ref.build(pos, mass, leaf)
+        off, idx, d2 = ref.within(centres, radii)
+        out[f"leaf{leaf}_offsets"], out[f"leaf{leaf}_index"], out[f"leaf{leaf}_dist2"] = off, idx, d2
+    np.savez_compressed(os.path.join(HERE, "neighbours.npz"), **out)
+    print("wrote neighbours.npz")
+
+
 def main():
     ref = Checker("ref")
+    if sys.argv[1:] == ["--only", "neighbours"]:
+        write_neighbours(ref)
+        return
+    write_neighbours(ref)
     tiny = {}
     # 1) the reference tests' own small systems
     cases = {
