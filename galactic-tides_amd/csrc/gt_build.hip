@@ -1,15 +1,15 @@
-// Octree construction on the GPU (reference: Octree::build, src/octree.cpp:298-402).
+// Octree construction on the GPU (reference: Octree::build, src/octree.cpp:169-273).
 //
 // The reference splits each node by a stable counting sort into octants,
 // recursively, until a node has at most leaf_size bodies, is 60 levels deep,
-// or is too small to split (:169-173).  The same tree falls out of a sort:
+// or is too small to split (:40-44).  The same tree falls out of a sort:
 //
-//  1. root cube from the bounding box (:313-334);
+//  1. root cube from the bounding box (:184-205);
 //  2. for every particle, its octant path: 3-bit octants, 21 levels per word, found
 //     by comparing against node centres that are updated with exactly the
-//     reference's FP64 operations (centre +- half/2, :258-266), so every
+//     reference's FP64 operations (centre +- half/2, :131-137), so every
 //     comparison matches the reference's; also the first depth at which the
-//     path's node is degenerate (:172);
+//     path's node is degenerate (:43);
 //  3. a stable radix sort of the 63-bit paths (equal paths keep particle
 //     order, as the counting sorts do); when a cluster of more than leaf_size
 //     bodies shares all the levels computed so far, the next 21 levels are
@@ -28,8 +28,8 @@
 //     each leaf put back into particle order (the reference's order inside a
 //     leaf is the input order, as no split has reordered them);
 //  7. FP64 moments level by level from the deepest: leaves sum their bodies
-//     in tree order (:175-201), inner nodes combine their children in order
-//     (:203-231), with no FMA contraction, so masses, centres of mass,
+//     in tree order (:46-72), inner nodes combine their children in order
+//     (:74-102), with no FMA contraction, so masses, centres of mass,
 //     quadrupoles and deltas are bit-identical to the reference's.
 //
 // A batch of independent systems (particles offsets[s] .. offsets[s + 1])
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void bbox_partial(const double *__restrict_
   if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// root[0..2] centre, root[3] half (src/octree.cpp:324-332).
+// root[0..2] centre, root[3] half (src/octree.cpp:184-205).
 __global__ __launch_bounds__(kBlock) void bbox_final(const double *__restrict__ part,
                                                      uint32_t nparts, double *__restrict__ root) {
   __shared__ double r[6][kBlock];

@@ -1,6 +1,6 @@
 // Grouped interaction-list evaluation on gfx950 (the hot loop of
 // GroupTreeGravity::compute; reference: gt_tree, src/metal/gravity_kernels.metal:53-112,
-// and evaluate_group_lists_cpu, src/gravity_backends.cpp:167-181).
+// and evaluate_group_lists_cpu, src/gravity_backends.cpp:70-84).
 //
 // Two kernels share the list format (GTGroup / GTCell / uint32 lists):
 //

@@ -6,7 +6,7 @@
 // keys and ranks each key among equal digits with wave ballots (match masks),
 // which keeps equal keys in input order.  Used for the octree's octant-path
 // keys, where stability is what reproduces the reference's stable counting
-// sort (src/octree.cpp:233-250).
+// sort (src/octree.cpp:106-121).
 #include <algorithm>
 #include <utility>
 

@@ -1,7 +1,7 @@
 // Per-particle FP64 kernels of a leapfrog step and the scatter from tree
 // order back to particle order (reference: LeapfrogKDK::step,
 // src/integrator.cpp:5-16, and GroupTreeGravity::compute's scatter,
-// src/gravity_backends.cpp:229-233).  Bandwidth-bound: each is one pass over
+// src/gravity_backends.cpp:132-136).  Bandwidth-bound: each is one pass over
 // the particle arrays.
 #include "gt_internal.h"
 

@@ -1,5 +1,5 @@
 // The per-particle FP64 tree solver on the GPU (reference: Octree::accel_pot,
-// src/octree.cpp:276-344, driven by compute_gravity_tree, src/gravity.cpp:7-26;
+// src/octree.cpp:275-344, driven by compute_gravity_tree, src/gravity.cpp:7-26;
 // the reference runs it on the CPU only).
 //
 // One lane per particle, particles in tree order (neighbouring lanes visit the

@@ -1,33 +1,33 @@
 // Interaction lists on the GPU (reference: build_group_lists,
-// src/group_lists.cpp:121-310).
+// src/group_lists.cpp:18-207).
 //
 //  link_up       parent of every node, then each node's depth and `skip`: the
 //                node a LIFO depth-first walk pops after finishing the node's
 //                subtree (previous sibling, else the parent's skip).  With it
 //                the walk needs no stack: open = go to the last child, done =
 //                go to skip.
-//  prep_lists    FP32 bodies and cells relative to the root centre (:143-166),
+//  prep_lists    FP32 bodies and cells relative to the root centre (:40-63),
 //                and one 64-byte walk record per node: FP64 centre, centre of
 //                mass and crit = 2 half / theta + delta, plus one word packing
 //                first child, skip, child count, depth and "at most two bodies".
 //                A node visit of the walk reads exactly one record (half a
 //                cache line); half is root_half / 2^depth, which is exactly the
 //                reference's repeated halving.
-//  group_starts  group selection (:168-216) as flags on tree positions: every
+//  group_starts  group selection (:65-113) as flags on tree positions: every
 //                node holding more than group_size bodies is one the
 //                reference's depth-first `visit` reaches, and each such node
 //                marks where its runs of small children (packed greedily while
 //                they fit) and its chunks start.  A scan turns the flags into
 //                groups in tree order.
 //  walk<MODE>    one thread per group: group bounding box, then the depth-
-//                first walk with the FP64 opening test (:227-268) evaluated
+//                first walk with the FP64 opening test (:147-154) evaluated
 //                in the reference's operation order with no FMA contraction,
 //                so accept/open decisions, and hence the lists, are identical.
 //                MODE kWalkCapacity writes each group's lists into fixed-size
 //                slots (sized from the previous evaluation) and counts; groups
 //                that do not fit are caught by the counts, and then the lists
 //                are rewritten by MODE kWalkExact at scanned offsets.
-//  list_stats    interaction counts, longest list and largest group (:290-309).
+//  list_stats    interaction counts, longest list and largest group (:187-206).
 #include "gt_internal.h"
 
 namespace gth {

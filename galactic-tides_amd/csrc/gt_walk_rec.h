@@ -1,5 +1,5 @@
 // The group walk's opening test, exact and as an FP32 filter (reference:
-// build_group_lists' acceptance test, src/group_lists.cpp:227-268).
+// build_group_lists' acceptance test, src/group_lists.cpp:147-154).
 //
 // The exact test runs in FP64 in the reference's operation order with no FMA
 // contraction, so its decisions are the reference's bit for bit.  The filter

@@ -145,7 +145,7 @@ def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, le
 
 
 def make_params(n, n_groups=0, G=1.0, eps=0.05, quadrupole=True):
-    """make_kernel_params (src/gravity_backends.cpp:122-129)."""
+    """make_kernel_params (src/gravity_backends.cpp:25-32)."""
     p = Params()
     p.n, p.n_groups = n, n_groups
     p.G = np.float32(G)

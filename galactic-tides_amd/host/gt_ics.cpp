@@ -3,7 +3,7 @@
 //
 // Positions follow the reference's sampler draw for draw (same std::mt19937_64
 // stream, same tables and interpolation, same operation order:
-// src/hernquist_ics.cpp:126-169,253-315,381-388,421-460 and
+// src/hernquist_ics.cpp:16-64,142-204,380-388,421-460 and
 // apps/galactic_tides_ics.cpp:78-97), so a galaxy built here has exactly the
 // reference's particle positions and the octree, groups and interaction lists
 // built from it can be compared with the reference's entry for entry.

@@ -102,7 +102,7 @@ typedef struct {
                         has SIMDs: a range's walk lasts as long as its longest group's */
 } gt_group_config;
 
-/* GroupLists statistics (group_lists.h:73-91) and tree size of the last call. */
+/* GroupLists statistics (group_lists.h:37-55) and tree size of the last call. */
 typedef struct {
   uint64_t n, n_nodes, depth, n_groups, n_cell_list, n_body_list;
   uint64_t cell_interactions, body_interactions, max_list, max_group;
@@ -140,7 +140,7 @@ int gt_hip_direct(gt_hip_context *ctx, const gt_float4 *bodies, const gt_params 
  * src/metal/gravity_kernels.metal:53-112): evaluates grouped interaction lists,
  * out[i] in tree order.  p->n must equal the number of bodies, p->n_groups the
  * number of groups; groups may hold at most 64 targets.  exact = 1 gives
- * results bit-identical to evaluate_group_lists_cpu (gravity_backends.cpp:167). */
+ * results bit-identical to evaluate_group_lists_cpu (gravity_backends.cpp:70-84). */
 int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
                 const gt_cell *cells, uint64_t n_cell_list, const uint32_t *cell_list,
                 uint64_t n_body_list, const uint32_t *body_list, const gt_group *groups,
@@ -148,22 +148,22 @@ int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
 
 /* ---- the whole force evaluation on the GPU ---- */
 
-/* GroupTreeGravity::compute (src/gravity_backends.cpp:206-234) with device = hip:
+/* GroupTreeGravity::compute (src/gravity_backends.cpp:109-137) with device = hip:
  * octree build, group selection, grouped walk, list evaluation and the scatter
  * back to particle order all run on the GPU.  pos/acc are n x 3 doubles. */
 int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                          const double *pos, const double *mass, double *acc, double *pot);
 /* TreeGravity::compute / compute_gravity_tree (include/galactic_tides/integrator.h:20-34,
- * src/gravity.cpp:7-26, walk Octree::accel_pot src/octree.cpp:276-344), which the
+ * src/gravity.cpp:7-26, walk Octree::accel_pot src/octree.cpp:275-344), which the
  * reference runs on the CPU only ("gravity = tree"): the per-particle FP64 tree
  * walk, here on the GPU over the GPU-built tree, with the reference's arithmetic
  * op for op, so acc/pot equal the reference's bit for bit.  Uses cfg->G, theta,
  * eps, quadrupole and leaf_size.  stats (may be NULL): [0] cell interactions,
- * [1] particle interactions (WalkStats, include/galactic_tides/octree.h:22-29). */
+ * [1] particle interactions (WalkStats, include/galactic_tides/octree.h:22-30). */
 int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                         const double *pos, const double *mass, double *acc, double *pot,
                         uint64_t *stats);
-/* Octree::build (src/octree.cpp:169-262) of the n particles with leaf_size, then
+/* Octree::build (src/octree.cpp:169-273) of the n particles with leaf_size, then
  * Octree::for_each_within (include/galactic_tides/octree.h:73-75,103-127) for nq
  * queries at once, on the GPU.  Query q has centre centres[3q .. 3q+2] and radius
  * radii[q] (n_radii = nq) or radii[0] (n_radii = 1).  Its neighbours are every
@@ -210,7 +210,7 @@ int gt_hip_autotune(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
  * quad xx yy zz xy xz yz, delta, 0; links n_nodes x 4 int32 = first_child,
  * n_children, begin, end; tree_pos/tree_mass in tree order).  Lists use the
  * caller's node numbering, so they compare raw with build_group_lists
- * (src/group_lists.cpp:121) on the same tree.  out (n x 4, tree order) may be NULL. */
+ * (src/group_lists.cpp:18-207) on the same tree.  out (n x 4, tree order) may be NULL. */
 int gt_hip_walk_tree(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                      uint64_t n_nodes, const double *geom, const int32_t *links,
                      const double *tree_pos, const double *tree_mass, gt_float4 *out);
@@ -226,7 +226,7 @@ int gt_hip_get_tree(gt_hip_context *ctx, double *geom, int32_t *links, uint64_t 
 int gt_hip_get_lists(gt_hip_context *ctx, gt_group *groups, uint32_t *cell_list,
                      uint32_t *body_list, gt_float4 *bodies, gt_cell *cells, double *origin);
 
-/* ---- device-resident leapfrog (LeapfrogKDK, include/galactic_tides/integrator.h:101-109) ---- */
+/* ---- device-resident leapfrog (LeapfrogKDK, include/galactic_tides/integrator.h:39-47) ---- */
 
 /* Upload particles (n x 3 pos/vel, n mass) to device memory kept by ctx. */
 int gt_hip_state_set(gt_hip_context *ctx, uint64_t n, const double *pos, const double *vel,
@@ -257,7 +257,7 @@ typedef struct {
  * sequential sums to FP64 rounding. */
 int gt_hip_state_diagnostics(gt_hip_context *ctx, double time, uint64_t max_out,
                              gt_diagnostics *out);
-/* LeapfrogKDK::prime (integrator.h:104): forces for the resident state. */
+/* LeapfrogKDK::prime (integrator.h:42): forces for the resident state. */
 int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
 /* nsteps x LeapfrogKDK::step (src/integrator.cpp:5-16) on the resident state:
  * FP64 kick and drift kernels around each GPU force evaluation.  An

@@ -39,9 +39,9 @@ struct Node {
   uint32_t begin = 0, end = 0;
 };
 
-constexpr int kDepthLimit = 60;  // src/octree.cpp:147
+constexpr int kDepthLimit = 60;  // src/octree.cpp:18
 
-// src/octree.cpp:169-173: small enough, deep enough, or a cube too small to split.
+// src/octree.cpp:40-44: small enough, deep enough, or a cube too small to split.
 bool leaf_rule(const Node &nd, uint32_t leaf, int depth) {
   const double scale = std::fabs(nd.c[0]) + std::fabs(nd.c[1]) + std::fabs(nd.c[2]) + 1e-300;
   return nd.end - nd.begin <= leaf || depth >= kDepthLimit || nd.half < 1e-12 * scale;
@@ -68,7 +68,7 @@ struct Builder {
     return (p[0] >= c[0] ? 1 : 0) | (p[1] >= c[1] ? 2 : 0) | (p[2] >= c[2] ? 4 : 0);
   }
 
-  // src/octree.cpp:175-201, bodies summed in tree order.
+  // src/octree.cpp:46-72, bodies summed in tree order.
   void leaf_moments(Node &nd) const {
     double m = 0, cx = 0, cy = 0, cz = 0;
     for (uint32_t k = nd.begin; k < nd.end; ++k) {
@@ -110,7 +110,7 @@ struct Builder {
     nd.delta = std::sqrt(ex * ex + ey * ey + ez * ez);
   }
 
-  // src/octree.cpp:203-231, children in stored order (parallel axis theorem).
+  // src/octree.cpp:74-102, children in stored order (parallel axis theorem).
   static void combine(std::vector<Node> &nodes, size_t ni) {
     Node &nd = nodes[ni];
     const size_t a = static_cast<size_t>(nd.first_child), b = a + nd.n_children;
@@ -152,7 +152,7 @@ struct Builder {
     nd.delta = std::sqrt(ex * ex + ey * ey + ez * ez);
   }
 
-  // src/octree.cpp:233-269: stable counting sort into octants; the non-empty
+  // src/octree.cpp:104-140: stable counting sort into octants; the non-empty
   // octants become children, appended to the node array in octant order.
   void split(size_t ni) {
     const uint32_t b = nodes[ni].begin, e = nodes[ni].end;
@@ -202,7 +202,7 @@ struct Builder {
   }
 };
 
-// src/octree.cpp:298-337 root cube, then the serial recursive build.
+// src/octree.cpp:169-208 root cube, then the serial recursive build.
 void build_tree(Tree &t, uint64_t n, const double *pos, const double *mass, int leaf_size) {
   t.nodes.clear();
   t.idx.resize(n);
@@ -310,7 +310,7 @@ struct WalkNode {
   uint32_t begin, end;
 };
 
-constexpr uint32_t kBodiesIfAtMost = 2;  // group_lists.h:94
+constexpr uint32_t kBodiesIfAtMost = 2;  // group_lists.h:58
 
 void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L) {
   const auto &nodes = t.nodes;
@@ -325,7 +325,7 @@ void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L
   if (n == 0) return;
   for (int a = 0; a < 3; ++a) L.origin[a] = nodes[0].c[a];
   const double inv_theta = 1.0 / theta;
-  // group_lists.cpp:147-166: FP32 bodies and cells relative to the root centre.
+  // group_lists.cpp:40-63: FP32 bodies and cells relative to the root centre.
   std::vector<WalkNode> wn(nodes.size());
   for (uint64_t k = 0; k < n; ++k) {
     L.bodies[k] = F4{float(t.tpos[3 * k] - L.origin[0]), float(t.tpos[3 * k + 1] - L.origin[1]),
@@ -351,7 +351,7 @@ void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L
     w.end = nd.end;
   }
 
-  // group_lists.cpp:168-216: groups in tree order.
+  // group_lists.cpp:65-113: groups in tree order.
   const uint32_t gs = static_cast<uint32_t>(group_size);
   std::vector<std::pair<uint32_t, uint32_t>> specs;
   auto cut = [&](uint32_t b, uint32_t e) {
@@ -387,7 +387,7 @@ void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L
   };
   visit(0);
 
-  // group_lists.cpp:218-269: one FP64 walk per group.
+  // group_lists.cpp:115-166: one FP64 walk per group.
   const long ng = static_cast<long>(specs.size());
   std::vector<std::vector<uint32_t>> gc_list(ng), gb_list(ng);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -439,7 +439,7 @@ void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L
     }
   }
 
-  // group_lists.cpp:271-309: concatenation and statistics.
+  // group_lists.cpp:168-206: concatenation and statistics.
   uint64_t nc = 0, nb = 0;
   L.groups.resize(ng);
   for (long g = 0; g < ng; ++g) {
@@ -466,7 +466,7 @@ void make_lists(const Tree &t, double theta, int group_size, bool pack, Lists &L
   }
 }
 
-// gravity_kernels.h:138-170 + gravity_backends.cpp:167-181
+// gravity_kernels.h:138-170 + gravity_backends.cpp:70-84
 void eval_lists(const Lists &L, float G, float eps2, bool quad, float *out) {
   const long ng = static_cast<long>(L.groups.size());
 #pragma omp parallel for schedule(dynamic, 8)
@@ -490,7 +490,7 @@ void eval_lists(const Lists &L, float G, float eps2, bool quad, float *out) {
   }
 }
 
-// src/octree.cpp:404-473 (Barnes 1994 offset criterion, FP64).
+// src/octree.cpp:275-344 (Barnes 1994 offset criterion, FP64).
 void tree_accel(const Tree &t, const double *x, uint64_t self, double G, double theta,
                 double eps, bool quad, double *acc, double &pot, uint64_t &ncell,
                 uint64_t &npart) {

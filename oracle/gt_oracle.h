@@ -32,10 +32,10 @@ typedef struct gto_run gto_run;
 
 gto_run *gto_new(void);
 void gto_free(gto_run *r);
-/* Octree::build (src/octree.cpp:298-402) with serial node numbering. */
+/* Octree::build (src/octree.cpp:169-273) with serial node numbering. */
 int gto_build(gto_run *r, uint64_t n, const double *pos, const double *mass, int leaf_size,
               uint64_t parallel_threshold);
-/* build_group_lists (src/group_lists.cpp:121-310) on the tree held by r. */
+/* build_group_lists (src/group_lists.cpp:18-207) on the tree held by r. */
 int gto_lists(gto_run *r, double theta, int group_size, int pack);
 void gto_info(const gto_run *r, uint64_t *info);
 void gto_nodes(const gto_run *r, double *geom, int32_t *links);
@@ -43,7 +43,7 @@ void gto_order(const gto_run *r, uint64_t *order);
 void gto_tree_positions(const gto_run *r, double *pos, double *mass);
 void gto_lists_get(const gto_run *r, uint32_t *groups, uint32_t *cell_list, uint32_t *body_list,
                    float *bodies, float *cells, double *origin);
-/* evaluate_group_lists_cpu (src/gravity_backends.cpp:167-181): out n x 4 floats, tree order. */
+/* evaluate_group_lists_cpu (src/gravity_backends.cpp:70-84): out n x 4 floats, tree order. */
 void gto_eval(const gto_run *r, double G, double eps, int quadrupole, float *out);
 /* Octree::accel_pot per particle = compute_gravity_tree (src/gravity.cpp:7-26). */
 void gto_tree_gravity(gto_run *r, uint64_t n, const double *pos, const double *mass, double G,
@@ -58,10 +58,10 @@ void gto_direct_f64_sample(uint64_t n, const double *pos, const double *mass, do
                            double eps, uint64_t ns, const uint64_t *sample, double *acc);
 /* gt_direct_accel for every target (include/.../gravity_kernels.h:173-182). */
 void gto_direct_f32(uint64_t n, const float *bodies, double G, double eps, float *out);
-/* pack_bodies (src/gravity_backends.cpp:112-120). */
+/* pack_bodies (src/gravity_backends.cpp:15-23). */
 void gto_pack_bodies(uint64_t n, const double *pos, const double *mass, const double *origin,
                      float *out);
-/* GroupTreeGravity::compute on the cpu device (src/gravity_backends.cpp:206-234). */
+/* GroupTreeGravity::compute on the cpu device (src/gravity_backends.cpp:109-137). */
 int gto_group_gravity(gto_run *r, uint64_t n, const double *pos, const double *mass, double G,
                       double theta, double eps, int quadrupole, int group_size, int leaf_size,
                       int pack, double *acc, double *pot);

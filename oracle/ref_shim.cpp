@@ -372,7 +372,7 @@ int gtref_max_threads(void) {
 #endif
 }
 
-// make_hernquist_galaxy (src/hernquist_ics.cpp:253-406) with the model's
+// make_hernquist_galaxy (src/hernquist_ics.cpp:142-419) with the model's
 // numeric fields in `model` (order: G, disk_n, disk_mass, disk_h, disk_z0,
 // disk_rmax, disk_Q, disk_Q_radius, bulge_n, bulge_mass, bulge_a, bulge_rmax,
 // halo_n, halo_mass, halo_core, halo_cutoff, halo_rmax, vesc_fraction, seed,

@@ -75,7 +75,7 @@ def test_ics_component_order_and_centre_of_mass():
     import gt_ics
     m = gt_ics.model(disk_n=300, bulge_n=100, halo_n=600, seed=7)
     pos, vel, mass, typ = gt_ics.galaxy(m)
-    # halo, then disk, then bulge (src/hernquist_ics.cpp:290-315), GADGET types
+    # halo, then disk, then bulge (src/hernquist_ics.cpp:175-204), GADGET types
     assert (typ[:600] == 1).all() and (typ[600:900] == 2).all() and (typ[900:] == 3).all()
     com = (mass[:, None] * pos).sum(0) / mass.sum()
     cv = (mass[:, None] * vel).sum(0) / mass.sum()

@@ -1,6 +1,6 @@
 """The per-particle FP64 tree solver on the GPU (gt_hip_tree_gravity) against
 the oracle's restatement of compute_gravity_tree (src/gravity.cpp:7-26,
-Octree::accel_pot src/octree.cpp:276-344), itself pinned to the reference's
+Octree::accel_pot src/octree.cpp:275-344), itself pinned to the reference's
 own build: accelerations, potentials and interaction counts bit for bit."""
 import numpy as np
 import pytest

@@ -77,7 +77,7 @@ def test_oracle_solvers_equal_reference(oracle, ref):
 
 def test_parallel_reference_build_is_the_same_tree(oracle, ref):
     # the reference numbers nodes differently when it builds in parallel
-    # (src/octree.cpp:337-395); canonically the trees are equal
+    # (src/octree.cpp:208-266); canonically the trees are equal
     pos, mass = oracle.test_system(0, 50000, 3)
     oracle.build(pos, mass, 8)
     ref.build(pos, mass, 8, 20000)
