@@ -103,6 +103,88 @@ def force_errors(ctx, pos, mass, acc, cfg, offsets=None, size=1000):
     return float(err[len(err) // 2]), float(err[len(err) * 99 // 100]), len(err)
 
 
+def bench_neighbours(args):
+    """Octree::build + Octree::for_each_within for every particle of the 2M
+    encounter, each with the radius that holds its 32 nearest other particles
+    (an SPH smoothing length; found with scipy's k-d tree before timing).  One
+    step = one gt_hip_neighbours call; value = queries per second of GPU time
+    (build, ordering, counting walk, scan, filling walk; HIP events), the
+    PCIe-inclusive wall rate alongside."""
+    import gt_hip
+    from scipy.spatial import cKDTree
+    pos, _, mass, _ = encounter_2m(args.halo)
+    n = len(mass)
+    t0 = time.time()
+    radii = cKDTree(pos).query(pos, k=33, workers=-1)[0][:, -1].copy()
+    print(f"# radii (33rd nearest, self included) in {time.time() - t0:.1f} s: "
+          f"median {np.median(radii):.4f}", file=sys.stderr, flush=True)
+    ctx = gt_hip.Context(0)
+    total = None
+    for _ in range(max(args.warmup, 1)):
+        off, idx, d2 = ctx.neighbours(pos, pos, radii, mass=mass, max_out=total)
+        total = int(off[-1])
+    gpu, wall, phases = [], [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        off, idx, d2 = ctx.neighbours(pos, pos, radii, mass=mass, max_out=total)
+        wall.append(time.perf_counter() - t)
+        tm = ctx.timings()
+        gpu.append(tm["total"])
+        phases.append((tm["tree"], tm["lists"], tm["evaluate"]))
+    ms = float(np.mean(gpu))
+    ph = np.mean(np.array(phases), axis=0)
+    print(f"# step (GPU) {ms:.2f} ms: build {ph[0]:.2f}, ordering + counting walk + scan "
+          f"{ph[1]:.2f}, filling walk {ph[2]:.2f}; wall {1e3 * np.mean(wall):.1f} ms; "
+          f"{total} neighbours", file=sys.stderr, flush=True)
+    # Filling walk: compulsory bytes = the lists written (8 B index + 8 B squared
+    # distance per neighbour), the queries read (24 B centre, 8 B radius) with
+    # their order, count and offset (12 B), the tree-order particles (24 B) and
+    # the node records (64 B), each once.
+    nodes = ctx.info()["n_nodes"]
+    traffic = 16 * total + 44 * n + 24 * n + 64 * nodes
+    achieved = traffic / (ph[2] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "within<true>",
+                "kernel_ms": round(float(ph[2]), 4), "bytes_per_launch": traffic,
+                "note": "a dependent walk per lane: latency-bound, not bandwidth-bound"}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from pyoracle import Checker, available
+        kind = "reference" if available("ref") else "port"
+        chk = Checker("ref" if kind == "reference" else "oracle")
+        chk.build(pos, mass, 8)
+        m = 200_000
+        t = time.perf_counter()
+        chk.within(pos[:m], radii[:m])
+        el = time.perf_counter() - t
+        cpu = {"value": round(m / el, 1), "unit": "queries/s",
+               "cores": 1 if kind == "reference" else chk.max_threads(), "kind": kind,
+               "sample": f"the first {m} queries of the same workload on the same tree "
+                         f"(Octree::for_each_within per query, {el:.1f} s)"}
+    line = {
+        "metric": "neighbour queries/s (Octree::for_each_within) at N=2M, 32 neighbours each",
+        "value": round(n / (ms * 1e-3), 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "wall_ms_per_call": round(1e3 * float(np.mean(wall)), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64",
+        "data": "synthetic: the 2M Hernquist encounter's initial positions",
+        "config": {"workload": "neighbours-2M", "n": n, "queries": n, "leaf_size": 8,
+                   "radius": "33rd nearest (self included)", "halo": args.halo},
+        "neighbours": total,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,10 +203,12 @@ def main():
     ap.add_argument("--split-len", type=int, default=0,
                     help="tuning: list entries per evaluation part (0: default, <0: never split)")
     ap.add_argument("--workload", default="encounter-2m",
-                    choices=["encounter-2m", "batch-128", "galaxy-4m"],
+                    choices=["encounter-2m", "batch-128", "galaxy-4m", "neighbours-2m"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
                          "independent galaxies of 2k-100k particles evaluated together; "
-                         "galaxy-4m: one 4M-particle galaxy (seed 6)")
+                         "galaxy-4m: one 4M-particle galaxy (seed 6); neighbours-2m: "
+                         "fixed-radius neighbour queries on the encounter's tree (every "
+                         "particle a query, radius = its distance to its 32nd neighbour)")
     ap.add_argument("--theta", type=float, default=0.75)
     ap.add_argument("--predict", type=int, default=0,
                     help="list slots: 0 uniform, 1 learned predictor (trained on the first "
@@ -154,6 +238,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
               "is design-only in this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "neighbours-2m":
+        bench_neighbours(args)
+        return
 
     import gt_hip
     cfg = dict(G=1.0, theta=args.theta, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,

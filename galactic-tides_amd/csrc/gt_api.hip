@@ -174,6 +174,7 @@ struct gt_hip_context {
   DBuf<uint32_t> nb_counts, nb_offs, nb_perm, nb_perm_alt;
   DBuf<uint64_t> nb_keys, nb_keys_alt;
   DBuf<unsigned long long> nb_index, nb_tmp, nb_sort_tmp;
+  DBuf<uint4> nb_nodes;  // 64-byte query-walk records, 4 uint4 each
 };
 
 namespace {
@@ -1280,6 +1281,10 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
   c->tree.set_counts(static_cast<uint32_t>(c->fw[frame::kNodes]),
                      static_cast<int>(c->fw[frame::kDepth]));
   c->lists_n = 0;  // (the last lists referred to the tree just replaced)
+  c->info = gt_lists_info{};
+  c->info.n = n;
+  c->info.n_nodes = c->fw[frame::kNodes];
+  c->info.depth = c->fw[frame::kDepth];
   return p;
 }
 
@@ -1541,11 +1546,11 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
     cfg.theta = 1.0;
     cfg.group_size = 32;
     cfg.leaf_size = leaf_size;
-    const Plan p = build_single_tree(c, &cfg, n32, dp, dm);
     double *dc = c->nb_centres.ensure(3 * nq);
     double *dr = c->nb_radii.ensure(n_radii);
     upload(dc, centres, 3 * nq, s);
     upload(dr, radii, n_radii, s);
+    const Plan p = build_single_tree(c, &cfg, n32, dp, dm);  // (times from here on)
     uint32_t *counts = c->nb_counts.ensure(nq), *offs = c->nb_offs.ensure(nq);
     unsigned long long *tmp = c->nb_tmp.ensure(scan_tmp_words(nq));
     unsigned long long *dev = c->misc.ensure(4);  // [0] error bits, [1] total
@@ -1557,44 +1562,48 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
     launch_query_keys(c->tree.geom(), frame, dc, nq32, keys, perm, s);
     radix_sort_u64(keys, perm, c->nb_keys_alt.ensure(nq), c->nb_perm_alt.ensure(nq), nq,
                    kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s);
+    void *recs = c->nb_nodes.ensure((nb_record_bytes(p.node_cap) + 15) / 16);
+    launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), frame, p.node_cap, recs, s);
     launch_fill_u64(dev, 2, 0ull, s);
-    launch_within(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
-                  c->tree.order(), n32, frame, p.node_cap, dc, dr, stride, nq32, perm, counts,
-                  nullptr, nullptr, nullptr, dev, s);
+    launch_within(recs, c->tree.tree_pos(), c->tree.order(), n32, frame, p.node_cap, dc, dr,
+                  stride, nq32, perm, counts, nullptr, nullptr, nullptr, dev, s);
     exclusive_scan_u32(counts, offs, nq, dev + 1, tmp, s);
+    GTH_CHECK(hipEventRecord(c->ev[3], s));
     unsigned long long h[2] = {0, 0};
     download(h, dev, 2, s);
-    std::vector<uint32_t> ho(nq);
-    download(ho.data(), offs, nq, s);
     GTH_CHECK(hipStreamSynchronize(s));
     if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: malformed tree");
     if (h[1] > 0xffffffffull)
       fail(GT_HIP_ERR_LIMIT, "more than 2^32 - 1 neighbours in one call: split the queries");
+    const bool fill = h[1] && (index || dist2) && h[1] <= max_out;
+    if (fill) {
+      unsigned long long *di = c->nb_index.ensure(h[1]);
+      double *dd = c->nb_dist2.ensure(h[1]);
+      GTH_CHECK(hipEventRecord(c->ev[4], s));
+      launch_within(recs, c->tree.tree_pos(), c->tree.order(), n32, frame, p.node_cap, dc, dr,
+                    stride, nq32, perm, counts, offs, di, dd, dev, s);
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+      if (index) download(reinterpret_cast<unsigned long long *>(index), di, h[1], s);
+      if (dist2) download(dist2, dd, h[1], s);
+    } else {
+      GTH_CHECK(hipEventRecord(c->ev[4], s));
+      GTH_CHECK(hipEventRecord(c->ev[2], s));
+    }
+    std::vector<uint32_t> ho(nq);
+    download(ho.data(), offs, nq, s);
+    download(h, dev, 1, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+    if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: the count and fill walks differ");
     for (uint64_t q = 0; q < nq; ++q) offsets[q] = ho[q];
     offsets[nq] = h[1];
     if (total) *total = h[1];
     if ((index || dist2) && h[1] > max_out)
       fail(GT_HIP_ERR_LIMIT, "max_out too small: the queries have " + std::to_string(h[1]) +
                                  " neighbours (offsets and total are written)");
-    if (h[1] && (index || dist2)) {
-      unsigned long long *di = c->nb_index.ensure(h[1]);
-      double *dd = c->nb_dist2.ensure(h[1]);
-      launch_within(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
-                    c->tree.order(), n32, frame, p.node_cap, dc, dr, stride, nq32, perm,
-                    counts, offs, di, dd, dev, s);
-      GTH_CHECK(hipEventRecord(c->ev[2], s));
-      if (index) download(reinterpret_cast<unsigned long long *>(index), di, h[1], s);
-      if (dist2) download(dist2, dd, h[1], s);
-      download(h, dev, 1, s);
-      GTH_CHECK(hipStreamSynchronize(s));
-      if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: the count and fill walks differ");
-    } else {
-      GTH_CHECK(hipEventRecord(c->ev[2], s));
-      GTH_CHECK(hipStreamSynchronize(s));
-    }
     c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
-    c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
-    c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);
+    c->tm.lists = elapsed_ms(c->ev[1], c->ev[3]);
+    c->tm.evaluate = elapsed_ms(c->ev[4], c->ev[2]);
+    c->tm.total = c->tm.tree + c->tm.lists + c->tm.evaluate;
     c->gpu_seconds = c->tm.total * 1e-3;
   });
 }

@@ -12,7 +12,9 @@
 //
 // Lanes take the queries in the order of a 30-bit Morton key of their centres
 // in the root cube (query_keys + a radix sort), so the lanes of a wave walk
-// nearby parts of the tree; results stay in the caller's query order.
+// nearby parts of the tree; results stay in the caller's query order.  A node
+// visit reads one 64-byte record (centre, half, children, body range, skip
+// link) instead of three arrays.
 #include "gt_internal.h"
 
 namespace gth {
@@ -20,18 +22,51 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// What a query walk reads of a node, in one 64-byte record: the cube (FP64
+// centre and half size), the first child and child count (0: a leaf), the
+// body range and the skip link.
+struct NbNode {
+  double cx, cy, cz, half;
+  uint32_t first, children, begin, end;
+  uint32_t skip, pad0, pad1, pad2;
+};
+static_assert(sizeof(NbNode) == 64, "NbNode layout");
+
+__global__ __launch_bounds__(kBlock) void nb_records(const double *__restrict__ geom,
+                                                     const int4 *__restrict__ lnk,
+                                                     const uint32_t *__restrict__ skip,
+                                                     const unsigned long long *__restrict__ frame,
+                                                     uint32_t node_cap, NbNode *__restrict__ out) {
+  const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= nn) return;
+  const double *g = geom + static_cast<size_t>(kGeomStride) * x;
+  const int4 L = lnk[x];
+  NbNode r;
+  r.cx = g[0];
+  r.cy = g[1];
+  r.cz = g[2];
+  r.half = g[3];
+  r.first = static_cast<uint32_t>(L.x);
+  r.children = static_cast<uint32_t>(L.y);
+  r.begin = static_cast<uint32_t>(L.z);
+  r.end = static_cast<uint32_t>(L.w);
+  r.skip = skip[x];
+  r.pad0 = r.pad1 = r.pad2 = 0;
+  out[x] = r;
+}
+
 #pragma clang fp contract(off)
 
 // Squared distance from c to the node's cube, as the reference sums it
 // (axes in order, only the axes where c is outside the slab).
-__device__ inline double box_d2(const double *g, double cx, double cy, double cz) {
-  const double half = g[3];
+__device__ inline double box_d2(const NbNode &g, double cx, double cy, double cz) {
   double d2 = 0.0;
-  const double dx = fabs(cx - g[0]) - half;
+  const double dx = fabs(cx - g.cx) - g.half;
   if (dx > 0) d2 += dx * dx;
-  const double dy = fabs(cy - g[1]) - half;
+  const double dy = fabs(cy - g.cy) - g.half;
   if (dy > 0) d2 += dy * dy;
-  const double dz = fabs(cz - g[2]) - half;
+  const double dz = fabs(cz - g.cz) - g.half;
   if (dz > 0) d2 += dz * dz;
   return d2;
 }
@@ -41,8 +76,7 @@ __device__ inline double box_d2(const double *g, double cx, double cy, double cz
 // Queries are taken in the order `perm` gives (null: as they are).
 template <bool kFill>
 __global__ __launch_bounds__(kBlock) void within(
-    const double *__restrict__ geom, const int4 *__restrict__ lnk,
-    const uint32_t *__restrict__ skip, const double *__restrict__ tpos,
+    const NbNode *__restrict__ nodes, const double *__restrict__ tpos,
     const uint32_t *__restrict__ order, uint32_t n, const unsigned long long *__restrict__ frame,
     uint32_t node_cap, const double *__restrict__ centres, const double *__restrict__ radii,
     uint32_t radius_stride, uint32_t nq, const uint32_t *__restrict__ perm,
@@ -73,14 +107,13 @@ __global__ __launch_bounds__(kBlock) void within(
       atomicOr(error, 1ull);  // malformed tree (never for a built one)
       break;
     }
-    const double *g = geom + static_cast<size_t>(kGeomStride) * cur;
+    const NbNode g = nodes[cur];
     if (box_d2(g, cx, cy, cz) > r2) {
-      cur = skip[cur];
+      cur = g.skip;
       continue;
     }
-    const int4 L = lnk[cur];
-    if (L.y == 0) {
-      const uint32_t b = static_cast<uint32_t>(L.z), e = min(static_cast<uint32_t>(L.w), n);
+    if (g.children == 0) {
+      const uint32_t b = g.begin, e = min(g.end, n);
       for (uint32_t j = b; j < e; ++j) {
         const double dx = tpos[3 * static_cast<size_t>(j)] - cx;
         const double dy = tpos[3 * static_cast<size_t>(j) + 1] - cy;
@@ -98,9 +131,9 @@ __global__ __launch_bounds__(kBlock) void within(
           }
         }
       }
-      cur = skip[cur];
+      cur = g.skip;
     } else {
-      cur = static_cast<uint32_t>(L.x + L.y - 1);  // the last child first
+      cur = g.first + g.children - 1;  // the last child first
     }
   }
   if (kFill) {
@@ -159,22 +192,36 @@ void launch_query_keys(const double *geom, const unsigned long long *frame,
   GTH_LAUNCH_CHECK();
 }
 
-void launch_within(const double *geom, const int4 *lnk, const uint32_t *skip, const double *tpos,
-                   const uint32_t *order, uint32_t n, const unsigned long long *frame,
-                   uint32_t node_cap, const double *centres, const double *radii,
-                   uint32_t radius_stride, uint32_t nq, const uint32_t *perm, uint32_t *counts,
-                   const uint32_t *offs, unsigned long long *index, double *dist2,
-                   unsigned long long *error, hipStream_t s) {
+size_t nb_record_bytes(uint32_t node_cap) {
+  return static_cast<size_t>(node_cap) * sizeof(NbNode);
+}
+
+void launch_nb_records(const double *geom, const int4 *lnk, const uint32_t *skip,
+                       const unsigned long long *frame, uint32_t node_cap, void *records,
+                       hipStream_t s) {
+  if (!node_cap) return;
+  hipLaunchKernelGGL(nb_records, dim3((node_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, geom,
+                     lnk, skip, frame, node_cap, static_cast<NbNode *>(records));
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_within(const void *records, const double *tpos, const uint32_t *order, uint32_t n,
+                   const unsigned long long *frame, uint32_t node_cap, const double *centres,
+                   const double *radii, uint32_t radius_stride, uint32_t nq,
+                   const uint32_t *perm, uint32_t *counts, const uint32_t *offs,
+                   unsigned long long *index, double *dist2, unsigned long long *error,
+                   hipStream_t s) {
   if (!nq) return;
   const dim3 grid((nq + kBlock - 1) / kBlock);
+  const NbNode *nodes = static_cast<const NbNode *>(records);
   if (offs) {
-    hipLaunchKernelGGL(within<true>, grid, dim3(kBlock), 0, s, geom, lnk, skip, tpos, order, n,
-                       frame, node_cap, centres, radii, radius_stride, nq, perm, counts, offs,
-                       index, dist2, error);
+    hipLaunchKernelGGL(within<true>, grid, dim3(kBlock), 0, s, nodes, tpos, order, n, frame,
+                       node_cap, centres, radii, radius_stride, nq, perm, counts, offs, index,
+                       dist2, error);
   } else {
-    hipLaunchKernelGGL(within<false>, grid, dim3(kBlock), 0, s, geom, lnk, skip, tpos, order, n,
-                       frame, node_cap, centres, radii, radius_stride, nq, perm, counts, offs,
-                       index, dist2, error);
+    hipLaunchKernelGGL(within<false>, grid, dim3(kBlock), 0, s, nodes, tpos, order, n, frame,
+                       node_cap, centres, radii, radius_stride, nq, perm, counts, offs, index,
+                       dist2, error);
   }
   GTH_LAUNCH_CHECK();
 }

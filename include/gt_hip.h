@@ -175,7 +175,10 @@ int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_
  * on it).  offsets and *total (may be NULL) are always written.  index and dist2
  * (each may be NULL; both NULL: the call only counts) are written when the total
  * fits max_out; otherwise the call returns GT_HIP_ERR_LIMIT and can be made again
- * with room for *total entries.  At most 2^32 - 1 neighbours in all per call. */
+ * with room for *total entries.  At most 2^32 - 1 neighbours in all per call.
+ * gt_hip_last_timings: tree = the build, lists = query ordering, counting walk
+ * and scan, evaluate = the filling walk, total = their sum (GPU time; the
+ * host copies and the wait for the count between the passes excluded). */
 int gt_hip_neighbours(gt_hip_context *ctx, int leaf_size, uint64_t n, const double *pos,
                       const double *mass, uint64_t nq, const double *centres,
                       const double *radii, uint64_t n_radii, uint64_t max_out,

@@ -42,6 +42,8 @@ def test_neighbours_equal_oracle(hip, oracle, kind, n, seed, leaf):
     for radii in (rng.uniform(0.0, 0.5, len(centres)), np.array([0.1])):
         got = hip.neighbours(pos, centres, radii, mass=mass, leaf_size=leaf)
         check_same(got, oracle.within(centres, radii))
+        assert hip.info()["n_nodes"] == oracle.info()["n_nodes"]
+        assert hip.info()["depth"] == oracle.info()["depth"]
 
 
 def test_neighbours_without_masses(hip, oracle):
