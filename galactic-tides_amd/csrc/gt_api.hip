@@ -1563,7 +1563,8 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
     radix_sort_u64(keys, perm, c->nb_keys_alt.ensure(nq), c->nb_perm_alt.ensure(nq), nq,
                    kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s);
     void *recs = c->nb_nodes.ensure((nb_record_bytes(p.node_cap) + 15) / 16);
-    launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), frame, p.node_cap, recs, s);
+    launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(), frame,
+                      p.node_cap, recs, s);
     launch_fill_u64(dev, 2, 0ull, s);
     launch_within(recs, c->tree.tree_pos(), c->tree.order(), n32, frame, p.node_cap, dc, dr,
                   stride, nq32, perm, counts, nullptr, nullptr, nullptr, dev, s);

@@ -351,8 +351,8 @@ void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *sk
 // malformed tree, bit 1 the passes disagree or perm is out of range.
 size_t nb_record_bytes(uint32_t node_cap);
 void launch_nb_records(const double *geom, const int4 *lnk, const uint32_t *skip,
-                       const unsigned long long *frame, uint32_t node_cap, void *records,
-                       hipStream_t s);
+                       const uint32_t *depth, const unsigned long long *frame, uint32_t node_cap,
+                       void *records, hipStream_t s);
 void launch_within(const void *records, const double *tpos, const uint32_t *order, uint32_t n,
                    const unsigned long long *frame, uint32_t node_cap, const double *centres,
                    const double *radii, uint32_t radius_stride, uint32_t nq,
