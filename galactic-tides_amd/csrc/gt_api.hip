@@ -1492,7 +1492,9 @@ int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t 
     unsigned long long *frame = c->frame.get();
     const Plan p = build_single_tree(c, cfg, n32, dp, dm);
     const uint32_t cap = p.node_cap;
-    launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(), c->tree.tree_pos(),
+    launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(),
+                        c->depth.get(),
+                        c->tree.tree_pos(),
                         c->tree.tree_mass(), c->tree.order(), n32, frame, cap, cfg->G, cfg->eps,
                         cfg->theta, cfg->quadrupole != 0, da, dq, st_dev, s);
     GTH_CHECK(hipEventRecord(c->ev[2], s));

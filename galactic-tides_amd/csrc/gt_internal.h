@@ -333,13 +333,13 @@ void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_
                         hipStream_t s);
 
 // gt_treegrav.hip: the per-particle FP64 tree solver over a built tree (one
-// system, root node 0): acc (N x 3) and pot in original order, stats[0..1] =
-// cell and particle interactions.
+// system, root node 0; skip and depth from launch_link_up): acc (N x 3) and
+// pot in original order, stats[0..1] = cell and particle interactions.
 void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *skip,
-                         const double *tpos, const double *tmass, const uint32_t *order,
-                         uint32_t n, const unsigned long long *frame, uint32_t node_cap, double G,
-                         double eps, double theta, int quadrupole, double *acc, double *pot,
-                         unsigned long long *stats, hipStream_t s);
+                         const uint32_t *depth, const double *tpos, const double *tmass,
+                         const uint32_t *order, uint32_t n, const unsigned long long *frame,
+                         uint32_t node_cap, double G, double eps, double theta, int quadrupole,
+                         double *acc, double *pot, unsigned long long *stats, hipStream_t s);
 
 // gt_neighbours.hip: fixed-radius queries over a built tree (one system, root
 // node 0).  launch_nb_records packs each node's cube, children, body range

@@ -19,38 +19,63 @@ constexpr int kBlock = 256;
 
 #pragma clang fp contract(off)
 
+// One wave walks for 64 consecutive particles (in tree order, so their walks
+// share most nodes): it visits every node any lane would visit, in the LIFO
+// order, and a lane takes part at a node only while every ancestor was opened
+// for it (active masks per depth in LDS).  Each lane therefore meets exactly
+// the nodes of its own walk, in order, and sums the same interactions in the
+// same order as a walk of its own.  Node data are read at wave-uniform
+// addresses and the wave's control flow is uniform: at 2M particles (theta
+// 0.75) 12.6 ms, against 28.2 ms with one independent walk per lane.
+constexpr int kMaxLevels = 64;
+
 __global__ __launch_bounds__(kBlock) void tree_gravity(
     const double *__restrict__ geom, const int4 *__restrict__ lnk,
-    const uint32_t *__restrict__ skip, const double *__restrict__ tpos,
-    const double *__restrict__ tmass, const uint32_t *__restrict__ order, uint32_t n,
-    const unsigned long long *__restrict__ frame, uint32_t node_cap, double G, double eps2,
-    double inv_theta, int quadrupole, double *__restrict__ acc, double *__restrict__ pot,
-    unsigned long long *__restrict__ stats) {
+    const uint32_t *__restrict__ skip, const uint32_t *__restrict__ depth,
+    const double *__restrict__ tpos, const double *__restrict__ tmass,
+    const uint32_t *__restrict__ order, uint32_t n, const unsigned long long *__restrict__ frame,
+    uint32_t node_cap, double G, double eps2, double inv_theta, int quadrupole,
+    double *__restrict__ acc, double *__restrict__ pot, unsigned long long *__restrict__ stats) {
+  __shared__ unsigned long long masks[kBlock / 64][kMaxLevels];
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = k < n;
   unsigned long long ncell = 0, npart = 0;
-  if (k < n && nn > 0) {
-    const uint32_t self = order[k];
-    const double x = tpos[3 * static_cast<size_t>(k)], y = tpos[3 * static_cast<size_t>(k) + 1],
-                 z = tpos[3 * static_cast<size_t>(k) + 2];
-    double ax = 0, ay = 0, az = 0, phi = 0;
-    uint32_t cur = 0, budget = nn;  // a walk visits each node once at most
-    while (cur != kNone) {
-      if (cur >= nn || budget-- == 0) {
-        phi = __builtin_nan("");  // malformed tree (never for a built one)
-        break;
-      }
-      const double *g = geom + static_cast<size_t>(kGeomStride) * cur;
-      const int4 L = lnk[cur];
-      const double dx = x - g[4], dy = y - g[5], dz = z - g[6];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double half = g[3];
-      const double crit = 2.0 * half * inv_theta + g[14];
-      const double slack = half * 1.0001;
-      const bool inside = fabs(x - g[0]) <= slack && fabs(y - g[1]) <= slack &&
-                          fabs(z - g[2]) <= slack;
-      if (!inside && r2 > crit * crit) {
-        // the cell as a multipole
+  uint32_t self = 0;
+  double x = 0, y = 0, z = 0;
+  if (valid) {
+    self = order[k];
+    x = tpos[3 * static_cast<size_t>(k)];
+    y = tpos[3 * static_cast<size_t>(k) + 1];
+    z = tpos[3 * static_cast<size_t>(k) + 2];
+  }
+  double ax = 0, ay = 0, az = 0, phi = 0;
+  const unsigned long long me = 1ull << lane;
+  unsigned long long *stack = masks[wave];
+  stack[0] = __ballot(valid);
+  uint32_t cur = (nn > 0 && stack[0] != 0) ? 0u : kNone, d = 0, budget = nn;
+  bool broken = false;
+  while (cur != kNone) {
+    if (cur >= nn || budget-- == 0 || d >= kMaxLevels - 1) {
+      broken = true;  // malformed tree (never for a built one)
+      break;
+    }
+    const uint32_t c = __builtin_amdgcn_readfirstlane(cur);
+    const double *g = geom + static_cast<size_t>(kGeomStride) * c;
+    const int4 L = lnk[c];
+    const bool act = (stack[d] & me) != 0;
+    const double dx = x - g[4], dy = y - g[5], dz = z - g[6];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double half = g[3];
+    const double crit = 2.0 * half * inv_theta + g[14];
+    const double slack = half * 1.0001;
+    const bool inside = fabs(x - g[0]) <= slack && fabs(y - g[1]) <= slack &&
+                        fabs(z - g[2]) <= slack;
+    const bool accept = act && !inside && r2 > crit * crit;
+    const bool open = act && !accept;
+    if (__ballot(accept)) {
+      if (accept) {  // the cell as a multipole
         const double m = g[7];
         const double r2s = r2 + eps2;
         const double inv = __ddiv_rn(1.0, __dsqrt_rn(r2s));
@@ -74,29 +99,43 @@ __global__ __launch_bounds__(kBlock) void tree_gravity(
           phi -= 0.5 * G * rqr * inv5;
         }
         ++ncell;
-        cur = skip[cur];
-      } else if (L.y == 0) {
-        const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-        for (uint32_t j = b; j < e && j < n; ++j) {
-          if (order[j] == self) continue;
+      }
+    }
+    const unsigned long long opened = __ballot(open);
+    if (L.y == 0) {
+      if (opened) {
+        const uint32_t b = static_cast<uint32_t>(L.z), e = min(static_cast<uint32_t>(L.w), n);
+        for (uint32_t j = b; j < e; ++j) {
+          const uint32_t oj = order[j];
           const double ex = x - tpos[3 * static_cast<size_t>(j)],
                        ey = y - tpos[3 * static_cast<size_t>(j) + 1],
                        ez = z - tpos[3 * static_cast<size_t>(j) + 2];
-          const double r2s = ex * ex + ey * ey + ez * ez + eps2;
-          const double inv = __ddiv_rn(1.0, __dsqrt_rn(r2s));
-          const double mj = tmass[j];
-          const double gm3 = G * mj * inv * inv * inv;
-          ax -= gm3 * ex;
-          ay -= gm3 * ey;
-          az -= gm3 * ez;
-          phi -= G * mj * inv;
-          ++npart;
+          if (open && oj != self) {
+            const double r2s = ex * ex + ey * ey + ez * ez + eps2;
+            const double inv = __ddiv_rn(1.0, __dsqrt_rn(r2s));
+            const double mj = tmass[j];
+            const double gm3 = G * mj * inv * inv * inv;
+            ax -= gm3 * ex;
+            ay -= gm3 * ey;
+            az -= gm3 * ez;
+            phi -= G * mj * inv;
+            ++npart;
+          }
         }
-        cur = skip[cur];
-      } else {
-        cur = static_cast<uint32_t>(L.x + L.y - 1);  // the last child first
       }
+      cur = skip[c];
+      d = cur < nn ? depth[cur] : 0u;
+    } else if (opened) {
+      stack[d + 1] = opened;  // lanes active below this node
+      cur = static_cast<uint32_t>(L.x + L.y - 1);  // the last child first
+      ++d;
+    } else {
+      cur = skip[c];
+      d = cur < nn ? depth[cur] : 0u;
     }
+  }
+  if (broken) phi = __builtin_nan("");
+  if (valid) {
     acc[3 * static_cast<size_t>(self)] = ax;
     acc[3 * static_cast<size_t>(self) + 1] = ay;
     acc[3 * static_cast<size_t>(self) + 2] = az;
@@ -106,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void tree_gravity(
     ncell += __shfl_xor(ncell, off, 64);
     npart += __shfl_xor(npart, off, 64);
   }
-  if ((threadIdx.x & 63) == 0 && (ncell | npart)) {
+  if (lane == 0 && (ncell | npart)) {
     atomicAdd(stats, ncell);
     atomicAdd(stats + 1, npart);
   }
@@ -117,15 +156,15 @@ __global__ __launch_bounds__(kBlock) void tree_gravity(
 }  // namespace
 
 void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *skip,
-                         const double *tpos, const double *tmass, const uint32_t *order,
-                         uint32_t n, const unsigned long long *frame, uint32_t node_cap, double G,
-                         double eps, double theta, int quadrupole, double *acc, double *pot,
-                         unsigned long long *stats, hipStream_t s) {
+                         const uint32_t *depth, const double *tpos, const double *tmass,
+                         const uint32_t *order, uint32_t n, const unsigned long long *frame,
+                         uint32_t node_cap, double G, double eps, double theta, int quadrupole,
+                         double *acc, double *pot, unsigned long long *stats, hipStream_t s) {
   launch_fill_u64(stats, 2, 0ull, s);
   if (!n) return;
   hipLaunchKernelGGL(tree_gravity, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, geom, lnk,
-                     skip, tpos, tmass, order, n, frame, node_cap, G, eps * eps, 1.0 / theta,
-                     quadrupole, acc, pot, stats);
+                     skip, depth, tpos, tmass, order, n, frame, node_cap, G, eps * eps,
+                     1.0 / theta, quadrupole, acc, pot, stats);
   GTH_LAUNCH_CHECK();
 }
 
