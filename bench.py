@@ -185,6 +185,78 @@ def bench_neighbours(args):
     print(json.dumps(line), flush=True)
 
 
+# FP64 operations per interaction of Octree::accel_pot as written (sqrt and
+# division one each; the opening tests not counted): a cell with quadrupole
+# (24 monopole + 49 quadrupole), a particle.
+TREE_CELL_FLOPS, TREE_PARTICLE_FLOPS = 73, 24
+FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 vector peak
+
+
+def bench_tree(args):
+    """compute_gravity_tree (Octree::accel_pot per particle, FP64) on the 2M
+    encounter's initial positions.  One step = one gt_hip_tree_gravity call
+    (tree build + walk); value = interactions per second of GPU time."""
+    import gt_hip
+    pos, _, mass, _ = encounter_2m(args.halo)
+    n = len(mass)
+    ctx = gt_hip.Context(0)
+    cfg = dict(theta=args.theta, eps=0.025, quadrupole=True, leaf_size=8)
+    for _ in range(max(args.warmup, 1)):
+        _, _, st = ctx.tree_gravity(pos, mass, **cfg)
+    tot, walk, build = [], [], []
+    for _ in range(args.steps):
+        _, _, st = ctx.tree_gravity(pos, mass, **cfg)
+        tm = ctx.timings()
+        tot.append(tm["total"])
+        walk.append(tm["evaluate"])
+        build.append(tm["tree"])
+    ms, walk_ms = float(np.mean(tot)), float(np.mean(walk))
+    cells, parts = int(st[0]), int(st[1])
+    print(f"# step (GPU) {ms:.2f} ms: build {np.mean(build):.2f}, walk {walk_ms:.2f}; "
+          f"{cells} cell + {parts} particle interactions", file=sys.stderr, flush=True)
+    flops = TREE_CELL_FLOPS * cells + TREE_PARTICLE_FLOPS * parts
+    achieved = flops / (walk_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "tree_gravity", "kernel_ms": round(walk_ms, 4),
+                "flops_per_launch": flops, "compute": "fp64 vector ALU"}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from pyoracle import Checker, available
+        if available("ref"):
+            chk = Checker("ref")
+            best = np.zeros(1)
+            t = time.time()
+            chk._time_tree(n, np.ascontiguousarray(pos), np.ascontiguousarray(mass), 1.0,
+                           args.theta, 0.025, 1, 8, 1, best)
+            cpu = {"value": round((cells + parts) / best[0], 1), "unit": "interactions/s",
+                   "cores": chk.max_threads(), "kind": "reference",
+                   "ms_per_force_eval": round(best[0] * 1e3, 1),
+                   "sample": f"the full 2M encounter: TreeGravity::compute (build + walk, "
+                             f"OpenMP), one warm-up then one timed call ({time.time() - t:.0f} s)"}
+    line = {
+        "metric": "FP64 tree-solver interactions/s (gravity = tree) at N=2M",
+        "value": round((cells + parts) / (ms * 1e-3), 1),
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64",
+        "data": "synthetic: the 2M Hernquist encounter's initial positions",
+        "config": {"workload": "tree-2M", "n": n, "theta": args.theta, "eps": 0.025,
+                   "leaf_size": 8, "quadrupole": True, "halo": args.halo},
+        "interactions": {"cells": cells, "particles": parts},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,12 +275,15 @@ def main():
     ap.add_argument("--split-len", type=int, default=0,
                     help="tuning: list entries per evaluation part (0: default, <0: never split)")
     ap.add_argument("--workload", default="encounter-2m",
-                    choices=["encounter-2m", "batch-128", "galaxy-4m", "neighbours-2m"],
+                    choices=["encounter-2m", "batch-128", "galaxy-4m", "neighbours-2m",
+                             "tree-2m"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
                          "independent galaxies of 2k-100k particles evaluated together; "
                          "galaxy-4m: one 4M-particle galaxy (seed 6); neighbours-2m: "
                          "fixed-radius neighbour queries on the encounter's tree (every "
-                         "particle a query, radius = its distance to its 32nd neighbour)")
+                         "particle a query, radius = its distance to its 32nd neighbour); "
+                         "tree-2m: the FP64 per-particle tree solver (gravity = tree) on the "
+                         "encounter")
     ap.add_argument("--theta", type=float, default=0.75)
     ap.add_argument("--predict", type=int, default=0,
                     help="list slots: 0 uniform, 1 learned predictor (trained on the first "
@@ -241,6 +316,9 @@ def main():
 
     if args.workload == "neighbours-2m":
         bench_neighbours(args)
+        return
+    if args.workload == "tree-2m":
+        bench_tree(args)
         return
 
     import gt_hip
