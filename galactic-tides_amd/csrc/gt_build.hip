@@ -499,7 +499,12 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
 // ---- 7. moments, one level at a time -------------------------------------------------
 
 // Moments of node x from its bodies (leaf) or its children, which are on the
-// level below and already done.
+// level below and already done.  The data of up to eight bodies or children
+// are loaded together before they are summed (one round of loads instead of
+// one per term: the level launches are latency-bound), and summed one by one
+// in order, so every sum is the reference's sequence.
+constexpr int kChunk = 8;
+
 __device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__restrict__ lnk,
                                     const double *__restrict__ tpos,
                                     const double *__restrict__ tmass,
@@ -511,12 +516,27 @@ __device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__
   double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (L.y == 0) {
     const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-    for (uint32_t k = b; k < e; ++k) {
-      const double mi = tmass[k];
-      m += mi;
-      cx += mi * tpos[3 * static_cast<size_t>(k)];
-      cy += mi * tpos[3 * static_cast<size_t>(k) + 1];
-      cz += mi * tpos[3 * static_cast<size_t>(k) + 2];
+    for (uint32_t k0 = b; k0 < e; k0 += kChunk) {
+      double mi[kChunk], px[kChunk], py[kChunk], pz[kChunk];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        const uint32_t k = k0 + i;
+        if (k < e) {
+          mi[i] = tmass[k];
+          px[i] = tpos[3 * static_cast<size_t>(k)];
+          py[i] = tpos[3 * static_cast<size_t>(k) + 1];
+          pz[i] = tpos[3 * static_cast<size_t>(k) + 2];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        if (k0 + i < e) {
+          m += mi[i];
+          cx += mi[i] * px[i];
+          cy += mi[i] * py[i];
+          cz += mi[i] * pz[i];
+        }
+      }
     }
     if (m > 0) {
       cx = __ddiv_rn(cx, m);
@@ -527,28 +547,56 @@ __device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__
       cy = ccy;
       cz = ccz;
     }
-    for (uint32_t k = b; k < e; ++k) {
-      const double mi = tmass[k];
-      const double dx = tpos[3 * static_cast<size_t>(k)] - cx,
-                   dy = tpos[3 * static_cast<size_t>(k) + 1] - cy,
-                   dz = tpos[3 * static_cast<size_t>(k) + 2] - cz;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      q[0] += mi * (3 * dx * dx - r2);
-      q[1] += mi * (3 * dy * dy - r2);
-      q[2] += mi * (3 * dz * dz - r2);
-      q[3] += mi * 3 * dx * dy;
-      q[4] += mi * 3 * dx * dz;
-      q[5] += mi * 3 * dy * dz;
+    for (uint32_t k0 = b; k0 < e; k0 += kChunk) {
+      double mi[kChunk], px[kChunk], py[kChunk], pz[kChunk];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        const uint32_t k = k0 + i;
+        if (k < e) {
+          mi[i] = tmass[k];
+          px[i] = tpos[3 * static_cast<size_t>(k)];
+          py[i] = tpos[3 * static_cast<size_t>(k) + 1];
+          pz[i] = tpos[3 * static_cast<size_t>(k) + 2];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        if (k0 + i < e) {
+          const double dx = px[i] - cx, dy = py[i] - cy, dz = pz[i] - cz;
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          q[0] += mi[i] * (3 * dx * dx - r2);
+          q[1] += mi[i] * (3 * dy * dy - r2);
+          q[2] += mi[i] * (3 * dz * dz - r2);
+          q[3] += mi[i] * 3 * dx * dy;
+          q[4] += mi[i] * 3 * dx * dz;
+          q[5] += mi[i] * 3 * dy * dz;
+        }
+      }
     }
   } else {
-    const uint32_t c0 = static_cast<uint32_t>(L.x), c1 = c0 + static_cast<uint32_t>(L.y);
-    if (L.x < 0 || c1 > n_nodes) return;  // malformed links: leave the row unset
-    for (uint32_t c = c0; c < c1; ++c) {
-      const double *h = geom + static_cast<size_t>(kGeomStride) * c;
-      m += h[7];
-      cx += h[7] * h[4];
-      cy += h[7] * h[5];
-      cz += h[7] * h[6];
+    const uint32_t c0 = static_cast<uint32_t>(L.x), nc = static_cast<uint32_t>(L.y);
+    if (L.x < 0 || nc > kChunk || c0 + nc > n_nodes) return;  // malformed links: leave the row unset
+    double hm[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], hq[kChunk][6];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      if (static_cast<uint32_t>(i) < nc) {
+        const double *h = geom + static_cast<size_t>(kGeomStride) * (c0 + i);
+        hm[i] = h[7];
+        hx[i] = h[4];
+        hy[i] = h[5];
+        hz[i] = h[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) hq[i][j] = h[8 + j];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      if (static_cast<uint32_t>(i) < nc) {
+        m += hm[i];
+        cx += hm[i] * hx[i];
+        cy += hm[i] * hy[i];
+        cz += hm[i] * hz[i];
+      }
     }
     if (m > 0) {
       cx = __ddiv_rn(cx, m);
@@ -559,17 +607,19 @@ __device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__
       cy = ccy;
       cz = ccz;
     }
-    for (uint32_t c = c0; c < c1; ++c) {
-      const double *h = geom + static_cast<size_t>(kGeomStride) * c;
-      const double mc = h[7];
-      const double dx = h[4] - cx, dy = h[5] - cy, dz = h[6] - cz;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      q[0] += h[8] + mc * (3 * dx * dx - r2);
-      q[1] += h[9] + mc * (3 * dy * dy - r2);
-      q[2] += h[10] + mc * (3 * dz * dz - r2);
-      q[3] += h[11] + mc * 3 * dx * dy;
-      q[4] += h[12] + mc * 3 * dx * dz;
-      q[5] += h[13] + mc * 3 * dy * dz;
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      if (static_cast<uint32_t>(i) < nc) {
+        const double mc = hm[i];
+        const double dx = hx[i] - cx, dy = hy[i] - cy, dz = hz[i] - cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        q[0] += hq[i][0] + mc * (3 * dx * dx - r2);
+        q[1] += hq[i][1] + mc * (3 * dy * dy - r2);
+        q[2] += hq[i][2] + mc * (3 * dz * dz - r2);
+        q[3] += hq[i][3] + mc * 3 * dx * dy;
+        q[4] += hq[i][4] + mc * 3 * dx * dz;
+        q[5] += hq[i][5] + mc * 3 * dy * dz;
+      }
     }
   }
   g[4] = cx;
