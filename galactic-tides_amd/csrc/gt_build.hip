@@ -382,10 +382,12 @@ __global__ __launch_bounds__(kBlock) void link_nodes(
   const uint32_t i = vals[b];
   const uint32_t system = sid ? sid[i] : 0u;
   const double *root = roots + 4 * static_cast<size_t>(system);
-  const uint64_t *w[3] = {w0, w1, w2};
+  // (the path words this depth needs, loaded once)
+  const uint64_t kw[3] = {w0[i], d > static_cast<uint32_t>(kLevels) ? w1[i] : 0ull,
+                          d > 2u * kLevels ? w2[i] : 0ull};
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   for (uint32_t l = 0; l < d; ++l) {
-    const uint64_t key = w[l / kLevels][i];
+    const uint64_t key = kw[l / kLevels];
     const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
     const double h2 = 0.5 * h;
     cx = cx + ((o & 1u) ? h2 : -h2);
