@@ -282,6 +282,13 @@ __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restri
 }
 
 // Leaf depth of sorted position p and the number of nodes beginning at p.
+// best = max over the windows j .. j+leaf of leaf+1 particles that contain p
+// and fit in [0, n) of the window's min lcp.  Up to kTileLeaf the block puts
+// its lcp values (with leaf on either side) in LDS and takes the minima of
+// all windows starting in its range once (W), then each position's maximum
+// over its leaf+1 windows; for larger leaves, the windows directly.
+constexpr uint32_t kTileLeaf = 64;
+
 __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict__ lcp,
                                                       const uint32_t *__restrict__ vals,
                                                       const uint8_t *__restrict__ dg, uint32_t n,
@@ -289,11 +296,34 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
                                                       uint8_t *__restrict__ ldep,
                                                       uint32_t *__restrict__ starts,
                                                       uint32_t *__restrict__ unresolved) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n) return;
-  // Windows j .. j+leaf of leaf+1 particles that contain p and fit in [0, n).
+  __shared__ int8_t sl[kBlock + 2 * kTileLeaf];  // shared(lcp[k]) for k = base - leaf ..
+  __shared__ int8_t win[kBlock + kTileLeaf];     // W(j) for j = base - leaf ..
+  const uint32_t base = blockIdx.x * kBlock;
+  const uint32_t p = base + threadIdx.x;
   int best = -1;
-  if (n > leaf) {
+  if (leaf <= kTileLeaf) {
+    // windows start at j in [base - leaf, base + kBlock) and read lcp[j .. j + leaf - 1]
+    const int64_t k0 = static_cast<int64_t>(base) - leaf;
+    const uint32_t nk = kBlock + 2 * leaf;
+    for (uint32_t t = threadIdx.x; t < nk; t += kBlock) {
+      const int64_t k = k0 + t;
+      sl[t] = static_cast<int8_t>(k >= 0 && k + 1 < static_cast<int64_t>(n) ? shared(lcp[k]) : 0);
+    }
+    __syncthreads();
+    const uint32_t nj = kBlock + leaf;
+    for (uint32_t t = threadIdx.x; t < nj; t += kBlock) {
+      int w = cap;
+      for (uint32_t i = 0; i < leaf; ++i) w = min(w, static_cast<int>(sl[t + i]));
+      win[t] = static_cast<int8_t>(w);
+    }
+    __syncthreads();
+    if (p < n && n > leaf) {
+      const uint32_t jlo = p >= leaf ? p - leaf : 0u;
+      const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
+      for (uint32_t j = jlo; j <= jhi; ++j)
+        best = max(best, static_cast<int>(win[j - (base - leaf)]));
+    }
+  } else if (p < n && n > leaf) {
     const uint32_t jlo = p >= leaf ? p - leaf : 0u;
     const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
     for (uint32_t j = jlo; j <= jhi; ++j) {
@@ -302,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
       best = max(best, w);
     }
   }
+  if (p >= n) return;
   int d = best + 1;
   d = min(d, static_cast<int>(dg[vals[p]]));
   d = min(d, kDepthLimit);

@@ -48,7 +48,8 @@ def _canonical_tree(geom, links):
 @pytest.mark.parametrize("n,kind,seed,leaf", [(3000, 0, 11, 8), (613, 1, 3, 4), (50000, 0, 3, 8),
                                               (4000, 0, 42, 1), (4000, 0, 42, 16), (9, 0, 1, 8),
                                               (1, 0, 1, 8), (2, 0, 1, 8),
-                                              (2_300_000, 0, 5, 8)])
+                                              (2_300_000, 0, 5, 8), (20000, 0, 7, 64),
+                                              (20000, 0, 7, 100)])
 def test_gpu_tree_equals_reference_tree(hip, oracle, n, kind, seed, leaf):
     pos, mass = oracle.test_system(kind, n, seed)
     oracle.build(pos, mass, leaf)
