@@ -60,7 +60,7 @@ struct GraphKey {
   Plan plan;
   double G = 0, theta = 0, eps = 0, dt = 0;
   int quadrupole = 0, group_size = 0, leaf_size = 0, pack = 0, exact = 0, eval_waves = 0,
-      walk_lanes = 0, hilo = 0;
+      walk_lanes = 0, hilo = 0, persistent = 0;
   uint64_t generation = 0;
 };
 
@@ -80,6 +80,7 @@ bool same_key(const GraphKey &a, const GraphKey &b) {
          a.dt == b.dt && a.quadrupole == b.quadrupole && a.group_size == b.group_size &&
          a.leaf_size == b.leaf_size && a.pack == b.pack && a.exact == b.exact &&
          a.eval_waves == b.eval_waves && a.walk_lanes == b.walk_lanes && a.hilo == b.hilo &&
+         a.persistent == b.persistent &&
          a.generation == b.generation;
 }
 
@@ -175,6 +176,7 @@ struct gt_hip_context {
   DBuf<uint64_t> nb_keys, nb_keys_alt;
   DBuf<unsigned long long> nb_index, nb_tmp, nb_sort_tmp;
   DBuf<uint4> nb_nodes;  // 64-byte query-walk records, 4 uint4 each
+  DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation
 };
 
 namespace {
@@ -734,10 +736,15 @@ void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
     hl.cells_lo = c->cells_lo.get();
     hl.group_hilo = c->group_hilo.get();
   }
+  const int waves = cfg->eval_waves == 2 ? 2 : 3;
+  // persistent grid (whole evaluation only): each CU holds `waves` blocks
+  const bool persistent = span == 0 && g0 == 0 && cfg->persistent == 1;
+  const uint32_t pblocks =
+      persistent ? static_cast<uint32_t>(c->prop.multiProcessorCount) * waves : 0u;
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
-              cfg->exact != 0, cfg->eval_waves == 2 ? 2 : 3, c->out.get(), s ? s : c->stream,
-              items ? &sp : nullptr, p.hilo ? &hl : nullptr, g0, span);
+              cfg->exact != 0, waves, c->out.get(), s ? s : c->stream, items ? &sp : nullptr,
+              p.hilo ? &hl : nullptr, g0, span, c->eval_queue.get(), pblocks);
 }
 
 // Results in tree order: split groups' parts added up.
@@ -1079,6 +1086,7 @@ GraphKey key_for(const gt_group_config *cfg, const Plan &p, double dt) {
   k.eval_waves = cfg->eval_waves == 2 ? 2 : 3;
   k.walk_lanes = walk_lanes(cfg, p);
   k.hilo = uses_hilo(cfg);
+  k.persistent = cfg->persistent == 1 && !cfg->exact;
   k.generation = alloc_generation;
   return k;
 }
@@ -1317,6 +1325,7 @@ int gt_hip_create(gt_hip_context **out, int device) {
     for (auto &e : c->walk_done) GTH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     GTH_CHECK(hipEventCreateWithFlags(&c->eval_done, hipEventDisableTiming));
     launch_fill_u64(c->frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
+    c->eval_queue.ensure(kXcds);
     launch_fill_u64(c->raw_frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
     GTH_CHECK(hipStreamSynchronize(c->stream));
     g_create_error.clear();
@@ -1709,6 +1718,7 @@ int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, c
       sweep(&gt_group_config::eval_waves, {2, 3});
       sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
       sweep(&gt_group_config::split_len, {2048, 8192, -1});
+      sweep(&gt_group_config::persistent, {0, 1});
     }
     *best = cur;
     if (best_ms) *best_ms = cur_ms;

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const uint32_t *__restrict__ split_off, float4 *__restrict__ partial, float G, float eps2,
     const float4 *__restrict__ bodies_lo, const float4 *__restrict__ cells_lo,
     const float4 *__restrict__ group_hilo, float4 *__restrict__ out, uint32_t group_cap,
-    uint32_t g_begin, uint32_t g_span) {
+    uint32_t g_begin, uint32_t g_span, uint32_t *__restrict__ queue) {
   __shared__ float4 targets[kBlock / kWave][32];
   if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
   // items: this range's work items; else groups g_begin .. g_begin + g_span
@@ -388,11 +388,31 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   // XCD-aware: each XCD evaluates a contiguous run of (spatially adjacent)
   // groups, whose lists gather the same cells, so its L2 holds them.
   const uint32_t waves_per_block = kBlock / kWave;
-  const uint32_t blk =
-      xcd_block_used(blockIdx.x, (n_items + waves_per_block - 1) / waves_per_block);
-  if (blk == kNone) return;
-  const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
-  if (wi >= n_items) return;
+  const uint32_t used = (n_items + waves_per_block - 1) / waves_per_block;
+  // queue: a persistent grid; the waves on each XCD take that XCD's share of
+  // the items (the same shares as the static mapping) one at a time from the
+  // XCD's counter, in order.
+  uint32_t wi_lo = 0, wi_hi = 0, xcd = 0;
+  if (queue) {
+    xcd = blockIdx.x % kXcds;
+    const uint32_t q = (used + kXcds - 1) / kXcds;
+    wi_lo = min(xcd * q * waves_per_block, n_items);
+    wi_hi = min((xcd + 1) * q * waves_per_block, n_items);
+  }
+  for (uint32_t step = 0;; ++step) {
+  uint32_t wi;
+  if (queue) {
+    uint32_t t = 0;
+    if (threadIdx.x % kWave == 0) t = atomicAdd(queue + xcd, 1u);
+    wi = __builtin_amdgcn_readfirstlane(wi_lo + __builtin_amdgcn_readfirstlane(t));
+    if (wi >= wi_hi) break;
+  } else {
+    if (step) break;
+    const uint32_t blk = xcd_block_used(blockIdx.x, used);
+    if (blk == kNone) break;
+    wi = __builtin_amdgcn_readfirstlane(blk * waves_per_block + threadIdx.x / kWave);
+    if (wi >= n_items) break;
+  }
   uint32_t gi = g_begin + wi, part = 0, parts = 1;
   if (items) {
     const uint2 it = items[order ? order[wi] : wi];
@@ -439,6 +459,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
       default: GTH_BATCH(32); break;
     }
 #undef GTH_BATCH
+  }
   }
 }
 
@@ -596,7 +617,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
                  const EvalSplit *split, const EvalHiLo *hilo, uint32_t g_begin,
-                 uint32_t g_span) {
+                 uint32_t g_span, uint32_t *queue, uint32_t persistent_blocks) {
   if (group_cap == 0) return;
   if (g_span == 0 || g_span > group_cap) g_span = group_cap;
   if (exact && (g_begin != 0 || g_span != group_cap))
@@ -605,7 +626,10 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
   const bool parts = split && !exact;
   const uint32_t item_cap = parts ? split->item_cap : g_span;
   const uint32_t blocks = (item_cap + waves_per_block - 1) / waves_per_block;
-  const dim3 grid(exact ? blocks : xcd_grid(blocks)), block(kBlock);
+  if (exact || !persistent_blocks) queue = nullptr;
+  if (queue) launch_fill_u32(queue, kXcds, 0u, s);
+  const dim3 grid(exact ? blocks : queue ? xcd_grid(persistent_blocks) : xcd_grid(blocks)),
+      block(kBlock);
   const uint2 *items = parts ? split->items : nullptr;
   const uint32_t *order = parts && split->longest_first ? split->order : nullptr;
   const uint32_t *split_off = parts ? split->split_off : nullptr;
@@ -615,7 +639,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                *ghl = hl ? hilo->group_hilo : nullptr;
 #define GTH_EVAL_ARGS                                                                \
   bodies, cells, cell_list, body_list, groups, frame, item_cap, items, order, split_off, \
-      partial, G, eps2, blo, clo, ghl, out, group_cap, g_begin, g_span
+      partial, G, eps2, blo, clo, ghl, out, group_cap, g_begin, g_span, queue
 #define GTH_EVAL(Q, W)                                                                    \
   if (hl)                                                                                \
     hipLaunchKernelGGL((eval_fast<Q, W, true>), grid, block, 0, s, GTH_EVAL_ARGS);  \

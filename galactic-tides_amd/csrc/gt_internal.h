@@ -267,7 +267,8 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
                  const EvalSplit *split = nullptr, const EvalHiLo *hilo = nullptr,
-                 uint32_t g_begin = 0, uint32_t g_span = 0);
+                 uint32_t g_begin = 0, uint32_t g_span = 0, uint32_t *queue = nullptr,
+                 uint32_t persistent_blocks = 0);
 // Adds the parts of split groups into out, in part order.
 void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
                           uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s,

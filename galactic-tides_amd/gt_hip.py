@@ -26,7 +26,8 @@ class GroupConfig(C.Structure):
                 ("quadrupole", C.c_int), ("group_size", C.c_int), ("leaf_size", C.c_int),
                 ("pack", C.c_int), ("exact", C.c_int), ("eval_waves", C.c_int),
                 ("walk_lanes", C.c_int), ("split_len", C.c_int), ("eval_order", C.c_int),
-                ("hilo", C.c_int), ("predict", C.c_int), ("overlap", C.c_int)]
+                ("hilo", C.c_int), ("predict", C.c_int), ("overlap", C.c_int),
+                ("persistent", C.c_int)]
 
 
 class Params(C.Structure):
@@ -134,13 +135,13 @@ def _f64(a, shape=None):
 
 def make_config(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                 pack=-1, exact=False, eval_waves=0, walk_lanes=0, split_len=0, eval_order=0,
-                hilo=False, predict=0, overlap=0):
+                hilo=False, predict=0, overlap=0, persistent=0):
     cfg = GroupConfig()
     cfg.G, cfg.theta, cfg.eps = G, theta, eps
     cfg.quadrupole, cfg.group_size, cfg.leaf_size = int(quadrupole), group_size, leaf_size
     cfg.pack, cfg.exact, cfg.eval_waves, cfg.walk_lanes = pack, int(exact), eval_waves, walk_lanes
     cfg.split_len, cfg.eval_order, cfg.hilo = split_len, eval_order, int(hilo)
-    cfg.predict, cfg.overlap = int(predict), int(overlap)
+    cfg.predict, cfg.overlap, cfg.persistent = int(predict), int(overlap), int(persistent)
     return cfg
 
 
@@ -306,7 +307,7 @@ class Context:
         ms = C.c_double()
         self._check(_autotune(self._h, C.byref(c), n, _ptr(pos), _ptr(mass),
                               int(search_group_size), C.byref(best), C.byref(ms)))
-        keys = ("group_size", "eval_waves", "walk_lanes", "split_len", "eval_order")
+        keys = ("group_size", "eval_waves", "walk_lanes", "split_len", "eval_order", "persistent")
         return {k: int(getattr(best, k)) for k in keys}, ms.value
 
     def walk_tree(self, geom, links, tree_pos, tree_mass, evaluate=True, **cfg):
