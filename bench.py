@@ -257,6 +257,62 @@ def bench_tree(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_direct(args):
+    """DirectGravity on the GPU (FP32 O(N^2), gt_direct) for the 200k galaxy
+    of the reference's M4 Max figure (README: 122 ms at 200k): one step = one
+    gt_hip_direct call; value = pair interactions per second of kernel time."""
+    import gt_hip
+    import gt_ics
+    m = gt_ics.model(disk_n=80_000, bulge_n=20_000, halo_n=100_000, bulge_mass=0.333,
+                     bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
+                     halo_profile=args.halo, halo_a=4.0, seed=2)
+    pos, _, mass, _ = gt_ics.galaxy(m)
+    n = len(mass)
+    centre = 0.5 * (pos.min(0) + pos.max(0))  # bounding_box_centre (gravity_backends.cpp:35-44)
+    bodies = np.zeros((n, 4), np.float32)
+    bodies[:, :3] = (pos - centre).astype(np.float32)
+    bodies[:, 3] = mass.astype(np.float32)
+    ctx = gt_hip.Context(0)
+    for _ in range(max(args.warmup, 1)):
+        out = ctx.direct(bodies, G=1.0, eps=0.025)
+    ms = []
+    for _ in range(args.steps):
+        out = ctx.direct(bodies, G=1.0, eps=0.025)
+        ms.append(1e3 * ctx.last_gpu_seconds)
+    kms = float(np.mean(ms))
+    pairs = n * (n - 1)
+    print(f"# direct {n}: {kms:.2f} ms per evaluation (kernel)", file=sys.stderr, flush=True)
+    flops = BODY_FLOPS * pairs
+    achieved = flops / (kms * 1e-3) / 1e12
+    sample = np.sort(np.random.default_rng(1).choice(n, 1000, replace=False)).astype(np.uint64)
+    ref = ctx.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
+    got = out[sample.astype(np.int64), :3].astype(np.float64)
+    err = np.sort(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1))
+    line = {
+        "metric": "FP32 direct-summation pair interactions/s (gravity = direct) at N=200k",
+        "value": round(pairs / (kms * 1e-3), 1),
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(kms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: a 200k Hernquist galaxy (seed 2)",
+        "config": {"workload": "direct-200k", "n": n, "eps": 0.025, "halo": args.halo},
+        "force_error": {"median": float(err[len(err) // 2]),
+                        "p99": float(err[len(err) * 99 // 100]), "sample": len(err)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "direct_fast", "kernel_ms": round(kms, 4),
+                     "flops_per_launch": flops, "compute": "fp32 vector ALU"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,14 +332,15 @@ def main():
                     help="tuning: list entries per evaluation part (0: default, <0: never split)")
     ap.add_argument("--workload", default="encounter-2m",
                     choices=["encounter-2m", "batch-128", "galaxy-4m", "neighbours-2m",
-                             "tree-2m"],
+                             "tree-2m", "direct-200k"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
                          "independent galaxies of 2k-100k particles evaluated together; "
                          "galaxy-4m: one 4M-particle galaxy (seed 6); neighbours-2m: "
                          "fixed-radius neighbour queries on the encounter's tree (every "
                          "particle a query, radius = its distance to its 32nd neighbour); "
                          "tree-2m: the FP64 per-particle tree solver (gravity = tree) on the "
-                         "encounter")
+                         "encounter; direct-200k: FP32 direct summation (gravity = direct) "
+                         "over a 200k-particle galaxy")
     ap.add_argument("--theta", type=float, default=0.75)
     ap.add_argument("--predict", type=int, default=0,
                     help="list slots: 0 uniform, 1 learned predictor (trained on the first "
@@ -319,6 +376,9 @@ def main():
         return
     if args.workload == "tree-2m":
         bench_tree(args)
+        return
+    if args.workload == "direct-200k":
+        bench_direct(args)
         return
 
     import gt_hip

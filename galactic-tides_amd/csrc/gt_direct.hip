@@ -16,28 +16,45 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Sources j0 .. j1 against this lane's target, into the sums (of m d / r^3
+// and m / r with d = source - target, so the signs and G come at the end).
+// kSelf: the range may hold the target itself, which is skipped.
+template <bool kSelf>
+__device__ inline void direct_range(const float4 *__restrict__ bodies, uint32_t j0, uint32_t j1,
+                                    uint32_t i, float4 bi, float eps2, float &sx, float &sy,
+                                    float &sz, float &sp) {
+#pragma unroll 8
+  for (uint32_t j = j0; j < j1; ++j) {
+    const float4 b = bodies[j];
+    const float dx = b.x - bi.x, dy = b.y - bi.y, dz = b.z - bi.z;
+    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    if (kSelf) r2 = (j == i) ? __builtin_inff() : r2;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float t = b.w * inv;  // m / r
+    const float t3 = t * (inv * inv);
+    sx = fmaf(t3, dx, sx);
+    sy = fmaf(t3, dy, sy);
+    sz = fmaf(t3, dz, sz);
+    sp += t;
+  }
+}
+
+// The source loop is wave-uniform; only the 64 sources with the wave's own
+// indices can be a lane's target, so only that stretch tests for it.
 __global__ __launch_bounds__(kBlock) void direct_fast(const float4 *__restrict__ bodies,
                                                       uint32_t n, float G, float eps2,
                                                       float4 *__restrict__ out) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const float4 bi = bodies[i < n ? i : n - 1];
-  const float inf = __builtin_inff();
-  float ax = 0.0f, ay = 0.0f, az = 0.0f, ap = 0.0f;
-#pragma unroll 8
-  for (uint32_t j = 0; j < n; ++j) {
-    const float4 b = bodies[j];
-    const float dx = bi.x - b.x, dy = bi.y - b.y, dz = bi.z - b.z;
-    float r2 = dx * dx + dy * dy + dz * dz + eps2;
-    r2 = (j == i) ? inf : r2;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float gm = G * b.w * inv;
-    const float gm3 = gm * inv * inv;
-    ax -= gm3 * dx;
-    ay -= gm3 * dy;
-    az -= gm3 * dz;
-    ap -= gm;
-  }
-  if (i < n) out[i] = make_float4(ax, ay, az, ap);
+  // (wave-uniform: the sources then come through the scalar cache)
+  const uint32_t w0 =
+      __builtin_amdgcn_readfirstlane(min(blockIdx.x * kBlock + (threadIdx.x & ~63u), n));
+  const uint32_t w1 = min(w0 + 64u, n);
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, sp = 0.0f;
+  direct_range<false>(bodies, 0, w0, i, bi, eps2, sx, sy, sz, sp);
+  direct_range<true>(bodies, w0, w1, i, bi, eps2, sx, sy, sz, sp);
+  direct_range<false>(bodies, w1, n, i, bi, eps2, sx, sy, sz, sp);
+  if (i < n) out[i] = make_float4(G * sx, G * sy, G * sz, -G * sp);
 }
 
 #pragma clang fp contract(off)
