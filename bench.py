@@ -108,8 +108,8 @@ def bench_neighbours(args):
     encounter, each with the radius that holds its 32 nearest other particles
     (an SPH smoothing length; found with scipy's k-d tree before timing).  One
     step = one gt_hip_neighbours call; value = queries per second of GPU time
-    (build, ordering, counting walk, scan, filling walk; HIP events), the
-    PCIe-inclusive wall rate alongside."""
+    (build, ordering, walk, scan, moving the lists to their places; HIP
+    events), the PCIe-inclusive wall rate alongside."""
     import gt_hip
     from scipy.spatial import cKDTree
     pos, _, mass, _ = encounter_2m(args.halo)
@@ -133,20 +133,21 @@ def bench_neighbours(args):
         phases.append((tm["tree"], tm["lists"], tm["evaluate"]))
     ms = float(np.mean(gpu))
     ph = np.mean(np.array(phases), axis=0)
-    print(f"# step (GPU) {ms:.2f} ms: build {ph[0]:.2f}, ordering + counting walk + scan "
-          f"{ph[1]:.2f}, filling walk {ph[2]:.2f}; wall {1e3 * np.mean(wall):.1f} ms; "
+    print(f"# step (GPU) {ms:.2f} ms: build {ph[0]:.2f}, ordering + walk + scan "
+          f"{ph[1]:.2f}, lists to their places {ph[2]:.2f}; wall {1e3 * np.mean(wall):.1f} ms; "
           f"{total} neighbours", file=sys.stderr, flush=True)
-    # Filling walk: compulsory bytes = the lists written (8 B index + 8 B squared
+    # The walk (each query's first neighbours into a slot of its own, all
+    # counted): compulsory bytes = the lists written (8 B index + 8 B squared
     # distance per neighbour), the queries read (24 B centre, 8 B radius) with
-    # their order, count and offset (12 B), the tree-order particles (24 B) and
-    # the node records (64 B), each once.
+    # their order and count (8 B), the tree-order particles (24 B) and the node
+    # records (64 B), each once; its phase also holds the ordering and the scan.
     nodes = ctx.info()["n_nodes"]
-    traffic = 16 * total + 44 * n + 24 * n + 64 * nodes
-    achieved = traffic / (ph[2] * 1e-3) / 1e9
+    traffic = 16 * total + 40 * n + 24 * n + 64 * nodes
+    achieved = traffic / (ph[1] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "within<true>",
-                "kernel_ms": round(float(ph[2]), 4), "bytes_per_launch": traffic,
-                "note": "a dependent walk per lane: latency-bound, not bandwidth-bound"}
+                "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "within<2>",
+                "kernel_ms": round(float(ph[1]), 4), "bytes_per_launch": traffic,
+                "note": "a dependent walk per wave: latency-bound, not bandwidth-bound"}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -176,6 +176,10 @@ struct gt_hip_context {
   DBuf<uint64_t> nb_keys, nb_keys_alt;
   DBuf<unsigned long long> nb_index, nb_tmp, nb_sort_tmp;
   DBuf<uint4> nb_nodes;  // 64-byte query-walk records, 4 uint4 each
+  DBuf<unsigned long long> nb_stage_index;  // per-query slots of the one-walk fill
+  DBuf<double> nb_stage_dist2;
+  DBuf<uint32_t> nb_over;  // queries whose lists outgrew their slots
+  uint32_t nb_stage = 64;  // slot entries per query (grows when many overflow)
   DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation
 };
 
@@ -1523,6 +1527,9 @@ int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t 
   });
 }
 
+constexpr uint64_t kNbStageBytes = 4ull << 30;  // slot memory of a one-walk neighbour fill
+constexpr uint32_t kNbStageMax = 256;
+
 int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double *pos,
                       const double *mass, uint64_t nq, const double *centres,
                       const double *radii, uint64_t n_radii, uint64_t max_out,
@@ -1576,29 +1583,52 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
     void *recs = c->nb_nodes.ensure((nb_record_bytes(p.node_cap) + 15) / 16);
     launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(), frame,
                       p.node_cap, recs, s);
-    launch_fill_u64(dev, 2, 0ull, s);
+    launch_fill_u64(dev, 3, 0ull, s);  // [2]: queries whose lists outgrew their slots
+    // A call that wants the lists walks once, writing each query's first
+    // nb_stage neighbours into a slot of its own and counting them all; the
+    // lists that fit are then moved to their CSR places and only the others
+    // are walked again.  Counting calls, and slots past kNbStageBytes, take a
+    // counting walk and a filling walk instead.
+    const bool want = index || dist2;
+    uint32_t stage = want ? c->nb_stage : 0u;
+    if (static_cast<uint64_t>(stage) * nq * 16 > kNbStageBytes) stage = 0;
+    unsigned long long *si = nullptr;
+    double *sd = nullptr;
+    if (stage) {
+      si = c->nb_stage_index.ensure(static_cast<size_t>(stage) * nq);
+      sd = c->nb_stage_dist2.ensure(static_cast<size_t>(stage) * nq);
+    }
     launch_within(recs, c->tree.tree_pos(), c->tree.order(), n32, frame, p.node_cap, dc, dr,
-                  stride, nq32, perm, counts, nullptr, nullptr, nullptr, dev, s);
+                  stride, nq32, nq32, perm, counts, nullptr, si, sd, stage, dev, s);
     exclusive_scan_u32(counts, offs, nq, dev + 1, tmp, s);
     GTH_CHECK(hipEventRecord(c->ev[3], s));
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[3] = {0, 0, 0};
     download(h, dev, 2, s);
     GTH_CHECK(hipStreamSynchronize(s));
     if (h[0]) fail(GT_HIP_ERR_DEVICE, "neighbour queries: malformed tree");
     if (h[1] > 0xffffffffull)
       fail(GT_HIP_ERR_LIMIT, "more than 2^32 - 1 neighbours in one call: split the queries");
-    const bool fill = h[1] && (index || dist2) && h[1] <= max_out;
+    const bool fill = h[1] && want && h[1] <= max_out;
+    GTH_CHECK(hipEventRecord(c->ev[4], s));
     if (fill) {
       unsigned long long *di = c->nb_index.ensure(h[1]);
       double *dd = c->nb_dist2.ensure(h[1]);
-      GTH_CHECK(hipEventRecord(c->ev[4], s));
+      uint32_t *lanes_perm = perm, lanes = nq32;
+      if (stage) {
+        uint32_t *over = c->nb_over.ensure(nq);
+        launch_unstage(counts, offs, nq32, stage, si, sd, di, dd, over, dev + 2, s);
+        download(h + 2, dev + 2, 1, s);
+        GTH_CHECK(hipStreamSynchronize(s));
+        lanes_perm = over;
+        lanes = static_cast<uint32_t>(h[2]);
+        if (h[2] > nq / 8 && stage < kNbStageMax) c->nb_stage = std::min(2 * stage, kNbStageMax);
+      }
       launch_within(recs, c->tree.tree_pos(), c->tree.order(), n32, frame, p.node_cap, dc, dr,
-                    stride, nq32, perm, counts, offs, di, dd, dev, s);
+                    stride, nq32, lanes, lanes_perm, counts, offs, di, dd, 0, dev, s);
       GTH_CHECK(hipEventRecord(c->ev[2], s));
       if (index) download(reinterpret_cast<unsigned long long *>(index), di, h[1], s);
       if (dist2) download(dist2, dd, h[1], s);
     } else {
-      GTH_CHECK(hipEventRecord(c->ev[4], s));
       GTH_CHECK(hipEventRecord(c->ev[2], s));
     }
     std::vector<uint32_t> ho(nq);

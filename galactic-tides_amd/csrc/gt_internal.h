@@ -345,8 +345,8 @@ void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *sk
 // gt_neighbours.hip: fixed-radius queries over a built tree (one system, root
 // node 0).  launch_nb_records packs each node's cube, children, body range
 // and skip link into `records` (nb_record_bytes(node_cap) bytes).  Query q
-// has centre centres[3q..3q+2] and radius radii[q * radius_stride]; perm (may
-// be null) is the order the lanes take the queries in.  offs null: counts[q] =
+// has centre centres[3q..3q+2] and radius radii[q * radius_stride]; lane k <
+// lanes takes query perm[k].  offs null: counts[q] =
 // its neighbours.  Otherwise its neighbours (original index, squared
 // distance) go to index / dist2 from offs[q], counts[q] of them.  error: bit 0
 // malformed tree, bit 1 the passes disagree or perm is out of range.
@@ -356,10 +356,18 @@ void launch_nb_records(const double *geom, const int4 *lnk, const uint32_t *skip
                        void *records, hipStream_t s);
 void launch_within(const void *records, const double *tpos, const uint32_t *order, uint32_t n,
                    const unsigned long long *frame, uint32_t node_cap, const double *centres,
-                   const double *radii, uint32_t radius_stride, uint32_t nq,
+                   const double *radii, uint32_t radius_stride, uint32_t nq, uint32_t lanes,
                    const uint32_t *perm, uint32_t *counts, const uint32_t *offs,
-                   unsigned long long *index, double *dist2, unsigned long long *error,
-                   hipStream_t s);
+                   unsigned long long *index, double *dist2, uint32_t stage,
+                   unsigned long long *error, hipStream_t s);
+// stage > 0 (launch_within): the first `stage` neighbours of query q go to
+// index / dist2 from q * stage and counts[q] gets all of them; launch_unstage
+// then moves every list of at most `stage` entries to its CSR place (offs)
+// and appends the other queries to `over` (count in *n_over).
+void launch_unstage(const uint32_t *counts, const uint32_t *offs, uint32_t nq, uint32_t stage,
+                    const unsigned long long *s_index, const double *s_dist2,
+                    unsigned long long *index, double *dist2, uint32_t *over,
+                    unsigned long long *n_over, hipStream_t s);
 
 // Sort keys of the queries: 30-bit Morton keys of the centres in the root cube
 // of the tree whose node 0 is geom's first row (0 for an empty tree), and

@@ -7,14 +7,19 @@
 // order in which the reference's callback receives them.
 // The box test and the squared distances are the reference's FP64 expressions
 // in its operation order with no contraction, so the neighbour lists and the
-// distances equal the reference's bit for bit.  Two passes over the same walk:
-// the counts, then, after an exclusive scan of the counts, the lists.
+// distances equal the reference's bit for bit.  One walk counts every query's
+// neighbours and writes the first ones into a slot per query; after an
+// exclusive scan of the counts the lists that fit are moved to their CSR
+// places and the queries with more are walked again straight into theirs
+// (a count pass and a fill pass when the slots would not fit).
 //
 // Lanes take the queries in the order of a 30-bit Morton key of their centres
 // in the root cube (query_keys + a radix sort), so the lanes of a wave walk
 // nearby parts of the tree; results stay in the caller's query order.  A node
 // visit reads one 64-byte record (centre, half, children, body range, skip
 // link) instead of three arrays.
+#include <algorithm>
+
 #include "gt_internal.h"
 
 namespace gth {
@@ -73,9 +78,10 @@ __device__ inline double box_d2(const NbNode &g, double cx, double cy, double cz
   return d2;
 }
 
-// kFill false: counts[q] = neighbours of query q.  kFill true: query q's
-// neighbours into index / dist2 from offs[q], at most counts[q] of them.
-// Lane k takes query perm[k].
+// kCount: counts[q] = neighbours of query q.  kFill: query q's neighbours
+// into index / dist2 from offs[q], at most counts[q] of them.  kStage: the
+// first `stride` of them into index / dist2 from q * stride, and counts[q] =
+// all of them.  Lane k < lanes takes query perm[k] (< nq).
 //
 // One wave walks for its 64 queries together (a packet walk): it visits, in
 // the depth-first order above, every node that any of its lanes would visit,
@@ -83,37 +89,44 @@ __device__ inline double box_d2(const NbNode &g, double cx, double cy, double cz
 // its radius (its active mask for each depth is kept in LDS), so each lane
 // sees exactly the nodes of its own walk, in order.  Node records and
 // particles are read at wave-uniform addresses.  Against one independent walk
-// per lane this takes 9.1 instead of 11.8 ms for 2M queries of 32 neighbours.
+// per lane, a counting and a filling walk for 2M queries of 32 neighbours took
+// 9.1 instead of 11.8 ms.
 constexpr int kMaxLevels = 64;
 
-template <bool kFill>
+constexpr int kCount = 0, kFill = 1, kStage = 2;
+
+template <int kMode>
 __global__ __launch_bounds__(kBlock) void within(
     const NbNode *__restrict__ nodes, const double *__restrict__ tpos,
     const uint32_t *__restrict__ order, uint32_t n, const unsigned long long *__restrict__ frame,
     uint32_t node_cap, const double *__restrict__ centres, const double *__restrict__ radii,
-    uint32_t radius_stride, uint32_t nq, const uint32_t *__restrict__ perm,
+    uint32_t radius_stride, uint32_t nq, uint32_t lanes, const uint32_t *__restrict__ perm,
     uint32_t *__restrict__ counts, const uint32_t *__restrict__ offs,
-    unsigned long long *__restrict__ index, double *__restrict__ dist2,
+    unsigned long long *__restrict__ index, double *__restrict__ dist2, uint32_t stride,
     unsigned long long *__restrict__ error) {
   __shared__ unsigned long long masks[kBlock / 64][kMaxLevels];
   const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = k < nq;
+  const bool valid = k < lanes;
   const uint32_t q = valid ? perm[k] : 0u;
   const bool ok = valid && q < nq;
   if (valid && !ok) atomicOr(error, 2ull);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   double cx = 0, cy = 0, cz = 0, r2 = 0;
-  uint32_t found = 0, at = 0, end = 0;
+  uint32_t found = 0;
+  size_t at = 0, end = 0;
   if (ok) {
     cx = centres[3 * static_cast<size_t>(q)];
     cy = centres[3 * static_cast<size_t>(q) + 1];
     cz = centres[3 * static_cast<size_t>(q) + 2];
     const double r = radii[static_cast<size_t>(q) * radius_stride];
     r2 = r * r;
-    if (kFill) {
+    if (kMode == kFill) {
       at = offs[q];
       end = at + counts[q];
+    } else if (kMode == kStage) {
+      at = static_cast<size_t>(q) * stride;
+      end = at + stride;
     }
   }
   const unsigned long long me = 1ull << lane;
@@ -138,15 +151,14 @@ __global__ __launch_bounds__(kBlock) void within(
           const double dz = tpos[3 * static_cast<size_t>(j) + 2] - cz;
           const double dd = dx * dx + dy * dy + dz * dz;
           if (pass && dd <= r2) {
-            if (kFill) {
+            if (kMode != kCount) {
               if (at < end) {
                 index[at] = order[j];
                 dist2[at] = dd;
               }
               ++at;
-            } else {
-              ++found;
             }
+            ++found;
           }
         }
       }
@@ -162,10 +174,36 @@ __global__ __launch_bounds__(kBlock) void within(
     }
   }
   if (!ok) return;
-  if (kFill) {
+  if (kMode == kFill) {
     if (at != end) atomicOr(error, 2ull);  // the passes disagree (never)
   } else {
     counts[q] = found;
+  }
+}
+
+// Staged lists to their CSR places: thread (q, i) copies entry i of query q
+// when q has at most `stride` entries; queries with more are appended to
+// `over` (their lists are written by a kFill walk).
+__global__ __launch_bounds__(kBlock) void unstage(const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ offs, uint32_t nq,
+                                                  uint32_t stride,
+                                                  const unsigned long long *__restrict__ s_index,
+                                                  const double *__restrict__ s_dist2,
+                                                  unsigned long long *__restrict__ index,
+                                                  double *__restrict__ dist2,
+                                                  uint32_t *__restrict__ over,
+                                                  unsigned long long *__restrict__ n_over) {
+  const size_t total = static_cast<size_t>(nq) * stride;
+  for (size_t t = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; t < total;
+       t += static_cast<size_t>(gridDim.x) * kBlock) {
+    const uint32_t q = static_cast<uint32_t>(t / stride), i = static_cast<uint32_t>(t % stride);
+    const uint32_t c = counts[q];
+    if (c > stride) {
+      if (i == 0) over[atomicAdd(n_over, 1ull)] = q;
+    } else if (i < c) {
+      index[static_cast<size_t>(offs[q]) + i] = s_index[t];
+      dist2[static_cast<size_t>(offs[q]) + i] = s_dist2[t];
+    }
   }
 }
 
@@ -233,22 +271,37 @@ void launch_nb_records(const double *geom, const int4 *lnk, const uint32_t *skip
 
 void launch_within(const void *records, const double *tpos, const uint32_t *order, uint32_t n,
                    const unsigned long long *frame, uint32_t node_cap, const double *centres,
-                   const double *radii, uint32_t radius_stride, uint32_t nq,
+                   const double *radii, uint32_t radius_stride, uint32_t nq, uint32_t lanes,
                    const uint32_t *perm, uint32_t *counts, const uint32_t *offs,
-                   unsigned long long *index, double *dist2, unsigned long long *error,
-                   hipStream_t s) {
-  if (!nq) return;
-  const dim3 grid((nq + kBlock - 1) / kBlock);
+                   unsigned long long *index, double *dist2, uint32_t stage,
+                   unsigned long long *error, hipStream_t s) {
+  if (!lanes) return;
+  const dim3 grid((lanes + kBlock - 1) / kBlock);
   const NbNode *nodes = static_cast<const NbNode *>(records);
-  if (offs) {
-    hipLaunchKernelGGL(within<true>, grid, dim3(kBlock), 0, s, nodes, tpos, order, n, frame,
-                       node_cap, centres, radii, radius_stride, nq, perm, counts, offs, index,
-                       dist2, error);
-  } else {
-    hipLaunchKernelGGL(within<false>, grid, dim3(kBlock), 0, s, nodes, tpos, order, n,
-                       frame, node_cap, centres, radii, radius_stride, nq, perm, counts, offs,
-                       index, dist2, error);
-  }
+#define GTH_WITHIN(M)                                                                          \
+  hipLaunchKernelGGL(within<M>, grid, dim3(kBlock), 0, s, nodes, tpos, order, n, frame, node_cap, \
+                     centres, radii, radius_stride, nq, lanes, perm, counts, offs, index, dist2,  \
+                     stage,                                                                     \
+                     error)
+  if (stage)
+    GTH_WITHIN(kStage);
+  else if (offs)
+    GTH_WITHIN(kFill);
+  else
+    GTH_WITHIN(kCount);
+#undef GTH_WITHIN
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_unstage(const uint32_t *counts, const uint32_t *offs, uint32_t nq, uint32_t stage,
+                    const unsigned long long *s_index, const double *s_dist2,
+                    unsigned long long *index, double *dist2, uint32_t *over,
+                    unsigned long long *n_over, hipStream_t s) {
+  if (!nq || !stage) return;
+  const size_t total = static_cast<size_t>(nq) * stage;
+  const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((total + kBlock - 1) / kBlock, 65536));
+  hipLaunchKernelGGL(unstage, dim3(grid), dim3(kBlock), 0, s, counts, offs, nq, stage, s_index,
+                     s_dist2, index, dist2, over, n_over);
   GTH_LAUNCH_CHECK();
 }
 

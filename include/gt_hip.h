@@ -181,9 +181,12 @@ int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_
  * (each may be NULL; both NULL: the call only counts) are written when the total
  * fits max_out; otherwise the call returns GT_HIP_ERR_LIMIT and can be made again
  * with room for *total entries.  At most 2^32 - 1 neighbours in all per call.
- * gt_hip_last_timings: tree = the build, lists = query ordering, counting walk
- * and scan, evaluate = the filling walk, total = their sum (GPU time; the
- * host copies and the wait for the count between the passes excluded). */
+ * gt_hip_last_timings: tree = the build, lists = query ordering, the walk
+ * (counting; when the lists are wanted, also writing each query's first
+ * neighbours into a slot of its own) and the scan, evaluate = the lists moved
+ * to their places and the queries whose lists outgrew their slots walked
+ * again (the filling walk when slots did not fit 4 GiB), total = their sum
+ * (GPU time; host copies and waits between the phases excluded). */
 int gt_hip_neighbours(gt_hip_context *ctx, int leaf_size, uint64_t n, const double *pos,
                       const double *mass, uint64_t nq, const double *centres,
                       const double *radii, uint64_t n_radii, uint64_t max_out,
