@@ -34,7 +34,7 @@ def test_struct_layouts_match_the_reference():
     # gravity_kernels.h:51-60 GTParams 32 B; metal_backend.mm:21-24
     assert C.sizeof(gt_hip.Params) == 32
     assert C.sizeof(gt_hip.ListsInfo) == 80
-    assert C.sizeof(gt_hip.GroupConfig) == 3 * 8 + 12 * 4
+    assert C.sizeof(gt_hip.GroupConfig) == 3 * 8 + 13 * 4 + 4  # 13 ints, padded to 8
 
 
 def test_compiled_in_and_clean_refusal_without_a_device():
