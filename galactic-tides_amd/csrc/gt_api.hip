@@ -21,6 +21,7 @@ struct History {
   uint32_t n = 0;  // particles of the evaluation described
   uint32_t systems = 1;
   int words = 1;
+  bool full_paths = false;  // a one-word build needed all 21 levels of the path sort
   uint32_t nodes = 0;
   int depth = -1;
   uint32_t groups = 0;
@@ -65,7 +66,8 @@ struct GraphKey {
 };
 
 bool same_plan(const Plan &a, const Plan &b) {
-  return a.n == b.n && a.systems == b.systems && a.words == b.words && a.node_cap == b.node_cap &&
+  return a.n == b.n && a.systems == b.systems && a.words == b.words &&
+         a.path_levels == b.path_levels && a.node_cap == b.node_cap &&
          a.depth_cap == b.depth_cap && a.group_cap == b.group_cap && a.cap_c == b.cap_c &&
          a.cap_b == b.cap_b && a.split_len == b.split_len && a.item_cap == b.item_cap &&
          a.split_cap == b.split_cap && a.longest_first == b.longest_first &&
@@ -252,6 +254,15 @@ uint32_t slot_for(uint64_t most) {
   return static_cast<uint32_t>(std::min<uint64_t>((most + most / 4 + 32 + 3) & ~3ull, 1u << 30));
 }
 
+// Path levels for the sort of a one-word build whose previous tree was depth
+// levels deep: at least one level more, and as many as the sort's 8-bit passes
+// for that hold (3 bits a level).
+int path_levels_for(int depth) {
+  const int need = std::min(depth + 1, 21);
+  const int passes = (3 * need + 7) / 8;
+  return std::min(21, 8 * passes / 3);
+}
+
 // Whether the bounds p still hold what the history says with some headroom
 // (then they are kept, so buffers and captured graphs stay as they are).
 bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
@@ -260,6 +271,8 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
     return false;
   if (static_cast<uint64_t>(h.nodes) + h.nodes / 16 > p.node_cap) return false;
   if (h.depth + 1 > p.depth_cap && p.depth_cap < 60) return false;
+  if (p.words == 1 && p.path_levels < 21 && (h.full_paths || h.depth + 1 > p.path_levels))
+    return false;
   // moment grids: sized once the levels are known, then kept while no level
   // needs more than two passes of its grid
   if (h.levels_known && !p.levels_sized) return false;
@@ -331,6 +344,7 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
   p.node_cap = same && h.nodes ? grown(h.nodes, 1024, kMaxNodes)
                                : static_cast<uint32_t>(std::min<uint64_t>(n + 1024ull, kMaxNodes));
   p.depth_cap = same && h.depth >= 0 ? std::min(60, h.depth + 3) : 60;
+  p.path_levels = same && h.depth >= 0 && !h.full_paths ? path_levels_for(h.depth) : 21;
   if (same && h.levels_known) {
     p.levels_sized = true;
     for (int d = 0; d < frame::kLevels; ++d) {
@@ -410,8 +424,12 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
   h.systems = p.systems;
   h.words = p.words;
   if (st & kAbortPaths) {
-    if (p.words >= 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
-    h.words = p.words + 1;
+    if (p.words == 1 && p.path_levels < 21) {
+      h.full_paths = true;
+    } else {
+      if (p.words >= 3) fail(GT_HIP_ERR_DEVICE, "octree build: leaf deeper than 63 levels");
+      h.words = p.words + 1;
+    }
   }
   if (fw[frame::kNodes] > kMaxNodes)
     fail(GT_HIP_ERR_LIMIT, "too many tree nodes for the packed walk records (2^26 - 1)");

@@ -202,11 +202,12 @@ __device__ inline bool degenerate(double cx, double cy, double cz, double half) 
 // Digits of levels [21 word, 21 word + 21) of every particle's octant path
 // (the descent from its system's root replays all earlier levels), and the
 // first depth <= 21 (word + 1) at which the path's node is degenerate.  Word 0
-// also seeds the sort: keys[i] = path, vals[i] = i.  sid may be null (one system).
+// also seeds the sort: keys[i] = path >> key_shift (its first levels only),
+// vals[i] = i.  sid may be null (one system).
 __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict__ pos, uint32_t n,
                                                        const double *__restrict__ roots,
                                                        const uint32_t *__restrict__ sid, int word,
-                                                       uint64_t *__restrict__ wkey,
+                                                       uint64_t *__restrict__ wkey, int key_shift,
                                                        uint64_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals,
                                                        uint8_t *__restrict__ dg) {
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict_
   wkey[i] = key;
   dg[i] = first_dg;
   if (keys) {
-    keys[i] = key;
+    keys[i] = key >> key_shift;
     vals[i] = i;
   }
 }
@@ -255,11 +256,12 @@ __global__ __launch_bounds__(kBlock) void gather_sid_keys(const uint32_t *__rest
 
 // ---- 4. shared prefixes and leaf depths ---------------------------------------------
 
-// Levels shared by the paths of sorted neighbours p, p+1 (at most 21 words).
+// Levels shared by the paths of sorted neighbours p, p+1 (at most 21 words),
+// at most cap (the levels the sort ordered by).
 __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restrict__ w0,
                                                         const uint64_t *__restrict__ w1,
                                                         const uint64_t *__restrict__ w2, int words,
-                                                        const uint32_t *__restrict__ vals,
+                                                        int cap, const uint32_t *__restrict__ vals,
                                                         const uint32_t *__restrict__ sid,
                                                         uint32_t n, uint8_t *__restrict__ lcp) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restri
       break;
     }
   }
-  lcp[p] = static_cast<uint8_t>(common);
+  lcp[p] = static_cast<uint8_t>(min(common, cap));
 }
 
 // Leaf depth of sorted position p and the number of nodes beginning at p.
@@ -679,6 +681,25 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
     node_moments(x, n_nodes, lnk, tpos, tmass, geom);
 }
 
+// Levels d_hi down to d_lo by one workgroup, with a barrier between levels:
+// for the run of small levels at the bottom of the tree, where a launch per
+// level costs more than the level's work.  Any level size is
+// handled (the loop strides), only more slowly.
+constexpr int kSerialBlock = 512;
+
+__global__ __launch_bounds__(kSerialBlock) void serial_level_moments(
+    int d_hi, int d_lo, const uint32_t *__restrict__ lstart,
+    const unsigned long long *__restrict__ frame, uint32_t node_cap, const int4 *__restrict__ lnk,
+    const double *__restrict__ tpos, const double *__restrict__ tmass, double *__restrict__ geom) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
+  for (int d = d_hi; d >= d_lo; --d) {
+    const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
+    for (uint32_t x = first + threadIdx.x; x < last; x += kSerialBlock)
+      node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+    __syncthreads();
+  }
+}
+
 #pragma clang fp contract(fast)
 
 }  // namespace
@@ -760,18 +781,23 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   // 2-3. octant paths (p.words 21-level words of each) and their stable sort:
   // by the last word first, then by each earlier one.  One word is enough
   // unless a cluster of more than leaf_size bodies shares all 21 levels; the
-  // leaf depths report it, and the host builds again with another word.
+  // leaf depths report it, and the host builds again with another word.  With
+  // one word the sort orders by its first p.path_levels levels only: the tree
+  // needs the bodies grouped down to the deepest leaf, and the bodies of a
+  // leaf go back into particle order anyway (leaf_order).  A leaf deeper than
+  // that is reported the same way, and the host builds again with all 21.
+  const int levels = p.words == 1 ? std::min(std::max(p.path_levels, 1), kLevels) : kLevels * p.words;
   uint64_t *keys = keys_.get();
   uint32_t *vals = vals_.get();
   uint8_t *dg = dg_.get();
   uint64_t *w[3] = {word0_.get(), p.words > 1 ? word1_.get() : nullptr,
                     p.words > 2 ? word2_.get() : nullptr};
   hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
-                     sid, 0, w[0], keys, vals, dg);
+                     sid, 0, w[0], p.words == 1 ? 3 * (kLevels - levels) : 0, keys, vals, dg);
   GTH_LAUNCH_CHECK();
   for (int k = 1; k < p.words; ++k) {
     hipLaunchKernelGGL(octant_paths, dim3(blocks_for(n)), dim3(kBlock), 0, s, pos, n, root_.get(),
-                       sid, k, w[k], static_cast<uint64_t *>(nullptr),
+                       sid, k, w[k], 0, static_cast<uint64_t *>(nullptr),
                        static_cast<uint32_t *>(nullptr), dg);
     GTH_LAUNCH_CHECK();
   }
@@ -780,8 +806,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
       hipLaunchKernelGGL(gather_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[k], vals, n, keys);
       GTH_LAUNCH_CHECK();
     }
-    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, 3 * kLevels, sort_tmp_.get(),
-                   s);
+    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n,
+                   p.words == 1 ? 3 * levels : 3 * kLevels, sort_tmp_.get(), s);
   }
   if (batch) {  // systems apart, most significant: one stable pass more
     int bits = 1;
@@ -799,10 +825,10 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint32_t *starts = starts_.get();
   uint32_t *offs = offs_.get();
   hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
-                     p.words, vals, sid, n, lcp);
+                     p.words, levels, vals, sid, n, lcp);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
-                     kLevels * p.words, ldep, starts,
+                     levels, ldep, starts,
                      reinterpret_cast<uint32_t *>(frame + frame::kUnresolved));
   GTH_LAUNCH_CHECK();
   exclusive_scan_u32(starts, offs, n, frame + frame::kNodes, scan_tmp_.get(), s);
@@ -851,9 +877,24 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   // allows (a deeper tree is reported by the depth in the frame).  Each level's
   // grid is sized from the previous tree's level (the launches stride, so any
   // grid is correct): most levels are small, and a full-size grid of idle
-  // workgroups per level made the moments launch-bound.
+  // workgroups per level made the moments launch-bound.  The deep tail of
+  // levels that the previous tree had small (with the empty levels the plan
+  // allows below the deepest) goes to one workgroup, level after level: 5 us
+  // instead of about 19 us for four launches.  (The small top levels are not
+  // done that way: their nodes have eight children each, and one workgroup
+  // took 61 us where five launches take 37 us.)
   const uint32_t mgrid = std::min<uint32_t>(blocks_for(cap), kMomentBlocks);
-  for (int d = p.depth_cap; d >= 0; --d) {
+  auto small = [&p](int d) {
+    return p.levels_sized && d < frame::kLevels && p.level_blocks[d] <= kSerialLevelBlocks;
+  };
+  int lo = p.depth_cap;  // levels (lo, depth_cap]: the small deep tail
+  while (lo >= 0 && small(lo)) --lo;
+  if (lo < p.depth_cap) {
+    hipLaunchKernelGGL(serial_level_moments, dim3(1), dim3(kSerialBlock), 0, s, p.depth_cap,
+                       lo + 1, lstart, frame, cap, lnk, tpos, tmass, geom);
+    GTH_LAUNCH_CHECK();
+  }
+  for (int d = lo; d >= 0; --d) {
     const uint32_t blocks =
         p.levels_sized && d < frame::kLevels ? std::min<uint32_t>(p.level_blocks[d], mgrid) : mgrid;
     hipLaunchKernelGGL(level_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0, s, d,

@@ -175,6 +175,10 @@ struct Plan {
   uint32_t n = 0;          // particles
   uint32_t systems = 1;    // independent systems (a batch), each with its own tree
   int words = 1;           // 21-level octant-path words computed per particle
+  // Levels of the path the sort orders by when words == 1 (at most 21): the
+  // deepest leaf's depth and some headroom; a deeper leaf is reported like one
+  // past the last word.
+  int path_levels = 21;
   uint32_t node_cap = 0;   // node arrays
   int depth_cap = 60;      // deepest level whose moments are computed
   uint32_t group_cap = 0;  // groups covered by the walk and evaluation grids
@@ -196,6 +200,9 @@ struct Plan {
   uint16_t level_blocks[frame::kLevels] = {};
 };
 constexpr uint16_t kMomentBlocks = 1024;
+// Levels sized at most this many workgroups get no launch of their own: runs
+// of them are done by one workgroup, level after level.
+constexpr uint16_t kSerialLevelBlocks = 2;
 
 // A device count that must not exceed cap; past it (an overflow the status
 // reports) the count reads as zero, so dependent kernels do nothing.

@@ -1,12 +1,15 @@
 // Stable LSD radix sort of (uint64 key, uint32 value) pairs, 8 bits per pass.
 //
-// Per pass: each workgroup histograms a tile of 2048 keys; an exclusive scan
-// over the digit-major histogram gives every (digit, tile) its output offset;
-// each workgroup then re-reads its tile in eight rounds of 256 consecutive
-// keys and ranks each key among equal digits with wave ballots (match masks),
-// which keeps equal keys in input order.  Used for the octree's octant-path
-// keys, where stability is what reproduces the reference's stable counting
-// sort (src/octree.cpp:106-121).
+// Per pass: each workgroup histograms a tile of 2048 keys; then one workgroup
+// per digit scans that digit's row of the digit-major tile histogram (its
+// counts over the tiles before each tile), starting from the keys of smaller
+// digits, which it adds up from the totals the other digits' workgroups
+// publish; then each workgroup ranks its tile's keys among equal digits with
+// wave ballots (match masks) in eight rounds of 256 consecutive keys, which
+// keeps equal keys in input order, stages them in LDS in digit order and
+// writes them out in that order.  Used for the octree's octant-path keys,
+// where stability is what reproduces the reference's stable counting sort
+// (src/octree.cpp:106-121).
 #include <algorithm>
 #include <utility>
 
@@ -48,6 +51,96 @@ __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restr
   counts[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Workgroup d (by ticket): counts[d * ntiles + t] becomes the output
+// position of digit d's first key of tile t, i.e. the keys of smaller digits
+// plus digit d's keys in tiles before t.  Each workgroup publishes its digit's
+// total in status[d] (epoch << 32 | total; status starts cleared and each
+// pass has its own epoch) and reads the totals of the smaller digits from
+// there.  Digits are handed out by ticket in the order workgroups start, so a
+// workgroup waits only for ones already running, which publish before they
+// wait.  A wait that outlasts kSpinLimit re-reads (it should never) sets
+// *error and counts that total as 0.
+constexpr int kScanItems = 4;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__global__ __launch_bounds__(kBlock) void digit_row_scan(uint32_t ntiles, uint32_t epoch,
+                                                         uint32_t *__restrict__ ticket,
+                                                         unsigned long long *status,
+                                                         unsigned long long *error,
+                                                         uint32_t *__restrict__ counts) {
+  __shared__ uint32_t part[kWaves];
+  __shared__ uint32_t digit;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0) digit = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t d = digit;
+  uint32_t *row = counts + static_cast<size_t>(d) * ntiles;
+  // the digit's total, published
+  uint32_t v = 0;
+  for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) v += row[t];
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  if (lane == 0) part[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (int w = 0; w < kWaves; ++w) total += part[w];
+    __hip_atomic_store(status + d, (static_cast<unsigned long long>(epoch) << 32) | total,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  // keys of smaller digits: thread j reads digit j's total
+  uint32_t below = 0;
+  if (threadIdx.x < d) {
+    unsigned long long w =
+        __hip_atomic_load(status + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t spin = 0; (w >> 32) != epoch && spin < kSpinLimit; ++spin) {
+      __builtin_amdgcn_s_sleep(1);
+      w = __hip_atomic_load(status + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if ((w >> 32) == epoch)
+      below = static_cast<uint32_t>(w);
+    else
+      atomicOr(error, 1ull);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) below += __shfl_xor(below, off, kWave);
+  if (lane == 0) part[wave] = below;
+  __syncthreads();
+  uint32_t carry = 0;
+  for (int w = 0; w < kWaves; ++w) carry += part[w];
+  __syncthreads();
+  // the row's exclusive scan, from carry
+  for (uint32_t t0 = 0; t0 < ntiles; t0 += kBlock * kScanItems) {
+    uint32_t c[kScanItems], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const uint32_t t = t0 + threadIdx.x * kScanItems + k;
+      c[k] = t < ntiles ? row[t] : 0u;
+      sum += c[k];
+    }
+    uint32_t incl = sum;
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += u;
+    }
+    if (lane == kWave - 1) part[wave] = incl;
+    __syncthreads();
+    uint32_t before = carry, total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      if (w < wave) before += part[w];
+      total += part[w];
+    }
+    uint32_t run = before + incl - sum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const uint32_t t = t0 + threadIdx.x * kScanItems + k;
+      if (t < ntiles) row[t] = run;
+      run += c[k];
+    }
+    carry += total;
+    __syncthreads();
+  }
+}
+
 // The tile's keys are ranked (stably) and placed in LDS in digit order first,
 // then written out in that order: consecutive threads write consecutive
 // addresses of a digit's run instead of scattering one key per digit bucket.
@@ -65,15 +158,23 @@ __global__ __launch_bounds__(kBlock) void tile_scatter(
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
   const uint32_t in_tile = base < n ? static_cast<uint32_t>(min(n - base, static_cast<size_t>(kTile))) : 0u;
+  // the tile's pairs, read once into registers (item k of a thread is the
+  // tile's pair k * 256 + thread)
+  uint64_t rk[kItems];
+  uint32_t rv[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = k * kBlock + threadIdx.x;
+    rk[k] = i < in_tile ? keys[base + i] : 0ull;
+    rv[k] = i < in_tile ? vals[base + i] : 0u;
+  }
   // 1. the tile's digit counts, then their exclusive sum: each digit's first
   // tile-local slot
   run[threadIdx.x] = 0;
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const uint32_t i = k * kBlock + threadIdx.x;
-    if (i < in_tile) atomicAdd(&run[(keys[base + i] >> shift) & 0xff], 1u);
-  }
+  for (int k = 0; k < kItems; ++k)
+    if (k * kBlock + threadIdx.x < in_tile) atomicAdd(&run[(rk[k] >> shift) & 0xff], 1u);
   __syncthreads();
   if (threadIdx.x < kWave) {  // one wave: four digits per lane
     const uint32_t a0 = run[4 * lane], a1 = run[4 * lane + 1], a2 = run[4 * lane + 2],
@@ -96,19 +197,14 @@ __global__ __launch_bounds__(kBlock) void tile_scatter(
   // 2. stable ranks: rounds of 256 consecutive keys, lanes of equal digit
   // matched with wave ballots, waves in order
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+#pragma unroll
   for (int round = 0; round < kItems; ++round) {
     for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t i = round * kBlock + threadIdx.x;
-    const bool valid = i < in_tile;
-    uint64_t key = 0;
-    uint32_t val = 0;
-    uint32_t d = 0;
-    if (valid) {
-      key = keys[base + i];
-      val = vals[base + i];
-      d = static_cast<uint32_t>((key >> shift) & 0xff);
-    }
+    const bool valid = round * kBlock + threadIdx.x < in_tile;
+    const uint64_t key = rk[round];
+    const uint32_t val = rv[round];
+    const uint32_t d = valid ? static_cast<uint32_t>((key >> shift) & 0xff) : 0u;
     // Lanes of this wave holding the same digit.
     uint64_t same = __ballot(valid);
 #pragma unroll
@@ -158,15 +254,22 @@ __global__ __launch_bounds__(kBlock) void copy_pairs(const uint64_t *__restrict_
   }
 }
 
+// Scratch (64-bit words): the digit totals' status words, one ticket counter
+// per pass (uint32), an error word, then the digit-major tile histogram of one
+// pass (uint32).
+constexpr int kMaxPasses = 8;  // 64-bit keys
+constexpr size_t kStatusWords = kDigits + kMaxPasses / 2 + 1;
+
 void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                  size_t n, const unsigned long long *n_dev, int bits, unsigned long long *tmp,
                  hipStream_t s) {
   const size_t ntiles = (n + kTile - 1) / kTile;
-  const size_t ncounts = static_cast<size_t>(kDigits) * ntiles;
-  uint32_t *counts = reinterpret_cast<uint32_t *>(tmp);
-  unsigned long long *scan_tmp = tmp + (ncounts + 1) / 2;
-  unsigned long long *total = scan_tmp + scan_tmp_words(ncounts);
-  const int passes = (bits + 7) / 8;
+  unsigned long long *status = tmp;
+  uint32_t *tickets = reinterpret_cast<uint32_t *>(tmp + kDigits);
+  unsigned long long *error = tmp + kDigits + kMaxPasses / 2;
+  uint32_t *counts = reinterpret_cast<uint32_t *>(tmp + kStatusWords);
+  const int passes = std::min((bits + 7) / 8, kMaxPasses);
+  launch_fill_u64(tmp, kStatusWords, 0ull, s);
   uint64_t *kin = keys, *kout = keys_alt;
   uint32_t *vin = vals, *vout = vals_alt;
   for (int p = 0; p < passes; ++p) {
@@ -174,7 +277,10 @@ void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *v
     hipLaunchKernelGGL(tile_histogram, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
                        kin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts);
     GTH_LAUNCH_CHECK();
-    exclusive_scan_u32(counts, counts, ncounts, total, scan_tmp, s);
+    hipLaunchKernelGGL(digit_row_scan, dim3(kDigits), dim3(kBlock), 0, s,
+                       static_cast<uint32_t>(ntiles), static_cast<uint32_t>(p + 1), tickets + p,
+                       status, error, counts);
+    GTH_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scatter, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, kin,
                        vin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts, kout, vout);
     GTH_LAUNCH_CHECK();
@@ -192,9 +298,7 @@ void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *v
 
 size_t radix_tmp_words(size_t n) {
   const size_t ntiles = (n + kTile - 1) / kTile;
-  const size_t counts = static_cast<size_t>(kDigits) * ntiles;
-  // counts (uint32, rounded up to 64-bit words), scan scratch, scan total
-  return (counts + 1) / 2 + scan_tmp_words(counts) + 1;
+  return kStatusWords + (static_cast<size_t>(kDigits) * ntiles + 1) / 2;
 }
 
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,

@@ -65,6 +65,44 @@ def test_gpu_tree_equals_reference_tree(hip, oracle, n, kind, seed, leaf):
     assert np.array_equal(g1.view(np.uint64), g2.view(np.uint64))
 
 
+def _assert_tree_equals_oracle(hip, oracle):
+    geom, links, order = hip.get_tree()
+    wg, wl = oracle.nodes()
+    assert hip.info()["n_nodes"] == oracle.info()["n_nodes"]
+    assert hip.info()["depth"] == oracle.info()["depth"]
+    assert np.array_equal(order, oracle.order())
+    g1, l1 = _canonical_tree(geom, links)
+    g2, l2 = _canonical_tree(wg, wl)
+    assert np.array_equal(l1, l2)
+    assert np.array_equal(g1.view(np.uint64), g2.view(np.uint64))
+
+
+def test_tree_deeper_than_the_sorted_path_levels(hip, oracle):
+    """Once a context knows its tree's depth, the path sort orders by that
+    many levels plus headroom only.  A tree that then needs much deeper leaves
+    (a tight cluster appears) is detected and built again with every level,
+    equal to the reference's; so is the shallow tree after it."""
+    n = 20000
+    pos, mass = oracle.test_system(0, n, 7)
+    cfg = dict(theta=0.7, eps=0.01, group_size=32, leaf_size=8, exact=True)
+    for _ in range(3):
+        hip.group_compute(pos, mass, **cfg)
+    shallow = hip.info()["depth"]
+    reruns = hip.pipeline_stats()["reruns"]
+    deep = pos.copy()
+    rng = np.random.default_rng(3)
+    deep[:300] = deep[0] + rng.normal(size=(300, 3)) * 1e-3  # depth 13 -> 17
+    oracle.build(deep, mass, 8)
+    hip.group_compute(deep, mass, **cfg)
+    assert hip.pipeline_stats()["reruns"] > reruns
+    assert hip.info()["depth"] > shallow + 3
+    _assert_tree_equals_oracle(hip, oracle)
+    oracle.build(pos, mass, 8)
+    for _ in range(2):
+        hip.group_compute(pos, mass, **cfg)
+        _assert_tree_equals_oracle(hip, oracle)
+
+
 @pytest.mark.parametrize("gs,pack", [(32, 1), (32, 0), (64, 1), (7, 1)])
 def test_group_compute_exact_is_bitwise(hip, oracle, gs, pack):
     pos, mass = oracle.test_system(0, 20000, 7)
