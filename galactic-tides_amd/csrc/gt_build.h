@@ -49,7 +49,7 @@ class TreeBuilder {
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, ndep_, ndep_alt_, word0_, word1_, word2_;
-  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, nchild_,
+  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, last_child_,
       sid_, tsid_, root_node_;
   DBuf<uint8_t> dg_, lcp_, ldep_;
   DBuf<unsigned long long> sort_tmp_, scan_tmp_;

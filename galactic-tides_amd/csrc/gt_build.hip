@@ -396,87 +396,135 @@ __global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restric
 
 // ---- 6. links, geometry, leaf order ------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void link_nodes(
-    const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
-    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
-    const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
-    const uint32_t *__restrict__ vals, const double *__restrict__ roots,
-    const uint32_t *__restrict__ sid, const unsigned long long *__restrict__ frame,
-    uint32_t node_cap, double *__restrict__ geom, int4 *__restrict__ lnk,
-    uint32_t *__restrict__ parent, uint32_t *__restrict__ nchild,
-    uint32_t *__restrict__ root_node) {
-  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  if (x >= n_nodes) return;
-  const uint32_t d = static_cast<uint32_t>(node_depth[x]);
-  const uint32_t b = node_begin[x];
-  // Centre and half size: replay the octant path of the node's first body
-  // from its system's root.
-  const uint32_t i = vals[b];
-  const uint32_t system = sid ? sid[i] : 0u;
-  const double *root = roots + 4 * static_cast<size_t>(system);
-  // (the path words this depth needs, loaded once)
-  const uint64_t kw[3] = {w0[i], d > static_cast<uint32_t>(kLevels) ? w1[i] : 0ull,
-                          d > 2u * kLevels ? w2[i] : 0ull};
-  double cx = root[0], cy = root[1], cz = root[2], h = root[3];
-  for (uint32_t l = 0; l < d; ++l) {
-    const uint64_t key = kw[l / kLevels];
-    const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
-    const double h2 = 0.5 * h;
-    cx = cx + ((o & 1u) ? h2 : -h2);
-    cy = cy + ((o & 2u) ? h2 : -h2);
-    cz = cz + ((o & 4u) ? h2 : -h2);
-    h = h2;
-  }
-  double *g = geom + static_cast<size_t>(kGeomStride) * x;
-  g[0] = cx;
-  g[1] = cy;
-  g[2] = cz;
-  g[3] = h;
-  g[15] = 0.0;
-  lnk[x] = make_int4(-1, 0, static_cast<int>(b), 0);
-  if (d == 0) {
-    parent[x] = kNone;
-    if (root_node) root_node[system] = x;
-    return;
-  }
-  // Parent: the last node of the level above beginning at or before b.
-  uint32_t lo = lstart[d - 1], hi = lstart[d] - 1;
+// Largest i in [lo, hi] with begin[i] <= b (begin[lo] <= b, begins ascending).
+__device__ inline uint32_t last_begin_at_or_before(const uint32_t *begin, uint32_t lo,
+                                                   uint32_t hi, uint32_t b) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1) / 2;
-    if (node_begin[mid] <= b)
+    if (begin[mid] <= b)
       lo = mid;
     else
       hi = mid - 1;
   }
-  parent[x] = lo;
-  atomicAdd(&nchild[lo], 1u);
+  return lo;
 }
 
+// Node x's centre and half size, from the octant path of its first body
+// replayed from its system's root, and its parent: the last node of the level
+// above beginning at or before x's begin.  With one path word and one system
+// (sorted_keys set) the path is the sorted key at the node's begin, shifted
+// back into place (it holds the levels the sort ordered by, which the tree
+// never exceeds), instead of the first body's path word gathered by particle
+// index.  A workgroup whose nodes are all on one level finds the parents of
+// its first and last node in global memory and the others in an LDS copy of
+// the begins between them.
+constexpr uint32_t kParentWindow = 2048;
+
+__global__ __launch_bounds__(kBlock) void link_nodes(
+    const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
+    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
+    const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
+    const uint32_t *__restrict__ vals, const uint64_t *__restrict__ sorted_keys, int key_shift,
+    const double *__restrict__ roots, const uint32_t *__restrict__ sid,
+    const unsigned long long *__restrict__ frame, uint32_t node_cap, double *__restrict__ geom,
+    int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ root_node) {
+  __shared__ uint32_t sbegin[kParentWindow];
+  __shared__ uint32_t bracket[2];
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t x0 = blockIdx.x * kBlock;
+  if (x0 >= n_nodes) return;  // the whole workgroup
+  const uint32_t x = x0 + threadIdx.x;
+  const uint32_t xl = min(x0 + kBlock, n_nodes) - 1;
+  const bool valid = x < n_nodes;
+  uint32_t d = 0, b = 0, system = 0;
+  if (valid) {
+    d = static_cast<uint32_t>(node_depth[x]);
+    b = node_begin[x];
+    // (the path words this depth needs, loaded once)
+    uint64_t kw[3] = {0ull, 0ull, 0ull};
+    if (sorted_keys) {
+      kw[0] = sorted_keys[b] << key_shift;
+    } else {
+      const uint32_t i = vals[b];
+      system = sid ? sid[i] : 0u;
+      kw[0] = w0[i];
+      kw[1] = d > static_cast<uint32_t>(kLevels) ? w1[i] : 0ull;
+      kw[2] = d > 2u * kLevels ? w2[i] : 0ull;
+    }
+    const double *root = roots + 4 * static_cast<size_t>(system);
+    double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+    for (uint32_t l = 0; l < d; ++l) {
+      const uint64_t key = kw[l / kLevels];
+      const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
+      const double h2 = 0.5 * h;
+      cx = cx + ((o & 1u) ? h2 : -h2);
+      cy = cy + ((o & 2u) ? h2 : -h2);
+      cz = cz + ((o & 4u) ? h2 : -h2);
+      h = h2;
+    }
+    double *g = geom + static_cast<size_t>(kGeomStride) * x;
+    g[0] = cx;
+    g[1] = cy;
+    g[2] = cz;
+    g[3] = h;
+    g[15] = 0.0;
+    lnk[x] = make_int4(-1, 0, static_cast<int>(b), 0);
+    if (d == 0) {
+      parent[x] = kNone;
+      if (root_node) root_node[system] = x;
+    }
+  }
+  const uint32_t dfirst = static_cast<uint32_t>(node_depth[x0]);
+  const bool one_level = dfirst > 0 && dfirst == static_cast<uint32_t>(node_depth[xl]);
+  uint32_t up = kNone;
+  if (one_level) {
+    const uint32_t lo = lstart[dfirst - 1], hi = lstart[dfirst] - 1;
+    if (threadIdx.x == 0) bracket[0] = last_begin_at_or_before(node_begin, lo, hi, b);
+    if (x == xl) bracket[1] = last_begin_at_or_before(node_begin, lo, hi, b);
+    __syncthreads();
+    const uint32_t p0 = bracket[0], p1 = bracket[1];
+    if (p1 - p0 < kParentWindow) {
+      for (uint32_t t = threadIdx.x; t <= p1 - p0; t += kBlock) sbegin[t] = node_begin[p0 + t];
+      __syncthreads();
+      if (valid) up = p0 + last_begin_at_or_before(sbegin, 0, p1 - p0, b);
+    } else if (valid) {
+      up = last_begin_at_or_before(node_begin, p0, p1, b);
+    }
+  } else if (valid && d > 0) {
+    up = last_begin_at_or_before(node_begin, lstart[d - 1], lstart[d] - 1, b);
+  }
+  if (up != kNone) parent[x] = up;
+}
+
+// Children of a node are consecutive on the level below: the first child
+// (the one beginning where its parent begins) goes to its parent's link, the
+// last one (whose successor has another parent) to last_child[parent].
 __global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restrict__ parent,
-                                                       const uint32_t *__restrict__ nchild,
                                                        const uint32_t *__restrict__ node_begin,
                                                        const unsigned long long *__restrict__ frame,
-                                                       uint32_t node_cap, int4 *__restrict__ lnk) {
+                                                       uint32_t node_cap, int4 *__restrict__ lnk,
+                                                       uint32_t *__restrict__ last_child) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t p = parent[x];
-  if (p != kNone && node_begin[p] == node_begin[x]) lnk[p].x = static_cast<int>(x);
-  lnk[x].y = static_cast<int>(nchild[x]);
+  if (p == kNone) return;
+  if (node_begin[p] == node_begin[x]) lnk[p].x = static_cast<int>(x);
+  if (x + 1 >= n_nodes || parent[x + 1] != p) last_child[p] = x;
 }
 
-// end of node x: the begin of the node after it in its parent's children, or
-// (for a last child) its parent's end; the root ends at n.
 __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__ parent,
                                                     const uint32_t *__restrict__ node_begin,
                                                     const uint64_t *__restrict__ node_depth,
+                                                    const uint32_t *__restrict__ last_child,
                                                     const unsigned long long *__restrict__ frame,
                                                     uint32_t node_cap, uint32_t n,
                                                     int4 *__restrict__ lnk) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
   if (x0 >= n_nodes) return;
+  const int first = lnk[x0].x;  // children: first .. last_child[x0]
+  lnk[x0].y = first >= 0 ? static_cast<int>(last_child[x0]) - first + 1 : 0;
   uint32_t x = x0;
   for (;;) {
     const uint32_t p = parent[x];
@@ -485,8 +533,7 @@ __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__
       lnk[x0].w = static_cast<int>(next_root ? node_begin[x + 1] : n);
       return;
     }
-    const int4 P = lnk[p];
-    if (static_cast<int>(x) != P.x + P.y - 1) {
+    if (x + 1 < n_nodes && parent[x + 1] == p) {  // not its parent's last child
       lnk[x0].w = static_cast<int>(x + 1 < n_nodes ? node_begin[x + 1] : n);
       return;
     }
@@ -737,7 +784,7 @@ void TreeBuilder::reserve(const Plan &p) {
   geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
   lnk_.ensure(nn);
   parent_.ensure(nn);
-  nchild_.ensure(nn);
+  last_child_.ensure(nn);
   tpos_.ensure(3 * n);
   tmass_.ensure(n);
 }
@@ -853,17 +900,18 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   double *geom = geom_.get();
   int4 *lnk = lnk_.get();
   uint32_t *parent = parent_.get();
-  uint32_t *nchild = nchild_.get();
-  launch_fill_u32(nchild, cap, 0u, s);
+  uint32_t *last_child = last_child_.get();
   hipLaunchKernelGGL(link_nodes, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
-                     w[0], w[1], w[2], vals, root_.get(), sid, frame, cap, geom, lnk, parent,
-                     nchild, root_node);
+                     w[0], w[1], w[2], vals,
+                     p.words == 1 && !batch ? keys : static_cast<const uint64_t *>(nullptr),
+                     p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, frame, cap,
+                     geom, lnk, parent, root_node);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finish_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nchild, nbeg,
-                     frame, cap, lnk);
+  hipLaunchKernelGGL(finish_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, frame,
+                     cap, lnk, last_child);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep, frame,
-                     cap, n, lnk);
+  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
+                     last_child, frame, cap, n, lnk);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, vals);
   GTH_LAUNCH_CHECK();
