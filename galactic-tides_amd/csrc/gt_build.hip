@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
 // in order, so every sum is the reference's sequence.
 constexpr int kChunk = 8;
 
-__device__ inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__restrict__ lnk,
+__device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__restrict__ lnk,
                                     const double *__restrict__ tpos,
                                     const double *__restrict__ tmass,
                                     double *__restrict__ geom) {
@@ -726,6 +726,31 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
   const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
   for (uint32_t x = first + blockIdx.x * kBlock + threadIdx.x; x < last; x += gridDim.x * kBlock)
     node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+}
+
+// Levels d and d - 1 in one launch: eight lanes per node of level d - 1, lane
+// j doing the node's child j (on level d), then, after a barrier, the node's
+// first lane the node itself.  For small levels, where a launch costs more
+// than the level's work.
+__global__ __launch_bounds__(kBlock) void level_pair_moments(
+    int d, const uint32_t *__restrict__ lstart, const unsigned long long *__restrict__ frame,
+    uint32_t node_cap, const int4 *__restrict__ lnk, const double *__restrict__ tpos,
+    const double *__restrict__ tmass, double *__restrict__ geom) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t first = lstart[d - 1], last = min(lstart[d], n_nodes);
+  const uint32_t lanes = last > first ? 8 * (last - first) : 0u;
+  // every thread of a workgroup takes the same number of rounds (barriers)
+  for (uint32_t base = blockIdx.x * kBlock; base < lanes; base += gridDim.x * kBlock) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t x = first + t / 8, j = t % 8;
+    if (t < lanes) {
+      const int4 L = lnk[x];
+      if (L.x >= 0 && j < static_cast<uint32_t>(L.y) && static_cast<uint32_t>(L.x) + j < n_nodes)
+        node_moments(static_cast<uint32_t>(L.x) + j, n_nodes, lnk, tpos, tmass, geom);
+    }
+    __syncthreads();
+    if (t < lanes && j == 0) node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+  }
 }
 
 // Levels d_hi down to d_lo by one workgroup, with a barrier between levels:
@@ -942,12 +967,24 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                        lo + 1, lstart, frame, cap, lnk, tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
   }
-  for (int d = lo; d >= 0; --d) {
+  auto paired = [&p](int d) {
+    return p.levels_sized && d < frame::kLevels && p.level_blocks[d] <= kPairLevelBlocks;
+  };
+  for (int d = lo; d >= 0;) {
+    if (d >= 1 && paired(d) && paired(d - 1)) {  // levels d and d - 1 together
+      const uint32_t blocks = std::min<uint32_t>(8u * p.level_blocks[d - 1], 8u * mgrid);
+      hipLaunchKernelGGL(level_pair_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0,
+                         s, d, lstart, frame, cap, lnk, tpos, tmass, geom);
+      GTH_LAUNCH_CHECK();
+      d -= 2;
+      continue;
+    }
     const uint32_t blocks =
         p.levels_sized && d < frame::kLevels ? std::min<uint32_t>(p.level_blocks[d], mgrid) : mgrid;
     hipLaunchKernelGGL(level_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0, s, d,
                        lstart, frame, cap, lnk, tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
+    --d;
   }
 }
 
