@@ -150,6 +150,13 @@ struct gt_hip_context {
   const double *lists_geom = nullptr;  // geometry rows the last lists refer to
   // particle state (resident leapfrog) and inputs of single evaluations
   DBuf<double> pos, vel, mass, acc, pot;
+  // The resident state is kept in the order of the tree of its last priming
+  // (gt_hip_prime), so the gathers into tree order and the force scatter back
+  // are nearly sequential; orig[k] is the caller's index of resident particle k.
+  DBuf<uint32_t> orig, orig_tmp;
+  DBuf<double> state_tmp;
+  bool permuted = false;       // orig is not the identity
+  bool tree_on_state = false;  // the last tree was built from the resident state
   DBuf<double> diag_partial, diag_total;  // diagnostics sums
   DBuf<double> in_pos, in_mass, in_acc, in_pot;
   uint64_t n_state = 0;
@@ -933,8 +940,10 @@ void predictor_after_eval(gt_hip_context *c, const Plan &p, bool overflowed, con
 // evaluated.  Any other bound too small: everything again with the sizes the
 // status reported.  Phase times go to c->tm.
 void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos,
-                const double *mass, uint32_t n, double *acc, double *pot) {
+                const double *mass, uint32_t n, double *acc, double *pot,
+                const uint32_t *orig = nullptr) {
   hipStream_t s = c->stream;
+  c->tree_on_state = orig != nullptr;
   unsigned long long *frame = c->frame.get();
   for (int attempt = 0;; ++attempt) {
     if (attempt == 8) fail(GT_HIP_ERR_DEVICE, "force evaluation: bounds did not settle");
@@ -943,7 +952,7 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
-    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s);
+    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig);
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     enqueue_tree_lists(c, cfg, p);
@@ -1019,7 +1028,7 @@ void eager_step(gt_hip_context *c, const gt_group_config *cfg, double dt) {
   unsigned long long *status = c->frame.get() + frame::kStatus;
   launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt, status, c->stream);
   force_eval(c, cfg, c->pos.get(), c->mass.get(), static_cast<uint32_t>(n), c->acc.get(),
-             c->pot.get());
+             c->pot.get(), c->orig.get());
   launch_kick(c->vel.get(), c->acc.get(), n, h, status, c->stream);
 }
 
@@ -1073,7 +1082,8 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt,
                         frame + frame::kStatus, s);
       clear_counts(frame, s);
-      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s);
+      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s,
+                      c->orig.get());
       break;
     case 1:
       // (overlapped steps launch this phase and the next from graphs per range)
@@ -1255,7 +1265,8 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
       set_margin(c, std::min(c->pred.margin + 0.1f, 3.0f));
     }
     learn(c, p, st);
-    force_eval(c, cfg, c->pos.get(), c->mass.get(), n, c->acc.get(), c->pot.get());
+    force_eval(c, cfg, c->pos.get(), c->mass.get(), n, c->acc.get(), c->pot.get(),
+               c->orig.get());
     launch_kick(c->vel.get(), c->acc.get(), n, 0.5 * dt, frame + frame::kStatus, s);
     ++done_total;
   }
@@ -1292,6 +1303,7 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
+    c->tree_on_state = false;
     c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     read_frame(c);
@@ -1316,6 +1328,50 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
   c->info.n_nodes = c->fw[frame::kNodes];
   c->info.depth = c->fw[frame::kDepth];
   return p;
+}
+
+// The resident state (and its forces) into the order of the tree just built
+// from it; orig follows, and the tree's order becomes the identity.
+void permute_state(gt_hip_context *c) {
+  const uint64_t n = c->n_state;
+  if (!n || !c->tree_on_state || c->tree.n() != n) return;
+  hipStream_t s = c->stream;
+  const uint32_t *order = c->tree.order();
+  double *tmp = c->state_tmp.ensure(3 * n);
+  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
+                                           {c->acc.get(), 3}, {c->mass.get(), 1},
+                                           {c->pot.get(), 1}};
+  for (const auto &r : rows) {
+    launch_gather_rows(r.first, order, n, r.second, tmp, s);
+    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  uint32_t *ot = c->orig_tmp.ensure(n);
+  launch_compose_u32(c->orig.get(), order, n, ot, s);
+  GTH_CHECK(hipMemcpyAsync(c->orig.get(), ot, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  launch_compose_u32(nullptr, nullptr, n, c->tree.order_mutable(), s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  c->permuted = true;
+}
+
+// The resident state back into the caller's order (orig the identity again).
+void unpermute_state(gt_hip_context *c) {
+  const uint64_t n = c->n_state;
+  if (!n || !c->permuted) return;
+  hipStream_t s = c->stream;
+  double *tmp = c->state_tmp.ensure(3 * n);
+  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
+                                           {c->acc.get(), 3}, {c->mass.get(), 1},
+                                           {c->pot.get(), 1}};
+  for (const auto &r : rows) {
+    launch_scatter_rows(r.first, c->orig.get(), n, r.second, tmp, s);
+    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  launch_compose_u32(nullptr, nullptr, n, c->orig.get(), s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  c->permuted = false;
+  c->tree_on_state = false;  // its order refers to the permuted rows
 }
 
 }  // namespace
@@ -1915,10 +1971,16 @@ int gt_hip_get_tree(gt_hip_context *c, double *geom, int32_t *links, uint64_t *o
     hipStream_t s = c->stream;
     if (geom) download(geom, c->tree.geom(), static_cast<size_t>(kGeomStride) * nn, s);
     if (links) download(reinterpret_cast<int4 *>(links), c->tree.links(), nn, s);
-    std::vector<uint32_t> o(order ? n : 0);
+    std::vector<uint32_t> o(order ? n : 0), orig;
     if (order) download(o.data(), c->tree.order(), n, s);
+    // a tree of the resident state: its positions' particles in the caller's order
+    const bool mapped = order && c->tree_on_state && c->permuted && c->n_state == n;
+    if (mapped) {
+      orig.resize(n);
+      download(orig.data(), c->orig.get(), n, s);
+    }
     GTH_CHECK(hipStreamSynchronize(s));
-    for (uint32_t k = 0; k < o.size(); ++k) order[k] = o[k];
+    for (uint32_t k = 0; k < o.size(); ++k) order[k] = mapped ? orig[o[k]] : o[k];
   });
 }
 
@@ -1980,10 +2042,14 @@ int gt_hip_state_set(gt_hip_context *c, uint64_t n, const double *pos, const dou
     c->mass.ensure(m);
     c->acc.ensure(3 * m);
     c->pot.ensure(m);
+    c->orig.ensure(m);
     upload(c->pos.get(), pos, 3 * n, s);
     upload(c->vel.get(), vel, 3 * n, s);
     upload(c->mass.get(), mass, n, s);
+    launch_compose_u32(nullptr, nullptr, n, c->orig.get(), s);  // the identity
     GTH_CHECK(hipStreamSynchronize(s));
+    c->permuted = false;
+    c->tree_on_state = false;
     c->n_state = n;
     c->state_systems = 1;
     c->state_forces = false;
@@ -1994,6 +2060,7 @@ int gt_hip_state_set_systems(gt_hip_context *c, uint64_t n_systems, const uint64
   if (!c) return GT_HIP_ERR_ARGUMENT;
   return guarded(c, [&] {
     GTH_CHECK(hipSetDevice(c->device));
+    unpermute_state(c);  // the offsets are ranges of the caller's order
     upload_systems(c, c->n_state, n_systems, offsets, c->state_off);
     GTH_CHECK(hipStreamSynchronize(c->stream));
     c->state_systems = static_cast<uint32_t>(n_systems);
@@ -2006,10 +2073,30 @@ int gt_hip_state_get(gt_hip_context *c, double *pos, double *vel, double *acc, d
   return guarded(c, [&] {
     const uint64_t n = c->n_state;
     hipStream_t s = c->stream;
-    if (pos) download(pos, c->pos.get(), 3 * n, s);
-    if (vel) download(vel, c->vel.get(), 3 * n, s);
-    if (acc) download(acc, c->acc.get(), 3 * n, s);
-    if (pot) download(pot, c->pot.get(), n, s);
+    if (!c->permuted) {
+      if (pos) download(pos, c->pos.get(), 3 * n, s);
+      if (vel) download(vel, c->vel.get(), 3 * n, s);
+      if (acc) download(acc, c->acc.get(), 3 * n, s);
+      if (pot) download(pot, c->pot.get(), n, s);
+      GTH_CHECK(hipStreamSynchronize(s));
+      return;
+    }
+    // back into the caller's order: row k of the resident state is particle orig[k]
+    std::vector<uint32_t> o(n);
+    download(o.data(), c->orig.get(), n, s);
+    std::vector<double> t;
+    auto get = [&](double *out, const double *dev, int w) {
+      if (!out) return;
+      t.resize(w * n);
+      download(t.data(), dev, w * n, s);
+      GTH_CHECK(hipStreamSynchronize(s));
+      for (uint64_t k = 0; k < n; ++k)
+        for (int a = 0; a < w; ++a) out[w * static_cast<uint64_t>(o[k]) + a] = t[w * k + a];
+    };
+    get(pos, c->pos.get(), 3);
+    get(vel, c->vel.get(), 3);
+    get(acc, c->acc.get(), 3);
+    get(pot, c->pot.get(), 1);
     GTH_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -2061,7 +2148,8 @@ int gt_hip_prime(gt_hip_context *c, const gt_group_config *cfg) {
     c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
     if (c->n_state)
       force_eval(c, cfg, c->pos.get(), c->mass.get(), static_cast<uint32_t>(c->n_state),
-                 c->acc.get(), c->pot.get());
+                 c->acc.get(), c->pot.get(), c->orig.get());
+    permute_state(c);
     c->state_forces = true;
   });
 }
@@ -2081,6 +2169,7 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     c->state_forces = false;
     c->systems = c->state_systems;
     c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
+    c->tree_on_state = true;
     if (use_graph)
       graph_steps(c, cfg, nsteps, dt);
     else

@@ -22,9 +22,11 @@ class TreeBuilder {
   // reads them after the evaluation and hands the node count back with
   // set_counts.  A tree with more nodes than p.node_cap is not built.
   // Batches (p.systems > 1) take the system offsets (p.systems + 1 entries,
-  // device memory).
+  // device memory).  orig (device, may be null: the identity) gives the
+  // caller's index of each particle in pos; bodies of a leaf are ordered by it.
   void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
-               const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s);
+               const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s,
+               const uint32_t *orig = nullptr);
   void set_counts(uint32_t n_nodes, int depth) {
     n_nodes_ = n_nodes;
     depth_ = depth;
@@ -36,6 +38,7 @@ class TreeBuilder {
   const double *geom() const { return geom_.get(); }
   const int4 *links() const { return lnk_.get(); }
   const uint32_t *order() const { return vals_.get(); }
+  uint32_t *order_mutable() { return vals_.get(); }
   const double *tree_pos() const { return tpos_.get(); }
   const double *tree_mass() const { return tmass_.get(); }
   const double *root() const { return root_.get(); }

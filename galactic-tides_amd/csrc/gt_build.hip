@@ -542,9 +542,13 @@ __global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__
 }
 
 // Bodies of a leaf back into particle order (insertion sort; leaves are small).
+constexpr uint32_t kLeafRegs = 16;  // leaves up to this size are sorted in registers
+// orig (may be null: the identity) maps the positions in pos to the caller's
+// particle indices, which decide the order.
 __global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ lnk,
                                                      const unsigned long long *__restrict__ frame,
                                                      uint32_t node_cap,
+                                                     const uint32_t *__restrict__ orig,
                                                      uint32_t *__restrict__ vals) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
@@ -552,10 +556,40 @@ __global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ ln
   const int4 L = lnk[x];
   if (L.y != 0) return;
   const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
+  if (e - b <= kLeafRegs) {  // the leaf in registers: its loads issue together
+    uint32_t v[kLeafRegs], key[kLeafRegs];
+#pragma unroll
+    for (uint32_t i = 0; i < kLeafRegs; ++i)
+      if (b + i < e) v[i] = vals[b + i];
+#pragma unroll
+    for (uint32_t i = 0; i < kLeafRegs; ++i)
+      if (b + i < e) key[i] = orig ? orig[v[i]] : v[i];
+    bool moved = false;
+#pragma unroll
+    for (uint32_t i = 1; i < kLeafRegs; ++i) {  // insertion sort by key, stable
+#pragma unroll
+      for (uint32_t j = i; j > 0; --j) {
+        if (b + i < e && key[j - 1] > key[j]) {
+          const uint32_t tk = key[j - 1], tv = v[j - 1];
+          key[j - 1] = key[j];
+          v[j - 1] = v[j];
+          key[j] = tk;
+          v[j] = tv;
+          moved = true;
+        }
+      }
+    }
+    if (moved) {
+#pragma unroll
+      for (uint32_t i = 0; i < kLeafRegs; ++i)
+        if (b + i < e) vals[b + i] = v[i];
+    }
+    return;
+  }
   for (uint32_t k = b + 1; k < e; ++k) {
-    const uint32_t v = vals[k];
+    const uint32_t v = vals[k], key = orig ? orig[v] : v;
     uint32_t j = k;
-    while (j > b && vals[j - 1] > v) {
+    while (j > b && (orig ? orig[vals[j - 1]] : vals[j - 1]) > key) {
       vals[j] = vals[j - 1];
       --j;
     }
@@ -816,7 +850,7 @@ void TreeBuilder::reserve(const Plan &p) {
 
 void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                           const unsigned long long *sys_off, unsigned long long *frame,
-                          hipStream_t s) {
+                          hipStream_t s, const uint32_t *orig) {
   const uint32_t n = p.n;
   n_ = n;
   node_cap_ = p.node_cap;
@@ -938,7 +972,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
                      last_child, frame, cap, n, lnk);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, vals);
+  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, orig,
+                     vals);
   GTH_LAUNCH_CHECK();
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();

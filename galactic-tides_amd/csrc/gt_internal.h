@@ -391,6 +391,14 @@ void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double
                     double *pot, const unsigned long long *status, hipStream_t s);
 void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,
                        double dt, const unsigned long long *status, hipStream_t s);
+// Permutations of the resident state: out[k] = in[order[k]] (rows of w
+// doubles); out[idx[k]] = in[k]; out[k] = a[b[k]] (a or b null: the identity).
+void launch_gather_rows(const double *in, const uint32_t *order, uint64_t n, int w, double *out,
+                        hipStream_t s);
+void launch_scatter_rows(const double *in, const uint32_t *idx, uint64_t n, int w, double *out,
+                         hipStream_t s);
+void launch_compose_u32(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *out,
+                        hipStream_t s);
 void launch_kick(double *vel, const double *acc, size_t n, double h,
                  const unsigned long long *status, hipStream_t s);
 // Unless frame[kStatus] is set: frame[kStepsDone] += 1 and the step's

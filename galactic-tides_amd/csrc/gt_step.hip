@@ -122,6 +122,39 @@ __global__ void count_step(unsigned long long *frame) {
   frame[frame::kStepInteractions] += frame[frame::kStats] + frame[frame::kStats + 1];
 }
 
+// out[k] = in[order[k]] for rows of w doubles (the resident state into the
+// order of a tree: order gives the position in `in` of tree position k).
+__global__ __launch_bounds__(kBlock) void gather_rows(const double *__restrict__ in,
+                                                      const uint32_t *__restrict__ order,
+                                                      uint64_t n, int w,
+                                                      double *__restrict__ out) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t i = order[k];
+  for (int a = 0; a < w; ++a) out[w * k + a] = in[w * i + a];
+}
+
+// out[idx[k]] = in[k] for rows of w doubles (idx a permutation).
+__global__ __launch_bounds__(kBlock) void scatter_rows(const double *__restrict__ in,
+                                                       const uint32_t *__restrict__ idx,
+                                                       uint64_t n, int w,
+                                                       double *__restrict__ out) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t i = idx[k];
+  for (int a = 0; a < w; ++a) out[w * i + a] = in[w * k + a];
+}
+
+// out[k] = (a ? a[b[k]] : b[k]); iota when b is null too.
+__global__ __launch_bounds__(kBlock) void compose_u32(const uint32_t *__restrict__ a,
+                                                      const uint32_t *__restrict__ b, uint64_t n,
+                                                      uint32_t *__restrict__ out) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t j = b ? b[k] : static_cast<uint32_t>(k);
+  out[k] = a ? a[j] : j;
+}
+
 }  // namespace
 
 void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
@@ -145,6 +178,27 @@ void launch_kick(double *vel, const double *acc, size_t n, double h,
   if (!n) return;
   hipLaunchKernelGGL(kick, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, vel, acc, 3 * n, h,
                      status);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_gather_rows(const double *in, const uint32_t *order, uint64_t n, int w, double *out,
+                        hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(gather_rows, dim3(blocks_for(n)), dim3(kBlock), 0, s, in, order, n, w, out);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_scatter_rows(const double *in, const uint32_t *idx, uint64_t n, int w, double *out,
+                         hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(scatter_rows, dim3(blocks_for(n)), dim3(kBlock), 0, s, in, idx, n, w, out);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_compose_u32(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *out,
+                        hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(compose_u32, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, b, n, out);
   GTH_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,64 @@
+"""The resident leapfrog state is kept in the order of the tree of its last
+priming (gt_hip_prime), so gathers into tree order are nearly sequential.
+None of that may show: downloads, forces, the tree's body order and a later
+change of systems all see the caller's particle order, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(theta=0.75, eps=0.01, group_size=32, leaf_size=8, exact=True)
+
+
+def same_bits(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def test_prime_and_steps_keep_the_callers_order(hip, oracle):
+    n = 6000
+    pos, mass = oracle.test_system(0, n, 9)
+    vel = np.random.default_rng(1).normal(size=pos.shape) * 0.05
+    hip.state_set(pos, vel, mass)
+    hip.prime(**CFG)
+    p, v, a, q = hip.state_get()
+    assert same_bits(p, pos) and same_bits(v, vel)
+    acc, pot = hip.group_compute(pos, mass, **CFG)
+    assert same_bits(a, acc) and same_bits(q, pot)
+    # the tree of the priming, in caller indices
+    hip.state_set(pos, vel, mass)
+    hip.prime(**CFG)
+    _, _, order = hip.get_tree()
+    oracle.build(pos, mass, 8)
+    assert np.array_equal(order, oracle.order())
+    # steps on the permuted state: the tree of the last step, and the state,
+    # against the same steps on an unpermuted start
+    hip.steps(3, 0.01, use_graph=True, **CFG)
+    p3, v3, a3, q3 = hip.state_get()
+    _, _, order = hip.get_tree()
+    want = oracle.kdk_group(pos, vel, mass, 3, 0.01, eps=0.01)
+    for got, ref in zip((p3, v3, a3, q3), want):
+        assert same_bits(got, ref)
+    oracle.build(p3, mass, 8)  # the last step's tree (the final kick moves nothing)
+    assert np.array_equal(order, oracle.order())
+
+
+def test_systems_set_after_priming(hip, oracle):
+    n = 4000
+    pos, mass = oracle.test_system(0, n, 4)
+    vel = np.zeros_like(pos)
+    hip.state_set(pos, vel, mass)
+    hip.prime(**CFG)  # the state is now in this tree's order
+    off = np.array([0, 1500, n])
+    hip.state_set_systems(off)  # ranges of the caller's order
+    hip.prime(**CFG)
+    p, v, a, q = hip.state_get()
+    assert same_bits(p, pos) and same_bits(v, vel)
+    acc, pot = hip.group_compute_batch(off, pos, mass, **CFG)
+    assert same_bits(a, acc) and same_bits(q, pot)
+    d = hip.state_diagnostics(0.0, systems=2)
+    assert len(d) == 2
+    for s in range(2):
+        m = mass[off[s]:off[s + 1]]
+        com = (m[:, None] * pos[off[s]:off[s + 1]]).sum(0) / m.sum()
+        assert np.allclose(d[s]["centre_of_mass"], com, rtol=1e-12, atol=1e-12)
