@@ -248,7 +248,7 @@ int gt_hip_state_set(gt_hip_context *ctx, uint64_t n, const double *pos, const d
  * prime/steps evaluate each system on its own.  gt_hip_state_set resets to
  * one system.  Forces must be primed again. */
 int gt_hip_state_set_systems(gt_hip_context *ctx, uint64_t n_systems, const uint64_t *offsets);
-/* Download; any pointer may be NULL. */
+/* Download, in the order the particles were set; any pointer may be NULL. */
 int gt_hip_state_get(gt_hip_context *ctx, double *pos, double *vel, double *acc, double *pot);
 
 /* Diagnostics (include/galactic_tides/integrator.h:49-58): total energy is
@@ -269,7 +269,11 @@ typedef struct {
  * sequential sums to FP64 rounding. */
 int gt_hip_state_diagnostics(gt_hip_context *ctx, double time, uint64_t max_out,
                              gt_diagnostics *out);
-/* LeapfrogKDK::prime (integrator.h:42): forces for the resident state. */
+/* LeapfrogKDK::prime (integrator.h:42): forces for the resident state.  The
+ * library then keeps the resident particles in the order of this force
+ * evaluation's tree (so each step's gathers into tree order are nearly
+ * sequential); results, downloads and gt_hip_get_tree's order are in the
+ * caller's order all the same. */
 int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
 /* nsteps x LeapfrogKDK::step (src/integrator.cpp:5-16) on the resident state:
  * FP64 kick and drift kernels around each GPU force evaluation.  An
