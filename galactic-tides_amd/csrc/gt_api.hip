@@ -87,6 +87,8 @@ bool same_key(const GraphKey &a, const GraphKey &b) {
 }
 
 constexpr int kTimedSteps = 64;  // graph steps whose phases are timed with events
+// Leapfrog steps between reorders of the resident state into tree order.
+constexpr int kReorderSteps = 64;
 constexpr int kPhases = 4;       // tree, lists, evaluation, scatter + kick
 
 constexpr int kMaxChunks = 8;  // overlap: walk chunks whose evaluation runs alongside
@@ -156,6 +158,7 @@ struct gt_hip_context {
   DBuf<uint32_t> orig, orig_tmp;
   DBuf<double> state_tmp;
   bool permuted = false;       // orig is not the identity
+  int steps_in_order = 0;      // steps since the state was last put in tree order
   bool tree_on_state = false;  // the last tree was built from the resident state
   DBuf<double> diag_partial, diag_total;  // diagnostics sums
   DBuf<double> in_pos, in_mass, in_acc, in_pot;
@@ -1022,6 +1025,51 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
 
 // ---- leapfrog steps ---------------------------------------------------------------
 
+// The resident state (and its forces) into the order of the tree just built
+// from it; orig follows, and the tree's order becomes the identity.
+void permute_state(gt_hip_context *c) {
+  const uint64_t n = c->n_state;
+  if (!n || !c->tree_on_state || c->tree.n() != n) return;
+  hipStream_t s = c->stream;
+  const uint32_t *order = c->tree.order();
+  double *tmp = c->state_tmp.ensure(3 * n);
+  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
+                                           {c->acc.get(), 3}, {c->mass.get(), 1},
+                                           {c->pot.get(), 1}};
+  for (const auto &r : rows) {
+    launch_gather_rows(r.first, order, n, r.second, tmp, s);
+    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  uint32_t *ot = c->orig_tmp.ensure(n);
+  launch_compose_u32(c->orig.get(), order, n, ot, s);
+  GTH_CHECK(hipMemcpyAsync(c->orig.get(), ot, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  launch_compose_u32(nullptr, nullptr, n, c->tree.order_mutable(), s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  c->permuted = true;
+  c->steps_in_order = 0;
+}
+
+// The resident state back into the caller's order (orig the identity again).
+void unpermute_state(gt_hip_context *c) {
+  const uint64_t n = c->n_state;
+  if (!n || !c->permuted) return;
+  hipStream_t s = c->stream;
+  double *tmp = c->state_tmp.ensure(3 * n);
+  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
+                                           {c->acc.get(), 3}, {c->mass.get(), 1},
+                                           {c->pot.get(), 1}};
+  for (const auto &r : rows) {
+    launch_scatter_rows(r.first, c->orig.get(), n, r.second, tmp, s);
+    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  launch_compose_u32(nullptr, nullptr, n, c->orig.get(), s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  c->permuted = false;
+  c->tree_on_state = false;  // its order refers to the permuted rows
+}
+
 void eager_step(gt_hip_context *c, const gt_group_config *cfg, double dt) {
   const uint64_t n = c->n_state;
   const double h = 0.5 * dt;
@@ -1206,7 +1254,9 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     }
     const Plan p = plan_for(c, n, cfg);
     prepare_step_graphs(c, cfg, p, dt);
-    const int k = nsteps - done_total;
+    // at most kReorderSteps steps between reorders of the state (the graphs
+    // stay valid: the state keeps its buffers)
+    const int k = std::min(nsteps - done_total, std::max(1, kReorderSteps - c->steps_in_order));
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     launch_fill_u64(frame + frame::kStepsDone, 2, 0ull, s);  // and kStepInteractions
     for (int i = 0; i < k; ++i) {
@@ -1246,6 +1296,8 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     if (st == 0) {
       // counts of the last step
       finish(c, p, c->tree.geom(), false);
+      c->steps_in_order += done;
+      if (c->steps_in_order >= kReorderSteps) permute_state(c);
       if (p.predicted) {
         Predictor &P = c->pred;
         P.predicted += done;
@@ -1328,50 +1380,6 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
   c->info.n_nodes = c->fw[frame::kNodes];
   c->info.depth = c->fw[frame::kDepth];
   return p;
-}
-
-// The resident state (and its forces) into the order of the tree just built
-// from it; orig follows, and the tree's order becomes the identity.
-void permute_state(gt_hip_context *c) {
-  const uint64_t n = c->n_state;
-  if (!n || !c->tree_on_state || c->tree.n() != n) return;
-  hipStream_t s = c->stream;
-  const uint32_t *order = c->tree.order();
-  double *tmp = c->state_tmp.ensure(3 * n);
-  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
-                                           {c->acc.get(), 3}, {c->mass.get(), 1},
-                                           {c->pot.get(), 1}};
-  for (const auto &r : rows) {
-    launch_gather_rows(r.first, order, n, r.second, tmp, s);
-    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
-                             hipMemcpyDeviceToDevice, s));
-  }
-  uint32_t *ot = c->orig_tmp.ensure(n);
-  launch_compose_u32(c->orig.get(), order, n, ot, s);
-  GTH_CHECK(hipMemcpyAsync(c->orig.get(), ot, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
-  launch_compose_u32(nullptr, nullptr, n, c->tree.order_mutable(), s);
-  GTH_CHECK(hipStreamSynchronize(s));
-  c->permuted = true;
-}
-
-// The resident state back into the caller's order (orig the identity again).
-void unpermute_state(gt_hip_context *c) {
-  const uint64_t n = c->n_state;
-  if (!n || !c->permuted) return;
-  hipStream_t s = c->stream;
-  double *tmp = c->state_tmp.ensure(3 * n);
-  const std::pair<double *, int> rows[] = {{c->pos.get(), 3}, {c->vel.get(), 3},
-                                           {c->acc.get(), 3}, {c->mass.get(), 1},
-                                           {c->pot.get(), 1}};
-  for (const auto &r : rows) {
-    launch_scatter_rows(r.first, c->orig.get(), n, r.second, tmp, s);
-    GTH_CHECK(hipMemcpyAsync(r.first, tmp, sizeof(double) * r.second * n,
-                             hipMemcpyDeviceToDevice, s));
-  }
-  launch_compose_u32(nullptr, nullptr, n, c->orig.get(), s);
-  GTH_CHECK(hipStreamSynchronize(s));
-  c->permuted = false;
-  c->tree_on_state = false;  // its order refers to the permuted rows
 }
 
 }  // namespace
@@ -2170,10 +2178,14 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     c->systems = c->state_systems;
     c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
     c->tree_on_state = true;
-    if (use_graph)
+    if (use_graph) {
       graph_steps(c, cfg, nsteps, dt);
-    else
-      for (int step = 0; step < nsteps; ++step) eager_step(c, cfg, dt);
+    } else {
+      for (int step = 0; step < nsteps; ++step) {
+        eager_step(c, cfg, dt);
+        if (++c->steps_in_order >= kReorderSteps) permute_state(c);
+      }
+    }
     GTH_CHECK(hipStreamSynchronize(c->stream));
     c->state_forces = true;
   });

@@ -62,3 +62,22 @@ def test_systems_set_after_priming(hip, oracle):
         m = mass[off[s]:off[s + 1]]
         com = (m[:, None] * pos[off[s]:off[s + 1]]).sum(0) / m.sum()
         assert np.allclose(d[s]["centre_of_mass"], com, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_reorders_during_long_runs(hip, oracle, graph):
+    """The state is put back into tree order every 64 steps within a run: 70
+    steps, graph or eager, equal the reference's steps bit for bit."""
+    n = 2000
+    pos, mass = oracle.test_system(0, n, 13)
+    vel = np.random.default_rng(2).normal(size=pos.shape) * 0.3
+    hip.state_set(pos, vel, mass)
+    hip.prime(**CFG)
+    hip.steps(70, 0.005, use_graph=graph, **CFG)
+    got = hip.state_get()
+    want = oracle.kdk_group(pos, vel, mass, 70, 0.005, eps=0.01)
+    for a, b in zip(got, want):
+        assert same_bits(a, b)
+    _, _, order = hip.get_tree()
+    oracle.build(got[0], mass, 8)
+    assert np.array_equal(order, oracle.order())
