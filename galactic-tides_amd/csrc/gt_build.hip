@@ -203,7 +203,9 @@ __device__ inline bool degenerate(double cx, double cy, double cz, double half) 
 // (the descent from its system's root replays all earlier levels), and the
 // first depth <= 21 (word + 1) at which the path's node is degenerate.  Word 0
 // also seeds the sort: keys[i] = path >> key_shift (its first levels only),
-// vals[i] = i.  sid may be null (one system).
+// vals[i] = i; then only those levels are computed (the word's others are 0,
+// and a degenerate depth past them is not looked for: the tree is cut there).
+// sid may be null (one system).
 __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict__ pos, uint32_t n,
                                                        const double *__restrict__ roots,
                                                        const uint32_t *__restrict__ sid, int word,
@@ -219,7 +221,9 @@ __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict_
   double cx = root[0], cy = root[1], cz = root[2], h = root[3];
   uint64_t key = 0;
   uint8_t first_dg = kNoDegenerate;
-  const int first = kLevels * word, last = first + kLevels;
+  // (with key_shift > 0 only the levels the sort orders by: the first
+  // kLevels - key_shift / 3)
+  const int first = kLevels * word, last = first + kLevels - key_shift / 3;
   for (int l = 0; l < last; ++l) {
     if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = static_cast<uint8_t>(l);
     const uint32_t o = (x >= cx ? 1u : 0u) | (y >= cy ? 2u : 0u) | (z >= cz ? 4u : 0u);
@@ -231,10 +235,10 @@ __global__ __launch_bounds__(kBlock) void octant_paths(const double *__restrict_
     h = h2;
   }
   if (first_dg == kNoDegenerate && degenerate(cx, cy, cz, h)) first_dg = static_cast<uint8_t>(last);
-  wkey[i] = key;
+  wkey[i] = key << key_shift;
   dg[i] = first_dg;
   if (keys) {
-    keys[i] = key >> key_shift;
+    keys[i] = key;
     vals[i] = i;
   }
 }
