@@ -32,12 +32,18 @@ __device__ inline size_t sort_count(size_t n, const unsigned long long *n_dev) {
   return v <= n ? static_cast<size_t>(v) : 0;
 }
 
+// Each tile's digit counts, digit-major.  Workgroup 0 also clears `clear`
+// words (the first pass: the status words the row scans use next).
 __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restrict__ keys,
                                                          size_t cap,
                                                          const unsigned long long *n_dev,
                                                          int shift, uint32_t ntiles,
-                                                         uint32_t *__restrict__ counts) {
+                                                         uint32_t *__restrict__ counts,
+                                                         unsigned long long *__restrict__ clear,
+                                                         uint32_t clear_words) {
   __shared__ uint32_t h[kDigits];
+  if (blockIdx.x == 0)
+    for (uint32_t t = threadIdx.x; t < clear_words; t += kBlock) clear[t] = 0ull;
   const size_t n = sort_count(cap, n_dev);
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -255,8 +261,8 @@ __global__ __launch_bounds__(kBlock) void copy_pairs(const uint64_t *__restrict_
 }
 
 // Scratch (64-bit words): the digit totals' status words, one ticket counter
-// per pass (uint32), an error word, then the digit-major tile histogram of one
-// pass (uint32).
+// per pass (uint32), an error word (these cleared by the first pass's
+// histogram launch), then the digit-major tile histogram of one pass (uint32).
 constexpr int kMaxPasses = 8;  // 64-bit keys
 constexpr size_t kStatusWords = kDigits + kMaxPasses / 2 + 1;
 
@@ -269,13 +275,13 @@ void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *v
   unsigned long long *error = tmp + kDigits + kMaxPasses / 2;
   uint32_t *counts = reinterpret_cast<uint32_t *>(tmp + kStatusWords);
   const int passes = std::min((bits + 7) / 8, kMaxPasses);
-  launch_fill_u64(tmp, kStatusWords, 0ull, s);
   uint64_t *kin = keys, *kout = keys_alt;
   uint32_t *vin = vals, *vout = vals_alt;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     hipLaunchKernelGGL(tile_histogram, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
-                       kin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts);
+                       kin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts, tmp,
+                       p == 0 ? static_cast<uint32_t>(kStatusWords) : 0u);
     GTH_LAUNCH_CHECK();
     hipLaunchKernelGGL(digit_row_scan, dim3(kDigits), dim3(kBlock), 0, s,
                        static_cast<uint32_t>(ntiles), static_cast<uint32_t>(p + 1), tickets + p,
