@@ -84,11 +84,14 @@ def cpu_baseline(pos, mass, cfg, offsets=None):
 
 
 def force_errors(ctx, pos, mass, acc, cfg, offsets=None, size=1000):
-    """|a - a64| / |a64| on a seeded sample (default_rng(1)) of `size` particles,
-    a64 by FP64 direct summation over the particle's own system."""
+    """|a - a64| / |a64| on the reference benchmark's sample of `size` particles
+    (std::shuffle of the indices with std::mt19937_64(1),
+    apps/galactic_tides_bench.cpp:74-78), a64 by FP64 direct summation over the
+    particle's own system (src/gravity.cpp:28-45, on the GPU); median err[n/2]
+    and p99 err[n*99/100] as apps/galactic_tides_bench.cpp:142-149."""
+    import gt_ics
     n = len(mass)
-    rng = np.random.default_rng(1)
-    sample = np.sort(rng.choice(n, size=min(size, n), replace=False)).astype(np.uint64)
+    sample = np.sort(gt_ics.bench_sample(n, size, 1))
     bounds = np.array([0, n], np.uint64) if offsets is None else np.asarray(offsets, np.uint64)
     err = []
     for k in range(len(bounds) - 1):
@@ -285,7 +288,7 @@ def bench_direct(args):
     print(f"# direct {n}: {kms:.2f} ms per evaluation (kernel)", file=sys.stderr, flush=True)
     flops = BODY_FLOPS * pairs
     achieved = flops / (kms * 1e-3) / 1e12
-    sample = np.sort(np.random.default_rng(1).choice(n, 1000, replace=False)).astype(np.uint64)
+    sample = gt_ics.bench_sample(n, 1000, 1)  # apps/galactic_tides_bench.cpp:74-78
     ref = ctx.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
     got = out[sample.astype(np.int64), :3].astype(np.float64)
     err = np.sort(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1))

@@ -1250,6 +1250,7 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
       // sizes not known yet, or the predictor still learning: a step without graphs
       eager_step(c, cfg, dt);
       ++done_total;
+      if (++c->steps_in_order >= kReorderSteps) permute_state(c);
       continue;
     }
     const Plan p = plan_for(c, n, cfg);
@@ -1321,6 +1322,9 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
                c->orig.get());
     launch_kick(c->vel.get(), c->acc.get(), n, 0.5 * dt, frame + frame::kStatus, s);
     ++done_total;
+    // the graph steps before the stop and the one finished here
+    c->steps_in_order += done + 1;
+    if (c->steps_in_order >= kReorderSteps) permute_state(c);
   }
 }
 
@@ -1661,7 +1665,8 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
     uint64_t *keys = c->nb_keys.ensure(nq);
     launch_query_keys(c->tree.geom(), frame, dc, nq32, keys, perm, s);
     radix_sort_u64(keys, perm, c->nb_keys_alt.ensure(nq), c->nb_perm_alt.ensure(nq), nq,
-                   kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s);
+                   kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s,
+                   frame + frame::kSortCtl);
     void *recs = c->nb_nodes.ensure((nb_record_bytes(p.node_cap) + 15) / 16);
     launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(), frame,
                       p.node_cap, recs, s);
@@ -1779,61 +1784,89 @@ int gt_hip_group_compute_batch(gt_hip_context *c, const gt_group_config *cfg, ui
   });
 }
 
+namespace {
+
+// The autotuner over one system (offsets NULL) or a batch of independent
+// systems (n_systems + 1 offsets): times real evaluations of the caller's
+// particles while varying one tuning setting at a time.
+void autotune_impl(gt_hip_context *c, const gt_group_config *cfg, uint64_t n_systems,
+                   const uint64_t *offsets, uint64_t n, const double *pos, const double *mass,
+                   int search_group_size, gt_group_config *best, double *best_ms) {
+  check_config(cfg);
+  require(best != nullptr, "best is NULL");
+  if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
+  *best = *cfg;
+  if (best_ms) *best_ms = 0.0;
+  GTH_CHECK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (offsets) upload_systems(c, n, n_systems, offsets, c->batch_off);
+  if (n == 0) {
+    GTH_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  require(pos && mass, "array is NULL");
+  double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+  double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
+  upload(dp, pos, 3 * n, s);
+  upload(dm, mass, n, s);
+  c->systems = offsets && n_systems > 1 ? static_cast<uint32_t>(n_systems) : 1;
+  c->sys_off = offsets && n_systems > 1 ? c->batch_off.get() : nullptr;
+  // One warm-up (bounds settle) then the best of two timed evaluations.
+  auto time_of = [&](const gt_group_config &t) {
+    force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
+    double ms = 1e300;
+    for (int r = 0; r < 2; ++r) {
+      force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
+      ms = std::min(ms, c->tm.total);
+    }
+    return ms;
+  };
+  gt_group_config cur = *cfg;
+  double cur_ms = time_of(cur);
+  auto sweep = [&](int gt_group_config::*field, std::initializer_list<int> values) {
+    const int start = cur.*field;
+    for (int v : values) {
+      if (v == start) continue;
+      gt_group_config t = cur;
+      t.*field = v;
+      const double ms = time_of(t);
+      if (ms < 0.99 * cur_ms) {  // a change must beat run-to-run noise
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  };
+  if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
+  sweep(&gt_group_config::walk_lanes, {2, 4, 16, 32, 64});
+  if (!cfg->exact) {
+    sweep(&gt_group_config::eval_waves, {2, 3});
+    sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
+    sweep(&gt_group_config::split_len, {2048, 8192, -1});
+    sweep(&gt_group_config::persistent, {0, 1});
+  }
+  *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+}
+
+}  // namespace
+
 int gt_hip_autotune(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, const double *pos,
                     const double *mass, int search_group_size, gt_group_config *best,
                     double *best_ms) {
   if (!c) return GT_HIP_ERR_ARGUMENT;
   return guarded(c, [&] {
-    check_config(cfg);
-    require(best != nullptr, "best is NULL");
-    if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
-    *best = *cfg;
-    if (best_ms) *best_ms = 0.0;
-    if (n == 0) return;
-    require(pos && mass, "array is NULL");
-    GTH_CHECK(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
-    double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
-    upload(dp, pos, 3 * n, s);
-    upload(dm, mass, n, s);
-    c->systems = 1;
-    c->sys_off = nullptr;
-    // One warm-up (bounds settle) then the best of two timed evaluations.
-    auto time_of = [&](const gt_group_config &t) {
-      force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
-      double ms = 1e300;
-      for (int r = 0; r < 2; ++r) {
-        force_eval(c, &t, dp, dm, static_cast<uint32_t>(n), da, dq);
-        ms = std::min(ms, c->tm.total);
-      }
-      return ms;
-    };
-    gt_group_config cur = *cfg;
-    double cur_ms = time_of(cur);
-    auto sweep = [&](int gt_group_config::*field, std::initializer_list<int> values) {
-      const int start = cur.*field;
-      for (int v : values) {
-        if (v == start) continue;
-        gt_group_config t = cur;
-        t.*field = v;
-        const double ms = time_of(t);
-        if (ms < 0.99 * cur_ms) {  // a change must beat run-to-run noise
-          cur = t;
-          cur_ms = ms;
-        }
-      }
-    };
-    if (search_group_size) sweep(&gt_group_config::group_size, {32, 64});
-    sweep(&gt_group_config::walk_lanes, {2, 4, 16, 32, 64});
-    if (!cfg->exact) {
-      sweep(&gt_group_config::eval_waves, {2, 3});
-      sweep(&gt_group_config::eval_order, {1, 2, 3, 4});
-      sweep(&gt_group_config::split_len, {2048, 8192, -1});
-      sweep(&gt_group_config::persistent, {0, 1});
-    }
-    *best = cur;
-    if (best_ms) *best_ms = cur_ms;
+    autotune_impl(c, cfg, 1, nullptr, n, pos, mass, search_group_size, best, best_ms);
+  });
+}
+
+int gt_hip_autotune_batch(gt_hip_context *c, const gt_group_config *cfg, uint64_t n_systems,
+                          const uint64_t *offsets, const double *pos, const double *mass,
+                          int search_group_size, gt_group_config *best, double *best_ms) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    require(offsets != nullptr && n_systems >= 1, "offsets is NULL or n_systems is 0");
+    autotune_impl(c, cfg, n_systems, offsets, offsets[n_systems], pos, mass, search_group_size,
+                  best, best_ms);
   });
 }
 
@@ -2221,9 +2254,20 @@ int gt_hip_debug_shrink_slots(gt_hip_context *c, uint32_t shrink) {
   return GT_HIP_OK;
 }
 
+int gt_hip_debug_sort_waits(gt_hip_context *c, uint32_t polls) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    GTH_CHECK(hipSetDevice(c->device));
+    const unsigned long long v = polls;
+    upload(c->frame.get() + frame::kSortCtl + 1, &v, 1, c->stream);
+    GTH_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
 int gt_hip_pipeline_stats(const gt_hip_context *c, gt_pipeline_stats *st) {
   if (!c || !st) return GT_HIP_ERR_ARGUMENT;
   *st = c->pstats;
+  st->sort_recounts = c->fw[frame::kSortCtl];
   return GT_HIP_OK;
 }
 

@@ -917,7 +917,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
       GTH_LAUNCH_CHECK();
     }
     radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n,
-                   p.words == 1 ? 3 * levels : 3 * kLevels, sort_tmp_.get(), s);
+                   p.words == 1 ? 3 * levels : 3 * kLevels, sort_tmp_.get(), s,
+                   frame + frame::kSortCtl);
   }
   if (batch) {  // systems apart, most significant: one stable pass more
     int bits = 1;
@@ -925,7 +926,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
     hipLaunchKernelGGL(gather_sid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, sid, vals, n,
                        keys);
     GTH_LAUNCH_CHECK();
-    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, bits, sort_tmp_.get(), s);
+    radix_sort_u64(keys, vals, keys_alt_.get(), vals_alt_.get(), n, bits, sort_tmp_.get(), s,
+                   frame + frame::kSortCtl);
   }
 
   // 4. leaf depths and node counts per sorted position; the node total goes
@@ -950,7 +952,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                      frame, cap, ndep, nbeg);
   GTH_LAUNCH_CHECK();
   radix_sort_u64_dev(ndep, nbeg, ndep_alt_.get(), nbeg_alt_.get(), frame + frame::kNodes, cap, 6,
-                     sort_tmp_.get(), s);
+                     sort_tmp_.get(), s, frame + frame::kSortCtl);
   uint32_t *lstart = lstart_.get();
   launch_fill_u32(lstart, kDepthLimit + 2, 0u, s);
   hipLaunchKernelGGL(level_starts, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, frame, cap,

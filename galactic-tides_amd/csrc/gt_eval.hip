@@ -705,7 +705,7 @@ void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32
     GTH_LAUNCH_CHECK();
     radix_sort_u64_dev(split.keys, split.order, split.keys_alt, split.order_alt,
                        frame + frame::kItems, split.item_cap, split.cost_buckets > 0 ? 11 : 27,
-                       split.sort_tmp, s);
+                       split.sort_tmp, s, frame + frame::kSortCtl);
   }
 }
 

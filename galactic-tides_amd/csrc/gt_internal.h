@@ -154,6 +154,7 @@ constexpr int kPredBodies = 18; // sum of the predicted body-list slots
 constexpr int kPredError = 19;  // 1e6 x sum over groups of |log(1 + length) - prediction|
 constexpr int kExactTests = 20; // opening tests the FP32 filter left to the FP64 test
 constexpr int kCountsEnd = 21;  // [kNodes, kCountsEnd): cleared before each evaluation
+constexpr int kSortCtl = 22;    // 2 words: the radix sorts' ctl (recount count, test setting)
 constexpr int kStepsDone = 30;  // leapfrog steps completed since the counter was cleared
 constexpr int kStepInteractions = 31;  // their cell + body interactions
 constexpr int kLevelNodes = 32;  // kLevels words: nodes on each tree level 0 .. kLevels - 1
@@ -223,14 +224,19 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned lo
 
 // gt_sort.hip: stable LSD radix sort of (key, value) pairs on key bits [0, bits).
 // The sorted pairs end in (keys, vals); keys_alt/vals_alt are scratch of n
-// entries; tmp must hold radix_tmp_words(n) 64-bit words.
+// entries; tmp must hold radix_tmp_words(n) 64-bit words.  ctl (may be null):
+// two device words, [0] counts digit totals recounted because a bounded wait
+// gave up (the result is right either way), [1] a test setting (0 normally;
+// k > 0: every such wait gives up after k polls).
 size_t radix_tmp_words(size_t n);
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                    size_t n, int bits, unsigned long long *tmp, hipStream_t s);
+                    size_t n, int bits, unsigned long long *tmp, hipStream_t s,
+                    unsigned long long *ctl = nullptr);
 // The same for the first bounded_count(n_dev, cap) pairs of arrays of cap entries.
 void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                         const unsigned long long *n_dev, size_t cap, int bits,
-                        unsigned long long *tmp, hipStream_t s);
+                        unsigned long long *tmp, hipStream_t s,
+                        unsigned long long *ctl = nullptr);
 
 // gt_eval.hip
 // Long lists evaluated in parts (fast evaluation only): per group its part

@@ -57,30 +57,37 @@ __global__ __launch_bounds__(kBlock) void tile_histogram(const uint64_t *__restr
   counts[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Workgroup d (by ticket): counts[d * ntiles + t] becomes the output
+// Workgroup d (by ticket): offsets[d * ntiles + t] becomes the output
 // position of digit d's first key of tile t, i.e. the keys of smaller digits
-// plus digit d's keys in tiles before t.  Each workgroup publishes its digit's
-// total in status[d] (epoch << 32 | total; status starts cleared and each
-// pass has its own epoch) and reads the totals of the smaller digits from
-// there.  Digits are handed out by ticket in the order workgroups start, so a
-// workgroup waits only for ones already running, which publish before they
-// wait.  A wait that outlasts kSpinLimit re-reads (it should never) sets
-// *error and counts that total as 0.
+// plus digit d's keys in tiles before t (counts, the tile histogram, is only
+// read).  Each workgroup publishes its digit's total in status[d]
+// (epoch << 32 | total; status starts cleared and each pass has its own epoch)
+// and reads the totals of the smaller digits from there.  Digits are handed
+// out by ticket in the order workgroups start, so a workgroup waits only for
+// ones already running, which publish before they wait.  A wait is bounded:
+// after kSpinLimit polls (it should never come to that) the waiting thread
+// adds up that digit's row of the histogram itself, so the result is right
+// either way, and counts the event in ctl[0].  ctl (may be null) is two words
+// kept by the caller: [0] a count of such recounts, [1] a test setting, 0
+// normally; k > 0 makes every wait give up after k polls (it waits for an
+// epoch no workgroup publishes), so the recount runs for every smaller digit.
 constexpr int kScanItems = 4;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 __global__ __launch_bounds__(kBlock) void digit_row_scan(uint32_t ntiles, uint32_t epoch,
                                                          uint32_t *__restrict__ ticket,
                                                          unsigned long long *status,
-                                                         unsigned long long *error,
-                                                         uint32_t *__restrict__ counts) {
+                                                         unsigned long long *ctl,
+                                                         const uint32_t *__restrict__ counts,
+                                                         uint32_t *__restrict__ offsets) {
   __shared__ uint32_t part[kWaves];
   __shared__ uint32_t digit;
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) digit = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t d = digit;
-  uint32_t *row = counts + static_cast<size_t>(d) * ntiles;
+  const uint32_t *row = counts + static_cast<size_t>(d) * ntiles;
+  uint32_t *out_row = offsets + static_cast<size_t>(d) * ntiles;
   // the digit's total, published
   uint32_t v = 0;
   for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) v += row[t];
@@ -97,16 +104,22 @@ __global__ __launch_bounds__(kBlock) void digit_row_scan(uint32_t ntiles, uint32
   // keys of smaller digits: thread j reads digit j's total
   uint32_t below = 0;
   if (threadIdx.x < d) {
+    const unsigned long long test = ctl ? ctl[1] : 0ull;
+    const uint32_t limit = test ? static_cast<uint32_t>(min(test, 1ull << 20)) : kSpinLimit;
+    const uint32_t want = test ? (epoch ^ 0x80000000u) : epoch;
     unsigned long long w =
         __hip_atomic_load(status + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t spin = 0; (w >> 32) != epoch && spin < kSpinLimit; ++spin) {
+    for (uint32_t spin = 0; (w >> 32) != want && spin < limit; ++spin) {
       __builtin_amdgcn_s_sleep(1);
       w = __hip_atomic_load(status + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if ((w >> 32) == epoch)
+    if ((w >> 32) == want) {
       below = static_cast<uint32_t>(w);
-    else
-      atomicOr(error, 1ull);
+    } else {  // the total, from the histogram row itself
+      const uint32_t *r = counts + static_cast<size_t>(threadIdx.x) * ntiles;
+      for (uint32_t t = 0; t < ntiles; ++t) below += r[t];
+      if (ctl) atomicAdd(ctl, 1ull);
+    }
   }
   for (int off = kWave / 2; off > 0; off >>= 1) below += __shfl_xor(below, off, kWave);
   if (lane == 0) part[wave] = below;
@@ -139,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void digit_row_scan(uint32_t ntiles, uint32
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
       const uint32_t t = t0 + threadIdx.x * kScanItems + k;
-      if (t < ntiles) row[t] = run;
+      if (t < ntiles) out_row[t] = run;
       run += c[k];
     }
     carry += total;
@@ -260,20 +273,23 @@ __global__ __launch_bounds__(kBlock) void copy_pairs(const uint64_t *__restrict_
   }
 }
 
-// Scratch (64-bit words): the digit totals' status words, one ticket counter
-// per pass (uint32), an error word (these cleared by the first pass's
-// histogram launch), then the digit-major tile histogram of one pass (uint32).
+// Scratch (64-bit words): the digit totals' status words and one ticket
+// counter per pass (uint32), cleared by the first pass's histogram launch;
+// then the digit-major tile histogram of one pass (uint32) and the output
+// positions computed from it (uint32, the same size).
 constexpr int kMaxPasses = 8;  // 64-bit keys
-constexpr size_t kStatusWords = kDigits + kMaxPasses / 2 + 1;
+constexpr size_t kStatusWords = kDigits + kMaxPasses / 2;
+
+size_t histogram_words(size_t ntiles) { return (static_cast<size_t>(kDigits) * ntiles + 1) / 2; }
 
 void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                  size_t n, const unsigned long long *n_dev, int bits, unsigned long long *tmp,
-                 hipStream_t s) {
+                 unsigned long long *ctl, hipStream_t s) {
   const size_t ntiles = (n + kTile - 1) / kTile;
   unsigned long long *status = tmp;
   uint32_t *tickets = reinterpret_cast<uint32_t *>(tmp + kDigits);
-  unsigned long long *error = tmp + kDigits + kMaxPasses / 2;
   uint32_t *counts = reinterpret_cast<uint32_t *>(tmp + kStatusWords);
+  uint32_t *offsets = reinterpret_cast<uint32_t *>(tmp + kStatusWords + histogram_words(ntiles));
   const int passes = std::min((bits + 7) / 8, kMaxPasses);
   uint64_t *kin = keys, *kout = keys_alt;
   uint32_t *vin = vals, *vout = vals_alt;
@@ -285,10 +301,10 @@ void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *v
     GTH_LAUNCH_CHECK();
     hipLaunchKernelGGL(digit_row_scan, dim3(kDigits), dim3(kBlock), 0, s,
                        static_cast<uint32_t>(ntiles), static_cast<uint32_t>(p + 1), tickets + p,
-                       status, error, counts);
+                       status, ctl, counts, offsets);
     GTH_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scatter, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, kin,
-                       vin, n, n_dev, shift, static_cast<uint32_t>(ntiles), counts, kout, vout);
+                       vin, n, n_dev, shift, static_cast<uint32_t>(ntiles), offsets, kout, vout);
     GTH_LAUNCH_CHECK();
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -304,20 +320,21 @@ void sort_passes(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *v
 
 size_t radix_tmp_words(size_t n) {
   const size_t ntiles = (n + kTile - 1) / kTile;
-  return kStatusWords + (static_cast<size_t>(kDigits) * ntiles + 1) / 2;
+  return kStatusWords + 2 * histogram_words(ntiles);
 }
 
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                    size_t n, int bits, unsigned long long *tmp, hipStream_t s) {
+                    size_t n, int bits, unsigned long long *tmp, hipStream_t s,
+                    unsigned long long *ctl) {
   if (n <= 1 || bits <= 0) return;
-  sort_passes(keys, vals, keys_alt, vals_alt, n, nullptr, bits, tmp, s);
+  sort_passes(keys, vals, keys_alt, vals_alt, n, nullptr, bits, tmp, ctl, s);
 }
 
 void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                         const unsigned long long *n_dev, size_t cap, int bits,
-                        unsigned long long *tmp, hipStream_t s) {
+                        unsigned long long *tmp, hipStream_t s, unsigned long long *ctl) {
   if (cap <= 1 || bits <= 0) return;
-  sort_passes(keys, vals, keys_alt, vals_alt, cap, n_dev, bits, tmp, s);
+  sort_passes(keys, vals, keys_alt, vals_alt, cap, n_dev, bits, tmp, ctl, s);
 }
 
 }  // namespace gth

@@ -56,7 +56,8 @@ class PredictorStats(C.Structure):
 class PipelineStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("evaluations", "reruns", "list_rewrites",
                                            "graph_captures", "graph_steps", "graph_stops",
-                                           "interactions", "last_exact_tests")]
+                                           "interactions", "last_exact_tests",
+                                           "sort_recounts")]
 
 
 _vp = C.c_void_p
@@ -99,6 +100,9 @@ _group_compute_batch = _sig("gt_hip_group_compute_batch", C.c_int,
 _state_set_systems = _sig("gt_hip_state_set_systems", C.c_int, [_vp, C.c_uint64, _vp])
 _autotune = _sig("gt_hip_autotune", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, C.c_int,
                                               _P(GroupConfig), _P(C.c_double)])
+_autotune_batch = _sig("gt_hip_autotune_batch", C.c_int,
+                       [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, C.c_int,
+                        _P(GroupConfig), _P(C.c_double)])
 _walk_tree = _sig("gt_hip_walk_tree", C.c_int, [_vp, _P(GroupConfig), C.c_uint64, C.c_uint64,
                                                 _vp, _vp, _vp, _vp, _vp])
 _last_info = _sig("gt_hip_last_info", C.c_int, [_vp, _P(ListsInfo)])
@@ -113,6 +117,7 @@ _pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)
 _predictor_reset = _sig("gt_hip_predictor_reset", C.c_int, [_vp, C.c_int])
 _predictor_stats = _sig("gt_hip_predictor_stats", C.c_int, [_vp, _P(PredictorStats)])
 _debug_shrink_slots = _sig("gt_hip_debug_shrink_slots", C.c_int, [_vp, C.c_uint32])
+_debug_sort_waits = _sig("gt_hip_debug_sort_waits", C.c_int, [_vp, C.c_uint32])
 _direct_f64_sample = _sig("gt_hip_direct_f64_sample", C.c_int,
                           [_vp, C.c_uint64, _vp, _vp, C.c_double, C.c_double, C.c_uint64, _vp,
                            _vp])
@@ -310,6 +315,22 @@ class Context:
         keys = ("group_size", "eval_waves", "walk_lanes", "split_len", "eval_order", "persistent")
         return {k: int(getattr(best, k)) for k in keys}, ms.value
 
+    def autotune_batch(self, offsets, pos, mass, search_group_size=False, **cfg):
+        """Fastest tuning settings for a batch of independent systems (offsets as
+        for group_compute_batch): (settings dict, ms per batched evaluation)."""
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        n = int(offsets[-1]) if len(offsets) else 0
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        c = make_config(**cfg)
+        best = GroupConfig()
+        ms = C.c_double()
+        self._check(_autotune_batch(self._h, C.byref(c), len(offsets) - 1, _ptr(offsets),
+                                    _ptr(pos), _ptr(mass), int(search_group_size),
+                                    C.byref(best), C.byref(ms)))
+        keys = ("group_size", "eval_waves", "walk_lanes", "split_len", "eval_order", "persistent")
+        return {k: int(getattr(best, k)) for k in keys}, ms.value
+
     def walk_tree(self, geom, links, tree_pos, tree_mass, evaluate=True, **cfg):
         n = len(tree_mass)
         geom = _f64(geom)
@@ -349,6 +370,10 @@ class Context:
     def debug_shrink_slots(self, shrink):
         """Test hook: list slots `shrink` entries too short (0: normal)."""
         self._check(_debug_shrink_slots(self._h, shrink))
+
+    def debug_sort_waits(self, polls):
+        """Test hook: radix-sort waits give up after `polls` polls (0: normal)."""
+        self._check(_debug_sort_waits(self._h, polls))
 
     def get_tree(self):
         inf = self.info()

@@ -27,6 +27,8 @@ _lib.gt_ics_galaxy.restype = C.c_uint64
 _lib.gt_ics_galaxy.argtypes = [C.POINTER(Model), C.c_double, C.c_double, _vp, _vp, _vp, _vp]
 _lib.gt_ics_encounter.restype = C.c_uint64
 _lib.gt_ics_encounter.argtypes = [C.POINTER(Model)] + [C.c_double] * 6 + [_vp] * 4
+_lib.gt_ics_bench_sample.restype = C.c_uint64
+_lib.gt_ics_bench_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _vp]
 
 
 def model(**kw):
@@ -67,6 +69,16 @@ def encounter(m, g1_incl=0.0, g1_omega=0.0, g2_incl=60.0, g2_omega=30.0, rp=2.0,
     if got != n:
         raise ValueError("encounter: start separation below pericentre")
     return pos, vel, mass, typ
+
+
+def bench_sample(n, size=1000, seed=1):
+    """The reference benchmark's force-error sample
+    (apps/galactic_tides_bench.cpp:74-78): std::shuffle of 0 .. n-1 with
+    std::mt19937_64(seed), the first `size` indices (in shuffled order)."""
+    out = np.empty(min(n, size), np.uint64)
+    got = _lib.gt_ics_bench_sample(n, seed, size, _p(out))
+    assert got == len(out)
+    return out
 
 
 def batch_of_galaxies(count=128, n_min=2000, n_max=100000, seed=5, halo="hernquist"):

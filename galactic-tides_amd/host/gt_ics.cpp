@@ -303,4 +303,16 @@ uint64_t gt_ics_encounter(const gt_ics_model *m, double g1_incl, double g1_omega
   return emit(P, pos, vel, mass, type);
 }
 
+uint64_t gt_ics_bench_sample(uint64_t n, uint64_t seed, uint64_t nsample, uint64_t *out) {
+  // apps/galactic_tides_bench.cpp:74-78: the indices 0 .. n-1 shuffled by
+  // std::shuffle with a std::mt19937_64 seeded with `seed`, cut to nsample.
+  std::mt19937_64 rng(seed);
+  std::vector<uint64_t> idx(n);
+  for (uint64_t i = 0; i < n; ++i) idx[i] = i;
+  std::shuffle(idx.begin(), idx.end(), rng);
+  const uint64_t m = std::min(n, nsample);
+  if (out) std::copy(idx.begin(), idx.begin() + static_cast<std::ptrdiff_t>(m), out);
+  return m;
+}
+
 }  // extern "C"

@@ -35,6 +35,11 @@ uint64_t gt_ics_galaxy(const gt_ics_model *m, double incl, double omega, double 
 uint64_t gt_ics_encounter(const gt_ics_model *m, double g1_incl, double g1_omega,
                           double g2_incl, double g2_omega, double rp, double r0, double *pos,
                           double *vel, double *mass, uint8_t *type);
+/* The force-error sample of the reference's benchmark
+ * (apps/galactic_tides_bench.cpp:74-78): the indices 0 .. n-1 put through
+ * std::shuffle with std::mt19937_64(seed), the first min(n, nsample) of them
+ * written to out (when not NULL); returns that count.  The benchmark uses seed 1. */
+uint64_t gt_ics_bench_sample(uint64_t n, uint64_t seed, uint64_t nsample, uint64_t *out);
 
 #ifdef __cplusplus
 }

@@ -217,6 +217,12 @@ int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
 int gt_hip_autotune(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                     const double *pos, const double *mass, int search_group_size,
                     gt_group_config *best, double *best_ms);
+/* The same autotuner (no counterpart in the reference) timing batched
+ * evaluations of independent systems (offsets as for gt_hip_group_compute_batch):
+ * the settings that evaluate the whole batch fastest. */
+int gt_hip_autotune_batch(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n_systems,
+                          const uint64_t *offsets, const double *pos, const double *mass,
+                          int search_group_size, gt_group_config *best, double *best_ms);
 /* Walk and evaluate on a tree supplied by the caller (the reference's node
  * layout: geom n_nodes x 16 doubles = center xyz, half, com xyz, mass,
  * quad xx yy zz xy xz yz, delta, 0; links n_nodes x 4 int32 = first_child,
@@ -302,6 +308,9 @@ typedef struct {
   uint64_t interactions;   /* cell + body interactions of all completed evaluations */
   uint64_t last_exact_tests; /* opening tests of the last evaluation's walk that the FP32
                                 filter left to the FP64 test */
+  uint64_t sort_recounts;  /* radix-sort digit totals added up again by a workgroup whose
+                              bounded wait for another's published total gave up (results
+                              are right either way; as of the last evaluation's end) */
 } gt_pipeline_stats;
 int gt_hip_pipeline_stats(const gt_hip_context *ctx, gt_pipeline_stats *stats);
 /* The learned list-length predictor (no counterpart in the reference): with
@@ -331,6 +340,12 @@ int gt_hip_predictor_stats(const gt_hip_context *ctx, gt_predictor_stats *stats)
  * graph-stop recovery run on every evaluation; results must not change.
  * 0 (the default) restores the normal bounds. */
 int gt_hip_debug_shrink_slots(gt_hip_context *ctx, uint32_t shrink);
+/* Test hook (no counterpart in the reference): with polls > 0, every wait of
+ * the radix sorts' digit scans for another workgroup's published total gives
+ * up after `polls` polls and the total is added up from the histogram instead
+ * (counted in gt_pipeline_stats.sort_recounts); results must not change.
+ * 0 (the default) restores the normal bounded wait. */
+int gt_hip_debug_sort_waits(gt_hip_context *ctx, uint32_t polls);
 
 /* ---- measurement helpers ---- */
 

@@ -54,6 +54,7 @@ _REF_ONLY = {
     "galaxy": (_u64t, [_dp, _d, _d, _dp, _dp, _dp, _u8]),
     "encounter": (_u64t, [_dp, _d, _d, _d, _d, _d, _d, _dp, _dp, _dp, _u8]),
     "time_tree": (_i, [_u64t, _dp, _dp, _d, _d, _d, _i, _i, _i, _dp]),
+    "bench_sample": (_u64t, [_u64t, _u64t, _u64t, _u64]),
 }
 
 INFO_KEYS = ("n", "n_nodes", "depth", "n_groups", "n_cell_list", "n_body_list",
@@ -235,6 +236,13 @@ class Checker:
 
     def max_threads(self):
         return int(self._max_threads())
+
+    def bench_sample(self, n, size=1000, seed=1):
+        """(reference build only) The benchmark's force-error sample,
+        apps/galactic_tides_bench.cpp:74-78."""
+        out = np.zeros(min(n, size), np.uint64)
+        self._bench_sample(n, seed, size, out)
+        return out
 
     # -- inputs -------------------------------------------------------------------
     def test_system(self, kind, n, seed):

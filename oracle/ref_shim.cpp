@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <vector>
 
 #ifdef _OPENMP
@@ -362,6 +363,19 @@ uint64_t gtref_within(const gtref_run *r, uint64_t nq, const double *centres,
       }
   }
   return total;
+}
+
+// The force-error sample of apps/galactic_tides_bench.cpp:74-78 (that file is a
+// main(), not part of the library): 0 .. n-1 shuffled by std::shuffle with
+// std::mt19937_64(seed), cut to nsample.
+uint64_t gtref_bench_sample(uint64_t n, uint64_t seed, uint64_t nsample, uint64_t *out) {
+  std::mt19937_64 rng(seed);
+  std::vector<std::size_t> sample(n);
+  for (std::size_t i = 0; i < sample.size(); ++i) sample[i] = i;
+  std::shuffle(sample.begin(), sample.end(), rng);
+  sample.resize(std::min<std::size_t>(sample.size(), static_cast<std::size_t>(nsample)));
+  for (std::size_t i = 0; i < sample.size(); ++i) out[i] = sample[i];
+  return sample.size();
 }
 
 int gtref_max_threads(void) {

@@ -12,6 +12,12 @@ Outputs:
   tiny.npz        full inputs and outputs for three small systems
   configs.json    SHA-256 digests and statistics for the 50k and 200k galaxies,
                   a small encounter, and 10 leapfrog steps on the 50k galaxy
+  encounter2m.npz the 2M-particle encounter (seeds 3 and 4): the benchmark's
+                  force-error sample (apps/galactic_tides_bench.cpp:74-78) and FP64
+                  direct-summation accelerations at it (src/gravity.cpp:28-45), for
+                  the reference's halo and for the configurations' Hernquist halo;
+                  its summary (counts, digests of trees, lists and FP32 forces) goes
+                  into configs.json under "encounter2m"
   neighbours.npz  Octree::for_each_within results (CSR: offsets, original
                   indices, squared distances, in callback order) on the 3000-particle
                   system of test_gpu_kernels.cpp, leaf sizes 1 and 8
@@ -113,10 +119,69 @@ def write_neighbours(ref):
     print("wrote neighbours.npz")
 
 
+def sample_errors(acc, ref_acc):
+    """Median and p99 of |a - a64| / |a64| (apps/galactic_tides_bench.cpp:142-149)."""
+    e = np.sort(np.linalg.norm(acc - ref_acc, axis=1) / np.linalg.norm(ref_acc, axis=1))
+    return float(e[len(e) // 2]), float(e[len(e) * 99 // 100])
+
+
+def write_encounter2m(ref):
+    """Configuration 3 at full size.  Reference halo: everything from the
+    reference's own code (ICs, tree, lists, FP32 evaluation, FP64 direct).
+    Hernquist halo (the configurations' halo, which the reference's ICs do not
+    offer): positions from this repository's generator, FP64 direct from the
+    reference's code."""
+    sys.path.insert(0, os.path.join(ROOT, "galactic-tides_amd"))
+    import gt_ics
+    pos, vel, mass, typ = ref_encounter(ref, 400000, 100000, 500000, 3)
+    n = len(mass)
+    c = summary(ref, pos, mass)
+    c["model"] = dict(disk_n=400000, bulge_n=100000, halo_n=500000, seed=3)
+    c["velocities"] = digest(vel)
+    sample = ref.bench_sample(n, 1000, 1)
+    a64 = ref.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
+    # the reference's own FP32 grouped evaluation (tree order) at the sample
+    out = ref.eval(1.0, 0.025, True)
+    order = ref.order().astype(np.int64)
+    acc = np.empty((n, 3))
+    acc[order] = out[:, :3].astype(np.float64)
+    c["sample_median"], c["sample_p99"] = sample_errors(acc[sample.astype(np.int64)], a64)
+    print("encounter2m", c["n_nodes"], c["n_groups"], c["n_cell_list"], c["n_body_list"],
+          "p50/p99", c["sample_median"], c["sample_p99"], flush=True)
+    m = gt_ics.model(disk_n=400000, bulge_n=100000, halo_n=500000, bulge_mass=0.333,
+                     bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
+                     halo_profile="hernquist", halo_a=4.0, seed=3)
+    hpos, _, hmass, _ = gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
+    h64 = ref.direct_f64_sample(hpos, hmass, sample, 1.0, 0.025)
+    hs = summary(ref, hpos, hmass)
+    hout = ref.eval(1.0, 0.025, True)
+    horder = ref.order().astype(np.int64)
+    hacc = np.empty((n, 3))
+    hacc[horder] = hout[:, :3].astype(np.float64)
+    hmed, hp99 = sample_errors(hacc[sample.astype(np.int64)], h64)
+    c["hernquist"] = {k: hs[k] for k in ("positions", "masses", "n_nodes", "n_groups",
+                                         "cell_interactions", "body_interactions", "groups",
+                                         "body_list", "cell_list_canonical", "forces_fp32")}
+    c["hernquist"].update(sample_median=hmed, sample_p99=hp99)
+    print("encounter2m hernquist p50/p99", hmed, hp99, flush=True)
+    np.savez_compressed(os.path.join(HERE, "encounter2m.npz"), sample=sample, acc64=a64,
+                        hernquist_acc64=h64)
+    return c
+
+
 def main():
     ref = Checker("ref")
     if sys.argv[1:] == ["--only", "neighbours"]:
         write_neighbours(ref)
+        return
+    if sys.argv[1:] == ["--only", "encounter2m"]:
+        path = os.path.join(HERE, "configs.json")
+        with open(path) as f:
+            configs = json.load(f)
+        configs["encounter2m"] = write_encounter2m(ref)
+        with open(path, "w") as f:
+            json.dump(configs, f, indent=1, sort_keys=True)
+        print("wrote encounter2m.npz and configs.json")
         return
     write_neighbours(ref)
     tiny = {}
@@ -173,6 +238,7 @@ def main():
     pos, vel, mass, typ = ref_galaxy(ref, 20000, 5000, 25000, 1)
     p, v, a, phi = ref.kdk_group(pos, np.zeros_like(pos), mass, 10, 0.01, 1.0, 0.75, 0.025, True,
                                  32, 8, True)
+    configs["encounter2m"] = write_encounter2m(ref)
     configs["kdk50k"] = {"steps": 10, "dt": 0.01, "pos": digest(p), "vel": digest(v),
                          "acc": digest(a), "pot": digest(phi)}
     with open(os.path.join(HERE, "configs.json"), "w") as f:

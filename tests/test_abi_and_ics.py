@@ -48,7 +48,8 @@ def test_compiled_in_and_clean_refusal_without_a_device():
 
 def test_host_library_exports():
     import gt_ics
-    for n in ("gt_ics_default_model", "gt_ics_galaxy", "gt_ics_encounter"):
+    for n in ("gt_ics_default_model", "gt_ics_galaxy", "gt_ics_encounter",
+              "gt_ics_bench_sample"):
         assert hasattr(gt_ics._lib, n)
 
 
@@ -69,6 +70,22 @@ def test_ics_positions_equal_the_reference(name):
     assert _digest(pos) == c["positions"]
     assert _digest(mass) == c["masses"]
     assert np.isfinite(vel).all()
+
+
+def test_bench_sample_is_the_reference_benchmarks():
+    """The force-error sample (apps/galactic_tides_bench.cpp:74-78) at the 2M
+    encounter equals the one the reference's code drew for the golden data."""
+    import gt_ics
+    g = np.load(os.path.join(GOLDEN, "encounter2m.npz"))
+    sample = gt_ics.bench_sample(2_000_000, 1000, 1)
+    assert sample.dtype == np.uint64 and np.array_equal(sample, g["sample"])
+    assert len(np.unique(sample)) == 1000
+
+
+@pytest.mark.parametrize("n,size", [(1, 1000), (10, 3), (50_000, 1000), (200_000, 1000)])
+def test_bench_sample_small_sizes_against_the_reference(ref, n, size):
+    import gt_ics
+    assert np.array_equal(gt_ics.bench_sample(n, size, 1), ref.bench_sample(n, size, 1))
 
 
 def test_ics_component_order_and_centre_of_mass():

@@ -1,8 +1,11 @@
 """Batches of independent systems (gt_hip_group_compute_batch and the
 resident state with gt_hip_state_set_systems) against each system evaluated
-on its own: in exact mode every system's forces must be bit-identical to
-gt_hip_group_compute's for that system alone (which the golden tests pin to
-the reference), and the statistics must add up."""
+on its own and against the oracle: in exact mode every system's forces must
+be bit-identical to the oracle's GroupTreeGravity on that system alone
+(src/gravity_backends.cpp:109-137) and to gt_hip_group_compute's for it; in
+fast mode within the reference's GPU tolerance of the oracle
+(test_metal.cpp:115-116); the statistics must add up; the autotuner sweeps
+the configuration-4 settings on the 128-galaxy batch."""
 import numpy as np
 import pytest
 
@@ -133,3 +136,69 @@ def test_batch_128_galaxies_full_size(hip):
     assert inf["cell_interactions"] == sum(i["cell_interactions"] for i in infos)
     assert np.array_equal(bits(got_a), bits(want_a))
     assert np.array_equal(bits(got_p), bits(want_p))
+
+
+def oracle_each(oracle, offsets, pos, mass, theta, eps, gs, pack):
+    acc, pot = np.zeros_like(pos), np.zeros(len(mass))
+    for s in range(len(offsets) - 1):
+        b, e = int(offsets[s]), int(offsets[s + 1])
+        if b == e:
+            continue
+        acc[b:e], pot[b:e] = oracle.group_gravity(pos[b:e], mass[b:e], 1.0, theta, eps, True,
+                                                  gs, 8, bool(pack))
+    return acc, pot
+
+
+@pytest.mark.parametrize("pack,gs", [(1, 32), (0, 7), (1, 64)])
+def test_batch_exact_equals_the_oracle(hip, oracle, pack, gs):
+    sizes = [5003, 1, 0, 9, 3, 20000, 1200]
+    offsets, pos, mass = systems(oracle, sizes, 31)
+    got_a, got_p = hip.group_compute_batch(offsets, pos, mass, theta=0.7, eps=0.01,
+                                           group_size=gs, pack=pack, exact=True)
+    want_a, want_p = oracle_each(oracle, offsets, pos, mass, 0.7, 0.01, gs, pack)
+    assert np.array_equal(bits(got_a), bits(want_a))
+    assert np.array_equal(bits(got_p), bits(want_p))
+
+
+def test_batch_128_galaxies_against_the_oracle(hip, oracle):
+    """Configuration 4's batch (128 galaxies, 2k-100k particles each): exact
+    batched evaluation equals the oracle galaxy by galaxy bit for bit; the fast
+    evaluation is within the reference's GPU tolerance of it."""
+    import gt_ics
+    offsets, pos, _, mass = gt_ics.batch_of_galaxies()
+    cfg = dict(theta=0.75, eps=0.025, group_size=32, pack=1)
+    want_a, want_p = oracle_each(oracle, offsets, pos, mass, 0.75, 0.025, 32, 1)
+    got_a, got_p = hip.group_compute_batch(offsets, pos, mass, exact=True, **cfg)
+    assert np.array_equal(bits(got_a), bits(want_a))
+    assert np.array_equal(bits(got_p), bits(want_p))
+    got_a, _ = hip.group_compute_batch(offsets, pos, mass, **cfg)
+    err = np.linalg.norm(got_a - want_a, axis=1) / np.linalg.norm(want_a, axis=1)
+    print(f"fast batch vs oracle: median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.median(err) < 1e-5 and err.max() < 1e-3
+
+
+def test_autotune_sweep_on_the_128_galaxy_batch(hip):
+    """The configuration-4 sweep: group size {32, 64}, waves per SIMD, walk
+    lanes, list chunk (split) length, evaluation order and the persistent grid,
+    timed on the whole batch.  The settings it picks give the same forces as
+    the defaults for the same group size and unsplit lists (bit for bit: tuning
+    settings change how, not what, is computed), and within the fast
+    evaluation's tolerance otherwise (parts of split lists are added in a fixed
+    order; group size 64 makes other lists)."""
+    import gt_ics
+    offsets, pos, _, mass = gt_ics.batch_of_galaxies()
+    cfg = dict(theta=0.75, eps=0.025, group_size=32, pack=1)
+    t0 = hip.group_compute_batch(offsets, pos, mass, **cfg)
+    default_ms = hip.timings()["total"]
+    best, ms = hip.autotune_batch(offsets, pos, mass, search_group_size=True, **cfg)
+    print(f"batch of 128: default {default_ms:.2f} ms, tuned {ms:.2f} ms with {best}", flush=True)
+    assert best["group_size"] in (32, 64)
+    assert best["eval_waves"] in (0, 2, 3) and best["persistent"] in (0, 1)
+    assert 0 < ms <= 1.05 * default_ms
+    a, p = hip.group_compute_batch(offsets, pos, mass, **dict(cfg, **best))
+    split = best["split_len"]
+    unsplit = split < 0 or hip.info()["max_list"] <= (split or 8192)
+    if best["group_size"] == 32 and unsplit:
+        assert np.array_equal(bits(a), bits(t0[0])) and np.array_equal(bits(p), bits(t0[1]))
+    err = np.linalg.norm(a - t0[0], axis=1) / np.linalg.norm(t0[0], axis=1)
+    assert np.median(err) < (1e-5 if best["group_size"] == 32 else 1e-3)

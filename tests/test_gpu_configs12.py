@@ -4,8 +4,8 @@
     and 10 leapfrog steps at dt 0.01, forces checked against FP64 direct
     summation on all 50,000 particles;
  2. the 200k galaxy (seed 2): grouped evaluation and the FP32 direct kernel,
-    both against FP64 direct on a seeded uniform sample of 1000 particles
-    (numpy default_rng(1)).
+    both against FP64 direct on the reference benchmark's sample of 1000
+    particles (std::shuffle with mt19937_64(1), apps/galactic_tides_bench.cpp:74-78).
 Error bounds: the reference's own accuracy tests (test_metal.cpp:117, median
 below 3e-3 theta^2; test_octree.cpp:66-89) and the grouped solver's measured
 p99 at this opening angle."""
@@ -48,8 +48,8 @@ def test_config1_50k_against_fp64_direct_on_all(hip):
 def test_config2_200k_group_and_direct(hip):
     pos, _, mass = galaxy(80000, 20000, 100000, 2)
     n = len(mass)
-    rng = np.random.default_rng(1)
-    sample = np.sort(rng.choice(n, size=1000, replace=False)).astype(np.uint64)
+    import gt_ics
+    sample = gt_ics.bench_sample(n, 1000, 1)
     ref = hip.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
     acc, _ = hip.group_compute(pos, mass, theta=0.75, eps=0.025, group_size=32, pack=1)
     t_group = hip.timings()["total"]
@@ -66,3 +66,27 @@ def test_config2_200k_group_and_direct(hip):
           f"direct: {t_direct:.1f} ms, median {e_direct[500]:.2e} p99 {e_direct[990]:.2e}")
     assert e_group[500] < 3e-3 * 0.75 ** 2 and e_group[990] < 2e-2
     assert e_direct[500] < 1e-5 and e_direct[990] < 1e-4
+
+
+def test_gpu_fp64_direct_equals_the_oracle(hip, oracle):
+    """The FP64 direct summation behind every force error on the GPU equals the
+    oracle's (pinned bit for bit to the reference's compute_gravity_direct,
+    src/gravity.cpp:28-45) to 1e-12 relative: on all 50,000 particles of
+    configuration 1 and on the benchmark sample of configuration 2's 200k.  The
+    GPU adds each target's terms in a different (fixed) order, so the sums
+    agree to rounding, not bit for bit."""
+    import gt_ics
+    pos, _, mass = galaxy(20000, 5000, 25000, 1)
+    every = np.arange(len(mass), dtype=np.uint64)
+    want, _ = oracle.direct_f64(pos, mass, 1.0, 0.025)
+    got = hip.direct_f64_sample(pos, mass, every, 1.0, 0.025)
+    e = rel_err(got, want)
+    print(f"50k, all particles: max relative difference {e.max():.2e}", flush=True)
+    assert e.max() <= 1e-12
+    pos, _, mass = galaxy(80000, 20000, 100000, 2)
+    sample = gt_ics.bench_sample(len(mass), 1000, 1)
+    want = oracle.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
+    got = hip.direct_f64_sample(pos, mass, sample, 1.0, 0.025)
+    e = rel_err(got, want)
+    print(f"200k, benchmark sample: max relative difference {e.max():.2e}", flush=True)
+    assert e.max() <= 1e-12

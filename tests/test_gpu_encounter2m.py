@@ -1,0 +1,184 @@
+"""Configuration 3 at full size: the parabolic encounter of two 1M-particle
+galaxies (seeds 3 and 4; N = 2M; theta 0.75, group size 32 packed, leaf 8,
+eps 0.025), against golden data written from the reference build
+(tests/golden/make_fixtures.py, configs.json "encounter2m" and encounter2m.npz).
+
+ * reference halo: exact mode reproduces the reference's tree, groups, lists
+   and FP32 forces bit for bit (build_group_lists, src/group_lists.cpp:18-207;
+   evaluate_group_lists_cpu, src/gravity_backends.cpp:70-84), with the
+   reference's counts (842,805 nodes, 87,301 groups, 88.70M + 38.75M entries);
+   the fast mode builds the same lists, and its force error on the reference
+   benchmark's sample (std::shuffle with mt19937_64(1),
+   apps/galactic_tides_bench.cpp:74-78) is the reference algorithm's own;
+ * the GPU's FP64 direct summation, the truth every force error here is
+   measured against, equals the reference's (src/gravity.cpp:28-45) on that
+   sample to 1e-12;
+ * Hernquist halo (the configurations' halo): the same list identity, then 20
+   leapfrog steps from captured graphs with hi/lo positions relative to the
+   group centres: p99 force error at the end below 1e-2, energy, momentum and
+   centre-of-mass drift bounded.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KEYS = ("n_nodes", "depth", "n_groups", "n_cell_list", "n_body_list", "cell_interactions",
+        "body_interactions", "max_list", "max_group")
+CFG = dict(theta=0.75, eps=0.025, group_size=32, leaf_size=8, pack=1)
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLDEN, "configs.json")) as f:
+        c = json.load(f)["encounter2m"]
+    return c, np.load(os.path.join(GOLDEN, "encounter2m.npz"))
+
+
+def encounter(halo):
+    import gt_ics
+    m = gt_ics.model(disk_n=400_000, bulge_n=100_000, halo_n=500_000, bulge_mass=0.333,
+                     bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
+                     halo_profile=halo, halo_a=4.0, seed=3)
+    return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
+
+
+@pytest.fixture(scope="module")
+def ref_halo():
+    pos, vel, mass, _ = encounter("reference")
+    return pos, vel, mass
+
+
+@pytest.fixture(scope="module")
+def hernquist_halo():
+    pos, vel, mass, _ = encounter("hernquist")
+    return pos, vel, mass
+
+
+def list_digests(hip):
+    from pyoracle import canonical_ids
+    _, links, _ = hip.get_tree()
+    canon = canonical_ids(links)
+    L = hip.get_lists()
+    return {"groups": digest(L["groups"]), "body_list": digest(L["body_list"]),
+            "cell_list_canonical": digest(canon[L["cell_list"]].astype(np.int64))}
+
+
+def test_exact_mode_equals_reference(hip, golden, ref_halo):
+    from pyoracle import canonical_ids
+    c, _ = golden
+    pos, _, mass = ref_halo
+    assert digest(pos) == c["positions"] and digest(mass) == c["masses"]
+    acc, pot = hip.group_compute(pos, mass, exact=True, **CFG)
+    inf = hip.info()
+    print({k: inf[k] for k in KEYS}, flush=True)
+    assert {k: inf[k] for k in KEYS} == {k: c[k] for k in KEYS}
+    # the reference's counts at this configuration (measured with its own code)
+    assert (inf["n_nodes"], inf["n_groups"]) == (842_805, 87_301)
+    assert round(inf["n_cell_list"] / 1e6, 2) == 88.70
+    assert round(inf["n_body_list"] / 1e6, 2) == 38.75
+    geom, links, order = hip.get_tree()
+    canon = canonical_ids(links)
+    inv = np.argsort(canon)
+    fc = np.where(links[:, 1] > 0, canon[np.maximum(links[:, 0], 0)], -1)
+    clinks = np.stack([fc, links[:, 1], links[:, 2], links[:, 3]], 1)[inv].astype(np.int64)
+    assert digest(order) == c["order"]
+    assert digest(geom[inv]) == c["tree_geometry"]
+    assert digest(clinks) == c["tree_links"]
+    assert list_digests(hip) == {k: c[k] for k in ("groups", "body_list",
+                                                   "cell_list_canonical")}
+    o = order.astype(np.int64)
+    out = np.empty((len(mass), 4), np.float32)
+    out[:, :3] = acc[o]
+    out[:, 3] = pot[o]
+    assert digest(out) == c["forces_fp32"]
+
+
+def test_fast_mode_lists_identical_and_error_is_the_references(hip, golden, ref_halo):
+    c, g = golden
+    pos, _, mass = ref_halo
+    acc, _ = hip.group_compute(pos, mass, **CFG)
+    inf = hip.info()
+    assert {k: inf[k] for k in KEYS} == {k: c[k] for k in KEYS}
+    assert list_digests(hip) == {k: c[k] for k in ("groups", "body_list",
+                                                   "cell_list_canonical")}
+    sample = g["sample"].astype(np.int64)
+    e = np.sort(rel_err(acc[sample], g["acc64"]))
+    med, p99 = e[len(e) // 2], e[len(e) * 99 // 100]
+    print(f"fast: median {med:.3e} p99 {p99:.3e}; reference FP32: median "
+          f"{c['sample_median']:.3e} p99 {c['sample_p99']:.3e}", flush=True)
+    assert p99 <= 1e-2
+    # the error is the multipole truncation's: the reference's own FP32 lists
+    # evaluation has the same median and p99 to a few percent
+    assert abs(med - c["sample_median"]) <= 0.05 * c["sample_median"]
+    assert abs(p99 - c["sample_p99"]) <= 0.05 * c["sample_p99"]
+
+
+@pytest.mark.parametrize("halo", ["reference", "hernquist"])
+def test_gpu_fp64_direct_equals_references(hip, golden, ref_halo, hernquist_halo, halo):
+    c, g = golden
+    pos, _, mass = ref_halo if halo == "reference" else hernquist_halo
+    want = g["acc64"] if halo == "reference" else g["hernquist_acc64"]
+    got = hip.direct_f64_sample(pos, mass, g["sample"], 1.0, 0.025)
+    err = rel_err(got, want)
+    print(f"{halo}: FP64 direct max relative difference {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_hernquist_encounter_hilo_graph_steps(hip, golden, hernquist_halo):
+    c, g = golden
+    h = c["hernquist"]
+    pos, vel, mass = hernquist_halo
+    assert digest(pos) == h["positions"] and digest(mass) == h["masses"]
+    cfg = dict(CFG, hilo=True)
+    # the lists do not depend on the evaluation's precision mode
+    hip.group_compute(pos, mass, **cfg)
+    inf = hip.info()
+    assert (inf["n_nodes"], inf["n_groups"], inf["cell_interactions"],
+            inf["body_interactions"]) == (h["n_nodes"], h["n_groups"], h["cell_interactions"],
+                                          h["body_interactions"])
+    assert list_digests(hip) == {k: h[k] for k in ("groups", "body_list",
+                                                   "cell_list_canonical")}
+    hip.state_set(pos, vel, mass)
+    hip.prime(**cfg)
+    d0 = hip.state_diagnostics(0.0)[0]
+    st0 = hip.pipeline_stats()
+    nsteps, dt = 20, 0.01
+    hip.steps(nsteps, dt, use_graph=True, **cfg)
+    st1 = hip.pipeline_stats()
+    assert st1["graph_steps"] - st0["graph_steps"] == nsteps
+    d1 = hip.state_diagnostics(nsteps * dt)[0]
+    p, v, a, _ = hip.state_get()
+    sample = g["sample"]
+    a64 = hip.direct_f64_sample(p, mass, sample, 1.0, 0.025)
+    e = np.sort(rel_err(a[sample.astype(np.int64)], a64))
+    med, p99 = e[len(e) // 2], e[len(e) * 99 // 100]
+    e0 = d0["kinetic"] + d0["potential"]
+    e1 = d1["kinetic"] + d1["potential"]
+    mtot = mass.sum()
+    pscale = float(np.sum(mass * np.linalg.norm(vel, axis=1)))  # sum m |v|
+    dp = np.linalg.norm(np.subtract(d1["momentum"], d0["momentum"]))
+    # centre of mass moves with the total momentum: x_cm(t) = x_cm(0) + P t / M
+    drift = np.subtract(d1["centre_of_mass"], d0["centre_of_mass"]) - \
+        np.asarray(d0["momentum"]) * (nsteps * dt) / mtot
+    print(f"after {nsteps} graph steps (hi/lo): median {med:.3e} p99 {p99:.3e}; "
+          f"|dE/E| {abs(e1 - e0) / abs(e0):.2e}, |dP| / sum m|v| {dp / pscale:.2e}, "
+          f"centre-of-mass drift {np.linalg.norm(drift):.2e}", flush=True)
+    assert p99 <= 1e-2
+    # leapfrog with tree forces: energy to the force errors' level; momentum
+    # is not exactly conserved (the forces are not pairwise antisymmetric),
+    # and the centre of mass follows it: both far below the encounter's scales
+    # (separation 30, speeds of order 1)
+    assert abs(e1 - e0) <= 1e-3 * abs(e0)
+    assert dp <= 1e-3 * pscale
+    assert np.linalg.norm(drift) <= 1e-4

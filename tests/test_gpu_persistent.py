@@ -2,7 +2,9 @@
 of their XCD's share from a counter) runs the same per-item code on the same
 items as one wave per item, so forces and leapfrog states must be identical
 bit for bit, for split lists, hi/lo positions, tree order, and steps from
-captured graphs (the counters are reset inside each replay)."""
+captured graphs (the counters are reset inside each replay); and the
+persistent grid's forces are within the reference's GPU tolerance of the
+oracle's evaluate_group_lists_cpu (test_metal.cpp:115-116)."""
 import numpy as np
 import pytest
 
@@ -45,3 +47,14 @@ def test_persistent_graph_steps_equal(hip, oracle):
     after = hip.pipeline_stats()
     assert same(got, want)
     assert after["graph_steps"] - before["graph_steps"] >= 4
+
+
+@pytest.mark.parametrize("extra", [dict(), dict(split_len=256), dict(hilo=1)])
+def test_persistent_against_the_oracle(hip, oracle, extra):
+    pos, mass = oracle.test_system(0, 40000, 17)
+    want, _ = oracle.group_gravity(pos, mass, 1.0, 0.6, 0.01, True, 32, 8, True)
+    got, _ = hip.group_compute(pos, mass, theta=0.6, eps=0.01, group_size=32, persistent=1,
+                               **extra)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    print(f"{extra}: median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.median(err) < 1e-5 and err.max() < 1e-3

@@ -81,3 +81,50 @@ def test_reorders_during_long_runs(hip, oracle, graph):
     _, _, order = hip.get_tree()
     oracle.build(got[0], mass, 8)
     assert np.array_equal(order, oracle.order())
+
+
+def test_batch_reorders_during_long_runs(hip, oracle):
+    """A batch of systems stepped 70 times from graphs (a reorder of the state
+    at 64 steps in the middle of the run) equals each system stepped alone
+    eagerly, and the oracle's steps, bit for bit."""
+    sizes = [2500, 9, 3000]
+    pos, mass, vel = [], [], []
+    for k, n in enumerate(sizes):
+        p, m = oracle.test_system(k % 2, n, 51 + k)
+        pos.append(p + 6.0 * k)
+        mass.append(m)
+        vel.append(np.random.default_rng(k).normal(size=p.shape) * 0.2)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    P, M, V = np.concatenate(pos), np.concatenate(mass), np.concatenate(vel)
+    hip.state_set(P, V, M)
+    hip.state_set_systems(off)
+    hip.prime(**CFG)
+    hip.steps(70, 0.005, use_graph=True, **CFG)
+    got = hip.state_get()
+    for s, n in enumerate(sizes):
+        b, e = int(off[s]), int(off[s + 1])
+        hip.state_set(pos[s], vel[s], mass[s])
+        hip.prime(**CFG)
+        hip.steps(70, 0.005, use_graph=False, **CFG)
+        alone = hip.state_get()
+        want = oracle.kdk_group(pos[s], vel[s], mass[s], 70, 0.005, eps=0.01)
+        for g, a, w in zip(got, alone, want):
+            assert same_bits(g[b:e], a) and same_bits(a, w)
+
+
+def test_fast_graph_steps_equal_eager_over_long_runs(hip, oracle):
+    """The fast evaluation (v_rsq_f32, FMA, the wave evaluator's fixed
+    summation order) stepped 70 times: from graphs (with the reorder at 64
+    steps) bit for bit what launched kernels give."""
+    n = 5000
+    pos, mass = oracle.test_system(1, n, 21)
+    vel = np.random.default_rng(3).normal(size=pos.shape) * 0.3
+    cfg = dict(CFG, exact=False)
+    runs = []
+    for graph in (False, True):
+        hip.state_set(pos, vel, mass)
+        hip.prime(**cfg)
+        hip.steps(70, 0.005, use_graph=graph, **cfg)
+        runs.append(hip.state_get())
+    for a, b in zip(*runs):
+        assert same_bits(a, b)
