@@ -193,6 +193,8 @@ struct gt_hip_context {
   DBuf<uint32_t> nb_over;  // queries whose lists outgrew their slots
   uint32_t nb_stage = 64;  // slot entries per query (grows when many overflow)
   DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation
+  // predictor kernels on caller data (test hooks)
+  DBuf<float> tk_in, tk_stats, tk_params, tk_m, tk_v, tk_grad, tk_loss, tk_out;
 };
 
 namespace {
@@ -832,6 +834,14 @@ void finish(gt_hip_context *c, const Plan &p, const double *geom, bool count = t
   c->info.n_cell_list = fw[frame::kStats + 6];
   c->info.n_body_list = fw[frame::kStats + 7];
   c->pstats.last_exact_tests = fw[frame::kExactTests];
+  // entries the slot layout spanned: predicted slots, uniform slots per
+  // group, or (two passes) the lists themselves
+  if (p.predicted)
+    c->pstats.last_slot_entries = fw[frame::kPredCells] + fw[frame::kPredBodies];
+  else if (p.cap_c && p.cap_b)
+    c->pstats.last_slot_entries = c->info.n_groups * (static_cast<uint64_t>(p.cap_c) + p.cap_b);
+  else
+    c->pstats.last_slot_entries = c->info.n_cell_list + c->info.n_body_list;
   c->lists_n = p.n;
   c->lists_nn = nn;
   c->lists_geom = geom;
@@ -2252,6 +2262,58 @@ int gt_hip_debug_shrink_slots(gt_hip_context *c, uint32_t shrink) {
   if (!c) return GT_HIP_ERR_ARGUMENT;
   c->debug_shrink = shrink;
   return GT_HIP_OK;
+}
+
+int gt_hip_predictor_forward(gt_hip_context *c, uint64_t n, const float *x, const float *params,
+                             float *y) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    require(n <= 0xffffffffull, "too many rows");
+    if (n == 0) return;
+    require(x && params && y, "array is NULL");
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    float *dx = c->tk_in.ensure(kPredFeatures * n), *dp = c->tk_params.ensure(kPredParams);
+    float *dy = c->tk_out.ensure(kPredOutputs * n);
+    upload(dx, x, kPredFeatures * n, s);
+    upload(dp, params, kPredParams, s);
+    launch_mlp_forward(dx, static_cast<uint32_t>(n), dp, dy, s);
+    download(y, dy, kPredOutputs * n, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int gt_hip_predictor_train_step(gt_hip_context *c, uint64_t n, const float *samples,
+                                const float *stats, float *params, float *m, float *v,
+                                uint32_t blocks, uint32_t iter, float lr, float *grad,
+                                float *loss) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    require(n >= 1 && n <= 0xffffffffull, "n must be 1 .. 2^32 - 1");
+    require(blocks >= 1 && blocks <= 4096, "blocks must be 1 .. 4096");
+    require(samples && stats && params && m && v && grad && loss, "array is NULL");
+    GTH_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const size_t cols = kPredFeatures + kPredOutputs;
+    float *ds = c->tk_in.ensure(cols * n), *dst = c->tk_stats.ensure(kPredStats);
+    float *dp = c->tk_params.ensure(kPredParams), *dm = c->tk_m.ensure(kPredParams),
+          *dv = c->tk_v.ensure(kPredParams);
+    float *dg = c->tk_grad.ensure(static_cast<size_t>(blocks) * kPredParams),
+          *dl = c->tk_loss.ensure(blocks);
+    upload(ds, samples, cols * n, s);
+    upload(dst, stats, kPredStats, s);
+    upload(dp, params, kPredParams, s);
+    upload(dm, m, kPredParams, s);
+    upload(dv, v, kPredParams, s);
+    launch_train_iteration(ds, static_cast<uint32_t>(n), dst, dp, dm, dv, dg, dl, blocks, iter,
+                           lr, s);
+    download(params, dp, kPredParams, s);
+    download(m, dm, kPredParams, s);
+    download(v, dv, kPredParams, s);
+    download(grad, dg, static_cast<size_t>(blocks) * kPredParams, s);
+    download(loss, dl, blocks, s);
+    GTH_CHECK(hipStreamSynchronize(s));
+  });
 }
 
 int gt_hip_debug_sort_waits(gt_hip_context *c, uint32_t polls) {

@@ -39,6 +39,9 @@ void launch_collect_samples(const unsigned long long *frame, uint32_t group_cap,
                             uint32_t base, uint32_t capacity, uint32_t count, float *samples,
                             hipStream_t s);
 void launch_sample_stats(const float *samples, uint32_t n, float *stats, hipStream_t s);
+// The network alone: y (n x kPredOutputs, standardised) from normalised x
+// (n x kPredFeatures).
+void launch_mlp_forward(const float *x, uint32_t n, const float *params, float *y, hipStream_t s);
 // One Adam iteration over blocks x 64 samples; grad holds blocks x kPredParams
 // floats, loss blocks floats (sums of squared standardised errors).
 void launch_train_iteration(const float *samples, uint32_t n, const float *stats, float *params,

@@ -401,7 +401,33 @@ __global__ __launch_bounds__(kBlock) void adam_step(const float *__restrict__ gr
   params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + 1e-8f);
 }
 
+// The network on normalised inputs, one row per thread (x: n x kF, y: n x kO,
+// standardised outputs): the inference of predict_slots without the features
+// and slots around it.
+__global__ __launch_bounds__(kBlock) void forward_rows(const float *__restrict__ x, uint32_t n,
+                                                       const float *__restrict__ params,
+                                                       float *__restrict__ y) {
+  __shared__ float P[kPredParams];
+  for (int k = threadIdx.x; k < kPredParams; k += kBlock) P[k] = params[k];
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float xi[kF], yi[kO];
+#pragma unroll
+  for (int k = 0; k < kF; ++k) xi[k] = x[static_cast<size_t>(kF) * i + k];
+  mlp_forward(P, xi, yi);
+#pragma unroll
+  for (int o = 0; o < kO; ++o) y[static_cast<size_t>(kO) * i + o] = yi[o];
+}
+
 }  // namespace
+
+void launch_mlp_forward(const float *x, uint32_t n, const float *params, float *y,
+                        hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(forward_rows, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, n, params, y);
+  GTH_LAUNCH_CHECK();
+}
 
 void launch_group_features(const gt_group *groups, const unsigned long long *frame,
                            uint32_t group_cap, uint32_t node_cap, const double *tpos,

@@ -57,7 +57,7 @@ class PipelineStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("evaluations", "reruns", "list_rewrites",
                                            "graph_captures", "graph_steps", "graph_stops",
                                            "interactions", "last_exact_tests",
-                                           "sort_recounts")]
+                                           "last_slot_entries", "sort_recounts")]
 
 
 _vp = C.c_void_p
@@ -116,6 +116,11 @@ _steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_doubl
 _pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)])
 _predictor_reset = _sig("gt_hip_predictor_reset", C.c_int, [_vp, C.c_int])
 _predictor_stats = _sig("gt_hip_predictor_stats", C.c_int, [_vp, _P(PredictorStats)])
+_predictor_forward = _sig("gt_hip_predictor_forward", C.c_int,
+                          [_vp, C.c_uint64, _vp, _vp, _vp])
+_predictor_train_step = _sig("gt_hip_predictor_train_step", C.c_int,
+                             [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
+                              C.c_float, _vp, _vp])
 _debug_shrink_slots = _sig("gt_hip_debug_shrink_slots", C.c_int, [_vp, C.c_uint32])
 _debug_sort_waits = _sig("gt_hip_debug_sort_waits", C.c_int, [_vp, C.c_uint32])
 _direct_f64_sample = _sig("gt_hip_direct_f64_sample", C.c_int,
@@ -366,6 +371,33 @@ class Context:
         self._check(_predictor_stats(self._h, C.byref(st)))
         return {k: (float(getattr(st, k)) if t is C.c_double else int(getattr(st, k)))
                 for k, t in PredictorStats._fields_}
+
+    PREDICTOR_PARAMS = 4866  # W1 64x8, b1 64, W2 64x64, b2 64, W3 2x64, b3 2
+
+    def predictor_forward(self, x, params):
+        """The predictor network's outputs (n x 2, standardised) at normalised
+        inputs x (n x 8), computed by the library's inference kernel."""
+        x = np.ascontiguousarray(x, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        y = np.empty((len(x), 2), np.float32)
+        self._check(_predictor_forward(self._h, len(x), _ptr(x), _ptr(params), _ptr(y)))
+        return y
+
+    def predictor_train_step(self, samples, stats, params, m, v, blocks, iteration, lr):
+        """One training iteration of the library's kernels: returns (params, m,
+        v after the Adam step, per-workgroup gradients (blocks x 4866),
+        per-workgroup squared-error sums)."""
+        samples = np.ascontiguousarray(samples, np.float32)
+        stats = np.ascontiguousarray(stats, np.float32)
+        params = np.array(params, np.float32)
+        m = np.array(m, np.float32)
+        v = np.array(v, np.float32)
+        grad = np.empty((blocks, self.PREDICTOR_PARAMS), np.float32)
+        loss = np.empty(blocks, np.float32)
+        self._check(_predictor_train_step(self._h, len(samples), _ptr(samples), _ptr(stats),
+                                          _ptr(params), _ptr(m), _ptr(v), blocks, iteration,
+                                          lr, _ptr(grad), _ptr(loss)))
+        return params, m, v, grad, loss
 
     def debug_shrink_slots(self, shrink):
         """Test hook: list slots `shrink` entries too short (0: normal)."""

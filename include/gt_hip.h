@@ -308,6 +308,9 @@ typedef struct {
   uint64_t interactions;   /* cell + body interactions of all completed evaluations */
   uint64_t last_exact_tests; /* opening tests of the last evaluation's walk that the FP32
                                 filter left to the FP64 test */
+  uint64_t last_slot_entries; /* list entries the last evaluation's slots spanned: the
+                                 predicted slots' total, groups x the uniform slots, or
+                                 (two passes, no slots) the lists' own entries */
   uint64_t sort_recounts;  /* radix-sort digit totals added up again by a workgroup whose
                               bounded wait for another's published total gave up (results
                               are right either way; as of the last evaluation's end) */
@@ -332,6 +335,27 @@ typedef struct {
   uint64_t list_entries;     /* ... and used */
 } gt_predictor_stats;
 int gt_hip_predictor_reset(gt_hip_context *ctx, int collect);
+/* The predictor's kernels on caller data (no counterpart in the reference; for
+ * checking them against a CPU restatement).  Parameter vector (params, m, v:
+ * 4866 floats): W1 (64 x 8, row-major), b1 (64), W2 (64 x 64), b2 (64),
+ * W3 (2 x 64), b3 (2).  gt_hip_predictor_forward: y (n x 2, standardised
+ * outputs) of the network at normalised inputs x (n x 8), as the slot
+ * inference computes it.  gt_hip_predictor_train_step: one training iteration
+ * as the library runs it: `blocks` workgroups of 64 samples each take
+ * samples (iter * blocks * 64 + b * 64 + t) * 2654435761 mod n of
+ * samples (n x 10: 8 raw features, log(1 + cells), log(1 + bodies)),
+ * standardised with stats (means of the 10 columns, then their deviations);
+ * each writes the gradient of its share of the loss (the mean over the
+ * minibatch and both outputs of the squared error) to grad[b] (blocks x 4866)
+ * and the sum of its squared errors to loss[b]; the gradients are added in
+ * workgroup order and one Adam step (beta 0.9 / 0.999, eps 1e-8, bias
+ * correction for step iter + 1) updates params, m and v in place. */
+int gt_hip_predictor_forward(gt_hip_context *ctx, uint64_t n, const float *x, const float *params,
+                             float *y);
+int gt_hip_predictor_train_step(gt_hip_context *ctx, uint64_t n, const float *samples,
+                                const float *stats, float *params, float *m, float *v,
+                                uint32_t blocks, uint32_t iter, float lr, float *grad,
+                                float *loss);
 int gt_hip_predictor_stats(const gt_hip_context *ctx, gt_predictor_stats *stats);
 
 /* Test hook (no counterpart in the reference): while shrink > 0, evaluations

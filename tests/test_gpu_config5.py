@@ -51,3 +51,46 @@ def test_galaxy_4m_200_steps_with_predictor(hip, galaxy4m, theta):
     assert st["overflows"] <= 0.1 * st["predicted"]
     assert st["mean_abs_log_error"] < 0.2
     assert after["graph_steps"] - before["graph_steps"] >= 100
+
+
+def test_what_the_trained_network_gains(hip, galaxy4m):
+    """The 4M galaxy at theta 0.75 under the three slot policies, on the same
+    trajectory (the states are bit-identical whichever runs): the plain
+    two-pass solver (count pass, scan, fill pass), uniform slots (the last
+    evaluation's longest lists + 25% for every group) and slots sized by the
+    trained network (first 50 evaluations gather samples).  Reported: lists
+    phase and step time, entries the slots reserve, the overflow rate.  The
+    network must reserve less than uniform slots and beat the two-pass lists
+    phase."""
+    pos, vel, mass = galaxy4m
+    base = dict(theta=0.75, eps=0.025, group_size=32, pack=1)
+    res = {}
+    for name, predict in (("two-pass", 2), ("uniform", 0), ("predicted", 1)):
+        cfg = dict(base, predict=predict)
+        if predict == 1:
+            hip.predictor_reset(50)
+        hip.state_set(pos, vel, mass)
+        hip.prime(**cfg)
+        hip.steps(55 if predict == 1 else 5, 0.01, use_graph=True, **cfg)
+        s0 = hip.pipeline_stats()
+        hip.steps(20, 0.01, use_graph=True, **cfg)
+        s1 = hip.pipeline_stats()
+        tm, inf = hip.timings(), hip.info()
+        res[name] = dict(lists_ms=round(tm["lists"], 3), step_ms=round(tm["total"], 3),
+                         slot_entries=s1["last_slot_entries"],
+                         list_entries=inf["n_cell_list"] + inf["n_body_list"],
+                         rewrites=s1["list_rewrites"] - s0["list_rewrites"],
+                         graph_steps=s1["graph_steps"] - s0["graph_steps"])
+    st = hip.predictor_stats()
+    for name, r in res.items():
+        print(f"{name:>9}: lists {r['lists_ms']:.3f} ms, step {r['step_ms']:.3f} ms, slots "
+              f"{r['slot_entries'] / 1e6:.1f}M entries for {r['list_entries'] / 1e6:.1f}M "
+              f"({r['slot_entries'] / r['list_entries']:.2f}x), rewrites {r['rewrites']}",
+              flush=True)
+    print("predictor:", st, flush=True)
+    p, u, t = res["predicted"], res["uniform"], res["two-pass"]
+    assert p["list_entries"] == u["list_entries"] == t["list_entries"]
+    assert st["trained"] == 1 and st["predicted"] >= 20
+    assert st["overflows"] <= 0.1 * st["predicted"]
+    assert p["slot_entries"] < u["slot_entries"]
+    assert p["lists_ms"] < t["lists_ms"]

@@ -67,3 +67,70 @@ def test_predictor_is_deterministic(hip, oracle):
         run(hip, pos, vel, mass, 5, **cfg)
         stats.append(hip.predictor_stats())
     assert stats[0] == stats[1]
+
+
+# ---- the network's kernels against the CPU restatement (oracle/mlp_oracle.py) ----
+
+def mlp_case(seed, n=5000):
+    import mlp_oracle as M
+    rng = np.random.default_rng(seed)
+    p = np.concatenate([rng.normal(0, 1 / np.sqrt(M.F), M.H * M.F), rng.normal(0, 0.1, M.H),
+                        rng.normal(0, 1 / np.sqrt(M.H), M.H * M.H), rng.normal(0, 0.1, M.H),
+                        rng.normal(0, 1 / np.sqrt(M.H), M.O * M.H), rng.normal(0, 0.1, M.O)])
+    p = p.astype(np.float32)
+    # raw features on their own scales and offsets, targets log(1 + length)
+    feat = rng.normal(size=(n, M.F)) * rng.uniform(0.5, 4, M.F) + rng.uniform(-3, 3, M.F)
+    targ = rng.uniform(3.0, 8.0, (n, M.O))
+    samples = np.concatenate([feat, targ], 1).astype(np.float32)
+    s64 = samples.astype(np.float64)
+    stats = np.concatenate([s64.mean(0), s64.std(0)]).astype(np.float32)
+    return p, samples, stats
+
+
+def test_forward_kernel_equals_the_oracle(hip):
+    import mlp_oracle as M
+    p, samples, stats = mlp_case(11)
+    x, _ = M.standardise(samples, stats)
+    got = hip.predictor_forward(x.astype(np.float32), p)
+    want = M.forward(p.astype(np.float64), x.astype(np.float32).astype(np.float64))[0]
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"forward: max difference {err:.2e} of the largest output", flush=True)
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("iteration", [0, 5])
+def test_gradient_and_adam_kernels_equal_the_oracle(hip, iteration):
+    """mlp_gradients: each workgroup's gradient of its 64 samples' share of the
+    minibatch loss against the oracle's analytic backprop (itself checked
+    against finite differences on the CPU), relative 1e-4; the squared-error
+    sums; and adam_step's update from the gradients added in workgroup order."""
+    import mlp_oracle as M
+    p, samples, stats = mlp_case(12 + iteration)
+    rng = np.random.default_rng(iteration)
+    m = (rng.normal(size=M.NPARAMS) * 1e-3 * (iteration > 0)).astype(np.float32)
+    v = (rng.uniform(0, 1e-5, M.NPARAMS) * (iteration > 0)).astype(np.float32)
+    blocks, lr = 8, 3e-3
+    p1, m1, v1, grad, loss = hip.predictor_train_step(samples, stats, p, m, v, blocks,
+                                                      iteration, lr)
+    idx = M.minibatch(len(samples), blocks, iteration)
+    x, t = M.standardise(samples, stats)
+    batch = blocks * M.BLOCK
+    worst = 0.0
+    for b in range(blocks):
+        rows = idx[b]
+        want = M.gradient(p.astype(np.float64), x[rows], t[rows], batch)
+        rel = np.linalg.norm(grad[b] - want) / np.linalg.norm(want)
+        worst = max(worst, rel)
+        assert rel <= 1e-4
+        assert np.all(np.abs(grad[b] - want) <= 1e-4 * np.abs(want).max() + 1e-4 * np.abs(want))
+        y = M.forward(p.astype(np.float64), x[rows])[0]
+        assert abs(loss[b] - np.sum((y - t[rows]) ** 2)) <= 1e-5 * np.sum((y - t[rows]) ** 2)
+    print(f"iteration {iteration}: worst workgroup gradient relative difference {worst:.2e}",
+          flush=True)
+    g = np.zeros(M.NPARAMS, np.float32)
+    for b in range(blocks):  # added in workgroup order, as adam_step does
+        g += grad[b]
+    pw, mw, vw = M.adam(p, m, v, g.astype(np.float64), iteration, lr)
+    assert np.allclose(m1, mw, rtol=1e-5, atol=1e-12)
+    assert np.allclose(v1, vw, rtol=1e-5, atol=1e-15)
+    assert np.abs(p1 - pw).max() <= 1e-3 * lr + 1e-6 * np.abs(pw).max()
