@@ -125,7 +125,7 @@ struct gt_hip_context {
   DBuf<Cell> cells;
   DBuf<float4> bodies, out;
   DBuf<uint32_t> parent, skip, depth, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
-      body_list, group_root;
+      body_list, group_root, flag_node, group_node;
   // systems of the current evaluation (a batch when more than one): count and
   // particle offsets in device memory
   uint32_t systems = 1;
@@ -509,10 +509,12 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
   c->cells.ensure(nn);
   c->bodies.ensure(n);
   c->flags.ensure(n);
+  c->flag_node.ensure(n);
   c->offs.ensure(n);
   c->starts.ensure(n);
   c->scan_tmp.ensure(scan_tmp_words(n) + 8);
   c->groups.ensure(gc);
+  c->group_node.ensure(gc);
   if (p.systems > 1) c->group_root.ensure(gc);
   c->ccount.ensure(gc);
   c->bcount.ensure(gc);
@@ -603,19 +605,21 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
                     hilo ? c->cells_lo.get() : nullptr, s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
   launch_group_starts(in.lnk, c->parent.get(), frame, cap, n,
-                      static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s);
+                      static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s,
+                      c->flag_node.get());
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
                      s);
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
   launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, in.tsid,
-                     in.root_node, c->starts.get(), c->groups.get(), groot, s);
+                     in.root_node, c->starts.get(), c->groups.get(), groot, s,
+                     c->flag_node.get(), c->group_node.get());
   if (hilo)
     launch_group_centres(c->groups.get(), frame, p.group_cap, in.tpos, in.tsid, in.roots, in.geom,
                          c->group_hilo.get(), s);
   if (p.predicted) {
     // slots from the learned lengths, packed at scanned offsets
     launch_group_features(c->groups.get(), frame, p.group_cap, cap, in.tpos, in.lnk, in.geom,
-                          groot, c->feat.get(), s);
+                          groot, c->group_node.get(), c->depth.get(), c->feat.get(), s);
     launch_predict_slots(c->feat.get(), frame, p.group_cap, c->pred_params.get(),
                          c->pred_stats.get(), c->slot_c.get(), c->slot_b.get(), c->pred_out.get(),
                          s);
@@ -936,7 +940,7 @@ void predictor_after_eval(gt_hip_context *c, const Plan &p, bool overflowed, con
   float *samples = c->pred_samples.ensure(static_cast<size_t>(kPredSampleFloats) * kSampleCap);
   const uint32_t *groot = p.systems > 1 ? c->group_root.get() : nullptr;
   launch_group_features(c->groups.get(), c->frame.get(), p.group_cap, p.node_cap, tpos, lnk, geom,
-                        groot, feat, s);
+                        groot, c->group_node.get(), c->depth.get(), feat, s);
   const uint32_t stride = static_cast<uint32_t>(std::max<uint64_t>(1, ng / kSamplesPerEval));
   const uint32_t count = static_cast<uint32_t>((ng + stride - 1) / stride);
   launch_collect_samples(c->frame.get(), p.group_cap, stride, feat, c->ccount.get(),

@@ -313,13 +313,17 @@ void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip
                        const uint32_t *tsid, const double *roots, WalkRec *recs,
                        WalkRec32 *images, uint2 *span, Cell *cells, float4 *bodies,
                        float4 *bodies_lo, float4 *cells_lo, hipStream_t s);
+// node_of (may be null): at each flagged start, the node the group is taken
+// from (the chunked node, a lone packed child, or the parent of packed
+// siblings); make_groups copies it per group into group_node (may be null).
 void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
                          uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
-                         hipStream_t s);
+                         hipStream_t s, uint32_t *node_of = nullptr);
 void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
                         const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
                         const uint32_t *root_node, uint32_t *starts, gt_group *groups,
-                        uint32_t *group_root, hipStream_t s);
+                        uint32_t *group_root, hipStream_t s, const uint32_t *node_of = nullptr,
+                        uint32_t *group_node = nullptr);
 // group_root: the node each group's walk starts from (null: node 0); geom
 // gives the root half sizes.  Coherent walks (lanes > 1) need a single tree.
 // Capacity mode: per-group slots of cap_c / cap_b entries at g * cap, or (with

@@ -13,11 +13,13 @@ constexpr int kPredParams = kPredHidden * kPredFeatures + kPredHidden + kPredHid
 constexpr int kPredStats = 2 * (kPredFeatures + kPredOutputs);  // means, then deviations
 constexpr int kPredSampleFloats = kPredFeatures + kPredOutputs;
 
-// feat: kPredFeatures floats per group (raw, not standardised).
+// feat: kPredFeatures floats per group (raw, not standardised); group_node:
+// the node each group was selected from, node_depth: every node's depth.
 void launch_group_features(const gt_group *groups, const unsigned long long *frame,
                            uint32_t group_cap, uint32_t node_cap, const double *tpos,
                            const int4 *lnk, const double *geom, const uint32_t *group_root,
-                           float *feat, hipStream_t s);
+                           const uint32_t *group_node, const uint32_t *node_depth, float *feat,
+                           hipStream_t s);
 // Per-group slots (multiples of 4; 0 past the group count) and the predicted
 // log(1 + length) per group and output (pred, 2 floats per group).  stats
 // holds kPredStats + 1 floats: the standardisation, then the log-space margin.

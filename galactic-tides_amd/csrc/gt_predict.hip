@@ -15,6 +15,8 @@
 // (deterministic) before the Adam update.
 #include "gt_predict.h"
 
+#include <algorithm>
+
 namespace gth {
 namespace {
 
@@ -31,15 +33,18 @@ inline uint32_t blocks_for(size_t n, int b = kBlock) {
 
 // ---- features ------------------------------------------------------------------------
 
-// f0 log size of the group's bounding box, f1 log size of the smallest node
-// holding the group, f2 that node's depth, f3 targets, f4 log distance of the
-// group centre from its root's centre, f5 log bodies per volume of that node,
-// f6 f7 the bounding box's middle and smallest extent over its largest.
+// f0 log size of the group's bounding box, f1 log size of the node the group
+// was selected from (the chunked node, a lone packed child, or the parent of
+// packed siblings: group_node), f2 that node's depth, f3 targets, f4 log
+// distance of the group centre from its root's centre, f5 log bodies per
+// volume of that node, f6 f7 the bounding box's middle and smallest extent
+// over its largest.
 __global__ __launch_bounds__(kBlock) void group_features(
     const gt_group *__restrict__ groups, const unsigned long long *__restrict__ frame,
     uint32_t group_cap, uint32_t node_cap, const double *__restrict__ tpos,
     const int4 *__restrict__ lnk, const double *__restrict__ geom,
-    const uint32_t *__restrict__ group_root, float *__restrict__ feat) {
+    const uint32_t *__restrict__ group_root, const uint32_t *__restrict__ group_node,
+    const uint32_t *__restrict__ node_depth, float *__restrict__ feat) {
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
@@ -54,27 +59,10 @@ __global__ __launch_bounds__(kBlock) void group_features(
       hi[a] = fmax(hi[a], v);
     }
   double e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-  // the smallest node holding the group: descend from the root
   const uint32_t root = group_root ? group_root[g] : 0u;
-  uint32_t x = root < nn ? root : 0u;
-  int depth = 0;
-  for (int guard = 0; guard < 64; ++guard) {
-    const int4 L = lnk[x];
-    if (L.y <= 0 || L.x < 0) break;
-    uint32_t next = kNone;
-    for (int c = 0; c < L.y; ++c) {
-      const uint32_t y = static_cast<uint32_t>(L.x + c);
-      if (y >= nn) break;
-      const int4 C = lnk[y];
-      if (static_cast<uint32_t>(C.z) <= G.begin && static_cast<uint32_t>(C.w) >= G.end) {
-        next = y;
-        break;
-      }
-    }
-    if (next == kNone) break;
-    x = next;
-    ++depth;
-  }
+  const uint32_t gn = group_node[g];
+  const uint32_t x = gn < nn ? gn : 0u;
+  const uint32_t depth = node_depth[x];
   const double *gx = geom + static_cast<size_t>(kGeomStride) * x;
   const double *gr = geom + static_cast<size_t>(kGeomStride) * (root < nn ? root : 0u);
   const double cell = 2.0 * gx[3], root_size = 2.0 * gr[3];
@@ -101,10 +89,13 @@ __global__ __launch_bounds__(kBlock) void group_features(
 
 // ---- inference -------------------------------------------------------------------------
 
-// Normalised features -> the two standardised outputs, for one sample.
+// Normalised features -> the two standardised outputs, for one sample.  P is
+// the parameter vector in global memory: every lane reads the same weights at
+// wave-uniform addresses, so they come through the scalar cache into SGPRs
+// and are scalar operands of the FMAs (no LDS read per weight).
 __device__ inline void mlp_forward(const float *__restrict__ P, const float *x, float *y) {
   float h1[kH];
-#pragma unroll 4
+#pragma unroll
   for (int i = 0; i < kH; ++i) {
     float a = P[kOffB1 + i];
 #pragma unroll
@@ -116,7 +107,7 @@ __device__ inline void mlp_forward(const float *__restrict__ P, const float *x, 
   for (int o = 0; o < kO; ++o) out[o] = P[kOffB3 + o];
   for (int i = 0; i < kH; ++i) {
     float a = P[kOffB2 + i];
-#pragma unroll 8
+#pragma unroll
     for (int j = 0; j < kH; ++j) a = fmaf(P[kOffW2 + i * kH + j], h1[j], a);
     const float h2 = fmaxf(a, 0.0f);
 #pragma unroll
@@ -142,9 +133,7 @@ __global__ __launch_bounds__(kBlock) void predict_slots(
     const float *__restrict__ feat, const unsigned long long *__restrict__ frame,
     uint32_t group_cap, const float *__restrict__ params, const float *__restrict__ stats,
     uint32_t *__restrict__ slot_c, uint32_t *__restrict__ slot_b, float *__restrict__ pred) {
-  __shared__ float P[kPredParams];
   __shared__ float S[kPredStats + 1];
-  for (int k = threadIdx.x; k < kPredParams; k += kBlock) P[k] = params[k];
   if (threadIdx.x < kPredStats + 1) S[threadIdx.x] = stats[threadIdx.x];
   __syncthreads();
   const float margin = S[kPredStats];  // in device memory: captured graphs see its updates
@@ -157,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void predict_slots(
   }
   float x[kF], y[kO];
   normalise(feat + static_cast<size_t>(kF) * g, S, x);
-  mlp_forward(P, x, y);
+  mlp_forward(params, x, y);
   uint32_t slots[kO];
 #pragma unroll
   for (int o = 0; o < kO; ++o) {
@@ -172,22 +161,32 @@ __global__ __launch_bounds__(kBlock) void predict_slots(
 }
 
 // |log(1 + true length) - prediction| summed over the groups (atomics on
-// integers: micro-units, so the sum does not depend on the order).
+// integers: micro-units, so the sum does not depend on the order).  A few
+// hundred blocks stride over the groups and each adds its total with one
+// atomic (one per wave made the atomics on the one word the kernel's cost).
+constexpr uint32_t kErrorBlocks = 256;
+
 __global__ __launch_bounds__(kBlock) void prediction_error(
     const unsigned long long *__restrict__ frame, uint32_t group_cap,
     const float *__restrict__ pred, const uint32_t *__restrict__ cc,
     const uint32_t *__restrict__ bc, unsigned long long *__restrict__ err) {
+  __shared__ unsigned long long part[kBlock / 64];
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   unsigned long long e = 0;
-  if (g < ng) {
+  for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < ng; g += kErrorBlocks * kBlock) {
     const float ec = fabsf(log1pf(static_cast<float>(cc[g])) - pred[2 * static_cast<size_t>(g)]);
     const float eb =
         fabsf(log1pf(static_cast<float>(bc[g])) - pred[2 * static_cast<size_t>(g) + 1]);
-    e = static_cast<unsigned long long>(1e6f * (ec + eb) * 0.5f);
+    e += static_cast<unsigned long long>(1e6f * (ec + eb) * 0.5f);
   }
   for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
-  if ((threadIdx.x & 63) == 0 && e) atomicAdd(err, e);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    if (t) atomicAdd(err, t);
+  }
 }
 
 // Walk ahead: the scanned slot totals must fit the list buffers.
@@ -205,9 +204,7 @@ __global__ __launch_bounds__(kBlock) void sample_residual_max(const float *__res
                                                               const float *__restrict__ params,
                                                               const float *__restrict__ stats,
                                                               unsigned int *__restrict__ out) {
-  __shared__ float P[kPredParams];
   __shared__ float S[kPredStats];
-  for (int k = threadIdx.x; k < kPredParams; k += kBlock) P[k] = params[k];
   if (threadIdx.x < kPredStats) S[threadIdx.x] = stats[threadIdx.x];
   __syncthreads();
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -215,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void sample_residual_max(const float *__res
   const float *smp = samples + static_cast<size_t>(kF + kO) * i;
   float x[kF], y[kO];
   normalise(smp, S, x);
-  mlp_forward(P, x, y);
+  mlp_forward(params, x, y);
   float r = 0.0f;
   for (int o = 0; o < kO; ++o) {
     const float lg = y[o] * S[kF + kO + kF + o] + S[kF + o];
@@ -407,15 +404,12 @@ __global__ __launch_bounds__(kBlock) void adam_step(const float *__restrict__ gr
 __global__ __launch_bounds__(kBlock) void forward_rows(const float *__restrict__ x, uint32_t n,
                                                        const float *__restrict__ params,
                                                        float *__restrict__ y) {
-  __shared__ float P[kPredParams];
-  for (int k = threadIdx.x; k < kPredParams; k += kBlock) P[k] = params[k];
-  __syncthreads();
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float xi[kF], yi[kO];
 #pragma unroll
   for (int k = 0; k < kF; ++k) xi[k] = x[static_cast<size_t>(kF) * i + k];
-  mlp_forward(P, xi, yi);
+  mlp_forward(params, xi, yi);
 #pragma unroll
   for (int o = 0; o < kO; ++o) y[static_cast<size_t>(kO) * i + o] = yi[o];
 }
@@ -432,10 +426,12 @@ void launch_mlp_forward(const float *x, uint32_t n, const float *params, float *
 void launch_group_features(const gt_group *groups, const unsigned long long *frame,
                            uint32_t group_cap, uint32_t node_cap, const double *tpos,
                            const int4 *lnk, const double *geom, const uint32_t *group_root,
-                           float *feat, hipStream_t s) {
+                           const uint32_t *group_node, const uint32_t *node_depth, float *feat,
+                           hipStream_t s) {
   if (!group_cap) return;
   hipLaunchKernelGGL(group_features, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, groups, frame,
-                     group_cap, node_cap, tpos, lnk, geom, group_root, feat);
+                     group_cap, node_cap, tpos, lnk, geom, group_root, group_node, node_depth,
+                     feat);
   GTH_LAUNCH_CHECK();
 }
 
@@ -467,8 +463,9 @@ void launch_prediction_error(const unsigned long long *frame, uint32_t group_cap
                              const uint32_t *cc, const uint32_t *bc, unsigned long long *err,
                              hipStream_t s) {
   if (!group_cap) return;
-  hipLaunchKernelGGL(prediction_error, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, frame,
-                     group_cap, pred, cc, bc, err);
+  const uint32_t blocks = std::min<uint32_t>(blocks_for(group_cap), kErrorBlocks);
+  hipLaunchKernelGGL(prediction_error, dim3(blocks), dim3(kBlock), 0, s, frame, group_cap, pred,
+                     cc, bc, err);
   GTH_LAUNCH_CHECK();
 }
 

@@ -713,7 +713,8 @@ __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ 
                                                        const uint32_t *__restrict__ parent,
                                                        const unsigned long long *__restrict__ frame,
                                                        uint32_t node_cap, uint32_t gs, int pack,
-                                                       uint32_t *__restrict__ flags) {
+                                                       uint32_t *__restrict__ flags,
+                                                       uint32_t *__restrict__ node_of) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
@@ -722,20 +723,26 @@ __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ 
   // not reached by visit(): a small child of its parent (roots always are)
   if (parent[x] != kNone && e - b <= gs) return;
   if (e - b <= gs || L.y == 0) {      // chunk(begin, end)
-    for (uint32_t s = b; s < e; s += gs) flags[s] = 1u;
+    for (uint32_t s = b; s < e; s += gs) {
+      flags[s] = 1u;
+      if (node_of) node_of[s] = x;
+    }
     return;
   }
   bool open = false;
-  uint32_t run = 0;
+  uint32_t run = 0, run_start = 0;
   for (int c = 0; c < L.y; ++c) {
     const int4 C = lnk[L.x + c];
     const uint32_t cc = static_cast<uint32_t>(C.w - C.z);
     if (cc <= gs) {
       if (pack && open && run + cc <= gs) {
         run += cc;
+        if (node_of) node_of[run_start] = x;  // siblings packed: their parent holds them
       } else {
         flags[C.z] = 1u;
+        if (node_of) node_of[C.z] = static_cast<uint32_t>(L.x + c);
         run = cc;
+        run_start = static_cast<uint32_t>(C.z);
         open = true;
       }
     } else {
@@ -757,7 +764,9 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ tsid,
                                                       const uint32_t *__restrict__ root_node,
                                                       gt_group *__restrict__ groups,
-                                                      uint32_t *__restrict__ group_root) {
+                                                      uint32_t *__restrict__ group_root,
+                                                      const uint32_t *__restrict__ node_of,
+                                                      uint32_t *__restrict__ group_node) {
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   if (g >= ng) return;
@@ -768,6 +777,7 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
   G.pad0 = G.pad1 = 0;
   groups[g] = G;
   if (tsid) group_root[g] = root_node[tsid[G.begin]];
+  if (group_node) group_node[g] = node_of[G.begin];
 }
 
 // stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group,
@@ -884,25 +894,26 @@ void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip
 
 void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
                          uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
-                         hipStream_t s) {
+                         hipStream_t s, uint32_t *node_of) {
   launch_fill_u32(flags, n, 0u, s);
   if (!node_cap) return;
   hipLaunchKernelGGL(group_starts, dim3(blocks_for(node_cap)), dim3(kBlock), 0, s, lnk, parent,
-                     frame, node_cap, gs, pack, flags);
+                     frame, node_cap, gs, pack, flags, node_of);
   GTH_LAUNCH_CHECK();
 }
 
 void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
                         const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
                         const uint32_t *root_node, uint32_t *starts, gt_group *groups,
-                        uint32_t *group_root, hipStream_t s) {
+                        uint32_t *group_root, hipStream_t s, const uint32_t *node_of,
+                        uint32_t *group_node) {
   if (!n) return;
   hipLaunchKernelGGL(compact_starts, dim3(blocks_for(n)), dim3(kBlock), 0, s, flags, offs, n,
                      starts);
   GTH_LAUNCH_CHECK();
   if (!group_cap) return;
   hipLaunchKernelGGL(make_groups, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, starts, frame,
-                     group_cap, n, tsid, root_node, groups, group_root);
+                     group_cap, n, tsid, root_node, groups, group_root, node_of, group_node);
   GTH_LAUNCH_CHECK();
 }
 

@@ -71,7 +71,7 @@ def test_what_the_trained_network_gains(hip, galaxy4m):
             hip.predictor_reset(50)
         hip.state_set(pos, vel, mass)
         hip.prime(**cfg)
-        hip.steps(55 if predict == 1 else 5, 0.01, use_graph=True, **cfg)
+        hip.steps(55, 0.01, use_graph=True, **cfg)  # the same trajectory for all three
         s0 = hip.pipeline_stats()
         hip.steps(20, 0.01, use_graph=True, **cfg)
         s1 = hip.pipeline_stats()

@@ -131,6 +131,9 @@ def test_gradient_and_adam_kernels_equal_the_oracle(hip, iteration):
     for b in range(blocks):  # added in workgroup order, as adam_step does
         g += grad[b]
     pw, mw, vw = M.adam(p, m, v, g.astype(np.float64), iteration, lr)
-    assert np.allclose(m1, mw, rtol=1e-5, atol=1e-12)
-    assert np.allclose(v1, vw, rtol=1e-5, atol=1e-15)
+    # FP32 rounding of each term: bounded by the operands' size (0.9 m and
+    # 0.1 g may nearly cancel)
+    g64 = g.astype(np.float64)
+    assert np.all(np.abs(m1 - mw) <= 1e-6 * (0.9 * np.abs(m) + 0.1 * np.abs(g64)) + 1e-15)
+    assert np.all(np.abs(v1 - vw) <= 1e-6 * (0.999 * np.abs(v) + 0.001 * g64 * g64) + 1e-20)
     assert np.abs(p1 - pw).max() <= 1e-3 * lr + 1e-6 * np.abs(pw).max()
