@@ -20,7 +20,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-result",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 SOURCES = ["gt_api.hip", "gt_build.hip", "gt_walk.hip", "gt_eval.hip", "gt_direct.hip",
-           "gt_scan.hip", "gt_sort.hip", "gt_step.hip", "gt_predict.hip", "gt_treegrav.hip", "gt_neighbours.hip"]
+           "gt_scan.hip", "gt_sort.hip", "gt_step.hip", "gt_predict.hip", "gt_treegrav.hip", "gt_neighbours.hip",
+           "gt_noalloc.hip"]
 
 
 def _compile(src):

@@ -193,6 +193,7 @@ struct gt_hip_context {
   DBuf<uint32_t> nb_over;  // queries whose lists outgrew their slots
   uint32_t nb_stage = 64;  // slot entries per query (grows when many overflow)
   DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation
+  void *noalloc_scratch = nullptr;  // this device's gt_hip_tree_noalloc scratch
   // predictor kernels on caller data (test hooks)
   DBuf<float> tk_in, tk_stats, tk_params, tk_m, tk_v, tk_grad, tk_loss, tk_out;
 };
@@ -1431,6 +1432,8 @@ int gt_hip_create(gt_hip_context **out, int device) {
     launch_fill_u64(c->frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
     c->eval_queue.ensure(kXcds);
     launch_fill_u64(c->raw_frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
+    // the no-allocation entry point's device scratch on this device (current)
+    c->noalloc_scratch = noalloc_scratch(device);
     GTH_CHECK(hipStreamSynchronize(c->stream));
     g_create_error.clear();
   } catch (const Error &e) {
@@ -1550,6 +1553,30 @@ int gt_hip_tree(gt_hip_context *c, const gt_float4 *bodies, uint64_t n_cells,
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     download(reinterpret_cast<float4 *>(out), dout, n, s);
     GTH_CHECK(hipStreamSynchronize(s));
+    c->gpu_seconds = elapsed_ms(c->ev[0], c->ev[1]) * 1e-3;
+  });
+}
+
+int gt_hip_tree_noalloc(gt_hip_context *c, const gt_float4 *bodies, uint64_t n_cells,
+                        const gt_cell *cells, uint64_t n_cell_list, const uint32_t *cell_list,
+                        uint64_t n_body_list, const uint32_t *body_list, const gt_group *groups,
+                        const gt_params *p, int exact, gt_float4 *out) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    require(p != nullptr, "params is NULL");
+    const uint32_t n = p->n, ng = p->n_groups;
+    c->gpu_seconds = 0.0;
+    if (n == 0 || ng == 0) return;
+    require(bodies && cells && groups && out, "array is NULL");
+    require(n_cell_list == 0 || cell_list, "cell_list is NULL");
+    require(n_body_list == 0 || body_list, "body_list is NULL");
+    if (n_cells > 0xffffffffull || n_cell_list > 0xffffffffull || n_body_list > 0xffffffffull)
+      fail(GT_HIP_ERR_LIMIT, "hip tree: more than 2^32 - 1 cells or list entries");
+    GTH_CHECK(hipSetDevice(c->device));
+    noalloc_tree(c->noalloc_scratch, reinterpret_cast<const float4 *>(bodies), n,
+                 reinterpret_cast<const Cell *>(cells), n_cells, cell_list, n_cell_list,
+                 body_list, n_body_list, groups, ng, p->G, p->eps2, p->quadrupole, exact != 0,
+                 reinterpret_cast<float4 *>(out), c->stream, c->ev[0], c->ev[1]);
     c->gpu_seconds = elapsed_ms(c->ev[0], c->ev[1]) * 1e-3;
   });
 }

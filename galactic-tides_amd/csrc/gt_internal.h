@@ -170,6 +170,7 @@ constexpr uint32_t kAbortLists = 16u;  // a group's lists do not fit the slots
 constexpr uint32_t kAbortWalk = 32u;   // malformed tree
 constexpr uint32_t kAbortItems = 64u;  // more work items or split parts than the plan's bounds
 constexpr uint32_t kAbortSlots = 128u; // predicted slots add up to more than the list buffers
+constexpr uint32_t kAbortInput = 256u; // caller-supplied device inputs failed their check
 
 // Host-side bounds of one evaluation, fixed before it is enqueued.
 struct Plan {
@@ -277,6 +278,20 @@ void launch_group_centres(const gt_group *groups, const unsigned long long *fram
                           uint32_t group_cap, const double *tpos, const uint32_t *tsid,
                           const double *roots, const double *geom, float4 *group_hilo,
                           hipStream_t s);
+// gt_noalloc.hip: the evaluation scratch of gt_hip_tree_noalloc, a fixed
+// device array (noalloc_scratch_bytes()); noalloc_scratch(device) is its
+// address on that device (looked up once per device; call with it current).
+// noalloc_tree checks the caller's device arrays on the device, evaluates them
+// into out, waits, and fails with GT_HIP_ERR_ARGUMENT when the check did;
+// t0 / t1 are recorded around the evaluation.
+void *noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+void noalloc_tree(void *scratch_ptr, const float4 *bodies, uint32_t n, const Cell *cells,
+                  uint64_t n_cells, const uint32_t *cell_list, uint64_t n_cell_list,
+                  const uint32_t *body_list, uint64_t n_body_list, const gt_group *groups,
+                  uint32_t n_groups, float G, float eps2, uint32_t quad, bool exact,
+                  float4 *out, hipStream_t s, hipEvent_t t0, hipEvent_t t1);
+
 void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_list,
                  const uint32_t *body_list, const gt_group *groups,
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,

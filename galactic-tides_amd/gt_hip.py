@@ -80,6 +80,9 @@ _compiled_in = _sig("gt_hip_compiled_in", C.c_int, [])
 _direct = _sig("gt_hip_direct", C.c_int, [_vp, _vp, _P(Params), _vp])
 _tree = _sig("gt_hip_tree", C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp, C.c_uint64,
                                       _vp, _vp, _P(Params), C.c_int, _vp])
+_tree_noalloc = _sig("gt_hip_tree_noalloc", C.c_int,
+                     [_vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp, C.c_uint64, _vp, _vp,
+                      _P(Params), C.c_int, _vp])
 _group_compute = _sig("gt_hip_group_compute", C.c_int,
                       [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp])
 class Diagnostics(C.Structure):
@@ -223,6 +226,14 @@ class Context:
         self._check(_tree(self._h, _ptr(b), len(cells), _ptr(cells), len(cl), _ptr(cl), len(bl),
                           _ptr(bl), _ptr(groups), C.byref(p), int(exact), _ptr(out)))
         return out
+
+    def tree_noalloc(self, bodies, n_cells, cells, n_cell_list, cell_list, n_body_list,
+                     body_list, groups, params, out, exact=False):
+        """gt_hip_tree on arrays already in device memory (every array argument
+        is a device address, an int); allocates nothing.  params: make_params."""
+        self._check(_tree_noalloc(self._h, bodies, n_cells, cells, n_cell_list, cell_list,
+                                  n_body_list, body_list, groups, C.byref(params), int(exact),
+                                  out))
 
     # -- the whole force evaluation -----------------------------------------------------
     def group_compute(self, pos, mass, **cfg):

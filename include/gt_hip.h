@@ -151,6 +151,24 @@ int gt_hip_tree(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
                 uint64_t n_body_list, const uint32_t *body_list, const gt_group *groups,
                 const gt_params *p, int exact, gt_float4 *out);
 
+/* gt_hip_tree for callers that cannot allocate device memory (no counterpart
+ * in the reference; for code that must not call hipMalloc: tight serving
+ * loops, harnesses that forbid it).  Same arguments and results as gt_hip_tree,
+ * but every array (bodies, cells, the two lists, groups and out) is in device
+ * memory, written by the caller before the call (its writes complete); p is a
+ * host struct.  The call allocates nothing: the evaluation's scratch (status
+ * and count words, the input check's flags: 1 KiB) is a fixed device array,
+ * one per GPU, that the library holds until the process ends.  The inputs
+ * are checked on the device as gt_hip_tree checks them on the host (same
+ * messages, GT_HIP_ERR_ARGUMENT), before anything is evaluated.  Calls on one
+ * GPU share that scratch: calls that overlap on one GPU (from several host
+ * threads or contexts) race.  Synchronous; limits: at most 2^32 - 1 cells
+ * and list entries (32-bit indices), groups of at most 64 targets. */
+int gt_hip_tree_noalloc(gt_hip_context *ctx, const gt_float4 *bodies, uint64_t n_cells,
+                        const gt_cell *cells, uint64_t n_cell_list, const uint32_t *cell_list,
+                        uint64_t n_body_list, const uint32_t *body_list, const gt_group *groups,
+                        const gt_params *p, int exact, gt_float4 *out);
+
 /* ---- the whole force evaluation on the GPU ---- */
 
 /* GroupTreeGravity::compute (src/gravity_backends.cpp:109-137) with device = hip:
