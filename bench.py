@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "galactic-tides_amd"))
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0     # MI355X published HBM3E bandwidth
 CELL_FLOPS, BODY_FLOPS = 71, 21  # per interaction, as written in the reference's kernels
 
 
@@ -60,27 +61,68 @@ def measured_traffic(kernel="eval_fast"):
     return None
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: model, logical CPUs, CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cpus_allowed": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def build_hash():
+    """SHA-256 (first 16 hex digits) of the HIP library this run loaded."""
+    import hashlib
+    path = os.path.join(ROOT, "galactic-tides_amd", "libgt_hip.so")
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def cpu_baseline(pos, mass, cfg, offsets=None):
-    """The reference's CPU group solver (packed lists, OpenMP on every host
-    core) timed on the same particles: one warm-up and best of 2 compute()
-    calls (for a batch: one solver per system, summed, best of 1).  Falls back
-    to the oracle's restatement if the reference build is absent."""
+    """The reference's CPU solvers timed on the same particles, on every host
+    core with OMP_PROC_BIND=close OMP_PLACES=cores (set here before the OpenMP
+    runtime loads, unless the environment sets them): GroupTreeGravity with
+    packed lists (the GPU's lists), with unpacked lists (the reference's CPU
+    default) and TreeGravity (gravity = tree), each one warm-up then the best
+    of 3 compute() calls, as apps/galactic_tides_bench.cpp:128-141 does.  For a
+    batch: one packed GroupTreeGravity per system, one warm-up and one call
+    each, summed.  The oracle's restatement stands in if the reference build
+    is absent."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    host = host_cpu()  # (before the OpenMP runtime binds this thread to one core)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Checker, available
     kind = "reference" if available("ref") else "port"
     chk = Checker("ref" if kind == "reference" else "oracle")
     t0 = time.time()
-    args = (cfg["G"], cfg["theta"], cfg["eps"], True, cfg["group_size"], cfg["leaf_size"], True)
+    args = (cfg["G"], cfg["theta"], cfg["eps"], True, cfg["group_size"], cfg["leaf_size"])
+    solvers = {}
     if offsets is None:
-        best = chk.time_group(pos, mass, *args, repeat=2)
+        best = chk.time_group(pos, mass, *args, True, repeat=3)
+        unpacked = chk.time_group(pos, mass, *args, False, repeat=3)
+        solvers = {"group_packed": round(best["total"] * 1e3, 1),
+                   "group_unpacked": round(unpacked["total"] * 1e3, 1)}
+        if kind == "reference":
+            solvers["tree"] = round(chk.time_tree(pos, mass, cfg["G"], cfg["theta"], cfg["eps"],
+                                                  True, cfg["leaf_size"], repeat=3) * 1e3, 1)
     else:
         best = None
         for k in range(len(offsets) - 1):
             b, e = int(offsets[k]), int(offsets[k + 1])
-            one = chk.time_group(pos[b:e], mass[b:e], *args, repeat=1)
+            one = chk.time_group(pos[b:e], mass[b:e], *args, True, repeat=1)
             best = one if best is None else {key: best[key] + one[key] for key in best}
+        solvers = {"group_packed": round(best["total"] * 1e3, 1)}
     wall = time.time() - t0
-    return kind, chk.max_threads(), best, wall
+    return kind, chk.max_threads(), best, solvers, host, wall
 
 
 def force_errors(ctx, pos, mass, acc, cfg, offsets=None, size=1000):
@@ -220,7 +262,7 @@ def bench_tree(args):
           f"{cells} cell + {parts} particle interactions", file=sys.stderr, flush=True)
     flops = TREE_CELL_FLOPS * cells + TREE_PARTICLE_FLOPS * parts
     achieved = flops / (walk_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "tree_gravity", "kernel_ms": round(walk_ms, 4),
                 "flops_per_launch": flops, "compute": "fp64 vector ALU"}
@@ -308,11 +350,85 @@ def bench_direct(args):
         "config": {"workload": "direct-200k", "n": n, "eps": 0.025, "halo": args.halo},
         "force_error": {"median": float(err[len(err) // 2]),
                         "p99": float(err[len(err) * 99 // 100]), "sample": len(err)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": None, "kernel": "direct_fast", "kernel_ms": round(kms, 4),
                      "flops_per_launch": flops, "compute": "fp32 vector ALU"},
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_plain_vs_full(args):
+    """The north star's comparison on one GPU, at the 2M encounter and the 4M
+    galaxy (theta 0.75, group size 32, eps 0.025): ms per leapfrog step (one
+    force evaluation each) of
+
+      plain: the plain GPU group solver, one launch after another with the host
+             waiting for each evaluation's status: lists by a count pass and a
+             fill pass (predict = 2), groups evaluated in tree order, no split
+             lists, no persistent grid, no graphs;
+      full:  the learned and batched solver: list slots sized by the trained
+             network (50 evaluations gather its samples first, outside the
+             timing), every group's work item in one launch on the persistent
+             grid taking them longest first, steps from captured HIP graphs;
+      uniform: the headline configuration (uniform slots, graphs) for reference.
+
+    All three give bit-identical states (checked here).  Value = plain / full."""
+    import gt_hip
+    out = {}
+    for name, make in (("encounter-2M", lambda: encounter_2m(args.halo)[:3]),
+                       ("galaxy-4M", lambda: galaxy_4m(args.halo))):
+        pos, vel, mass = make()
+        ctx = gt_hip.Context(0)
+        base = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
+                    pack=1)
+        modes = {"plain": (dict(base, predict=2, eval_order=1, split_len=-1, persistent=0), False),
+                 "full": (dict(base, predict=1, persistent=1), True),
+                 "uniform": (dict(base, predict=0), True)}
+        res, states = {}, {}
+        for mode, (cfg, graph) in modes.items():
+            if cfg["predict"] == 1:
+                ctx.predictor_reset(50)
+            ctx.state_set(pos, vel, mass)
+            ctx.prime(**cfg)
+            ctx.steps(55, 0.01, use_graph=graph, **cfg)  # (trains the network in "full")
+            t = time.perf_counter()
+            ctx.steps(args.steps, 0.01, use_graph=graph, **cfg)
+            ms = 1e3 * (time.perf_counter() - t) / args.steps
+            tm = ctx.timings()
+            res[mode] = {"ms_per_step": round(ms, 3), "lists_ms": round(tm["lists"], 3),
+                         "evaluate_ms": round(tm["evaluate"], 3), "tree_ms": round(tm["tree"], 3)}
+            if mode == "full":
+                st = ctx.predictor_stats()
+                res[mode]["predictor"] = {k: st[k] for k in ("predicted", "overflows",
+                                                             "mean_abs_log_error",
+                                                             "slot_entries", "list_entries")}
+            states[mode] = ctx.state_get()
+            print(f"# {name} {mode}: {res[mode]}", file=sys.stderr, flush=True)
+        same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64))
+                   for m in ("full", "uniform") for a, b in zip(states[m], states["plain"]))
+        out[name] = dict(res, ratio_plain_over_full=round(res["plain"]["ms_per_step"]
+                                                          / res["full"]["ms_per_step"], 3),
+                         identical_states=bool(same), n=len(mass))
+        ctx.close()
+    line = {
+        "metric": "plain / learned+batched group solver, ms per force evaluation (leapfrog step)",
+        "value": out["encounter-2M"]["ratio_plain_over_full"],
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 55,
+        "ms_per_step": out["encounter-2M"]["full"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (lists, evaluation); fp64 (tree, walk, kick-drift)",
+        "data": "synthetic: the 2M Hernquist encounter and the 4M galaxy (seed 6)",
+        "config": {"workload": "plain-vs-full", "theta": 0.75, "group_size": 32, "eps": 0.025,
+                   "halo": args.halo},
+        "results": out,
+        "build": {"libgt_hip_sha16": build_hash()},
     }
     print(json.dumps(line), flush=True)
 
@@ -336,7 +452,7 @@ def main():
                     help="tuning: list entries per evaluation part (0: default, <0: never split)")
     ap.add_argument("--workload", default="encounter-2m",
                     choices=["encounter-2m", "batch-128", "galaxy-4m", "neighbours-2m",
-                             "tree-2m", "direct-200k"],
+                             "tree-2m", "direct-200k", "plain-vs-full"],
                     help="encounter-2m: the 2M-particle encounter (headline); batch-128: 128 "
                          "independent galaxies of 2k-100k particles evaluated together; "
                          "galaxy-4m: one 4M-particle galaxy (seed 6); neighbours-2m: "
@@ -344,7 +460,8 @@ def main():
                          "particle a query, radius = its distance to its 32nd neighbour); "
                          "tree-2m: the FP64 per-particle tree solver (gravity = tree) on the "
                          "encounter; direct-200k: FP32 direct summation (gravity = direct) "
-                         "over a 200k-particle galaxy")
+                         "over a 200k-particle galaxy; plain-vs-full: the plain GPU group "
+                         "solver against the learned, batched one at 2M and 4M")
     ap.add_argument("--theta", type=float, default=0.75)
     ap.add_argument("--predict", type=int, default=0,
                     help="list slots: 0 uniform, 1 learned predictor (trained on the first "
@@ -383,6 +500,9 @@ def main():
         return
     if args.workload == "direct-200k":
         bench_direct(args)
+        return
+    if args.workload == "plain-vs-full":
+        bench_plain_vs_full(args)
         return
 
     import gt_hip
@@ -443,31 +563,40 @@ def main():
     p_now, _, a_now, _ = ctx.state_get()
     med, p99, n_err = force_errors(ctx, p_now, mass, a_now, cfg, offsets)
 
-    # Roofline of the list-evaluation kernel (FP32 VALU bound): algorithmic
-    # flops of the last evaluation / its kernel time measured with HIP events
-    # on the stream it runs on.
+    # Roofline of the list-evaluation kernel: algorithmic flops of the last
+    # evaluation / its kernel time measured with HIP events on the stream it
+    # runs on.  It is bound by the FP32 vector ALUs (pairwise gravity has no
+    # matrix operation to put on MFMA), priced against the published FP32
+    # vector peak; its compulsory HBM bytes (the lists, groups, cells, the
+    # targets in and the results out) over the same time give the HBM fraction.
     flops = CELL_FLOPS * inf["cell_interactions"] + BODY_FLOPS * inf["body_interactions"]
     eval_ms = tm["evaluate"]
     achieved = flops / (eval_ms * 1e-3) / 1e12
-    # Compute-bound: priced against the FP32 peak (the kernel runs on the
-    # vector ALUs; pairwise gravity has no matrix operation to put on MFMA).
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+    compulsory = (4 * (inf["n_cell_list"] + inf["n_body_list"]) + 32 * inf["n_groups"]
+                  + 48 * inf["n_nodes"] + 32 * n)
+    hbm_gbs = compulsory / (eval_ms * 1e-3) / 1e9
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": measured_traffic() if args.workload == "encounter-2m" else None,
                 "kernel": "eval_fast",
                 "kernel_ms": round(eval_ms, 4),
-                "flops_per_launch": flops, "compute": "fp32 vector ALU"}
+                "flops_per_launch": flops, "compute": "fp32 vector ALU",
+                "hbm": {"compulsory_bytes": compulsory, "achieved": round(hbm_gbs, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}}
 
     cpu = None
     if not args.no_cpu:
-        kind, cores, best, wall = cpu_baseline(pos, mass, cfg, offsets)
+        kind, cores, best, solvers, host, wall = cpu_baseline(pos, mass, cfg, offsets)
         cpu = {"value": round(best["interactions"] / best["total"], 1), "unit": "interactions/s",
                "cores": cores, "kind": kind,
                "ms_per_force_eval": round(best["total"] * 1e3, 1),
                "split_ms": {k: round(best[k] * 1e3, 1) for k in ("tree", "lists", "evaluate")},
+               "solvers_ms": solvers, "host": host,
                "sample": (("the full 2M encounter at the initial positions: GroupTreeGravity on "
-                           "the cpu device, packed lists, one warm-up then best of 2 compute() "
-                           if offsets is None else
+                           "the cpu device with packed lists (value; the GPU's lists), with "
+                           "unpacked lists, and TreeGravity, each one warm-up then best of 3 "
+                           "compute() " if offsets is None else
                            "all 128 galaxies at their initial positions: one GroupTreeGravity "
                            "per galaxy on the cpu device, packed lists, one warm-up then one "
                            "compute() each, summed ") + f"({wall:.0f} s wall)")}
@@ -498,6 +627,7 @@ def main():
         "force_error": {"median": med, "p99": p99, "sample": n_err},
         "predictor": ctx.predictor_stats() if args.predict == 1 else None,
         "interactions_per_step": interactions,
+        "build": {"libgt_hip_sha16": build_hash()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

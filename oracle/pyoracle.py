@@ -219,6 +219,18 @@ class Checker:
         return dict(total=best[0], tree=best[1], lists=best[2], evaluate=best[3],
                     interactions=best[4] + best[5])
 
+    def time_tree(self, pos, mass, G=1.0, theta=0.75, eps=0.025, quadrupole=True, leaf_size=8,
+                  repeat=3):
+        """(reference build only) TreeGravity::compute wall time: one warm-up,
+        then the best of `repeat` calls, in seconds."""
+        best = np.zeros(1)
+        rc = self._time_tree(len(mass), np.ascontiguousarray(pos, np.float64),
+                             np.ascontiguousarray(mass, np.float64), G, theta, eps,
+                             int(quadrupole), leaf_size, repeat, best)
+        if rc:
+            raise RuntimeError("timing failed")
+        return float(best[0])
+
     def within(self, centres, radii):
         """Octree::for_each_within for each query on the last build's tree: CSR
         offsets (nq + 1), original indices and squared distances, in the order
