@@ -36,6 +36,15 @@ inline gt_group_config config(double G, double theta, double eps, bool quadrupol
   return c;
 }
 
+// The same from a GravityParams (include/galactic_tides/octree.h:13-18: G,
+// theta, eps, quadrupole) and GroupTreeGravity's own arguments
+// (gravity_backends.h:62-63), as GroupTreeGravity::compute has them.
+template <class GravityParams>
+inline gt_group_config config(const GravityParams &gp, int group_size, int leaf_size, bool pack,
+                              bool exact = false) {
+  return config(gp.G, gp.theta, gp.eps, gp.quadrupole, group_size, leaf_size, pack, exact);
+}
+
 class Context {
  public:
   explicit Context(int device = 0) {
