@@ -194,6 +194,7 @@ struct gt_hip_context {
   uint32_t nb_stage = 64;  // slot entries per query (grows when many overflow)
   DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation
   void *noalloc_scratch = nullptr;  // this device's gt_hip_tree_noalloc scratch
+  DBuf<double> pt_pos;  // gt_hip_tree_gravity_at's points
   // predictor kernels on caller data (test hooks)
   DBuf<float> tk_in, tk_stats, tk_params, tk_m, tk_v, tk_grad, tk_loss, tk_out;
 };
@@ -1608,49 +1609,79 @@ int gt_hip_group_compute(gt_hip_context *c, const gt_group_config *cfg, uint64_t
   });
 }
 
+namespace {
+
+// The FP64 tree solver of n particles, for the particles themselves (points
+// null) or at np points; outputs in the caller's order (n or np rows).
+void tree_solve(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, const double *pos,
+                const double *mass, uint64_t np, const double *points, double *acc,
+                double *pot, uint64_t *stats) {
+  check_config(cfg);
+  if (n > 0xffffffffull || np > 0xffffffffull)
+    fail(GT_HIP_ERR_LIMIT, "too many particles or points for the 32 bit GPU indices");
+  c->tm = gt_timings{};
+  const uint64_t targets = points ? np : n;
+  if (n == 0 || targets == 0) {
+    if (stats) stats[0] = stats[1] = 0;
+    for (uint64_t k = 0; k < targets; ++k) {  // no particles: nothing attracts
+      if (acc) acc[3 * k] = acc[3 * k + 1] = acc[3 * k + 2] = 0.0;
+      if (pot) pot[k] = 0.0;
+    }
+    return;
+  }
+  require(pos && mass && acc && pot, "array is NULL");
+  GTH_CHECK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
+  double *da = c->in_acc.ensure(3 * targets), *dq = c->in_pot.ensure(targets);
+  double *dt = points ? c->pt_pos.ensure(3 * np) : nullptr;
+  unsigned long long *st_dev = c->misc.ensure(4);
+  upload(dp, pos, 3 * n, s);
+  upload(dm, mass, n, s);
+  if (points) upload(dt, points, 3 * np, s);
+  const uint32_t n32 = static_cast<uint32_t>(n);
+  unsigned long long *frame = c->frame.get();
+  const Plan p = build_single_tree(c, cfg, n32, dp, dm);
+  launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(),
+                      c->tree.tree_pos(), c->tree.tree_mass(), c->tree.order(), n32, frame,
+                      p.node_cap, cfg->G, cfg->eps, cfg->theta, cfg->quadrupole != 0, da, dq,
+                      st_dev, s, dt, static_cast<uint32_t>(np));
+  GTH_CHECK(hipEventRecord(c->ev[2], s));
+  download(acc, da, 3 * targets, s);
+  download(pot, dq, targets, s);
+  unsigned long long counts[2] = {0, 0};
+  download(counts, st_dev, 2, s);
+  GTH_CHECK(hipStreamSynchronize(s));
+  if (stats) {
+    stats[0] = counts[0];
+    stats[1] = counts[1];
+  }
+  c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
+  c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
+  c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);
+  c->gpu_seconds = c->tm.total * 1e-3;
+}
+
+}  // namespace
+
 int gt_hip_tree_gravity(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
                         const double *pos, const double *mass, double *acc, double *pot,
                         uint64_t *stats) {
   if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] { tree_solve(c, cfg, n, pos, mass, 0, nullptr, acc, pot, stats); });
+}
+
+int gt_hip_tree_gravity_at(gt_hip_context *c, const gt_group_config *cfg, uint64_t n,
+                           const double *pos, const double *mass, uint64_t np,
+                           const double *points, double *acc, double *pot, uint64_t *stats) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
   return guarded(c, [&] {
-    check_config(cfg);
-    if (n > 0xffffffffull) fail(GT_HIP_ERR_LIMIT, "too many particles for the 32 bit GPU indices");
-    c->tm = gt_timings{};
-    if (n == 0) {
+    require(np == 0 || points != nullptr, "points is NULL");
+    if (np == 0) {
       if (stats) stats[0] = stats[1] = 0;
       return;
     }
-    require(pos && mass && acc && pot, "array is NULL");
-    GTH_CHECK(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    double *dp = c->in_pos.ensure(3 * n), *dm = c->in_mass.ensure(n);
-    double *da = c->in_acc.ensure(3 * n), *dq = c->in_pot.ensure(n);
-    unsigned long long *st_dev = c->misc.ensure(4);
-    upload(dp, pos, 3 * n, s);
-    upload(dm, mass, n, s);
-    const uint32_t n32 = static_cast<uint32_t>(n);
-    unsigned long long *frame = c->frame.get();
-    const Plan p = build_single_tree(c, cfg, n32, dp, dm);
-    const uint32_t cap = p.node_cap;
-    launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(),
-                        c->depth.get(),
-                        c->tree.tree_pos(),
-                        c->tree.tree_mass(), c->tree.order(), n32, frame, cap, cfg->G, cfg->eps,
-                        cfg->theta, cfg->quadrupole != 0, da, dq, st_dev, s);
-    GTH_CHECK(hipEventRecord(c->ev[2], s));
-    download(acc, da, 3 * n, s);
-    download(pot, dq, n, s);
-    unsigned long long counts[2] = {0, 0};
-    download(counts, st_dev, 2, s);
-    GTH_CHECK(hipStreamSynchronize(s));
-    if (stats) {
-      stats[0] = counts[0];
-      stats[1] = counts[1];
-    }
-    c->tm.tree = elapsed_ms(c->ev[0], c->ev[1]);
-    c->tm.evaluate = elapsed_ms(c->ev[1], c->ev[2]);
-    c->tm.total = elapsed_ms(c->ev[0], c->ev[2]);
-    c->gpu_seconds = c->tm.total * 1e-3;
+    tree_solve(c, cfg, n, pos, mass, np, points, acc, pot, stats);
   });
 }
 

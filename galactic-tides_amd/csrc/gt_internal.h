@@ -374,7 +374,8 @@ void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *sk
                          const uint32_t *depth, const double *tpos, const double *tmass,
                          const uint32_t *order, uint32_t n, const unsigned long long *frame,
                          uint32_t node_cap, double G, double eps, double theta, int quadrupole,
-                         double *acc, double *pot, unsigned long long *stats, hipStream_t s);
+                         double *acc, double *pot, unsigned long long *stats, hipStream_t s,
+                         const double *points = nullptr, uint32_t n_points = 0);
 
 // gt_neighbours.hip: fixed-radius queries over a built tree (one system, root
 // node 0).  launch_nb_records packs each node's cube, children, body range

@@ -3,7 +3,9 @@
 // the reference runs it on the CPU only).
 //
 // One lane per particle, particles in tree order (neighbouring lanes visit the
-// same nodes).  Each lane walks the tree depth-first with the reference's LIFO
+// same nodes); or one lane per caller-given point, nobody left out (the
+// reference's accel_pot with self = Octree::kNoSelf, which its ICs generator
+// samples the potential with, src/hernquist_ics.cpp:206-330).  Each lane walks the tree depth-first with the reference's LIFO
 // order (children last to first) without a stack: `skip` is the node the LIFO
 // walk pops after a subtree.  Every opening test, multipole and body
 // interaction is the reference's FP64 expression in its operation order, with
@@ -35,20 +37,24 @@ __global__ __launch_bounds__(kBlock) void tree_gravity(
     const double *__restrict__ tpos, const double *__restrict__ tmass,
     const uint32_t *__restrict__ order, uint32_t n, const unsigned long long *__restrict__ frame,
     uint32_t node_cap, double G, double eps2, double inv_theta, int quadrupole,
-    double *__restrict__ acc, double *__restrict__ pot, unsigned long long *__restrict__ stats) {
+    const double *__restrict__ points, uint32_t n_points, double *__restrict__ acc,
+    double *__restrict__ pot, unsigned long long *__restrict__ stats) {
   __shared__ unsigned long long masks[kBlock / 64][kMaxLevels];
   const uint32_t nn = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = k < n;
+  // targets: the particles in tree order (each leaves itself out), or, with
+  // points, the points in the given order (nobody left out: self = kNone)
+  const bool valid = k < (points ? n_points : n);
   unsigned long long ncell = 0, npart = 0;
-  uint32_t self = 0;
+  uint32_t self = kNone;
   double x = 0, y = 0, z = 0;
   if (valid) {
-    self = order[k];
-    x = tpos[3 * static_cast<size_t>(k)];
-    y = tpos[3 * static_cast<size_t>(k) + 1];
-    z = tpos[3 * static_cast<size_t>(k) + 2];
+    const double *t = points ? points + 3 * static_cast<size_t>(k) : tpos + 3 * static_cast<size_t>(k);
+    if (!points) self = order[k];
+    x = t[0];
+    y = t[1];
+    z = t[2];
   }
   double ax = 0, ay = 0, az = 0, phi = 0;
   const unsigned long long me = 1ull << lane;
@@ -136,10 +142,11 @@ __global__ __launch_bounds__(kBlock) void tree_gravity(
   }
   if (broken) phi = __builtin_nan("");
   if (valid) {
-    acc[3 * static_cast<size_t>(self)] = ax;
-    acc[3 * static_cast<size_t>(self) + 1] = ay;
-    acc[3 * static_cast<size_t>(self) + 2] = az;
-    pot[self] = phi;
+    const size_t o = points ? k : self;  // particles: in the caller's order
+    acc[3 * o] = ax;
+    acc[3 * o + 1] = ay;
+    acc[3 * o + 2] = az;
+    pot[o] = phi;
   }
   for (int off = 32; off > 0; off >>= 1) {
     ncell += __shfl_xor(ncell, off, 64);
@@ -159,12 +166,14 @@ void launch_tree_gravity(const double *geom, const int4 *lnk, const uint32_t *sk
                          const uint32_t *depth, const double *tpos, const double *tmass,
                          const uint32_t *order, uint32_t n, const unsigned long long *frame,
                          uint32_t node_cap, double G, double eps, double theta, int quadrupole,
-                         double *acc, double *pot, unsigned long long *stats, hipStream_t s) {
+                         double *acc, double *pot, unsigned long long *stats, hipStream_t s,
+                         const double *points, uint32_t n_points) {
   launch_fill_u64(stats, 2, 0ull, s);
-  if (!n) return;
-  hipLaunchKernelGGL(tree_gravity, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, geom, lnk,
-                     skip, depth, tpos, tmass, order, n, frame, node_cap, G, eps * eps,
-                     1.0 / theta, quadrupole, acc, pot, stats);
+  const uint32_t targets = points ? n_points : n;
+  if (!n || !targets) return;
+  hipLaunchKernelGGL(tree_gravity, dim3((targets + kBlock - 1) / kBlock), dim3(kBlock), 0, s, geom,
+                     lnk, skip, depth, tpos, tmass, order, n, frame, node_cap, G, eps * eps,
+                     1.0 / theta, quadrupole, points, n_points, acc, pot, stats);
   GTH_LAUNCH_CHECK();
 }
 

@@ -95,6 +95,9 @@ _state_diagnostics = _sig("gt_hip_state_diagnostics", C.c_int,
                           [_vp, C.c_double, C.c_uint64, _vp])
 _tree_gravity = _sig("gt_hip_tree_gravity", C.c_int,
                      [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, _vp, _vp, _vp])
+_tree_gravity_at = _sig("gt_hip_tree_gravity_at", C.c_int,
+                        [_vp, _P(GroupConfig), C.c_uint64, _vp, _vp, C.c_uint64, _vp, _vp, _vp,
+                         _vp])
 _neighbours = _sig("gt_hip_neighbours", C.c_int,
                    [_vp, C.c_int, C.c_uint64, _vp, _vp, C.c_uint64, _vp, _vp, C.c_uint64,
                     C.c_uint64, _vp, _vp, _vp, _vp])
@@ -266,6 +269,22 @@ class Context:
         c = make_config(**cfg)
         self._check(_tree_gravity(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), _ptr(acc),
                                   _ptr(pot), _ptr(st)))
+        return acc, pot, st
+
+    def tree_gravity_at(self, pos, mass, points, **cfg):
+        """Octree::accel_pot at points (nobody left out) on the tree of the
+        particles, FP64: (acc, pot, [cells, particles])."""
+        n = len(mass)
+        pos = _f64(pos, (n, 3))
+        mass = _f64(mass)
+        points = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        npnt = len(points)
+        acc = np.empty((npnt, 3))
+        pot = np.empty(npnt)
+        st = np.zeros(2, np.uint64)
+        c = make_config(**cfg)
+        self._check(_tree_gravity_at(self._h, C.byref(c), n, _ptr(pos), _ptr(mass), npnt,
+                                     _ptr(points), _ptr(acc), _ptr(pot), _ptr(st)))
         return acc, pot, st
 
     def neighbours(self, pos, centres, radii, mass=None, leaf_size=8, max_out=None):

@@ -18,7 +18,10 @@ class Model(C.Structure):
                 ("bulge_mass", C.c_double), ("bulge_a", C.c_double), ("bulge_rmax", C.c_double),
                 ("halo_n", C.c_uint64), ("halo_mass", C.c_double), ("halo_core", C.c_double),
                 ("halo_cutoff", C.c_double), ("halo_rmax", C.c_double),
-                ("halo_profile", C.c_int), ("halo_a", C.c_double), ("seed", C.c_uint64)]
+                ("halo_profile", C.c_int), ("halo_a", C.c_double), ("seed", C.c_uint64),
+                ("vesc_fraction", C.c_double), ("preserve_dispersion", C.c_int),
+                ("disk_sigma_z_jeans", C.c_int), ("eps", C.c_double), ("theta", C.c_double),
+                ("quadrupole", C.c_int)]
 
 
 _vp = C.c_void_p
@@ -27,6 +30,15 @@ _lib.gt_ics_galaxy.restype = C.c_uint64
 _lib.gt_ics_galaxy.argtypes = [C.POINTER(Model), C.c_double, C.c_double, _vp, _vp, _vp, _vp]
 _lib.gt_ics_encounter.restype = C.c_uint64
 _lib.gt_ics_encounter.argtypes = [C.POINTER(Model)] + [C.c_double] * 6 + [_vp] * 4
+# int tree(user, model, n, pos, mass, np, points, acc, pot) (gt_ics.h gt_ics_tree_fn)
+TREE_FN = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(Model), C.c_uint64, _vp, _vp, C.c_uint64, _vp, _vp,
+                      _vp)
+_lib.gt_ics_galaxy_measured.restype = C.c_uint64
+_lib.gt_ics_galaxy_measured.argtypes = [C.POINTER(Model), C.c_double, C.c_double, TREE_FN, _vp,
+                                        _vp, _vp, _vp, _vp]
+_lib.gt_ics_encounter_measured.restype = C.c_uint64
+_lib.gt_ics_encounter_measured.argtypes = ([C.POINTER(Model)] + [C.c_double] * 6
+                                           + [TREE_FN, _vp] + [_vp] * 4)
 _lib.gt_ics_bench_sample.restype = C.c_uint64
 _lib.gt_ics_bench_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _vp]
 
@@ -53,19 +65,77 @@ def _p(a):
     return a.ctypes.data_as(_vp)
 
 
-def galaxy(m, incl=0.0, omega=0.0):
+def gpu_tree(ctx, leaf_size=8):
+    """The velocity stage's tree solver on the GPU (a gt_hip.Context): the
+    reference's FP64 tree walk (gt_hip_tree_gravity, gt_hip_tree_gravity_at),
+    bit for bit its compute_gravity_tree and Octree::accel_pot."""
+    def solve(m, pos, mass, points):
+        cfg = dict(G=m.G, theta=m.theta, eps=m.eps, quadrupole=bool(m.quadrupole),
+                   leaf_size=leaf_size)
+        if points is None:
+            acc, pot, _ = ctx.tree_gravity(pos, mass, **cfg)
+        else:
+            acc, pot, _ = ctx.tree_gravity_at(pos, mass, points, **cfg)
+        return acc, pot
+    return solve
+
+
+def _callback(solve, errors):
+    """A gt_ics_tree_fn around solve(model, pos, mass, points or None) -> (acc, pot)."""
+    def fn(user, mp, n, pos_p, mass_p, npts, pts_p, acc_p, pot_p):
+        try:
+            m = mp.contents
+            pos = np.ctypeslib.as_array(C.cast(pos_p, C.POINTER(C.c_double)), (n, 3))
+            mass = np.ctypeslib.as_array(C.cast(mass_p, C.POINTER(C.c_double)), (n,))
+            rows = npts if npts else n
+            points = (np.ctypeslib.as_array(C.cast(pts_p, C.POINTER(C.c_double)), (npts, 3))
+                      if npts else None)
+            acc, pot = solve(m, pos.copy(), mass.copy(), None if points is None else points.copy())
+            np.ctypeslib.as_array(C.cast(acc_p, C.POINTER(C.c_double)), (rows, 3))[:] = acc
+            np.ctypeslib.as_array(C.cast(pot_p, C.POINTER(C.c_double)), (rows,))[:] = pot
+            return 0
+        except Exception as e:  # reported after the library returns
+            errors.append(e)
+            return 1
+    return TREE_FN(fn)
+
+
+def galaxy(m, incl=0.0, omega=0.0, tree=None):
+    """make_hernquist_galaxy + orient.  tree (e.g. gpu_tree(ctx)): velocities
+    measured as the reference measures them (its velocities bit for bit with its
+    tree solver); None: the analytic velocities."""
     n = int(m.disk_n + m.bulge_n + m.halo_n)
     pos, vel, mass, typ = _arrays(n)
-    got = _lib.gt_ics_galaxy(C.byref(m), incl, omega, _p(pos), _p(vel), _p(mass), _p(typ))
+    if tree is None:
+        got = _lib.gt_ics_galaxy(C.byref(m), incl, omega, _p(pos), _p(vel), _p(mass), _p(typ))
+    else:
+        errors = []
+        cb = _callback(tree, errors)
+        got = _lib.gt_ics_galaxy_measured(C.byref(m), incl, omega, cb, None, _p(pos), _p(vel),
+                                          _p(mass), _p(typ))
+        if errors:
+            raise errors[0]
     assert got == n
     return pos, vel, mass, typ
 
 
-def encounter(m, g1_incl=0.0, g1_omega=0.0, g2_incl=60.0, g2_omega=30.0, rp=2.0, r0=30.0):
+def encounter(m, g1_incl=0.0, g1_omega=0.0, g2_incl=60.0, g2_omega=30.0, rp=2.0, r0=30.0,
+              tree=None):
+    """The encounter mode of galactic-tides-ics (galaxy 2 with seed + 1);
+    tree as for galaxy()."""
     n = 2 * int(m.disk_n + m.bulge_n + m.halo_n)
     pos, vel, mass, typ = _arrays(n)
-    got = _lib.gt_ics_encounter(C.byref(m), g1_incl, g1_omega, g2_incl, g2_omega, rp, r0,
-                                _p(pos), _p(vel), _p(mass), _p(typ))
+    if tree is None:
+        got = _lib.gt_ics_encounter(C.byref(m), g1_incl, g1_omega, g2_incl, g2_omega, rp, r0,
+                                    _p(pos), _p(vel), _p(mass), _p(typ))
+    else:
+        errors = []
+        cb = _callback(tree, errors)
+        got = _lib.gt_ics_encounter_measured(C.byref(m), g1_incl, g1_omega, g2_incl, g2_omega,
+                                             rp, r0, cb, None, _p(pos), _p(vel), _p(mass),
+                                             _p(typ))
+        if errors:
+            raise errors[0]
     if got != n:
         raise ValueError("encounter: start separation below pericentre")
     return pos, vel, mass, typ

@@ -186,6 +186,14 @@ int gt_hip_group_compute(gt_hip_context *ctx, const gt_group_config *cfg, uint64
 int gt_hip_tree_gravity(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
                         const double *pos, const double *mass, double *acc, double *pot,
                         uint64_t *stats);
+/* Octree::accel_pot (src/octree.cpp:275-344) at np points with nobody left out
+ * (self = Octree::kNoSelf) on the tree of the n particles, built and walked
+ * as gt_hip_tree_gravity does: acc np x 3 and pot np, bit for bit the
+ * reference's.  The reference's ICs generator samples the realised potential
+ * this way (src/hernquist_ics.cpp:206-330).  stats as for gt_hip_tree_gravity. */
+int gt_hip_tree_gravity_at(gt_hip_context *ctx, const gt_group_config *cfg, uint64_t n,
+                           const double *pos, const double *mass, uint64_t np,
+                           const double *points, double *acc, double *pot, uint64_t *stats);
 /* Octree::build (src/octree.cpp:169-273) of the n particles with leaf_size, then
  * Octree::for_each_within (include/galactic_tides/octree.h:73-75,103-127) for nq
  * queries at once, on the GPU.  Query q has centre centres[3q .. 3q+2] and radius

@@ -660,6 +660,19 @@ void gto_tree_gravity(gto_run *r, uint64_t n, const double *pos, const double *m
   }
 }
 
+// Octree::accel_pot (src/octree.cpp:275-344) at np points with nobody left out
+// (self = Octree::kNoSelf) on the last built tree.
+void gto_accel_at(const gto_run *r, uint64_t np, const double *points, double G, double theta,
+                  double eps, int quadrupole, double *acc, double *pot) {
+  const long nl = static_cast<long>(np);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long k = 0; k < nl; ++k) {
+    uint64_t c = 0, p = 0;
+    tree_accel(r->tree, points + 3 * k, UINT64_MAX, G, theta, eps, quadrupole != 0, acc + 3 * k,
+               pot[k], c, p);
+  }
+}
+
 void gto_direct_f64(uint64_t n, const double *pos, const double *mass, double G, double eps,
                     double *acc, double *pot) {
   const long nl = static_cast<long>(n);

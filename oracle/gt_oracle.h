@@ -49,6 +49,9 @@ void gto_eval(const gto_run *r, double G, double eps, int quadrupole, float *out
 void gto_tree_gravity(gto_run *r, uint64_t n, const double *pos, const double *mass, double G,
                       double theta, double eps, int quadrupole, int leaf_size, double *acc,
                       double *pot, uint64_t *stats);
+/* Octree::accel_pot at np points, nobody left out, on the last built tree. */
+void gto_accel_at(const gto_run *r, uint64_t np, const double *points, double G, double theta,
+                  double eps, int quadrupole, double *acc, double *pot);
 
 /* compute_gravity_direct (src/gravity.cpp:28-45), FP64. */
 void gto_direct_f64(uint64_t n, const double *pos, const double *mass, double G, double eps,

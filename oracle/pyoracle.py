@@ -39,6 +39,7 @@ _SIGS = {
     "lists_get": (None, [_vp, _u32, _u32, _u32, _fp, _fp, _dp]),
     "eval": (None, [_vp, _d, _d, _i, _fp]),
     "tree_gravity": (None, [_vp, _u64t, _dp, _dp, _d, _d, _d, _i, _i, _dp, _dp, _u64]),
+    "accel_at": (None, [_vp, _u64t, _dp, _d, _d, _d, _i, _dp, _dp]),
     "direct_f64": (None, [_u64t, _dp, _dp, _d, _d, _dp, _dp]),
     "direct_f64_sample": (None, [_u64t, _dp, _dp, _d, _d, _u64t, _u64, _dp]),
     "direct_f32": (None, [_u64t, _fp, _d, _d, _fp]),
@@ -152,6 +153,14 @@ class Checker:
                            np.ascontiguousarray(mass, np.float64), G, theta, eps,
                            int(quadrupole), leaf_size, acc, pot, st)
         return acc, pot, st
+
+    def accel_at(self, points, G=1.0, theta=0.6, eps=0.05, quadrupole=True):
+        """Octree::accel_pot at points (nobody left out) on the last built tree."""
+        points = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        acc = np.zeros((len(points), 3))
+        pot = np.zeros(len(points))
+        self._accel_at(self.h, len(points), points, G, theta, eps, int(quadrupole), acc, pot)
+        return acc, pot
 
     def direct_f64(self, pos, mass, G=1.0, eps=0.05):
         n = len(mass)

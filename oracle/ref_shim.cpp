@@ -187,6 +187,21 @@ void gtref_tree_gravity(gtref_run *r, uint64_t n, const double *pos, const doubl
   }
 }
 
+void gtref_accel_at(const gtref_run *r, uint64_t np, const double *points, double G,
+                    double theta, double eps, int quadrupole, double *acc, double *pot) {
+  const GravityParams gp = gparams(G, theta, eps, quadrupole);
+  for (uint64_t k = 0; k < np; ++k) {
+    Vec3 a;
+    double phi;
+    r->tree.accel_pot(Vec3(points[3 * k], points[3 * k + 1], points[3 * k + 2]),
+                      Octree::kNoSelf, gp, a, phi);
+    acc[3 * k] = a.x;
+    acc[3 * k + 1] = a.y;
+    acc[3 * k + 2] = a.z;
+    pot[k] = phi;
+  }
+}
+
 void gtref_direct_f64(uint64_t n, const double *pos, const double *mass, double G, double eps,
                       double *acc, double *pot) {
   Particles p = make_particles(n, pos, nullptr, mass);

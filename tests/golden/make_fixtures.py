@@ -169,10 +169,34 @@ def write_encounter2m(ref):
     return c
 
 
+def add_velocities(ref, configs):
+    """Digests of make_hernquist_galaxy's velocities (src/hernquist_ics.cpp:206-388,
+    measured with the reference's tree at eps 0.025, theta 0.75) for the
+    galaxies and the small encounter."""
+    for name in ("galaxy50k", "galaxy200k"):
+        md = configs[name]["model"]
+        pos, vel, mass, typ = ref_galaxy(ref, md["disk_n"], md["bulge_n"], md["halo_n"], md["seed"])
+        assert digest(pos) == configs[name]["positions"]
+        configs[name]["velocities"] = digest(vel)
+    md = configs["encounter20k"]["model"]
+    pos, vel, mass, typ = ref_encounter(ref, md["disk_n"], md["bulge_n"], md["halo_n"], md["seed"])
+    assert digest(pos) == configs["encounter20k"]["positions"]
+    configs["encounter20k"]["velocities"] = digest(vel)
+
+
 def main():
     ref = Checker("ref")
     if sys.argv[1:] == ["--only", "neighbours"]:
         write_neighbours(ref)
+        return
+    if sys.argv[1:] == ["--only", "velocities"]:
+        path = os.path.join(HERE, "configs.json")
+        with open(path) as f:
+            configs = json.load(f)
+        add_velocities(ref, configs)
+        with open(path, "w") as f:
+            json.dump(configs, f, indent=1, sort_keys=True)
+        print("wrote velocity digests into configs.json")
         return
     if sys.argv[1:] == ["--only", "encounter2m"]:
         path = os.path.join(HERE, "configs.json")
@@ -239,6 +263,7 @@ def main():
     p, v, a, phi = ref.kdk_group(pos, np.zeros_like(pos), mass, 10, 0.01, 1.0, 0.75, 0.025, True,
                                  32, 8, True)
     configs["encounter2m"] = write_encounter2m(ref)
+    add_velocities(ref, configs)
     configs["kdk50k"] = {"steps": 10, "dt": 0.01, "pos": digest(p), "vel": digest(v),
                          "acc": digest(a), "pot": digest(phi)}
     with open(os.path.join(HERE, "configs.json"), "w") as f:

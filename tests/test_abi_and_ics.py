@@ -98,3 +98,53 @@ def test_ics_component_order_and_centre_of_mass():
     cv = (mass[:, None] * vel).sum(0) / mass.sum()
     assert np.abs(com).max() < 1e-12 and np.abs(cv).max() < 1e-12
     assert np.isclose(mass.sum(), m.disk_mass + m.bulge_mass + m.halo_mass)
+
+
+def oracle_tree(chk):
+    """The velocity stage's tree solver from a CPU checker (compute_gravity_tree
+    and Octree::accel_pot at points, restated; pinned to the reference)."""
+    def solve(m, pos, mass, points):
+        if points is None:
+            acc, pot, _ = chk.tree_gravity(pos, mass, m.G, m.theta, m.eps, bool(m.quadrupole), 8)
+            return acc, pot
+        chk.build(pos, mass, 8)
+        return chk.accel_at(points, m.G, m.theta, m.eps, bool(m.quadrupole))
+    return solve
+
+
+@pytest.mark.parametrize("name", ["galaxy50k", "galaxy200k", "encounter20k"])
+def test_measured_velocities_equal_the_reference(oracle, name):
+    """The reference's velocity stage (src/hernquist_ics.cpp:206-388) with the
+    oracle's tree: make_hernquist_galaxy's velocities bit for bit (digests
+    written from the reference build), eps 0.025 and theta 0.75 as the
+    configurations' ICs measure their potential."""
+    import gt_ics
+    with open(os.path.join(GOLDEN, "configs.json")) as f:
+        c = json.load(f)[name]
+    m = gt_ics.model(bulge_mass=0.333, bulge_rmax=2.0, eps=0.025, theta=0.75, **c["model"])
+    if name.startswith("encounter"):
+        pos, vel, mass, typ = gt_ics.encounter(m, tree=oracle_tree(oracle))
+    else:
+        pos, vel, mass, typ = gt_ics.galaxy(m, tree=oracle_tree(oracle))
+    assert _digest(pos) == c["positions"] and _digest(mass) == c["masses"]
+    assert _digest(vel) == c["velocities"]
+
+
+def test_measured_velocities_small_against_the_reference_build(ref):
+    """Every particle type's velocities against the reference's generator
+    itself, and a tree callback that fails is reported."""
+    import sys
+    import gt_ics
+    sys.path.insert(0, GOLDEN)
+    from make_fixtures import ref_galaxy
+    rp, rv, rm, rt = ref_galaxy(ref, 1200, 300, 1500, 9)
+    m = gt_ics.model(disk_n=1200, bulge_n=300, halo_n=1500, bulge_mass=0.333, bulge_rmax=2.0,
+                     seed=9, eps=0.025, theta=0.75)
+    p, v, ms, t = gt_ics.galaxy(m, tree=oracle_tree(ref))
+    assert np.array_equal(p, rp) and np.array_equal(v.view(np.uint64), rv.view(np.uint64))
+    assert np.array_equal(t, rt)
+
+    def broken(m, pos, mass, points):
+        raise RuntimeError("tree unavailable")
+    with pytest.raises(RuntimeError, match="tree unavailable"):
+        gt_ics.galaxy(m, tree=broken)
