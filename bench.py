@@ -31,20 +31,26 @@ HBM_PEAK_GBS = 8000.0     # MI355X published HBM3E bandwidth
 CELL_FLOPS, BODY_FLOPS = 71, 21  # per interaction, as written in the reference's kernels
 
 
-def encounter_2m(halo="hernquist"):
+def encounter_2m(halo="hernquist", ctx=None):
+    """The encounter's ICs as galactic-tides-ics makes them (eps 0.025 and
+    theta 0.75 measure the realised potential); with a GPU context the
+    velocities are measured with its tree (the reference's velocities),
+    without one they follow the analytic recipe (positions are the same)."""
     import gt_ics
     m = gt_ics.model(disk_n=400_000, bulge_n=100_000, halo_n=500_000, bulge_mass=0.333,
                      bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
-                     halo_profile=halo, halo_a=4.0, seed=3)
-    return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
+                     halo_profile=halo, halo_a=4.0, seed=3, eps=0.025, theta=0.75)
+    tree = gt_ics.gpu_tree(ctx) if ctx is not None else None
+    return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0, tree=tree)
 
 
-def galaxy_4m(halo="hernquist"):
+def galaxy_4m(halo="hernquist", ctx=None):
     import gt_ics
     m = gt_ics.model(disk_n=1_600_000, bulge_n=400_000, halo_n=2_000_000, bulge_mass=0.333,
                      bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
-                     halo_profile=halo, halo_a=4.0, seed=6)
-    pos, vel, mass, _ = gt_ics.galaxy(m)
+                     halo_profile=halo, halo_a=4.0, seed=6, eps=0.025, theta=0.75)
+    tree = gt_ics.gpu_tree(ctx) if ctx is not None else None
+    pos, vel, mass, _ = gt_ics.galaxy(m, tree=tree)
     return pos, vel, mass
 
 
@@ -377,10 +383,10 @@ def bench_plain_vs_full(args):
     All three give bit-identical states (checked here).  Value = plain / full."""
     import gt_hip
     out = {}
-    for name, make in (("encounter-2M", lambda: encounter_2m(args.halo)[:3]),
-                       ("galaxy-4M", lambda: galaxy_4m(args.halo))):
-        pos, vel, mass = make()
+    for name, make in (("encounter-2M", lambda c: encounter_2m(args.halo, c)[:3]),
+                       ("galaxy-4M", lambda c: galaxy_4m(args.halo, c))):
         ctx = gt_hip.Context(0)
+        pos, vel, mass = make(ctx)
         base = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                     pack=1)
         modes = {"plain": (dict(base, predict=2, eval_order=1, split_len=-1, persistent=0), False),
@@ -510,20 +516,19 @@ def main():
                pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
                eval_order=args.eval_order, hilo=args.hilo, predict=args.predict,
                walk_lanes=args.walk_lanes, overlap=args.overlap)
+    ctx = gt_hip.Context(0)
+    print(f"# device: {ctx.device_name}", file=sys.stderr, flush=True)
     t0 = time.time()
     offsets = None
     if args.workload == "batch-128":
         import gt_ics
         offsets, pos, vel, mass = gt_ics.batch_of_galaxies(halo=args.halo)
     elif args.workload == "galaxy-4m":
-        pos, vel, mass = galaxy_4m(args.halo)
+        pos, vel, mass = galaxy_4m(args.halo, ctx)
     else:
-        pos, vel, mass, _ = encounter_2m(args.halo)
+        pos, vel, mass, _ = encounter_2m(args.halo, ctx)
     n = len(mass)
     print(f"# ICs: {n} particles in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
-
-    ctx = gt_hip.Context(0)
-    print(f"# device: {ctx.device_name}", file=sys.stderr, flush=True)
     tuning = None
     if not args.no_autotune and args.workload == "encounter-2m" and not args.exact:
         # Only settings that leave the lists alone (group_size stays 32).
@@ -614,8 +619,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (lists, evaluation); fp64 (tree, walk, kick-drift)",
-        "data": "synthetic: Hernquist galaxies generated in-process (positions identical to "
-                "the reference ICs generator)",
+        "data": "synthetic: Hernquist galaxies generated in-process as galactic-tides-ics "
+                "generates them (positions; velocities measured with the GPU tree, bit for bit "
+                "the reference generator's with its halo; batch-128: analytic velocities)",
         "config": {"workload": {"encounter-2m": "encounter-2M", "batch-128": "batch-128-galaxies",
                                 "galaxy-4m": "galaxy-4M"}[args.workload],
                    "systems": 1 if offsets is None else len(offsets) - 1,

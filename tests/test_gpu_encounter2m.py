@@ -45,12 +45,15 @@ def golden():
     return c, np.load(os.path.join(GOLDEN, "encounter2m.npz"))
 
 
-def encounter(halo):
+def encounter(halo, hip=None):
+    """The encounter as galactic-tides-ics makes it; with hip the velocities are
+    measured with the GPU tree (the reference's), else analytic."""
     import gt_ics
     m = gt_ics.model(disk_n=400_000, bulge_n=100_000, halo_n=500_000, bulge_mass=0.333,
                      bulge_a=0.2, bulge_rmax=2.0, halo_mass=5.8, halo_rmax=30.0,
-                     halo_profile=halo, halo_a=4.0, seed=3)
-    return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0)
+                     halo_profile=halo, halo_a=4.0, seed=3, eps=0.025, theta=0.75)
+    tree = gt_ics.gpu_tree(hip) if hip is not None else None
+    return gt_ics.encounter(m, 0.0, 0.0, 60.0, 30.0, 2.0, 30.0, tree=tree)
 
 
 @pytest.fixture(scope="module")
@@ -60,8 +63,8 @@ def ref_halo():
 
 
 @pytest.fixture(scope="module")
-def hernquist_halo():
-    pos, vel, mass, _ = encounter("hernquist")
+def hernquist_halo(hip):
+    pos, vel, mass, _ = encounter("hernquist", hip)
     return pos, vel, mass
 
 
