@@ -751,56 +751,43 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   g[14] = __dsqrt_rn(ex * ex + ey * ey + ez * ez);
 }
 
-// Levels d_top .. d_top + k - 1 (k <= kBandLevels) in one launch: workgroup w
-// takes an equal share of level d_top's nodes, finds the index ranges of
-// their children and grandchildren (children are consecutive on the level
-// below, so the children of consecutive nodes are one range: the smallest
-// first child to the largest last child of the range's inner nodes), then
-// computes the moments from the band's deepest level up, with a barrier
-// between levels.  Every node in the band is done by the workgroup that owns
-// its band-top ancestor, so a workgroup only reads rows its own threads wrote
-// (or earlier launches did): one launch instead of three, and node_moments as
-// in the level launches, so the moments are the same bits.
-constexpr int kBandLevels = 3;
-
-__global__ __launch_bounds__(kBlock) void band_moments(int d_top, int k,
-                                                       const uint32_t *__restrict__ lstart,
-                                                       const unsigned long long *__restrict__ frame,
-                                                       uint32_t node_cap,
-                                                       const int4 *__restrict__ lnk,
-                                                       const double *__restrict__ tpos,
-                                                       const double *__restrict__ tmass,
-                                                       double *__restrict__ geom) {
-  __shared__ uint32_t lo[kBandLevels], hi[kBandLevels];
+// Every node of level d: lstart[d] .. lstart[d + 1] (empty for levels past the
+// deepest, whose lstart entries are 0).
+__global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *__restrict__ lstart,
+                                                        const unsigned long long *__restrict__ frame,
+                                                        uint32_t node_cap,
+                                                        const int4 *__restrict__ lnk,
+                                                        const double *__restrict__ tpos,
+                                                        const double *__restrict__ tmass,
+                                                        double *__restrict__ geom) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t f = min(lstart[d_top], n_nodes), l = min(lstart[d_top + 1], n_nodes);
-  const uint32_t count = l > f ? l - f : 0u;
-  const uint32_t per = (count + gridDim.x - 1) / gridDim.x;
-  if (threadIdx.x == 0) {
-    lo[0] = min(f + blockIdx.x * per, l);
-    hi[0] = min(lo[0] + per, l);
-    for (int j = 1; j < kBandLevels; ++j) {
-      lo[j] = 0xffffffffu;
-      hi[j] = 0u;
-    }
-  }
-  __syncthreads();
-  for (int j = 1; j < k; ++j) {  // the children of level j - 1's range
-    const uint32_t b = min(hi[j - 1], n_nodes);  // (malformed links stay in bounds)
-    for (uint32_t x = lo[j - 1] + threadIdx.x; x < b; x += kBlock) {
+  const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
+  for (uint32_t x = first + blockIdx.x * kBlock + threadIdx.x; x < last; x += gridDim.x * kBlock)
+    node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+}
+
+// Levels d and d - 1 in one launch: eight lanes per node of level d - 1, lane
+// j doing the node's child j (on level d), then, after a barrier, the node's
+// first lane the node itself.  For small levels, where a launch costs more
+// than the level's work.
+__global__ __launch_bounds__(kBlock) void level_pair_moments(
+    int d, const uint32_t *__restrict__ lstart, const unsigned long long *__restrict__ frame,
+    uint32_t node_cap, const int4 *__restrict__ lnk, const double *__restrict__ tpos,
+    const double *__restrict__ tmass, double *__restrict__ geom) {
+  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
+  const uint32_t first = lstart[d - 1], last = min(lstart[d], n_nodes);
+  const uint32_t lanes = last > first ? 8 * (last - first) : 0u;
+  // every thread of a workgroup takes the same number of rounds (barriers)
+  for (uint32_t base = blockIdx.x * kBlock; base < lanes; base += gridDim.x * kBlock) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t x = first + t / 8, j = t % 8;
+    if (t < lanes) {
       const int4 L = lnk[x];
-      if (L.y > 0 && L.x >= 0) {
-        atomicMin(&lo[j], static_cast<uint32_t>(L.x));
-        atomicMax(&hi[j], static_cast<uint32_t>(L.x + L.y));
-      }
+      if (L.x >= 0 && j < static_cast<uint32_t>(L.y) && static_cast<uint32_t>(L.x) + j < n_nodes)
+        node_moments(static_cast<uint32_t>(L.x) + j, n_nodes, lnk, tpos, tmass, geom);
     }
     __syncthreads();
-  }
-  for (int j = k - 1; j >= 0; --j) {
-    const uint32_t b = min(hi[j], n_nodes);
-    for (uint32_t x = lo[j] + threadIdx.x; x < b; x += kBlock)
-      node_moments(x, n_nodes, lnk, tpos, tmass, geom);
-    __syncthreads();
+    if (t < lanes && j == 0) node_moments(x, n_nodes, lnk, tpos, tmass, geom);
   }
 }
 
@@ -1001,14 +988,15 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan
-  // allows (a deeper tree is reported by the depth in the frame).  The deep
-  // tail of levels that the previous tree had small (with the empty levels the
-  // plan allows below the deepest) goes to one workgroup, level after level;
-  // the levels above go in bands of three levels per launch (band_moments),
-  // each band's grid sized from the previous tree's populations of its levels
-  // (a full-size grid of idle workgroups made the moments launch-bound).  At
-  // the 2M encounter's depth 15 that is 6 launches where one launch per level
-  // (pairs for small ones) took 18.
+  // allows (a deeper tree is reported by the depth in the frame).  Each level's
+  // grid is sized from the previous tree's level (the launches stride, so any
+  // grid is correct): most levels are small, and a full-size grid of idle
+  // workgroups per level made the moments launch-bound.  The deep tail of
+  // levels that the previous tree had small (with the empty levels the plan
+  // allows below the deepest) goes to one workgroup, level after level: 5 us
+  // instead of about 19 us for four launches.  (The small top levels are not
+  // done that way: their nodes have eight children each, and one workgroup
+  // took 61 us where five launches take 37 us.)
   const uint32_t mgrid = std::min<uint32_t>(blocks_for(cap), kMomentBlocks);
   auto small = [&p](int d) {
     return p.levels_sized && d < frame::kLevels && p.level_blocks[d] <= kSerialLevelBlocks;
@@ -1020,21 +1008,24 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                        lo + 1, lstart, frame, cap, lnk, tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
   }
-  // The levels above: bands of up to three levels per launch, from the deepest
-  // up; each band's grid from the previous tree's populations of its levels
-  // (about 512 nodes a workgroup).
+  auto paired = [&p](int d) {
+    return p.levels_sized && d < frame::kLevels && p.level_blocks[d] <= kPairLevelBlocks;
+  };
   for (int d = lo; d >= 0;) {
-    const int top = std::max(d - (kBandLevels - 1), 0), k = d - top + 1;
-    uint32_t blocks = mgrid;
-    if (p.levels_sized) {
-      uint32_t sum = 0;
-      for (int j = top; j <= d; ++j) sum += j < frame::kLevels ? p.level_blocks[j] : kMomentBlocks;
-      blocks = std::min<uint32_t>(std::max<uint32_t>(sum / 2, 1u), 2048u);
+    if (d >= 1 && paired(d) && paired(d - 1)) {  // levels d and d - 1 together
+      const uint32_t blocks = std::min<uint32_t>(8u * p.level_blocks[d - 1], 8u * mgrid);
+      hipLaunchKernelGGL(level_pair_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0,
+                         s, d, lstart, frame, cap, lnk, tpos, tmass, geom);
+      GTH_LAUNCH_CHECK();
+      d -= 2;
+      continue;
     }
-    hipLaunchKernelGGL(band_moments, dim3(blocks), dim3(kBlock), 0, s, top, k, lstart, frame, cap,
-                       lnk, tpos, tmass, geom);
+    const uint32_t blocks =
+        p.levels_sized && d < frame::kLevels ? std::min<uint32_t>(p.level_blocks[d], mgrid) : mgrid;
+    hipLaunchKernelGGL(level_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0, s, d,
+                       lstart, frame, cap, lnk, tpos, tmass, geom);
     GTH_LAUNCH_CHECK();
-    d = top - 1;
+    --d;
   }
 }
 

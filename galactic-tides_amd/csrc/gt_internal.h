@@ -205,6 +205,8 @@ constexpr uint16_t kMomentBlocks = 1024;
 // Levels sized at most this many workgroups get no launch of their own: runs
 // of them are done by one workgroup, level after level.
 constexpr uint16_t kSerialLevelBlocks = 2;
+// Two consecutive levels sized at most this many workgroups share a launch.
+constexpr uint16_t kPairLevelBlocks = 64;
 
 // A device count that must not exceed cap; past it (an overflow the status
 // reports) the count reads as zero, so dependent kernels do nothing.
