@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   uint32_t wi;
   if (queue) {
     uint32_t t = 0;
-    if (threadIdx.x % kWave == 0) t = atomicAdd(queue + xcd, 1u);
+    if (threadIdx.x % kWave == 0) t = atomicAdd(queue + xcd * kQueueStride, 1u);
     wi = __builtin_amdgcn_readfirstlane(wi_lo + __builtin_amdgcn_readfirstlane(t));
     if (wi >= wi_hi) break;
   } else {
@@ -627,7 +627,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
   const uint32_t item_cap = parts ? split->item_cap : g_span;
   const uint32_t blocks = (item_cap + waves_per_block - 1) / waves_per_block;
   if (exact || !persistent_blocks) queue = nullptr;
-  if (queue) launch_fill_u32(queue, kXcds, 0u, s);
+  if (queue) launch_fill_u32(queue, kXcds * kQueueStride, 0u, s);
   const dim3 grid(exact ? blocks : queue ? xcd_grid(persistent_blocks) : xcd_grid(blocks)),
       block(kBlock);
   const uint2 *items = parts ? split->items : nullptr;

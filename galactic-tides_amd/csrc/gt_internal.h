@@ -201,6 +201,10 @@ struct Plan {
   bool levels_sized = false;
   uint16_t level_blocks[frame::kLevels] = {};
 };
+// The persistent evaluation's per-XCD item counters sit this many uint32 apart
+// (128 bytes: each on a cache line of its own, so the XCDs' atomics do not
+// contend for one line).
+constexpr uint32_t kQueueStride = 32;
 constexpr uint16_t kMomentBlocks = 1024;
 // Levels sized at most this many workgroups get no launch of their own: runs
 // of them are done by one workgroup, level after level.
