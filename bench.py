@@ -584,11 +584,15 @@ def main():
     # matrix operation to put on MFMA), priced against the published FP32
     # vector peak; its compulsory HBM bytes (the lists, groups, cells, the
     # targets in and the results out) over the same time give the HBM fraction.
+    # With hi/lo positions every cell and body also has a 16 B low part and
+    # every group a 32 B hi/lo centre.
     flops = CELL_FLOPS * inf["cell_interactions"] + BODY_FLOPS * inf["body_interactions"]
     eval_ms = tm["evaluate"]
     achieved = flops / (eval_ms * 1e-3) / 1e12
     compulsory = (4 * (inf["n_cell_list"] + inf["n_body_list"]) + 32 * inf["n_groups"]
                   + 48 * inf["n_nodes"] + 32 * n)
+    if hilo:
+        compulsory += 16 * inf["n_nodes"] + 16 * n + 32 * inf["n_groups"]
     hbm_gbs = compulsory / (eval_ms * 1e-3) / 1e9
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
