@@ -7,6 +7,7 @@
 #include <new>
 #include <random>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gt_build.h"
@@ -164,7 +165,11 @@ struct gt_hip_context {
   DBuf<double> in_pos, in_mass, in_acc, in_pot;
   uint64_t n_state = 0;
   bool state_forces = false;
-  // captured step graphs
+  // captured step graphs: the four phases, and k steps of them chained into
+  // one graph (run_exec[k], made when a run of k steps first needs it)
+  hipGraph_t step_graph[kPhases] = {};
+  hipGraphExec_t run_exec[kReorderSteps + 1] = {};
+  // overlapped steps: the tree and scatter phases as graphs of their own
   hipGraphExec_t step_exec[kPhases] = {};
   // overlapped steps: the lists phase and the evaluation as graphs per walk
   // range, launched on the two streams with the events between them
@@ -1103,6 +1108,11 @@ void destroy_graph(hipGraphExec_t &e) {
 
 void destroy_step_graphs(gt_hip_context *c) {
   for (auto &e : c->step_exec) destroy_graph(e);
+  for (auto &e : c->run_exec) destroy_graph(e);
+  for (auto &g : c->step_graph) {
+    if (g) (void)hipGraphDestroy(g);
+    g = nullptr;
+  }
   destroy_graph(c->ov_prep);
   destroy_graph(c->ov_finish);
   for (hipGraphExec_t *arr : {c->ov_walk, c->ov_items, c->ov_eval, c->ov_comb})
@@ -1111,9 +1121,9 @@ void destroy_step_graphs(gt_hip_context *c) {
   c->step_valid = false;
 }
 
-// What f enqueues on stream st, as an instantiated graph.
+// What f enqueues on stream st, as a graph.
 template <class F>
-hipGraphExec_t capture_graph(hipStream_t st, F &&f) {
+hipGraph_t capture_only(hipStream_t st, F &&f) {
   GTH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   try {
     f();
@@ -1125,6 +1135,13 @@ hipGraphExec_t capture_graph(hipStream_t st, F &&f) {
   }
   hipGraph_t g = nullptr;
   GTH_CHECK(hipStreamEndCapture(st, &g));
+  return g;
+}
+
+// What f enqueues on stream st, as an instantiated graph.
+template <class F>
+hipGraphExec_t capture_graph(hipStream_t st, F &&f) {
+  hipGraph_t g = capture_only(st, std::forward<F>(f));
   hipGraphExec_t e = nullptr;
   const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -1197,8 +1214,11 @@ void prepare_step_graphs(gt_hip_context *c, const gt_group_config *cfg, const Pl
   hipStream_t s = c->stream, b = c->stream2;
   const bool overlap = p.chunks > 1;
   for (int phase = 0; phase < kPhases; ++phase) {
-    if (overlap && (phase == 1 || phase == 2)) continue;
-    c->step_exec[phase] = capture_graph(s, [&] { enqueue_step_phase(c, cfg, p, dt, phase); });
+    if (!overlap)
+      c->step_graph[phase] =
+          capture_only(s, [&] { enqueue_step_phase(c, cfg, p, dt, phase); });
+    else if (phase == 0 || phase == 3)
+      c->step_exec[phase] = capture_graph(s, [&] { enqueue_step_phase(c, cfg, p, dt, phase); });
   }
   if (overlap) {
     const ListsInput in = tree_input(c);
@@ -1231,6 +1251,44 @@ void launch_graph(hipGraphExec_t e, hipStream_t st) {
   if (e) GTH_CHECK(hipGraphLaunch(e, st));
 }
 
+// k steps (k <= kTimedSteps) as one graph: each step's four phase graphs as
+// child graphs in a chain, with the step's timing events recorded between
+// them as event record nodes (events recorded on a capturing stream are not
+// captured, so they are added as nodes).  Made once per k and kept until the
+// phase graphs change.
+hipGraphExec_t run_graph(gt_hip_context *c, int k) {
+  if (k < 1 || k > kReorderSteps || k > kTimedSteps)
+    fail(GT_HIP_ERR_DEVICE, "step graph: run length out of range");
+  if (c->run_exec[k]) return c->run_exec[k];
+  for (hipGraph_t g : c->step_graph)
+    if (!g) fail(GT_HIP_ERR_DEVICE, "step graph: phases not captured");
+  hipGraph_t g = nullptr;
+  GTH_CHECK(hipGraphCreate(&g, 0));
+  hipGraphExec_t e = nullptr;
+  try {
+    hipGraphNode_t prev = nullptr;
+    for (int i = 0; i < k; ++i) {
+      hipEvent_t *ev = &c->step_ev[static_cast<size_t>(i) * (kPhases + 1)];
+      for (int phase = 0;; ++phase) {
+        hipGraphNode_t node = nullptr;
+        GTH_CHECK(hipGraphAddEventRecordNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0,
+                                             ev[phase]));
+        prev = node;
+        if (phase == kPhases) break;
+        GTH_CHECK(hipGraphAddChildGraphNode(&node, g, &prev, 1, c->step_graph[phase]));
+        prev = node;
+      }
+    }
+    GTH_CHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  } catch (...) {
+    (void)hipGraphDestroy(g);
+    throw;
+  }
+  (void)hipGraphDestroy(g);
+  c->run_exec[k] = e;
+  return e;
+}
+
 // The lists phase and the evaluation of an overlapped step from their graphs:
 // each walk range's evaluation on the second stream as soon as the range is
 // walked, with the same events as the launches without graphs.
@@ -1252,6 +1310,18 @@ void launch_overlapped_lists_eval(gt_hip_context *c) {
   GTH_CHECK(hipStreamWaitEvent(s, c->eval_done, 0));
 }
 
+// The graphs for the next run of graph steps (at most nsteps of them): the
+// phases for bounds p and, without overlap, the run's steps as one graph.
+// Returns the run's length: at most kReorderSteps steps between reorders of
+// the state (the graphs stay valid: the state keeps its buffers).
+int prepare_run(gt_hip_context *c, const gt_group_config *cfg, const Plan &p, int nsteps,
+                double dt) {
+  prepare_step_graphs(c, cfg, p, dt);
+  const int k = std::min(nsteps, std::max(1, kReorderSteps - c->steps_in_order));
+  if (!c->ov_chunks) run_graph(c, k);
+  return k;
+}
+
 // nsteps leapfrog steps from captured graphs, launched back to back; a step
 // that outgrows the bounds stops the rest (their kernels see the status), is
 // finished here without graphs, and the remaining steps go on from new graphs.
@@ -1270,20 +1340,21 @@ void graph_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
       continue;
     }
     const Plan p = plan_for(c, n, cfg);
-    prepare_step_graphs(c, cfg, p, dt);
-    // at most kReorderSteps steps between reorders of the state (the graphs
-    // stay valid: the state keeps its buffers)
-    const int k = std::min(nsteps - done_total, std::max(1, kReorderSteps - c->steps_in_order));
+    const int k = prepare_run(c, cfg, p, nsteps - done_total, dt);
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     launch_fill_u64(frame + frame::kStepsDone, 2, 0ull, s);  // and kStepInteractions
-    for (int i = 0; i < k; ++i) {
-      hipEvent_t *ev = i < kTimedSteps ? &c->step_ev[static_cast<size_t>(i) * (kPhases + 1)]
-                                       : nullptr;
-      if (ev) GTH_CHECK(hipEventRecord(ev[0], s));
-      for (int phase = 0; phase < kPhases; ++phase) {
-        if (phase == 1 && c->ov_chunks) launch_overlapped_lists_eval(c);
-        if (c->step_exec[phase]) GTH_CHECK(hipGraphLaunch(c->step_exec[phase], s));
-        if (ev) GTH_CHECK(hipEventRecord(ev[phase + 1], s));
+    if (!c->ov_chunks) {
+      GTH_CHECK(hipGraphLaunch(run_graph(c, k), s));
+    } else {
+      for (int i = 0; i < k; ++i) {
+        hipEvent_t *ev = i < kTimedSteps ? &c->step_ev[static_cast<size_t>(i) * (kPhases + 1)]
+                                         : nullptr;
+        if (ev) GTH_CHECK(hipEventRecord(ev[0], s));
+        for (int phase = 0; phase < kPhases; ++phase) {
+          if (phase == 1) launch_overlapped_lists_eval(c);
+          if (c->step_exec[phase]) GTH_CHECK(hipGraphLaunch(c->step_exec[phase], s));
+          if (ev) GTH_CHECK(hipEventRecord(ev[phase + 1], s));
+        }
       }
     }
     read_frame(c);
@@ -2310,6 +2381,24 @@ int gt_hip_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, doub
     }
     GTH_CHECK(hipStreamSynchronize(c->stream));
     c->state_forces = true;
+  });
+}
+
+int gt_hip_prepare_steps(gt_hip_context *c, const gt_group_config *cfg, int nsteps, double dt) {
+  if (!c) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    check_config(cfg);
+    require(nsteps >= 0, "nsteps must be non-negative");
+    require(c->state_forces, "forces are not current: call gt_hip_prime first");
+    GTH_CHECK(hipSetDevice(c->device));
+    const uint32_t n = static_cast<uint32_t>(c->n_state);
+    if (n == 0 || nsteps == 0) return;
+    c->systems = c->state_systems;
+    c->sys_off = c->state_systems > 1 ? c->state_off.get() : nullptr;
+    // what gt_hip_steps(use_graph = 1) would launch first, made now
+    const bool collecting = cfg->predict == 1 && !c->pred.trained;
+    if (collecting || !sizes_known(c->hist, n, c->systems)) return;
+    prepare_run(c, cfg, plan_for(c, n, cfg), nsteps, dt);
   });
 }
 

@@ -119,6 +119,8 @@ _state_set = _sig("gt_hip_state_set", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp])
 _state_get = _sig("gt_hip_state_get", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _prime = _sig("gt_hip_prime", C.c_int, [_vp, _P(GroupConfig)])
 _steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_double, C.c_int])
+_prepare_steps = _sig("gt_hip_prepare_steps", C.c_int,
+                      [_vp, _P(GroupConfig), C.c_int, C.c_double])
 _pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)])
 _predictor_reset = _sig("gt_hip_predictor_reset", C.c_int, [_vp, C.c_int])
 _predictor_stats = _sig("gt_hip_predictor_stats", C.c_int, [_vp, _P(PredictorStats)])
@@ -484,6 +486,12 @@ class Context:
     def steps(self, nsteps, dt, use_graph=False, **cfg):
         c = make_config(**cfg)
         self._check(_steps(self._h, C.byref(c), nsteps, dt, int(use_graph)))
+
+    def prepare_steps(self, nsteps, dt, **cfg):
+        """The graphs steps(nsteps, dt, use_graph=True, **cfg) launches first,
+        captured and instantiated now (nothing runs)."""
+        c = make_config(**cfg)
+        self._check(_prepare_steps(self._h, C.byref(c), nsteps, dt))
 
     # -- measurement -------------------------------------------------------------------
     def direct_f64_sample(self, pos, mass, sample, G=1.0, eps=0.025):

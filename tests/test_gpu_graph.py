@@ -2,7 +2,8 @@
 one by one: same kernels, same bounds, so the results must agree bit for bit
 (exact and fast evaluation alike), over several calls with host work between
 them, and when a step outgrows its bounds inside a graph and is finished
-without graphs.  The 50k galaxy's 10 exact steps must also reproduce the
+without graphs; prepared ahead, a call's steps run as one graph and capture
+nothing.  The 50k galaxy's 10 exact steps must also reproduce the
 digests of the reference's own LeapfrogKDK (tests/golden/configs.json)."""
 import hashlib
 import json
@@ -52,6 +53,33 @@ def test_graph_steps_equal_eager_steps(hip, oracle, exact, lanes):
     assert after["graph_captures"] > before["graph_captures"] or before["graph_captures"] > 0
     t = hip.timings()
     assert t["evaluate"] > 0 and t["total"] >= t["evaluate"]
+
+
+def test_prepared_steps_run_as_one_graph(hip, oracle):
+    # gt_hip_prepare_steps captures and instantiates what the next graph call
+    # launches: that call then captures nothing, runs its steps as one graph
+    # (events between the phases inside it), and equals eager steps bit for bit.
+    pos, mass = oracle.test_system(1, 30000, 25)
+    rng = np.random.default_rng(5)
+    vel = 0.3 * rng.standard_normal(pos.shape)
+    cfg = dict(theta=0.75, eps=0.01, group_size=32, hilo=True, persistent=1)
+    want = run(hip, pos, vel, mass, [9], False, **cfg)
+    hip.state_set(pos, vel, mass)
+    hip.prime(**cfg)
+    hip.steps(2, 0.01, use_graph=True, **cfg)
+    hip.prepare_steps(7, 0.01, **cfg)
+    before = hip.pipeline_stats()
+    hip.steps(7, 0.01, use_graph=True, **cfg)
+    after = hip.pipeline_stats()
+    got = hip.state_get()
+    assert same_state(got, want)
+    assert after["graph_captures"] == before["graph_captures"]
+    assert after["graph_steps"] - before["graph_steps"] == 7
+    t = hip.timings()
+    phases = t["tree"] + t["lists"] + t["evaluate"] + t["scatter"]
+    print(t, flush=True)
+    assert min(t["tree"], t["lists"], t["evaluate"], t["scatter"]) > 0
+    assert abs(phases - t["total"]) <= 1e-3 * t["total"] + 1e-4
 
 
 def test_graph_stop_and_resume(hip, oracle):
