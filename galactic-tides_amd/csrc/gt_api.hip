@@ -301,7 +301,7 @@ bool plan_holds(const Plan &p, const History &h, uint32_t n, uint32_t systems) {
   if (p.group_cap < n && static_cast<uint64_t>(h.groups) + h.groups / 16 > p.group_cap) return false;
   if (h.most_c + h.most_c / 16 + 8 > p.cap_c || h.most_b + h.most_b / 16 + 8 > p.cap_b)
     return false;
-  if (p.split_len && (!h.items_known || h.items + h.items / 16 > p.item_cap ||
+  if (p.longest_first && (!h.items_known || h.items + h.items / 16 > p.item_cap ||
                       h.split_parts + h.split_parts / 16 > p.split_cap))
     return false;
   if (p.predicted && (!h.pred_known || h.pred_c + h.pred_c / 16 > p.slot_total_c ||
@@ -320,13 +320,18 @@ bool sizes_known(const History &h, uint32_t n, uint32_t systems) {
 bool uses_hilo(const gt_group_config *cfg) { return cfg->hilo != 0 && !cfg->exact; }
 
 constexpr uint32_t kDefaultSplitLen = 8192;
-// Quarter-octave cost classes, longest first: at 2M the evaluation takes
-// 2.45 ms against 2.79 in tree order and 2.46 fully sorted (with less L2
-// miss traffic than the full sort, since a class keeps tree order).
+// One wave per work item: quarter-octave cost classes, longest first (at 2M
+// the evaluation takes 2.45 ms against 2.79 in tree order and 2.46 fully
+// sorted, with less L2 miss traffic than the full sort, since a class keeps
+// tree order).  Persistent grid: tree order, with no work items to make or
+// sort (2.42 ms either way at 2M, since the waves' queue balances the load,
+// and the lists phase is ~70 us shorter).
 constexpr int kDefaultEvalOrder = 4;
+constexpr int kDefaultPersistentOrder = 1;
 
 int eval_order_for(const gt_group_config *cfg) {
-  return cfg->eval_order == 0 ? kDefaultEvalOrder : cfg->eval_order;
+  if (cfg->eval_order != 0) return cfg->eval_order;
+  return cfg->persistent == 1 ? kDefaultPersistentOrder : kDefaultEvalOrder;
 }
 
 bool longest_first_for(const gt_group_config *cfg) {
@@ -392,7 +397,7 @@ Plan bounds_for(gt_hip_context *c, uint32_t n, uint32_t split_len, bool longest_
   p.split_len = split_len;
   p.longest_first = longest_first && split_len != 0;
   p.cost_buckets = p.longest_first ? buckets : 0;
-  if (split_len) {
+  if (p.longest_first) {  // work items
     const bool known = same && h.items_known;
     p.split_cap = known ? grown(h.split_parts, 64, 1u << 26) : p.group_cap / 8 + 64;
     const uint64_t extra = known && h.items > h.groups ? h.items - h.groups : p.split_cap;
@@ -476,7 +481,7 @@ void learn(gt_hip_context *c, const Plan &p, unsigned long long st) {
     h.pred_c = fw[frame::kPredCells];
     h.pred_b = fw[frame::kPredBodies];
   }
-  if ((st & ~static_cast<unsigned long long>(kAbortItems)) == 0 && p.split_len) {
+  if ((st & ~static_cast<unsigned long long>(kAbortItems)) == 0 && p.longest_first) {
     // (with the walk in chunks the frame holds the last chunk's counts: after
     // an overflow the bounds double instead)
     const bool over = st & kAbortItems;
@@ -539,7 +544,7 @@ void reserve_all(gt_hip_context *c, const Plan &p) {
     c->body_list.ensure(std::max<uint64_t>(static_cast<uint64_t>(gc) * p.cap_b, 1));
   }
   c->out.ensure(n);
-  if (p.split_len) {
+  if (p.longest_first) {
     c->parts.ensure(gc);
     c->split_count.ensure(gc);
     c->item_off.ensure(gc);
@@ -755,7 +760,18 @@ EvalSplit split_of(gt_hip_context *c, const Plan &p) {
   return sp;
 }
 
-bool uses_items(const gt_group_config *cfg, const Plan &p) { return p.split_len && !cfg->exact; }
+// Work items (made, and sorted longest first, in the lists phase) for the
+// longest-first orders; in tree order the evaluation cuts long lists into
+// the same parts itself, with the same results.
+bool uses_items(const gt_group_config *cfg, const Plan &p) {
+  return p.longest_first && !cfg->exact;
+}
+
+// Entries per part of the lists that the evaluation cuts itself (0: none).
+uint32_t inline_split_for(const gt_group_config *cfg, const Plan &p) {
+  if (cfg->exact || uses_items(cfg, p) || p.split_len == 0xffffffffu) return 0;
+  return p.split_len;
+}
 
 // Work items of the fast evaluation (long lists in parts, each XCD's share
 // longest first): part of the lists phase.
@@ -789,7 +805,8 @@ void enqueue_eval(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
   launch_eval(c->bodies.get(), c->cells.get(), c->cell_list.get(), c->body_list.get(),
               c->groups.get(), c->frame.get(), p.group_cap, G, eps2, cfg->quadrupole ? 1u : 0u,
               cfg->exact != 0, waves, c->out.get(), s ? s : c->stream, items ? &sp : nullptr,
-              p.hilo ? &hl : nullptr, g0, span, c->eval_queue.get(), pblocks);
+              p.hilo ? &hl : nullptr, g0, span, c->eval_queue.get(), pblocks,
+              inline_split_for(cfg, p));
 }
 
 // Results in tree order: split groups' parts added up.

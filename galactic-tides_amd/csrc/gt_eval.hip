@@ -245,13 +245,16 @@ __device__ inline uint32_t opaque_zero() {
 // the whole list.  Targets sit in LDS and are read as broadcasts; lane k holds
 // list entry base + k.  Targets past nt repeat the last one and are not stored.
 // The list gathers are software-pipelined: the index of chunk c+2 and the
-// entry of chunk c+1 are in flight while chunk c is evaluated.
+// entry of chunk c+1 are in flight while chunk c is evaluated.  accumulate:
+// the batch's sums are added to what dst holds (dst + sums, as combine_parts
+// adds a part to the parts before it) instead of stored.
 template <int TB, bool QUAD, bool HILO>
 __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
                                   const uint32_t *__restrict__ cell_list,
                                   const uint32_t *__restrict__ body_list, const Range &r,
                                   uint32_t tb, uint32_t nt, float G, float eps2, float4 *tgt,
-                                  uint32_t lane, float4 *__restrict__ dst, const HiLo &h) {
+                                  uint32_t lane, float4 *__restrict__ dst, const HiLo &h,
+                                  bool accumulate) {
   if (lane < TB) {
     const uint32_t t = tb + min(lane, nt - 1);
     float4 p = bodies[t];
@@ -352,7 +355,14 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
       rp = sp;
     }
   }
-  if (lane < nt) dst[lane] = make_float4(rx, ry, rz, rp);
+  if (lane < nt) {
+    float4 v = make_float4(rx, ry, rz, rp);
+    if (accumulate) {
+      const float4 o = dst[lane];
+      v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+    }
+    dst[lane] = v;
+  }
   __builtin_amdgcn_wave_barrier();  // targets of the next batch overwrite tgt
 }
 
@@ -365,6 +375,10 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
 // in one part writes its targets' results to out; a part of a split group
 // writes them to its slot of `partial` (kMaxGroup float4 each, slot
 // split_off[group] + part), which combine_parts adds up in part order.
+// Without items a group whose list is longer than split_len entries (0:
+// none) is cut into the same parts, which this wave evaluates one after
+// another, adding each part's sums to the results in part order: the sums
+// combine_parts forms from parts evaluated on different waves, bit for bit.
 template <bool QUAD, int WAVES, bool HILO>
 __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const float4 *__restrict__ bodies, const Cell *__restrict__ cells,
@@ -374,7 +388,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
     const uint32_t *__restrict__ split_off, float4 *__restrict__ partial, float G, float eps2,
     const float4 *__restrict__ bodies_lo, const float4 *__restrict__ cells_lo,
     const float4 *__restrict__ group_hilo, float4 *__restrict__ out, uint32_t group_cap,
-    uint32_t g_begin, uint32_t g_span, uint32_t *__restrict__ queue) {
+    uint32_t g_begin, uint32_t g_span, uint32_t *__restrict__ queue, uint32_t split_len) {
   __shared__ float4 targets[kBlock / kWave][32];
   if (frame[frame::kStatus]) return;  // lists incomplete: evaluate nothing
   // items: this range's work items; else groups g_begin .. g_begin + g_span
@@ -423,7 +437,12 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   const uint32_t lane = threadIdx.x % kWave;
   float4 *tgt = targets[threadIdx.x / kWave];
   const gt_group g = groups[gi];
-  const Range r = part_range(g, part, parts);
+  uint32_t inline_parts = 1;
+  if (!items && split_len) {
+    const uint32_t len = (g.cell_end - g.cell_begin) + (g.body_end - g.body_begin);
+    inline_parts = len > split_len ? min((len + split_len - 1) / split_len, 0xffffu) : 1u;
+    inline_parts = __builtin_amdgcn_readfirstlane(inline_parts);
+  }
   HiLo h = {bodies_lo, cells_lo, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (HILO) {
     const float4 hi = group_hilo[2 * static_cast<size_t>(gi)];
@@ -439,13 +458,15 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
                                           kMaxGroup
                            : out + g.begin;
   const uint32_t ntot = g.end - g.begin;
+  for (uint32_t ip = 0; ip < inline_parts; ++ip) {
+  const Range r = inline_parts > 1 ? part_range(g, ip, inline_parts) : part_range(g, part, parts);
   for (uint32_t t0 = 0; t0 < ntot; t0 += 32) {
     const uint32_t nt = min(32u, ntot - t0);
     const uint32_t tb = g.begin + t0;
     float4 *dst = dst0 + t0;
 #define GTH_BATCH(TB)                                                                      \
   eval_batch<TB, QUAD, HILO>(bodies, cells, cell_list, body_list, r, tb, nt, G, eps2, tgt, lane, \
-                             dst, h)
+                             dst, h, ip > 0)
     // Targets per batch rounded up to a multiple of 4: each path is
     // branch-free over its targets, and at most 3 targets are padding.
     switch ((nt + 3) / 4) {
@@ -459,6 +480,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
       default: GTH_BATCH(32); break;
     }
 #undef GTH_BATCH
+  }
   }
   }
 }
@@ -617,7 +639,8 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  const unsigned long long *frame, uint32_t group_cap, float G, float eps2,
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
                  const EvalSplit *split, const EvalHiLo *hilo, uint32_t g_begin,
-                 uint32_t g_span, uint32_t *queue, uint32_t persistent_blocks) {
+                 uint32_t g_span, uint32_t *queue, uint32_t persistent_blocks,
+                 uint32_t inline_split) {
   if (group_cap == 0) return;
   if (g_span == 0 || g_span > group_cap) g_span = group_cap;
   if (exact && (g_begin != 0 || g_span != group_cap))
@@ -639,7 +662,8 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                *ghl = hl ? hilo->group_hilo : nullptr;
 #define GTH_EVAL_ARGS                                                                \
   bodies, cells, cell_list, body_list, groups, frame, item_cap, items, order, split_off, \
-      partial, G, eps2, blo, clo, ghl, out, group_cap, g_begin, g_span, queue
+      partial, G, eps2, blo, clo, ghl, out, group_cap, g_begin, g_span, queue, \
+      parts ? 0u : inline_split
 #define GTH_EVAL(Q, W)                                                                    \
   if (hl)                                                                                \
     hipLaunchKernelGGL((eval_fast<Q, W, true>), grid, block, 0, s, GTH_EVAL_ARGS);  \

@@ -272,7 +272,9 @@ void launch_eval_items(const gt_group *groups, unsigned long long *frame, uint32
                        uint32_t g_begin = 0, uint32_t g_span = 0);
 // Evaluates the first frame[kGroups] groups (at most group_cap) unless
 // frame[kStatus] is set; with split (fast evaluation), the work items that
-// launch_eval_items made (split groups' parts then need launch_combine_parts).
+// launch_eval_items made (split groups' parts then need launch_combine_parts);
+// without, lists longer than inline_split entries (0: none) are evaluated in
+// the same parts by one wave each, with the same results.
 // Hi/lo precision mode: lo parts of the bodies' and cells' FP32 positions
 // (prep_lists), and each group's centre as hi, lo float4 pairs.
 struct EvalHiLo {
@@ -302,7 +304,7 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
                  uint32_t quad, bool exact, int waves, float4 *out, hipStream_t s,
                  const EvalSplit *split = nullptr, const EvalHiLo *hilo = nullptr,
                  uint32_t g_begin = 0, uint32_t g_span = 0, uint32_t *queue = nullptr,
-                 uint32_t persistent_blocks = 0);
+                 uint32_t persistent_blocks = 0, uint32_t inline_split = 0);
 // Adds the parts of split groups into out, in part order.
 void launch_combine_parts(const gt_group *groups, const unsigned long long *frame,
                           uint32_t group_cap, const EvalSplit &split, float4 *out, hipStream_t s,

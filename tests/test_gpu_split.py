@@ -63,15 +63,20 @@ def test_split_steps_from_graphs_equal_launched(hip, oracle):
         assert np.array_equal(bits(a), bits(b))
 
 
-@pytest.mark.parametrize("split", [-1, 512])
-def test_longest_first_order_changes_nothing(hip, oracle, split):
+@pytest.mark.parametrize("split", [-1, 512, 128])
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_evaluation_order_changes_nothing(hip, oracle, split, persistent):
     # Evaluating each XCD's groups longest first only reorders whole work
-    # items: every target's sum is formed the same way, so bit-identical.
+    # items, and in tree order (no work items) a long list is cut into the
+    # same parts by one wave, which adds them in part order as combine_parts
+    # does: every target's sum is formed the same way, so bit-identical.
     pos, mass = oracle.test_system(1, 30000, 54)
-    cfg = dict(theta=0.4, eps=0.01, group_size=32, split_len=split)
-    a1, p1 = hip.group_compute(pos, mass, eval_order=1, **cfg)
-    a2, p2 = hip.group_compute(pos, mass, eval_order=2, **cfg)
-    assert np.array_equal(bits(a1), bits(a2)) and np.array_equal(bits(p1), bits(p2))
+    cfg = dict(theta=0.4, eps=0.01, group_size=32, split_len=split, persistent=persistent)
+    got = [hip.group_compute(pos, mass, eval_order=o, **cfg) for o in (1, 2, 4, 0)]
+    if split > 0:
+        assert hip.info()["max_list"] > 2 * split  # the lists really were split
+    for a, p in got[1:]:
+        assert np.array_equal(bits(a), bits(got[0][0])) and np.array_equal(bits(p), bits(got[0][1]))
 
 
 def test_autotune_picks_valid_settings(hip, oracle):
