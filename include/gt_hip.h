@@ -78,12 +78,15 @@ typedef struct {
                         records ahead; 16, 32 or 64: that many groups share one
                         wave-coherent walk.  Every choice gives the same lists */
   int split_len;     /* tuning: lists longer than this many entries are evaluated in
-                        parts by several waves (fast evaluation only; parts added in a
-                        fixed order); 0 = library default (8192), < 0 = never split */
+                        parts, by several waves with the longest-first orders and by
+                        one wave in tree order (fast evaluation only; parts added in
+                        part order, so the results are the same either way); 0 =
+                        library default (8192), < 0 = never split */
   int eval_order;    /* tuning (fast evaluation): the order in which each XCD takes its
                         share of the groups: 1 = tree order, 2 = longest first, 3 / 4 =
                         longest first by cost classes of half / quarter octaves (tree
-                        order within a class); 0 = library default (4).  Results do
+                        order within a class); 0 = library default (4; 1 with the
+                        persistent grid, whose queue balances the load).  Results do
                         not depend on it. */
   int hilo;          /* 1: fast evaluation with every position as an FP32 hi + lo pair
                         taken relative to its group's centre (bounding-box centre in
