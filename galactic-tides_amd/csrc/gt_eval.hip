@@ -22,7 +22,8 @@
 //    read from LDS as broadcasts.  Group sizes are bucketed (multiples of 4
 //    targets per batch) so each path is branch-free over its targets.
 //    v_rsq_f32 and FMA chains; G is folded into the entry's mass and
-//    quadrupole once per entry; per-target sums are reduced with DPP.
+//    quadrupole once per entry; the per-target sums are reduced all together
+//    by a reduce-scatter butterfly (permlane swaps and DPP).
 #include "gt_internal.h"
 
 namespace gth {
@@ -127,23 +128,81 @@ __global__ __launch_bounds__(kBlock) void eval_exact(
 
 // ---- fast: entries across lanes, targets uniform ------------------------------------
 
-// Sum over the 64 lanes with DPP moves (no LDS traffic): pairs, quads, half
-// rows and rows by permutes within a row, then rows combined with the row
-// broadcasts; the total ends in lane 63.  Fixed order, so deterministic.
-template <int CTRL, int ROW_MASK>
-__device__ inline float dpp_add(float v) {
-  const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
-  return v + __int_as_float(moved);
+// Wave sums of many per-lane values at once, scattered over the lanes (a
+// reduce-scatter butterfly, no LDS traffic).  Each step pairs every lane
+// with a partner whose lane number differs in bit log2(D) and halves the
+// values each lane holds: the lane keeps the lower or the upper half (by that
+// bit), adds the partner's copy of it, and hands over the other half.  Lanes
+// 32 and 16 apart trade halves with v_permlane32_swap / v_permlane16_swap
+// (one instruction per pair of values); within a row the partner's value
+// comes by DPP (row_mirror, row_half_mirror, quad permutes).  Once a lane
+// holds an odd number of values the remaining steps add the partner's copy
+// of every value (an all-reduce), so 2^r lanes end with the same sums.
+// Fixed pairing and order, so deterministic.
+
+// The value of partner lane (lane ^ ...) in a step of distance D within a row.
+template <int D>
+__device__ inline float partner_in_row(float v) {
+  // D = 8: row_mirror (i <-> 15 - i), 4: row_half_mirror (i <-> 7 - i),
+  // 2: quad_perm [2, 3, 0, 1], 1: quad_perm [1, 0, 3, 2]
+  constexpr int ctrl = D == 8 ? 0x140 : D == 4 ? 0x141 : D == 2 ? 0x4e : 0xb1;
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false));
 }
 
-__device__ inline float wave_total(float v) {
-  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
-  v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
-  v = dpp_add<0x140, 0xf>(v);  // row_mirror
-  v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+// Both values of a pair after a swap of lanes D = 32 or 16 apart: the lane
+// holds its own value of the half it keeps and its partner's.
+template <int D>
+__device__ inline float swapped_sum(float keep_lo, float keep_hi) {
+  const auto r =
+      D == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(keep_lo), __float_as_uint(keep_hi),
+                                                 false, false)
+              : __builtin_amdgcn_permlane16_swap(__float_as_uint(keep_lo), __float_as_uint(keep_hi),
+                                                 false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// Halving steps a butterfly over n values makes (while n is even, at most 6).
+constexpr int rs_halvings(int n) {
+  int h = 0;
+  while (h < 6 && n % 2 == 0) {
+    n /= 2;
+    ++h;
+  }
+  return h;
+}
+
+// Step S (distance 32 >> S) of the butterfly over the M values acc[0 .. M);
+// base: the index of acc[0] among the wave's values so far.
+template <int M, int S>
+__device__ inline void reduce_scatter_step(float *acc, uint32_t lane, uint32_t &base) {
+  if constexpr (S < 6) {
+    constexpr int D = 32 >> S;
+    if constexpr (M % 2 == 0) {
+      constexpr int H = M / 2;
+      const bool hi = (lane & D) != 0;
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        if constexpr (D >= 16) {
+          acc[i] = swapped_sum<D>(acc[i], acc[i + H]);
+        } else {
+          const float send = hi ? acc[i] : acc[i + H];
+          const float keep = hi ? acc[i + H] : acc[i];
+          acc[i] = keep + partner_in_row<D>(send);
+        }
+      }
+      base += hi ? H : 0;
+      reduce_scatter_step<H, S + 1>(acc, lane, base);
+    } else {
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        if constexpr (D >= 16)
+          acc[j] = swapped_sum<D>(acc[j], acc[j]);
+        else
+          acc[j] = acc[j] + partner_in_row<D>(acc[j]);
+      }
+      reduce_scatter_step<M, S + 1>(acc, lane, base);
+    }
+  }
 }
 
 // Position used for lanes past the end of a list chunk: far enough that the
@@ -262,9 +321,10 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
     tgt[lane] = p;
   }
   __builtin_amdgcn_wave_barrier();
-  float ax[TB], ay[TB], az[TB], ap[TB];
+  // per-target partial sums: acc[4 t + c], c = x, y, z, potential
+  float acc[4 * TB];
 #pragma unroll
-  for (int t = 0; t < TB; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+  for (int k = 0; k < 4 * TB; ++k) acc[k] = 0.0f;
 
   {
     const uint32_t cb = r.cb, ce = r.ce;
@@ -295,15 +355,15 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
           const float inv5 = inv3 * inv2;
           const float h = rqr * inv5;
           const float s = fmaf(-2.5f * h, inv2, m3);
-          ax[t] = fmaf(s, dx, fmaf(qx, inv5, ax[t]));
-          ay[t] = fmaf(s, dy, fmaf(qy, inv5, ay[t]));
-          az[t] = fmaf(s, dz, fmaf(qz, inv5, az[t]));
-          ap[t] = fmaf(-0.5f, h, fmaf(e.ngm, inv, ap[t]));
+          acc[4 * t] = fmaf(s, dx, fmaf(qx, inv5, acc[4 * t]));
+          acc[4 * t + 1] = fmaf(s, dy, fmaf(qy, inv5, acc[4 * t + 1]));
+          acc[4 * t + 2] = fmaf(s, dz, fmaf(qz, inv5, acc[4 * t + 2]));
+          acc[4 * t + 3] = fmaf(-0.5f, h, fmaf(e.ngm, inv, acc[4 * t + 3]));
         } else {
-          ax[t] = fmaf(m3, dx, ax[t]);
-          ay[t] = fmaf(m3, dy, ay[t]);
-          az[t] = fmaf(m3, dz, az[t]);
-          ap[t] = fmaf(e.ngm, inv, ap[t]);
+          acc[4 * t] = fmaf(m3, dx, acc[4 * t]);
+          acc[4 * t + 1] = fmaf(m3, dy, acc[4 * t + 1]);
+          acc[4 * t + 2] = fmaf(m3, dz, acc[4 * t + 2]);
+          acc[4 * t + 3] = fmaf(e.ngm, inv, acc[4 * t + 3]);
         }
       }
       e = en;
@@ -332,10 +392,10 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float gi = ngm * inv;
         const float g3 = gi * inv * inv;
-        ax[t] = fmaf(g3, dx, ax[t]);
-        ay[t] = fmaf(g3, dy, ay[t]);
-        az[t] = fmaf(g3, dz, az[t]);
-        ap[t] += gi;
+        acc[4 * t] = fmaf(g3, dx, acc[4 * t]);
+        acc[4 * t + 1] = fmaf(g3, dy, acc[4 * t + 1]);
+        acc[4 * t + 2] = fmaf(g3, dz, acc[4 * t + 2]);
+        acc[4 * t + 3] += gi;
       }
       b = bn;
       j = j_next;
@@ -343,25 +403,22 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
     }
   }
 
-  float rx = 0.0f, ry = 0.0f, rz = 0.0f, rp = 0.0f;
+  // The wave's sums, scattered: after H halvings lane l holds M = 4 TB >> H
+  // sums, of values base .. base + M - 1, shared by R = 64 >> H lanes; each
+  // of those lanes stores every R-th of them (value k = 4 t + c is target
+  // t's component c), targets past nt are not stored.
+  constexpr int H = rs_halvings(4 * TB), M = (4 * TB) >> H, R = 64 >> H;
+  uint32_t base = 0;
+  reduce_scatter_step<4 * TB, 0>(acc, lane, base);
+  float *dstf = reinterpret_cast<float *>(dst);
+  const uint32_t rho = lane & (R - 1);
 #pragma unroll
-  for (int t = 0; t < TB; ++t) {
-    const float sx = wave_total(ax[t]), sy = wave_total(ay[t]), sz = wave_total(az[t]),
-                sp = wave_total(ap[t]);
-    if (lane == static_cast<uint32_t>(t)) {
-      rx = sx;
-      ry = sy;
-      rz = sz;
-      rp = sp;
+  for (int j = 0; j < M; ++j) {
+    const uint32_t k = base + j;
+    if ((static_cast<uint32_t>(j) & (R - 1)) == rho && (k >> 2) < nt) {
+      const float v = acc[j];
+      dstf[k] = accumulate ? dstf[k] + v : v;
     }
-  }
-  if (lane < nt) {
-    float4 v = make_float4(rx, ry, rz, rp);
-    if (accumulate) {
-      const float4 o = dst[lane];
-      v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
-    }
-    dst[lane] = v;
   }
   __builtin_amdgcn_wave_barrier();  // targets of the next batch overwrite tgt
 }
