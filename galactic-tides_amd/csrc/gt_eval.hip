@@ -345,21 +345,25 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float inv2 = inv * inv;
-        const float inv3 = inv * inv2;
-        const float m3 = e.ngm * inv3;  // -G m / r^3
         if (QUAD) {
+          // The reference's monopole and quadrupole terms (gravity_kernels.h:
+          // 109-135) with the powers of 1/r shared, 33 operations:
+          //   a   = r^-5 (u r^2 d + q),   u = -G m - 2.5 rqr r^-4
+          //   pot = r^-1 (-G m - 0.5 rqr r^-4),   q = G Q d, rqr = d . q
           const float qx = fmaf(e.q0, dx, fmaf(e.q3, dy, e.q4 * dz));
           const float qy = fmaf(e.q3, dx, fmaf(e.q1, dy, e.q5 * dz));
           const float qz = fmaf(e.q4, dx, fmaf(e.q5, dy, e.q2 * dz));
           const float rqr = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
-          const float inv5 = inv3 * inv2;
-          const float h = rqr * inv5;
-          const float s = fmaf(-2.5f * h, inv2, m3);
-          acc[4 * t] = fmaf(s, dx, fmaf(qx, inv5, acc[4 * t]));
-          acc[4 * t + 1] = fmaf(s, dy, fmaf(qy, inv5, acc[4 * t + 1]));
-          acc[4 * t + 2] = fmaf(s, dz, fmaf(qz, inv5, acc[4 * t + 2]));
-          acc[4 * t + 3] = fmaf(-0.5f, h, fmaf(e.ngm, inv, acc[4 * t + 3]));
+          const float inv4 = inv2 * inv2;
+          const float inv5 = inv4 * inv;
+          const float t4 = rqr * inv4;
+          const float w = fmaf(-2.5f, t4, e.ngm) * r2;
+          acc[4 * t] = fmaf(fmaf(w, dx, qx), inv5, acc[4 * t]);
+          acc[4 * t + 1] = fmaf(fmaf(w, dy, qy), inv5, acc[4 * t + 1]);
+          acc[4 * t + 2] = fmaf(fmaf(w, dz, qz), inv5, acc[4 * t + 2]);
+          acc[4 * t + 3] = fmaf(fmaf(-0.5f, t4, e.ngm), inv, acc[4 * t + 3]);
         } else {
+          const float m3 = e.ngm * (inv * inv2);  // -G m / r^3
           acc[4 * t] = fmaf(m3, dx, acc[4 * t]);
           acc[4 * t + 1] = fmaf(m3, dy, acc[4 * t + 1]);
           acc[4 * t + 2] = fmaf(m3, dz, acc[4 * t + 2]);
