@@ -577,6 +577,13 @@ int walk_lanes(const gt_group_config *cfg, const Plan &p) {
   return l == 16 || l == 32 || l == 64 ? l : per_lane;
 }
 
+// Whether the walk for these settings (a walk over child sets) writes the
+// hi/lo group centres itself; the other walks leave them to group_centres.
+bool walk_writes_centres(const gt_group_config *cfg, const Plan &p) {
+  const int l = walk_lanes(cfg, p);
+  return l == 3 || l == 4;
+}
+
 // Stack entries per lane of the walk over child sets: one per level below the
 // root (the tree is at most depth_cap deep, or the tree phase stops first).
 uint32_t levels_for(const Plan &p) {
@@ -625,7 +632,8 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
   launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, in.tsid,
                      in.root_node, c->starts.get(), c->groups.get(), groot, s,
                      c->flag_node.get(), c->group_node.get());
-  if (hilo)
+  // (the child-set walks write the hi/lo group centres themselves)
+  if (hilo && !walk_writes_centres(cfg, p))
     launch_group_centres(c->groups.get(), frame, p.group_cap, in.tpos, in.tsid, in.roots, in.geom,
                          c->group_hilo.get(), s);
   if (p.predicted) {
@@ -650,19 +658,20 @@ void enqueue_walk_range(gt_hip_context *c, const gt_group_config *cfg, const Pla
   const uint32_t chunks = std::max<uint32_t>(p.chunks, 1);
   const uint32_t span = (p.group_cap + chunks - 1) / chunks;
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
+  float4 *ghl = p.hilo && walk_writes_centres(cfg, p) ? c->group_hilo.get() : nullptr;
   if (p.predicted)
     launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->images.get(),
                 levels_for(p), c->span.get(), in.tpos,
                 c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, 0, 0,
                 c->slot_off_c.get(), c->slot_off_b.get(), c->ccount.get(), c->bcount.get(),
                 c->cell_list.get(), c->body_list.get(), s, c->slot_c.get(), c->slot_b.get(),
-                k * span, span);
+                k * span, span, ghl);
   else
     launch_walk(kWalkCapacity, walk_lanes(cfg, p), c->recs.get(), c->images.get(),
                 levels_for(p), c->span.get(), in.tpos,
                 c->groups.get(), frame, p.group_cap, p.node_cap, in.geom, groot, p.cap_c, p.cap_b,
                 nullptr, nullptr, c->ccount.get(), c->bcount.get(), c->cell_list.get(),
-                c->body_list.get(), s, nullptr, nullptr, k * span, span);
+                c->body_list.get(), s, nullptr, nullptr, k * span, span, ghl);
 }
 
 // After the walk: the predictor's error, the statistics and the status check.
@@ -1193,9 +1202,9 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       break;
     default:
       if (p.chunks <= 1) enqueue_combine(c, cfg, p);
-      launch_scatter(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
-                     c->pot.get(), frame + frame::kStatus, s);
-      launch_kick(c->vel.get(), c->acc.get(), n, h, frame + frame::kStatus, s);
+      // the scatter and the closing kick in one pass over the rows
+      launch_scatter_kick(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
+                          c->pot.get(), c->vel.get(), h, frame + frame::kStatus, s);
       launch_count_step(frame, s);
       break;
   }

@@ -654,7 +654,8 @@ __global__ __launch_bounds__(kBlock) void group_centres(const gt_group *__restri
   const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[G.begin]) : geom;
   float h[3], l[3];
   for (int a = 0; a < 3; ++a) {
-    const double c = 0.5 * (lo[a] + hi[a]) - o[a];
+    // each operation rounded (no contraction), as the walk forms the centre
+    const double c = __dsub_rn(__dmul_rn(0.5, __dadd_rn(lo[a], hi[a])), o[a]);
     h[a] = static_cast<float>(c);
     l[a] = static_cast<float>(c - static_cast<double>(h[a]));
   }

@@ -364,7 +364,8 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
                  uint32_t cap_b, const uint32_t *cell_off, const uint32_t *body_off,
                  uint32_t *cell_count, uint32_t *body_count, uint32_t *cell_list,
                  uint32_t *body_list, hipStream_t s, const uint32_t *slot_c = nullptr,
-                 const uint32_t *slot_b = nullptr, uint32_t g_begin = 0, uint32_t g_span = 0);
+                 const uint32_t *slot_b = nullptr, uint32_t g_begin = 0, uint32_t g_span = 0,
+                 float4 *group_hilo = nullptr);
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s);
@@ -421,6 +422,10 @@ constexpr int kQueryKeyBits = 30;
 // gt_step.hip: each does nothing once status (a frame's kStatus word) is set.
 void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
                     double *pot, const unsigned long long *status, hipStream_t s);
+// launch_scatter then launch_kick(vel, acc, n, h) as one kernel.
+void launch_scatter_kick(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+                         double *pot, double *vel, double h, const unsigned long long *status,
+                         hipStream_t s);
 void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,
                        double dt, const unsigned long long *status, hipStream_t s);
 // Permutations of the resident state: out[k] = in[order[k]] (rows of w

@@ -30,6 +30,29 @@ __global__ __launch_bounds__(kBlock) void scatter_forces(const float4 *__restric
   pot[i] = a.w;
 }
 
+// The scatter and the step's closing kick in one pass: acc and pot as
+// scatter_forces writes them, then v += h a for the same row, the same
+// operations as kick (so the result is bit for bit the two kernels').
+__global__ __launch_bounds__(kBlock) void scatter_kick(const float4 *__restrict__ out,
+                                                       const uint32_t *__restrict__ order,
+                                                       uint32_t n, double *__restrict__ acc,
+                                                       double *__restrict__ pot,
+                                                       double *__restrict__ vel, double h,
+                                                       const unsigned long long *status) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n || *status) return;
+  const float4 a = out[k];
+  const size_t i = order[k];
+  const double ax = a.x, ay = a.y, az = a.z;
+  acc[3 * i] = ax;
+  acc[3 * i + 1] = ay;
+  acc[3 * i + 2] = az;
+  pot[i] = a.w;
+  vel[3 * i] = vel[3 * i] + h * ax;
+  vel[3 * i + 1] = vel[3 * i + 1] + h * ay;
+  vel[3 * i + 2] = vel[3 * i + 2] + h * az;
+}
+
 // v += h a; x += dt v   (first half of the step)
 __global__ __launch_bounds__(kBlock) void kick_drift(double *__restrict__ pos,
                                                      double *__restrict__ vel,
@@ -162,6 +185,15 @@ void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double
   if (!n) return;
   hipLaunchKernelGGL(scatter_forces, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, order, n, acc,
                      pot, status);
+  GTH_LAUNCH_CHECK();
+}
+
+void launch_scatter_kick(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+                         double *pot, double *vel, double h, const unsigned long long *status,
+                         hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(scatter_kick, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, order, n, acc,
+                     pot, vel, h, status);
   GTH_LAUNCH_CHECK();
 }
 

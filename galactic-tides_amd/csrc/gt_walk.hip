@@ -361,7 +361,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
     const uint32_t *__restrict__ body_off, uint32_t *__restrict__ cell_count,
     uint32_t *__restrict__ body_count, uint32_t *__restrict__ cell_list,
     uint32_t *__restrict__ body_list, const uint32_t *__restrict__ slot_c,
-    const uint32_t *__restrict__ slot_b, uint32_t g_begin, uint32_t g_span) {
+    const uint32_t *__restrict__ slot_b, uint32_t g_begin, uint32_t g_span,
+    float4 *__restrict__ group_hilo) {
   constexpr uint32_t kPer = kSetMax / GL;            // children per lane
   constexpr uint32_t kGroups = kWalkBlock / GL;      // groups per block (one wave)
   constexpr uint32_t kLaneMask = (1u << GL) - 1u;
@@ -408,6 +409,21 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
   const GroupImage gim =
       group_image(origin[0], origin[1], origin[2], root_half, gcx, gcy, gcz, ghx, ghy, ghz);
   const float root_half32 = static_cast<float>(root_half);
+  // Hi/lo precision mode: the centre of the group's box relative to its
+  // system's origin (the root node's centre) as FP32 hi + lo, for the
+  // evaluation (group_hilo[2g], group_hilo[2g + 1]).
+  if (group_hilo && valid && k == 0) {
+    // (rounded subtractions, as group_centres forms them)
+    const double c[3] = {__dsub_rn(gcx, origin[0]), __dsub_rn(gcy, origin[1]),
+                         __dsub_rn(gcz, origin[2])};
+    float hv[3], lv[3];
+    for (int a = 0; a < 3; ++a) {
+      hv[a] = static_cast<float>(c[a]);
+      lv[a] = static_cast<float>(c[a] - static_cast<double>(hv[a]));
+    }
+    group_hilo[2 * static_cast<size_t>(g)] = make_float4(hv[0], hv[1], hv[2], 0.0f);
+    group_hilo[2 * static_cast<size_t>(g) + 1] = make_float4(lv[0], lv[1], lv[2], 0.0f);
+  }
   uint32_t cpos = 0, bpos = 0;
   if (valid) {
     if (MODE == kWalkExact) {
@@ -925,8 +941,10 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
                  const uint32_t *cell_off, const uint32_t *body_off, uint32_t *cell_count,
                  uint32_t *body_count, uint32_t *cell_list, uint32_t *body_list, hipStream_t s,
                  const uint32_t *slot_c, const uint32_t *slot_b, uint32_t g_begin,
-                 uint32_t g_span) {
+                 uint32_t g_span, float4 *group_hilo) {
   if (!group_cap) return;
+  if (group_hilo && lanes != 3 && lanes != 4)
+    fail(GT_HIP_ERR_ARGUMENT, "walk: only the child-set walks write group centres");
   if (g_span == 0 || g_span > group_cap) g_span = group_cap;
   const bool coherent = lanes >= 16;
   if (coherent && (g_begin != 0 || g_span != group_cap))
@@ -945,7 +963,7 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
 #define GTH_SETS_ARGS                                                                        \
   recs, images, levels_cap, span, tpos, groups, frame, group_cap, node_cap, geom, group_root, \
       cap_c, cap_b, cell_off, body_off, cell_count, body_count, cell_list, body_list, slot_c,  \
-      slot_b, g_begin, g_span
+      slot_b, g_begin, g_span, group_hilo
     if (mode == kWalkExact && gl == 8)
       hipLaunchKernelGGL((walk_sets<kWalkExact, 8>), grid, block, lds, s, GTH_SETS_ARGS);
     else if (mode == kWalkExact)

@@ -61,3 +61,24 @@ def test_hilo_graph_steps_equal_launched(hip, oracle):
         out.append(hip.state_get())
     for a, b in zip(*out):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("systems", [1, 3])
+def test_hilo_group_centres_the_same_from_every_walk(hip, oracle, systems):
+    # The walks over child sets write the hi/lo group centres as they compute
+    # the group boxes; the other walks leave them to a kernel of their own.
+    # Both round each operation the same way, so every walk gives the same
+    # forces bit for bit (one system, and a batch whose systems have their
+    # own origins).
+    pos, mass = wide_system(oracle, 9000, 64)
+    cfg = dict(theta=0.5, eps=1e-3, group_size=32, hilo=True)
+    offsets = None if systems == 1 else np.array([0, 2500, 6000, 9000], np.uint64)
+    got = []
+    for lanes in (0, 4, 2, 1):
+        if offsets is None:
+            got.append(hip.group_compute(pos, mass, walk_lanes=lanes, **cfg))
+        else:
+            got.append(hip.group_compute_batch(offsets, pos, mass, walk_lanes=lanes, **cfg))
+    for a, p in got[1:]:
+        assert np.array_equal(a.view(np.uint64), got[0][0].view(np.uint64))
+        assert np.array_equal(p.view(np.uint64), got[0][1].view(np.uint64))
