@@ -389,7 +389,7 @@ def bench_plain_vs_full(args):
         pos, vel, mass = make(ctx)
         base = dict(G=1.0, theta=0.75, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                     pack=1)
-        modes = {"plain": (dict(base, predict=2, eval_order=1, split_len=-1, persistent=0), False),
+        modes = {"plain": (dict(base, predict=2, eval_order=1, split_len=-1, persistent=-1), False),
                  "full": (dict(base, predict=1, persistent=1), True),
                  "uniform": (dict(base, predict=0), True)}
         res, states = {}, {}
@@ -480,9 +480,9 @@ def main():
                     help="hi/lo FP32 positions relative to group centres: 1 on, 0 off "
                          "(default: on for encounter-2m, as its configuration specifies)")
     ap.add_argument("--persistent", type=int, default=-1,
-                    help="the persistent evaluation grid taking work items longest first: 1 "
-                         "on, 0 off (default: on for encounter-2m, as its configuration "
-                         "specifies; the autotuner does not change it)")
+                    help="the persistent evaluation grid (waves taking their XCD's work "
+                         "items from a counter): 1 on (default, as the encounter's "
+                         "configuration specifies; the autotuner does not change it), 0 off")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tuning settings (otherwise the autotuner "
                          "picks them on the initial particles before the timed steps)")
@@ -521,7 +521,9 @@ def main():
     import gt_hip
     headline = args.workload == "encounter-2m"
     hilo = bool(args.hilo) if args.hilo >= 0 else headline
-    persistent = args.persistent if args.persistent >= 0 else int(headline)
+    # the library's default is the persistent grid (persistent 0); 0 here asks
+    # for one wave per work item (-1)
+    persistent = 1 if args.persistent < 0 else (1 if args.persistent else -1)
     cfg = dict(G=1.0, theta=args.theta, eps=0.025, quadrupole=True, group_size=32, leaf_size=8,
                pack=1, exact=args.exact, eval_waves=args.eval_waves, split_len=args.split_len,
                eval_order=args.eval_order, hilo=hilo, predict=args.predict,
@@ -646,7 +648,7 @@ def main():
                    "systems": 1 if offsets is None else len(offsets) - 1,
                    "n": n, "theta": args.theta, "group_size": 32, "predict": args.predict,
                    "leaf_size": 8, "eps": 0.025, "dt": 0.01, "halo": args.halo,
-                   "exact": bool(args.exact), "hilo": hilo, "persistent": bool(persistent),
+                   "exact": bool(args.exact), "hilo": hilo, "persistent": persistent > 0,
                    "graphs": graph,
                    "tuning": tuning,
                    "parallelism": "single GPU"},

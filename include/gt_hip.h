@@ -103,11 +103,11 @@ typedef struct {
                         next is walked; 1 = no overlap, 0 = library default (1), at most
                         8.  Pays only where the walk has many more groups than the GPU
                         has SIMDs: a range's walk lasts as long as its longest group's */
-  int persistent;    /* tuning (fast evaluation, whole evaluations): 1 = a persistent
-                        grid (the GPU's CUs x waves per SIMD) whose waves take the work
-                        items of their XCD's share one at a time from a counter, in the
-                        eval_order; 0 = one wave per work item, dealt out by the
-                        hardware dispatcher (default).  Results do not depend on it. */
+  int persistent;    /* tuning (fast evaluation, whole evaluations): 0 (default) or 1 = a
+                        persistent grid (the GPU's CUs x waves per SIMD) whose waves
+                        take the work items of their XCD's share one at a time from a
+                        counter, in the eval_order; -1 = one wave per work item, dealt
+                        out by the hardware dispatcher.  Results do not depend on it. */
 } gt_group_config;
 
 /* GroupLists statistics (group_lists.h:37-55) and tree size of the last call. */
@@ -234,7 +234,7 @@ int gt_hip_group_compute_batch(gt_hip_context *ctx, const gt_group_config *cfg,
  * given particles on the GPU (HIP events, best of 2 after a warm-up) while it
  * varies one tuning setting at a time (eval_waves {2, 3}, walk_lanes {2, 4,
  * 16, 32, 64}, eval_order {1, 2, 3, 4}, split_len {2048, 8192, never},
- * persistent {0, 1}, and group_size
+ * persistent {-1, 1}, and group_size
  * {32, 64} when search_group_size is 1), keeping each change that is at least 1%
  * faster,
  * and writes the fastest settings into *best (the other fields copied from

@@ -1,6 +1,7 @@
-"""The persistent evaluation grid (`persistent = 1`: waves take the work items
-of their XCD's share from a counter) runs the same per-item code on the same
-items as one wave per item, so forces and leapfrog states must be identical
+"""The persistent evaluation grid (the default, `persistent` 0 or 1: waves take
+the work items of their XCD's share from a counter) runs the same per-item
+code on the same items as one wave per item (`persistent = -1`), so forces
+and leapfrog states must be identical
 bit for bit, for split lists, hi/lo positions, tree order, and steps from
 captured graphs (the counters are reset inside each replay); and the
 persistent grid's forces are within the reference's GPU tolerance of the
@@ -24,9 +25,10 @@ def same(a, b):
 def test_persistent_forces_equal(hip, oracle, extra):
     pos, mass = oracle.test_system(0, 40000, 17)
     cfg = dict(theta=0.6, eps=0.01, group_size=32, **extra)
-    want = hip.group_compute(pos, mass, persistent=0, **cfg)
+    want = hip.group_compute(pos, mass, persistent=-1, **cfg)
     got = hip.group_compute(pos, mass, persistent=1, **cfg)
-    assert same(got, want)
+    default = hip.group_compute(pos, mass, **cfg)
+    assert same(got, want) and same(default, want)
 
 
 def test_persistent_graph_steps_equal(hip, oracle):
@@ -41,7 +43,7 @@ def test_persistent_graph_steps_equal(hip, oracle):
         hip.steps(6, 0.01, use_graph=use_graph, persistent=persistent, **cfg)
         return hip.state_get()
 
-    want = run(0, False)
+    want = run(-1, False)
     before = hip.pipeline_stats()
     got = run(1, True)
     after = hip.pipeline_stats()
