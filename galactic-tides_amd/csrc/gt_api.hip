@@ -182,6 +182,7 @@ struct gt_hip_context {
   std::vector<hipEvent_t> step_ev;  // kTimedSteps x (kPhases + 1)
   // raw-array entry points and sampling
   DBuf<float4> raw_bodies, raw_out;
+  DBuf<float> probe_out;
   DBuf<Cell> raw_cells;
   DBuf<uint32_t> raw_cl, raw_bl;
   DBuf<gt_group> raw_groups;
@@ -2522,6 +2523,29 @@ int gt_hip_debug_sort_waits(gt_hip_context *c, uint32_t polls) {
     const unsigned long long v = polls;
     upload(c->frame.get() + frame::kSortCtl + 1, &v, 1, c->stream);
     GTH_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gt_hip_debug_valu_rate(gt_hip_context *c, int rsq, double *lane_ops_per_s) {
+  if (!c || !lane_ops_per_s) return GT_HIP_ERR_ARGUMENT;
+  return guarded(c, [&] {
+    GTH_CHECK(hipSetDevice(c->device));
+    // eight 256-lane workgroups per CU: eight waves per SIMD
+    const uint32_t blocks = static_cast<uint32_t>(c->prop.multiProcessorCount) * 8u;
+    const int iters = 20000;
+    float *out = c->probe_out.ensure(static_cast<size_t>(blocks) * 256);
+    hipStream_t s = c->stream;
+    launch_valu_probe(out, blocks, iters / 10, rsq != 0, s);  // warm-up
+    double best = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {
+      GTH_CHECK(hipEventRecord(c->ev[0], s));
+      launch_valu_probe(out, blocks, iters, rsq != 0, s);
+      GTH_CHECK(hipEventRecord(c->ev[1], s));
+      GTH_CHECK(hipEventSynchronize(c->ev[1]));
+      const double ms = elapsed_ms(c->ev[0], c->ev[1]);
+      if (rep == 0 || ms < best) best = ms;
+    }
+    *lane_ops_per_s = static_cast<double>(blocks) * 256.0 * 16.0 * iters / (best * 1e-3);
   });
 }
 

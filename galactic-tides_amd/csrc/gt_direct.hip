@@ -9,6 +9,9 @@
 //    gravity is not a dense contraction, so there is no MFMA here.
 //  * direct_f64_sample: FP64 reference forces for a sample of targets
 //    (apps/galactic_tides_bench.cpp:80-95), used to measure force errors.
+//  * valu_probe: the GPU's sustained FP32 VALU rate (non-packed, as all the
+//    kernels here are built), the practical ceiling of the VALU-bound
+//    kernels: 16 independent FMA chains per lane, eight waves per SIMD.
 #include "gt_internal.h"
 
 namespace gth {
@@ -91,7 +94,38 @@ __global__ __launch_bounds__(kBlock) void direct_f64_sample(const double *__rest
 }
 #pragma clang fp contract(fast)
 
+// iters x 16 operations per lane: FMAs, or with RSQ every fourth one a
+// v_rsq_f32 (the evaluation's transcendental).  The sums are written so the
+// chains are not dead code.
+template <bool RSQ>
+__global__ __launch_bounds__(kBlock) void valu_probe(float *__restrict__ out, int iters, float x,
+                                                     float y) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  float a[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = static_cast<float>(t) * 1e-7f + static_cast<float>(i);
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      a[i] = (RSQ && (i & 3) == 0) ? __builtin_amdgcn_rsqf(a[i] + y) : fmaf(a[i], x, y);
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) sum += a[i];
+  out[t] = sum;
+}
+
 }  // namespace
+
+void launch_valu_probe(float *out, uint32_t blocks, int iters, bool rsq, hipStream_t s) {
+  if (rsq)
+    hipLaunchKernelGGL(valu_probe<true>, dim3(blocks), dim3(kBlock), 0, s, out, iters, 0.999f,
+                       1e-3f);
+  else
+    hipLaunchKernelGGL(valu_probe<false>, dim3(blocks), dim3(kBlock), 0, s, out, iters, 0.999f,
+                       1e-3f);
+  GTH_LAUNCH_CHECK();
+}
 
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
                    hipStream_t s) {

@@ -311,6 +311,9 @@ void launch_combine_parts(const gt_group *groups, const unsigned long long *fram
                           uint32_t g_begin = 0, uint32_t g_span = 0);
 
 // gt_direct.hip
+// The VALU throughput probe: blocks x 256 lanes, iters x 16 operations each
+// (out: blocks x 256 floats).
+void launch_valu_probe(float *out, uint32_t blocks, int iters, bool rsq, hipStream_t s);
 void launch_direct(const float4 *bodies, uint32_t n, float G, float eps2, float4 *out,
                    hipStream_t s);
 void launch_direct_f64_sample(const double *pos, const double *mass, uint64_t n, double G,

@@ -119,6 +119,7 @@ _state_set = _sig("gt_hip_state_set", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp])
 _state_get = _sig("gt_hip_state_get", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _prime = _sig("gt_hip_prime", C.c_int, [_vp, _P(GroupConfig)])
 _steps = _sig("gt_hip_steps", C.c_int, [_vp, _P(GroupConfig), C.c_int, C.c_double, C.c_int])
+_valu_rate = _sig("gt_hip_debug_valu_rate", C.c_int, [_vp, C.c_int, _P(C.c_double)])
 _prepare_steps = _sig("gt_hip_prepare_steps", C.c_int,
                       [_vp, _P(GroupConfig), C.c_int, C.c_double])
 _pipeline_stats = _sig("gt_hip_pipeline_stats", C.c_int, [_vp, _P(PipelineStats)])
@@ -494,6 +495,12 @@ class Context:
         self._check(_prepare_steps(self._h, C.byref(c), nsteps, dt))
 
     # -- measurement -------------------------------------------------------------------
+    def valu_rate(self, rsq=False):
+        """Sustained non-packed FP32 VALU rate of the GPU, lane operations per second."""
+        r = C.c_double(0.0)
+        self._check(_valu_rate(self._h, int(rsq), C.byref(r)))
+        return r.value
+
     def direct_f64_sample(self, pos, mass, sample, G=1.0, eps=0.025):
         n = len(mass)
         sample = np.ascontiguousarray(sample, np.uint64)

@@ -408,6 +408,13 @@ int gt_hip_debug_sort_waits(gt_hip_context *ctx, uint32_t polls);
 
 /* ---- measurement helpers ---- */
 
+/* The GPU's sustained FP32 VALU rate, in lane operations per second (no
+ * counterpart in the reference; the practical ceiling of the VALU-bound
+ * kernels, which are built without packed FP32): every lane runs 16
+ * independent FMA chains at eight waves per SIMD, best of 3 timed launches;
+ * rsq = 1 makes every fourth operation a v_rsq_f32. */
+int gt_hip_debug_valu_rate(gt_hip_context *ctx, int rsq, double *lane_ops_per_s);
+
 /* FP64 direct summation for a sample of targets (apps/galactic_tides_bench.cpp:80-95):
  * acc is ns x 3: the reference solution force errors are measured against. */
 int gt_hip_direct_f64_sample(gt_hip_context *ctx, uint64_t n, const double *pos,
