@@ -74,7 +74,8 @@ constexpr int kUndecided = -1;
 struct GroupImage {
   float cx, cy, cz, hx, hy, hz;
   float eo, ed;
-  bool usable;  // H in the filter's range
+  float es1, es0;  // the distance test's bound: es1 sum |d| + es0 + ... (accept_filter)
+  bool usable;     // H in the filter's range
 };
 
 #pragma clang fp contract(off)
@@ -115,6 +116,10 @@ GTH_HD inline GroupImage group_image(double ox, double oy, double oz, double H, 
   const double u = 5.9604644775390625e-08;  // 2^-24
   g.eo = static_cast<float>(16.0 * u * H);
   g.ed = static_cast<float>(8.0 * u * H);
+  // 1.5 (2 ed) and 1.5 (6 ed^2), a thousandth larger (the bound is far from
+  // tight; the margin covers their own rounding)
+  g.es1 = static_cast<float>(1.001 * 3.0 * 8.0 * u * H);
+  g.es0 = static_cast<float>(1.001 * 9.0 * (8.0 * u * H) * (8.0 * u * H));
   const double lim = 1.01 * H;
   g.usable = H > 9.094947017729282e-13 && H < 1099511627776.0 &&  // 2^-40, 2^40
              fabs(gcx - ox) <= lim && fabs(gcy - oy) <= lim && fabs(gcz - oz) <= lim &&
@@ -152,29 +157,30 @@ GTH_HD inline float half_image(float root_half32, int depth) { return ldexpf(roo
 // 1 accept, 0 no, kUndecided: the exact test decides.  half32 is the node's
 // half size image (half_image).
 GTH_HD inline int accept_filter(const WalkRec32 &r, float half32, const GroupImage &g) {
-  const float b = 1.0001f;
-  const float ax = fabsf(r.cx - g.cx) - (half32 * b + g.hx);
-  const float ay = fabsf(r.cy - g.cy) - (half32 * b + g.hy);
-  const float az = fabsf(r.cz - g.cz) - (half32 * b + g.hz);
+  const float hb = half32 * 1.0001f;
+  const float ax = fabsf(r.cx - g.cx) - (hb + g.hx);
+  const float ay = fabsf(r.cy - g.cy) - (hb + g.hy);
+  const float az = fabsf(r.cz - g.cz) - (hb + g.hz);
   // overlap fails on an axis where a > 0: sure when a > eo on some axis,
-  // sure not when a < -eo on every axis
-  const bool apart = ax > g.eo || ay > g.eo || az > g.eo;
-  const bool overlaps = ax < -g.eo && ay < -g.eo && az < -g.eo;
-  if (overlaps) return 0;
+  // sure not when a < -eo on every axis (both read off the largest a)
+  const float amax = fmaxf(fmaxf(ax, ay), az);
+  if (amax < -g.eo) return 0;
+  const bool apart = amax > g.eo;
   const float dx = fabsf(r.mx - g.cx) - g.hx;
   const float dy = fabsf(r.my - g.cy) - g.hy;
   const float dz = fabsf(r.mz - g.cz) - g.hz;
-  float d2 = 0.0f;
-  if (dx > 0.0f) d2 += dx * dx;
-  if (dy > 0.0f) d2 += dy * dy;
-  if (dz > 0.0f) d2 += dz * dz;
+  // the squares of the positive parts (FMAs: one rounding each, which keeps
+  // within the error bound below)
+  const float px = fmaxf(dx, 0.0f), py = fmaxf(dy, 0.0f), pz = fmaxf(dz, 0.0f);
+  const float d2 = fmaf(pz, pz, fmaf(py, py, px * px));
   const float crit = fabsf(r.crit);
   const float c2 = crit * crit;
   if (!(c2 > 1e-30f)) return kUndecided;
   const float s = d2 - c2;
   const float sum = fabsf(dx) + fabsf(dy) + fabsf(dz);
-  const float es =
-      1.5f * (2.0f * g.ed * sum + 6.0f * g.ed * g.ed + 2.384185791015625e-07f * (d2 + c2));
+  // 1.5 (2 ed sum + 6 ed^2 + 2^-22 (d2 + c2)), with the per-group parts
+  // folded into es1 and es0
+  const float es = fmaf(g.es1, sum, fmaf(3.5762786865234375e-07f, d2 + c2, g.es0));
   if (s < -es) return 0;
   if (apart && s > es) return 1;
   return kUndecided;
