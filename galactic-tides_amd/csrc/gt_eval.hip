@@ -237,27 +237,32 @@ __device__ inline void relative(float &x, float &y, float &z, const float4 *__re
   }
 }
 
+// Cell idx of a list.  A lane past the end of the list (valid false) holds
+// the list's last cell (idx is that cell) with its mass and quadrupole set
+// to zero: an accepted cell is apart from every target of the group, so its
+// separation is nonzero and a massless copy adds exactly nothing.  (No
+// branch: the entry's registers are the same on every lane.)
 template <bool QUAD, bool HILO>
 __device__ inline Entry load_cell(const Cell *__restrict__ cells, uint32_t idx, bool valid, float G,
                                   const HiLo &h) {
-  Entry e = {kFar, kFar, kFar, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (valid) {
-    const float4 cm = cells[idx].cm;
-    e.x = cm.x;
-    e.y = cm.y;
-    e.z = cm.z;
-    relative<HILO>(e.x, e.y, e.z, h.cells_lo, idx, h);
-    e.ngm = -G * cm.w;
-    if (QUAD) {
-      const float4 a = cells[idx].q0;
-      const float2 b = *reinterpret_cast<const float2 *>(&cells[idx].q1);
-      e.q0 = G * a.x;
-      e.q1 = G * a.y;
-      e.q2 = G * a.z;
-      e.q3 = G * a.w;
-      e.q4 = G * b.x;
-      e.q5 = G * b.y;
-    }
+  Entry e;
+  const float4 cm = cells[idx].cm;
+  e.x = cm.x;
+  e.y = cm.y;
+  e.z = cm.z;
+  relative<HILO>(e.x, e.y, e.z, h.cells_lo, idx, h);
+  const float gv = valid ? G : 0.0f;
+  e.ngm = -gv * cm.w;
+  e.q0 = e.q1 = e.q2 = e.q3 = e.q4 = e.q5 = 0.0f;
+  if (QUAD) {
+    const float4 a = cells[idx].q0;
+    const float2 b = *reinterpret_cast<const float2 *>(&cells[idx].q1);
+    e.q0 = gv * a.x;
+    e.q1 = gv * a.y;
+    e.q2 = gv * a.z;
+    e.q3 = gv * a.w;
+    e.q4 = gv * b.x;
+    e.q5 = gv * b.y;
   }
   return e;
 }
@@ -327,16 +332,23 @@ __device__ inline void eval_batch(const float4 *__restrict__ bodies, const Cell 
   for (int k = 0; k < 4 * TB; ++k) acc[k] = 0.0f;
 
   {
-    const uint32_t cb = r.cb, ce = r.ce;
+    // (zb keeps the compiler from hoisting the cell list's addresses out of
+    // the batch loop, where they would be spilled: recomputing them is cheaper
+    // and takes ~110 MB of spill writes off a 2M evaluation; the same for the
+    // body list made the evaluation slower)
+    const uint32_t zb = opaque_zero();
+    const uint32_t cb = r.cb + zb, ce = r.ce;
     uint32_t k = cb + lane;
-    uint32_t idx = k < ce ? cell_list[k] : 0u;
-    uint32_t idx_next = k + kWave < ce ? cell_list[k + kWave] : 0u;
+    // lanes past the end take the list's last cell (see load_cell)
+    const uint32_t last = ce > cb ? cell_list[ce - 1] : 0u;
+    uint32_t idx = k < ce ? cell_list[k] : last;
+    uint32_t idx_next = k + kWave < ce ? cell_list[k + kWave] : last;
     Entry e = load_cell<QUAD, HILO>(cells, idx, k < ce, G, h);
     for (uint32_t base = cb; base < ce; base += kWave) {
       const uint32_t kn = base + kWave + lane;
       const Entry en = load_cell<QUAD, HILO>(cells, idx_next, kn < ce, G, h);
       const uint32_t k2 = kn + kWave;
-      idx_next = k2 < ce ? cell_list[k2] : 0u;
+      idx_next = k2 < ce ? cell_list[k2] : last;
       const uint32_t z = opaque_zero();
 #pragma unroll
       for (int t = 0; t < TB; ++t) {
