@@ -330,6 +330,38 @@ __global__ __launch_bounds__(kWalkBlock) void walk(
   groups[g] = G;
 }
 
+// Sums over the GL lanes that serve one group (GL = 4 or 8 consecutive lanes,
+// inside one row of 16) by DPP moves: the group's total (quad swaps, then the
+// other quad of the eight by the half-row mirror), and the sum over this lane
+// (k of the group) and the lanes above it (row shifts, masked at the group's
+// end).
+template <int CTRL>
+__device__ inline uint32_t dpp_u32(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xf, 0xf, false));
+}
+template <int GL>
+__device__ inline uint32_t lanes_total(uint32_t v) {
+  static_assert(GL == 4 || GL == 8, "4 or 8 lanes per group");
+  v += dpp_u32<0xb1>(v);  // quad_perm [1,0,3,2]
+  v += dpp_u32<0x4e>(v);  // quad_perm [2,3,0,1]
+  if (GL == 8) v += dpp_u32<0x141>(v);  // row_half_mirror
+  return v;
+}
+template <int GL>
+__device__ inline uint32_t lanes_suffix(uint32_t v, uint32_t k) {
+  static_assert(GL == 4 || GL == 8, "4 or 8 lanes per group");
+  const uint32_t u1 = dpp_u32<0x101>(v);  // row_shl:1
+  if (k + 1 < GL) v += u1;
+  const uint32_t u2 = dpp_u32<0x102>(v);  // row_shl:2
+  if (k + 2 < GL) v += u2;
+  if (GL == 8) {
+    const uint32_t u4 = dpp_u32<0x104>(v);  // row_shl:4
+    if (k + 4 < GL) v += u4;
+  }
+  return v;
+}
+
 // Walk over child sets, eight lanes per group.  The nodes of one set (a
 // node's children, first to last) are tested together, one per lane of the
 // group's octet: their 32-byte images come in one round of independent loads,
@@ -449,42 +481,42 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
   uint32_t budget = 2 * nn + 2;
   const uint32_t shift = threadIdx.x & ~(GL - 1u);
   for (;;) {
-    if (!done && (budget-- == 0 || n > kSetMax || fc >= nn || n > nn - fc || d > 63))
+    if (!done &&
+        (budget-- == 0 || n == 0 || n > kSetMax || fc >= nn || n > nn - fc || d > 63))
       failed = done = true;
     if (!__any(!done)) break;
     if (done) continue;
     WalkRec32 im[kPer];
     uint2 bs[kPer];
     int code[kPer];
+    // lanes past the end of the set read its last node again (no branch, no
+    // defaults to set; their results are masked below)
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r) {
-      const uint32_t c = k + GL * r;
-      im[r] = WalkRec32{};
-      bs[r] = uint2{};
-      if (c < n) {
-        im[r] = images[fc + c];
-        bs[r] = span[fc + c];
-      }
+      const uint32_t at = fc + min(k + GL * r, n - 1u);
+      im[r] = images[at];
+      bs[r] = span[at];
     }
     uint32_t open_bits = 0, cell_bits = 0;
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r) {
       const uint32_t c = k + GL * r;
-      code[r] = -1;
-      if (c < n) {
-        int dec = gim.usable
-                      ? accept_filter(im[r], half_image(root_half32, static_cast<int>(d)), gim)
-                      : kUndecided;
-        if (dec == kUndecided) {
-          const WalkRec x = recs[fc + c];
-          dec = accept_exact(x.cx, x.cy, x.cz, x.mx, x.my, x.mz,
-                             ldexp(root_half, -static_cast<int>(d)), x.crit, gcx, gcy, gcz, ghx,
-                             ghy, ghz);
-          ++exact_tests;
-        }
-        const bool small = signbit(im[r].crit), leaf = (im[r].meta & kMetaLeaf) != 0u;
-        code[r] = (dec == 1 && !small) ? kCellCode : (dec == 1 || leaf) ? kBodiesCode : kOpenCode;
+      const bool in_set = c < n;
+      int dec = gim.usable
+                    ? accept_filter(im[r], half_image(root_half32, static_cast<int>(d)), gim)
+                    : kUndecided;
+      if (in_set && dec == kUndecided) {
+        const WalkRec x = recs[fc + c];
+        dec = accept_exact(x.cx, x.cy, x.cz, x.mx, x.my, x.mz,
+                           ldexp(root_half, -static_cast<int>(d)), x.crit, gcx, gcy, gcz, ghx,
+                           ghy, ghz);
+        ++exact_tests;
       }
+      const bool small = signbit(im[r].crit), leaf = (im[r].meta & kMetaLeaf) != 0u;
+      code[r] = !in_set                  ? -1
+                : (dec == 1 && !small)   ? kCellCode
+                : (dec == 1 || leaf)     ? kBodiesCode
+                                         : kOpenCode;
       open_bits |= (static_cast<uint32_t>(__ballot(code[r] == kOpenCode) >> shift) & kLaneMask)
                    << (GL * r);
       cell_bits |= (static_cast<uint32_t>(__ballot(code[r] == kCellCode) >> shift) & kLaneMask)
@@ -500,24 +532,29 @@ __global__ __launch_bounds__(kWalkBlock) void walk_sets(
       const uint32_t c = k + GL * r;
       const bool mine = c < n && ((emit >> c) & 1u);
       cnt[r] = (mine && code[r] == kBodiesCode) ? bs[r].y - bs[r].x : 0u;
-      suffix[r] = cnt[r];
-      for (uint32_t off = 1; off < GL; off <<= 1) {
-        const uint32_t u = __shfl_down(suffix[r], off, GL);
-        if (k + off < GL) suffix[r] += u;
-      }
-      slot_total[r] = __shfl(suffix[r], 0, GL);
+      suffix[r] = lanes_suffix<GL>(cnt[r], k);
+      slot_total[r] = lanes_total<GL>(cnt[r]);
       if (mine && code[r] == kCellCode) {
         const uint32_t at = nc + __popc(cell_bits & emit & (0xfeu << c));
         if (MODE == kWalkExact || at < cap_c) cell_list[cpos + at] = fc + c;
       }
     }
+    // Each lane writes its own node's bodies: up to eight as straight-line
+    // predicated stores at immediate offsets (a leaf holds at most leaf_size
+    // bodies, eight by default), the rest of a larger leaf in a loop.
     uint32_t higher_slots = 0;  // bodies of the slots above
 #pragma unroll
     for (int r = static_cast<int>(kPer) - 1; r >= 0; --r) {
       const uint32_t first = nb + higher_slots + (suffix[r] - cnt[r]);
-      for (uint32_t j = 0; j < cnt[r]; ++j) {
-        const uint32_t at = first + j;
-        if (MODE == kWalkExact || at < cap_b) body_list[bpos + at] = bs[r].x + j;
+      uint32_t m = cnt[r];
+      if (MODE != kWalkExact) m = min(m, __builtin_elementwise_sub_sat(cap_b, first));  // room left
+      if (m) {
+        uint32_t *const out = body_list + (bpos + first);
+        const uint32_t x = bs[r].x;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+          if (j < m) out[j] = x + j;
+        for (uint32_t j = 8; j < m; ++j) out[j] = x + j;
       }
       higher_slots += slot_total[r];
     }
