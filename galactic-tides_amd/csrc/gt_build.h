@@ -51,11 +51,11 @@ class TreeBuilder {
   uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0, systems_ = 1;
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
-  DBuf<uint64_t> keys_, keys_alt_, ndep_, ndep_alt_, word0_, word1_, word2_;
-  DBuf<uint32_t> vals_, vals_alt_, starts_, offs_, nbeg_, nbeg_alt_, lstart_, parent_, last_child_,
-      sid_, tsid_, root_node_;
-  DBuf<uint8_t> dg_, lcp_, ldep_;
-  DBuf<unsigned long long> sort_tmp_, scan_tmp_;
+  DBuf<uint64_t> keys_, keys_alt_, word0_, word1_, word2_;
+  DBuf<uint32_t> vals_, vals_alt_, level_counts_, level_offsets_, nbeg_, lstart_, parent_, sid_,
+      tsid_, root_node_;
+  DBuf<uint8_t> dg_, lcp_, ldep_, ndep_;
+  DBuf<unsigned long long> sort_tmp_, level_scan_;
   DBuf<int4> lnk_;
 };
 

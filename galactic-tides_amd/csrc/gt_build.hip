@@ -287,262 +287,309 @@ __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restri
   lcp[p] = static_cast<uint8_t>(min(common, cap));
 }
 
-// Leaf depth of sorted position p and the number of nodes beginning at p.
-// best = max over the windows j .. j+leaf of leaf+1 particles that contain p
-// and fit in [0, n) of the window's min lcp.  Up to kTileLeaf the block puts
-// its lcp values (with leaf on either side) in LDS and takes the minima of
-// all windows starting in its range once (W), then each position's maximum
-// over its leaf+1 windows; for larger leaves, the windows directly.
+// Leaf depth of sorted position p, and the tile's count of the nodes that
+// begin in it per depth (depth-major, for the row scan; workgroup 0 also
+// clears the scan's clear_words status words).  best = max over the windows
+// j .. j+leaf of leaf+1 particles that contain p and fit in [0, n) of the
+// window's min lcp.  Up to kTileLeaf the block puts its lcp values (with leaf
+// on either side) in LDS and takes the minima of all windows starting in its
+// range once (W), then each position's maximum over its leaf+1 windows; for
+// larger leaves, the windows directly.
 constexpr uint32_t kTileLeaf = 64;
+
+// Positions per workgroup of leaf_depths and emit_levels (a tile of the
+// per-depth node counts): four per thread, thread k's being base + i * kBlock
+// + k.
+constexpr uint32_t kLevelItems = 4;
+constexpr uint32_t kLevelTile = kBlock * kLevelItems;
 
 __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict__ lcp,
                                                       const uint32_t *__restrict__ vals,
                                                       const uint8_t *__restrict__ dg, uint32_t n,
                                                       uint32_t leaf, int cap,
-                                                      uint8_t *__restrict__ ldep,
-                                                      uint32_t *__restrict__ starts,
+                                                      uint8_t *__restrict__ ldep, uint32_t rows,
+                                                      uint32_t *__restrict__ level_counts,
+                                                      unsigned long long *__restrict__ clear,
+                                                      uint32_t clear_words,
                                                       uint32_t *__restrict__ unresolved) {
-  __shared__ int8_t sl[kBlock + 2 * kTileLeaf];  // shared(lcp[k]) for k = base - leaf ..
-  __shared__ int8_t win[kBlock + kTileLeaf];     // W(j) for j = base - leaf ..
-  const uint32_t base = blockIdx.x * kBlock;
-  const uint32_t p = base + threadIdx.x;
-  int best = -1;
+  __shared__ int8_t sl[kLevelTile + 2 * kTileLeaf];  // shared(lcp[k]) for k = base - leaf ..
+  __shared__ int8_t win[kLevelTile + kTileLeaf];     // W(j) for j = base - leaf ..
+  __shared__ uint32_t hist[kDepthLimit + 2];         // the tile's nodes per depth
+  if (blockIdx.x == 0)
+    for (uint32_t t = threadIdx.x; t < clear_words; t += kBlock) clear[t] = 0ull;
+  for (uint32_t t = threadIdx.x; t < rows; t += kBlock) hist[t] = 0u;
+  const uint32_t base = blockIdx.x * kLevelTile;
+  int best[kLevelItems];
+#pragma unroll
+  for (uint32_t i = 0; i < kLevelItems; ++i) best[i] = -1;
   if (leaf <= kTileLeaf) {
-    // windows start at j in [base - leaf, base + kBlock) and read lcp[j .. j + leaf - 1]
+    // windows start at j in [base - leaf, base + kLevelTile) and read lcp[j .. j + leaf - 1]
     const int64_t k0 = static_cast<int64_t>(base) - leaf;
-    const uint32_t nk = kBlock + 2 * leaf;
+    const uint32_t nk = kLevelTile + 2 * leaf;
     for (uint32_t t = threadIdx.x; t < nk; t += kBlock) {
       const int64_t k = k0 + t;
       sl[t] = static_cast<int8_t>(k >= 0 && k + 1 < static_cast<int64_t>(n) ? shared(lcp[k]) : 0);
     }
     __syncthreads();
-    const uint32_t nj = kBlock + leaf;
+    const uint32_t nj = kLevelTile + leaf;
     for (uint32_t t = threadIdx.x; t < nj; t += kBlock) {
       int w = cap;
       for (uint32_t i = 0; i < leaf; ++i) w = min(w, static_cast<int>(sl[t + i]));
       win[t] = static_cast<int8_t>(w);
     }
     __syncthreads();
-    if (p < n && n > leaf) {
-      const uint32_t jlo = p >= leaf ? p - leaf : 0u;
-      const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
-      for (uint32_t j = jlo; j <= jhi; ++j)
-        best = max(best, static_cast<int>(win[j - (base - leaf)]));
+#pragma unroll
+    for (uint32_t i = 0; i < kLevelItems; ++i) {
+      const uint32_t p = base + i * kBlock + threadIdx.x;
+      if (p < n && n > leaf) {
+        const uint32_t jlo = p >= leaf ? p - leaf : 0u;
+        const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
+        for (uint32_t j = jlo; j <= jhi; ++j)
+          best[i] = max(best[i], static_cast<int>(win[j - (base - leaf)]));
+      }
     }
-  } else if (p < n && n > leaf) {
-    const uint32_t jlo = p >= leaf ? p - leaf : 0u;
-    const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
-    for (uint32_t j = jlo; j <= jhi; ++j) {
-      int w = cap;
-      for (uint32_t k = j; k < j + leaf; ++k) w = min(w, shared(lcp[k]));
-      best = max(best, w);
+  } else {
+#pragma unroll
+    for (uint32_t i = 0; i < kLevelItems; ++i) {
+      const uint32_t p = base + i * kBlock + threadIdx.x;
+      if (p < n && n > leaf) {
+        const uint32_t jlo = p >= leaf ? p - leaf : 0u;
+        const uint32_t jhi = p < n - 1 - leaf ? p : n - 1 - leaf;
+        for (uint32_t j = jlo; j <= jhi; ++j) {
+          int w = cap;
+          for (uint32_t k = j; k < j + leaf; ++k) w = min(w, shared(lcp[k]));
+          best[i] = max(best[i], w);
+        }
+      }
     }
   }
-  if (p >= n) return;
-  int d = best + 1;
-  d = min(d, static_cast<int>(dg[vals[p]]));
-  d = min(d, kDepthLimit);
-  // Deeper than the path words computed: reported, and the tree is cut at
-  // `cap` levels (nodes exist only where their octant digits do), to be built
-  // again with another word.
-  if (d > cap) atomicOr(unresolved, 1u);
-  d = min(d, cap);
-  ldep[p] = static_cast<uint8_t>(d);
-  const int first = first_level(lcp, p);
-  starts[p] = d >= first ? static_cast<uint32_t>(d - first + 1) : 0u;
+  __syncthreads();  // (hist cleared)
+#pragma unroll
+  for (uint32_t i = 0; i < kLevelItems; ++i) {
+    const uint32_t p = base + i * kBlock + threadIdx.x;
+    if (p >= n) continue;
+    int d = best[i] + 1;
+    d = min(d, static_cast<int>(dg[vals[p]]));
+    d = min(d, kDepthLimit);
+    // Deeper than the path words computed: reported, and the tree is cut at
+    // `cap` levels (nodes exist only where their octant digits do), to be
+    // built again with another word.
+    if (d > cap) atomicOr(unresolved, 1u);
+    d = min(d, cap);
+    ldep[p] = static_cast<uint8_t>(d);
+    // the nodes beginning at p: depths first .. d
+    for (int t = first_level(lcp, p); t <= d; ++t) atomicAdd(&hist[t], 1u);
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < rows; t += kBlock)
+    level_counts[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = hist[t];
 }
 
-// ---- 5. preorder enumeration ---------------------------------------------------------
+// ---- 5-6. nodes in level order, links, geometry ---------------------------------------
 
-__global__ __launch_bounds__(kBlock) void emit_nodes(const uint8_t *__restrict__ lcp,
-                                                     const uint32_t *__restrict__ starts,
-                                                     const uint32_t *__restrict__ offs, uint32_t n,
-                                                     const unsigned long long *__restrict__ frame,
-                                                     uint32_t node_cap,
-                                                     uint64_t *__restrict__ node_depth,
-                                                     uint32_t *__restrict__ node_begin) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n || frame[frame::kNodes] > node_cap) return;  // over capacity: no tree
-  const uint32_t c = starts[p];
-  const uint32_t first = static_cast<uint32_t>(first_level(lcp, p));
-  for (uint32_t t = 0; t < c; ++t) {
-    node_depth[offs[p] + t] = first + t;
-    node_begin[offs[p] + t] = p;
+// Nodes straight into level order: position p (sorted) begins the nodes of
+// depths first_level(p) .. ldep[p] (none when first > ldep), and a level's
+// nodes are in order of the position they begin at, which is the reference's
+// child order (octant order) under every parent.  level_offsets, the row scan
+// of the tiles' node counts per depth (rows = deepest possible level + 2; the
+// last row is empty, so its start is the node total), gives the index of the
+// tile's first node of each depth; a node's rank among its tile's nodes of
+// that depth comes from wave ballots over the depths the wave's nodes span.
+// Per node it writes its depth, its begin, its parent (p's node one level up,
+// or for the first node at p the last node one level up that begins before
+// p: the one containing p) and its first child (p's node one level down; a
+// node at p's leaf depth is a leaf).  Workgroup 0 writes the level starts
+// (lstart[d]; levels past the deepest 0, lstart[deepest + 1] = the total), the
+// level sizes, the node total and the depth into the frame.  A tree with more
+// nodes than node_cap is not written.
+constexpr int kBuildWaves = kBlock / 64;
+constexpr int kRankSlots = kLevelItems * kBuildWaves;  // (item, wave) in position order
+
+__device__ inline int wave_min(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void emit_levels(
+    const uint8_t *__restrict__ lcp, const uint8_t *__restrict__ ldep, uint32_t n,
+    uint32_t rows, const uint32_t *__restrict__ level_offsets,
+    unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t *__restrict__ lstart,
+    int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ node_begin,
+    uint8_t *__restrict__ node_depth) {
+  __shared__ uint32_t rank0[kRankSlots][kDepthLimit + 1];  // counts, then their prefix
+  const uint32_t ntiles = gridDim.x, tile = blockIdx.x;
+  const int tmax = static_cast<int>(rows) - 2;  // deepest possible level
+  const uint32_t total = level_offsets[static_cast<size_t>(rows - 1) * ntiles];
+  if (tile == 0) {
+    // the deepest level with nodes (every thread finds it; rows <= 62)
+    int deepest = 0;
+    for (int t = 1; t <= tmax; ++t)
+      if (level_offsets[static_cast<size_t>(t + 1) * ntiles] >
+          level_offsets[static_cast<size_t>(t) * ntiles])
+        deepest = t;
+    for (int t = static_cast<int>(threadIdx.x); t < kDepthLimit + 2; t += kBlock)
+      lstart[t] = t <= deepest + 1 ? level_offsets[static_cast<size_t>(t) * ntiles] : 0u;
+    for (int t = static_cast<int>(threadIdx.x); t < frame::kLevels; t += kBlock)
+      frame[frame::kLevelNodes + t] =
+          t <= deepest ? level_offsets[static_cast<size_t>(t + 1) * ntiles] -
+                             level_offsets[static_cast<size_t>(t) * ntiles]
+                       : 0u;
+    if (threadIdx.x == 0) {
+      frame[frame::kNodes] = total;
+      frame[frame::kDepth] = static_cast<unsigned long long>(deepest);
+    }
+  }
+  if (total > node_cap) return;  // over capacity: no tree (the whole grid)
+  for (uint32_t k = threadIdx.x; k < kRankSlots * (kDepthLimit + 1); k += kBlock)
+    (&rank0[0][0])[k] = 0u;
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const uint32_t base = tile * kLevelTile + threadIdx.x;
+  int first[kLevelItems], d[kLevelItems];
+  int lo = kDepthLimit + 1, hi = -1;  // the depths of the wave's nodes
+#pragma unroll
+  for (uint32_t i = 0; i < kLevelItems; ++i) {
+    const uint32_t p = base + i * kBlock;
+    first[i] = 1;
+    d[i] = 0;  // (no nodes past n)
+    if (p < n) {
+      first[i] = first_level(lcp, p);
+      d[i] = ldep[p];
+    }
+    if (first[i] <= d[i]) {
+      lo = min(lo, first[i]);
+      hi = max(hi, d[i]);
+    }
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  __syncthreads();  // (counts cleared)
+#pragma unroll
+  for (uint32_t i = 0; i < kLevelItems; ++i)
+    for (int t = lo; t <= hi; ++t) {
+      const uint64_t m = __ballot(first[i] <= t && t <= d[i]);
+      if (lane == 0) rank0[i * kBuildWaves + wave][t] = static_cast<uint32_t>(__popcll(m));
+    }
+  __syncthreads();
+  for (int t = static_cast<int>(threadIdx.x); t <= tmax; t += kBlock) {
+    uint32_t run = 0;
+    for (int k = 0; k < kRankSlots; ++k) {
+      const uint32_t c = rank0[k][t];
+      rank0[k][t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  if (hi < 0) return;  // no nodes begin in this wave
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
+  const int t0 = lo > 0 ? lo - 1 : 0;  // from the level above (the first nodes' parents)
+#pragma unroll
+  for (uint32_t i = 0; i < kLevelItems; ++i) {
+    const uint32_t p = base + i * kBlock;
+    uint32_t above = kNone;  // the last node of depth t - 1 that begins at or before p
+    uint32_t prev = kNone;   // p's node of depth t - 1
+    for (int t = t0; t <= hi; ++t) {
+      const bool in = first[i] <= t && t <= d[i];
+      const uint64_t m = __ballot(in);
+      const uint32_t at = level_offsets[static_cast<size_t>(t) * ntiles + tile] +
+                          rank0[i * kBuildWaves + wave][t] +
+                          static_cast<uint32_t>(__popcll(m & below));
+      if (in) {  // p's node of depth t is node `at`
+        parent[at] = t == 0 ? kNone : above;
+        node_begin[at] = p;
+        node_depth[at] = static_cast<uint8_t>(t);
+        if (t > first[i]) lnk[prev] = make_int4(static_cast<int>(at), 0, static_cast<int>(p), 0);
+        if (t == d[i]) lnk[at] = make_int4(-1, 0, static_cast<int>(p), 0);
+        prev = at;
+      }
+      above = in ? at : at - 1u;  // (at - 1: this tile's or an earlier tile's last before p)
+    }
   }
 }
 
-// lstart[d] = first node of level d (levels past the deepest stay 0), and the
-// depth of the tree into the frame.
-// frame[kLevelNodes + d] = nodes on level d (the host sizes the next moments
-// launches from them).  One workgroup of 64.
-__global__ void level_counts(const uint32_t *__restrict__ lstart,
-                             unsigned long long *__restrict__ frame) {
-  for (int d = threadIdx.x; d < frame::kLevels; d += blockDim.x) {
-    const uint32_t a = lstart[d], b = lstart[d + 1];
-    frame[frame::kLevelNodes + d] = b > a ? b - a : 0u;
-  }
-}
+// Per node: its centre and half size (the octant path of its first body
+// replayed from its system's root with the reference's FP64 centre updates:
+// with one path word and one system, sorted_keys set, the path is the sorted
+// key at the node's begin shifted back into place, else the path words of
+// particle vals[begin]), its child count (its children are consecutive on the
+// level below, from its first child on while their parent is the node: at
+// most eight, read together), each system's root node, and a leaf's end: the
+// first position q + 1 after its begin whose predecessor q shares fewer than
+// the leaf's depth levels with it (lcp, read sixteen at a time).  An inner
+// node's end is its last child's, set with its moments.
+constexpr uint32_t kEndChunk = 16;
 
-__global__ __launch_bounds__(kBlock) void level_starts(const uint64_t *__restrict__ node_depth,
-                                                       unsigned long long *__restrict__ frame,
-                                                       uint32_t node_cap,
-                                                       uint32_t *__restrict__ lstart) {
+__global__ __launch_bounds__(kBlock) void node_links(
+    const uint32_t *__restrict__ parent, const uint32_t *__restrict__ node_begin,
+    const uint8_t *__restrict__ node_depth, const uint8_t *__restrict__ lcp,
+    const unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t n,
+    const uint64_t *__restrict__ w0, const uint64_t *__restrict__ w1,
+    const uint64_t *__restrict__ w2, const uint32_t *__restrict__ vals,
+    const uint64_t *__restrict__ sorted_keys, int key_shift, const double *__restrict__ roots,
+    const uint32_t *__restrict__ sid, double *__restrict__ geom, int4 *__restrict__ lnk,
+    uint32_t *__restrict__ root_node) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
-  const uint32_t d = static_cast<uint32_t>(node_depth[x]);
-  if (x == 0 || static_cast<uint32_t>(node_depth[x - 1]) != d) lstart[d] = x;
-  if (x == n_nodes - 1) {
-    lstart[d + 1] = n_nodes;
-    frame[frame::kDepth] = d;
+  const uint32_t d = node_depth[x], b = node_begin[x];
+  const int first = lnk[x].x;
+  // centre and half size
+  uint64_t kw[3] = {0ull, 0ull, 0ull};
+  uint32_t system = 0;
+  if (sorted_keys) {
+    kw[0] = sorted_keys[b] << key_shift;
+  } else {
+    const uint32_t i = vals[b];
+    system = sid ? sid[i] : 0u;
+    kw[0] = w0[i];
+    kw[1] = d > static_cast<uint32_t>(kLevels) ? w1[i] : 0ull;
+    kw[2] = d > 2u * kLevels ? w2[i] : 0ull;
   }
-}
-
-// ---- 6. links, geometry, leaf order ------------------------------------------------
-
-// Largest i in [lo, hi] with begin[i] <= b (begin[lo] <= b, begins ascending).
-__device__ inline uint32_t last_begin_at_or_before(const uint32_t *begin, uint32_t lo,
-                                                   uint32_t hi, uint32_t b) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) / 2;
-    if (begin[mid] <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
+  const double *root = roots + 4 * static_cast<size_t>(system);
+  double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+  for (uint32_t l = 0; l < d; ++l) {
+    const uint64_t key = kw[l / kLevels];
+    const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
+    const double h2 = 0.5 * h;
+    cx = cx + ((o & 1u) ? h2 : -h2);
+    cy = cy + ((o & 2u) ? h2 : -h2);
+    cz = cz + ((o & 4u) ? h2 : -h2);
+    h = h2;
   }
-  return lo;
-}
-
-// Node x's centre and half size, from the octant path of its first body
-// replayed from its system's root, and its parent: the last node of the level
-// above beginning at or before x's begin.  With one path word and one system
-// (sorted_keys set) the path is the sorted key at the node's begin, shifted
-// back into place (it holds the levels the sort ordered by, which the tree
-// never exceeds), instead of the first body's path word gathered by particle
-// index.  A workgroup whose nodes are all on one level finds the parents of
-// its first and last node in global memory and the others in an LDS copy of
-// the begins between them.
-constexpr uint32_t kParentWindow = 2048;
-
-__global__ __launch_bounds__(kBlock) void link_nodes(
-    const uint64_t *__restrict__ node_depth, const uint32_t *__restrict__ node_begin,
-    const uint32_t *__restrict__ lstart, const uint64_t *__restrict__ w0,
-    const uint64_t *__restrict__ w1, const uint64_t *__restrict__ w2,
-    const uint32_t *__restrict__ vals, const uint64_t *__restrict__ sorted_keys, int key_shift,
-    const double *__restrict__ roots, const uint32_t *__restrict__ sid,
-    const unsigned long long *__restrict__ frame, uint32_t node_cap, double *__restrict__ geom,
-    int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ root_node) {
-  __shared__ uint32_t sbegin[kParentWindow];
-  __shared__ uint32_t bracket[2];
-  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t x0 = blockIdx.x * kBlock;
-  if (x0 >= n_nodes) return;  // the whole workgroup
-  const uint32_t x = x0 + threadIdx.x;
-  const uint32_t xl = min(x0 + kBlock, n_nodes) - 1;
-  const bool valid = x < n_nodes;
-  uint32_t d = 0, b = 0, system = 0;
-  if (valid) {
-    d = static_cast<uint32_t>(node_depth[x]);
-    b = node_begin[x];
-    // (the path words this depth needs, loaded once)
-    uint64_t kw[3] = {0ull, 0ull, 0ull};
-    if (sorted_keys) {
-      kw[0] = sorted_keys[b] << key_shift;
-    } else {
-      const uint32_t i = vals[b];
-      system = sid ? sid[i] : 0u;
-      kw[0] = w0[i];
-      kw[1] = d > static_cast<uint32_t>(kLevels) ? w1[i] : 0ull;
-      kw[2] = d > 2u * kLevels ? w2[i] : 0ull;
-    }
-    const double *root = roots + 4 * static_cast<size_t>(system);
-    double cx = root[0], cy = root[1], cz = root[2], h = root[3];
-    for (uint32_t l = 0; l < d; ++l) {
-      const uint64_t key = kw[l / kLevels];
-      const uint32_t o = static_cast<uint32_t>(key >> (3 * (kLevels - 1 - l % kLevels))) & 7u;
-      const double h2 = 0.5 * h;
-      cx = cx + ((o & 1u) ? h2 : -h2);
-      cy = cy + ((o & 2u) ? h2 : -h2);
-      cz = cz + ((o & 4u) ? h2 : -h2);
-      h = h2;
-    }
-    double *g = geom + static_cast<size_t>(kGeomStride) * x;
-    g[0] = cx;
-    g[1] = cy;
-    g[2] = cz;
-    g[3] = h;
-    g[15] = 0.0;
-    lnk[x] = make_int4(-1, 0, static_cast<int>(b), 0);
-    if (d == 0) {
-      parent[x] = kNone;
-      if (root_node) root_node[system] = x;
-    }
+  double *g = geom + static_cast<size_t>(kGeomStride) * x;
+  g[0] = cx;
+  g[1] = cy;
+  g[2] = cz;
+  g[3] = h;
+  g[15] = 0.0;
+  if (d == 0 && root_node) root_node[system] = x;
+  if (first >= 0) {  // children
+    const uint32_t c0 = static_cast<uint32_t>(first);
+    uint32_t up[7];
+#pragma unroll
+    for (uint32_t k = 0; k < 7; ++k) up[k] = c0 + 1 + k < n_nodes ? parent[c0 + 1 + k] : kNone;
+    int children = 1;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (children == k + 1 && up[k] == x) ++children;
+    lnk[x].y = children;
+    return;
   }
-  const uint32_t dfirst = static_cast<uint32_t>(node_depth[x0]);
-  const bool one_level = dfirst > 0 && dfirst == static_cast<uint32_t>(node_depth[xl]);
-  uint32_t up = kNone;
-  if (one_level) {
-    const uint32_t lo = lstart[dfirst - 1], hi = lstart[dfirst] - 1;
-    if (threadIdx.x == 0) bracket[0] = last_begin_at_or_before(node_begin, lo, hi, b);
-    if (x == xl) bracket[1] = last_begin_at_or_before(node_begin, lo, hi, b);
-    __syncthreads();
-    const uint32_t p0 = bracket[0], p1 = bracket[1];
-    if (p1 - p0 < kParentWindow) {
-      for (uint32_t t = threadIdx.x; t <= p1 - p0; t += kBlock) sbegin[t] = node_begin[p0 + t];
-      __syncthreads();
-      if (valid) up = p0 + last_begin_at_or_before(sbegin, 0, p1 - p0, b);
-    } else if (valid) {
-      up = last_begin_at_or_before(node_begin, p0, p1, b);
-    }
-  } else if (valid && d > 0) {
-    up = last_begin_at_or_before(node_begin, lstart[d - 1], lstart[d] - 1, b);
-  }
-  if (up != kNone) parent[x] = up;
-}
-
-// Children of a node are consecutive on the level below: the first child
-// (the one beginning where its parent begins) goes to its parent's link, the
-// last one (whose successor has another parent) to last_child[parent].
-__global__ __launch_bounds__(kBlock) void finish_links(const uint32_t *__restrict__ parent,
-                                                       const uint32_t *__restrict__ node_begin,
-                                                       const unsigned long long *__restrict__ frame,
-                                                       uint32_t node_cap, int4 *__restrict__ lnk,
-                                                       uint32_t *__restrict__ last_child) {
-  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  if (x >= n_nodes) return;
-  const uint32_t p = parent[x];
-  if (p == kNone) return;
-  if (node_begin[p] == node_begin[x]) lnk[p].x = static_cast<int>(x);
-  if (x + 1 >= n_nodes || parent[x + 1] != p) last_child[p] = x;
-}
-
-__global__ __launch_bounds__(kBlock) void node_ends(const uint32_t *__restrict__ parent,
-                                                    const uint32_t *__restrict__ node_begin,
-                                                    const uint64_t *__restrict__ node_depth,
-                                                    const uint32_t *__restrict__ last_child,
-                                                    const unsigned long long *__restrict__ frame,
-                                                    uint32_t node_cap, uint32_t n,
-                                                    int4 *__restrict__ lnk) {
-  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t x0 = blockIdx.x * kBlock + threadIdx.x;
-  if (x0 >= n_nodes) return;
-  const int first = lnk[x0].x;  // children: first .. last_child[x0]
-  lnk[x0].y = first >= 0 ? static_cast<int>(last_child[x0]) - first + 1 : 0;
-  uint32_t x = x0;
+  // a leaf (lnk[x].y stays 0): its end
+  uint32_t q = b;
   for (;;) {
-    const uint32_t p = parent[x];
-    if (p == kNone) {  // a root ends where the next system's root begins
-      const bool next_root = x + 1 < n_nodes && node_depth[x + 1] == 0;
-      lnk[x0].w = static_cast<int>(next_root ? node_begin[x + 1] : n);
-      return;
-    }
-    if (x + 1 < n_nodes && parent[x + 1] == p) {  // not its parent's last child
-      lnk[x0].w = static_cast<int>(x + 1 < n_nodes ? node_begin[x + 1] : n);
-      return;
-    }
-    x = p;
+    uint8_t c[kEndChunk];
+#pragma unroll
+    for (uint32_t k = 0; k < kEndChunk; ++k) c[k] = q + k + 1 < n ? lcp[q + k] : kNoShare;
+    uint32_t k = 0;
+    while (k < kEndChunk && shared(c[k]) >= static_cast<int>(d)) ++k;
+    q += k;
+    if (k < kEndChunk) break;
   }
+  lnk[x].w = static_cast<int>(q + 1);
 }
 
 // Bodies of a leaf back into particle order (insertion sort; leaves are small).
@@ -625,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
 // in order, so every sum is the reference's sequence.
 constexpr int kChunk = 8;
 
-__device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, uint32_t n_nodes, const int4 *__restrict__ lnk,
+__device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, uint32_t n_nodes, int4 *__restrict__ lnk,
                                     const double *__restrict__ tpos,
                                     const double *__restrict__ tmass,
                                     double *__restrict__ geom) {
@@ -696,6 +743,7 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   } else {
     const uint32_t c0 = static_cast<uint32_t>(L.x), nc = static_cast<uint32_t>(L.y);
     if (L.x < 0 || nc > kChunk || c0 + nc > n_nodes) return;  // malformed links: leave the row unset
+    lnk[x].w = lnk[c0 + nc - 1].w;  // the node ends where its last child does
     double hm[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], hq[kChunk][6];
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
@@ -756,7 +804,7 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
 __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *__restrict__ lstart,
                                                         const unsigned long long *__restrict__ frame,
                                                         uint32_t node_cap,
-                                                        const int4 *__restrict__ lnk,
+                                                        int4 *__restrict__ lnk,
                                                         const double *__restrict__ tpos,
                                                         const double *__restrict__ tmass,
                                                         double *__restrict__ geom) {
@@ -772,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
 // than the level's work.
 __global__ __launch_bounds__(kBlock) void level_pair_moments(
     int d, const uint32_t *__restrict__ lstart, const unsigned long long *__restrict__ frame,
-    uint32_t node_cap, const int4 *__restrict__ lnk, const double *__restrict__ tpos,
+    uint32_t node_cap, int4 *__restrict__ lnk, const double *__restrict__ tpos,
     const double *__restrict__ tmass, double *__restrict__ geom) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t first = lstart[d - 1], last = min(lstart[d], n_nodes);
@@ -799,7 +847,7 @@ constexpr int kSerialBlock = 512;
 
 __global__ __launch_bounds__(kSerialBlock) void serial_level_moments(
     int d_hi, int d_lo, const uint32_t *__restrict__ lstart,
-    const unsigned long long *__restrict__ frame, uint32_t node_cap, const int4 *__restrict__ lnk,
+    const unsigned long long *__restrict__ frame, uint32_t node_cap, int4 *__restrict__ lnk,
     const double *__restrict__ tpos, const double *__restrict__ tmass, double *__restrict__ geom) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   for (int d = d_hi; d >= d_lo; --d) {
@@ -833,21 +881,20 @@ void TreeBuilder::reserve(const Plan &p) {
   word0_.ensure(n);
   if (p.words > 1) word1_.ensure(n);
   if (p.words > 2) word2_.ensure(n);
-  sort_tmp_.ensure(radix_tmp_words(std::max(n, nn)));
+  sort_tmp_.ensure(radix_tmp_words(n));
   lcp_.ensure(n);
   ldep_.ensure(n);
-  starts_.ensure(n);
-  offs_.ensure(n);
-  scan_tmp_.ensure(scan_tmp_words(n));
-  ndep_.ensure(nn);
+  const size_t level_cells =
+      static_cast<size_t>(kDepthLimit + 2) * ((n + kLevelTile - 1) / kLevelTile);
+  level_counts_.ensure(level_cells);
+  level_offsets_.ensure(level_cells);
+  level_scan_.ensure(kDepthLimit + 3);
   nbeg_.ensure(nn);
-  ndep_alt_.ensure(nn);
-  nbeg_alt_.ensure(nn);
+  ndep_.ensure(nn);
   lstart_.ensure(kDepthLimit + 2);
   geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
   lnk_.ensure(nn);
   parent_.ensure(nn);
-  last_child_.ensure(nn);
   tpos_.ensure(3 * n);
   tmass_.ensure(n);
 }
@@ -930,53 +977,41 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                    frame + frame::kSortCtl);
   }
 
-  // 4. leaf depths and node counts per sorted position; the node total goes
-  // to frame[kNodes]
+  // 4. leaf depths, and each tile's node count per depth
   uint8_t *lcp = lcp_.get();
   uint8_t *ldep = ldep_.get();
-  uint32_t *starts = starts_.get();
-  uint32_t *offs = offs_.get();
   hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
                      p.words, levels, vals, sid, n, lcp);
   GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(leaf_depths, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf,
-                     levels, ldep, starts,
+  const uint32_t ntiles = static_cast<uint32_t>((n + kLevelTile - 1) / kLevelTile);
+  const uint32_t rows = static_cast<uint32_t>(std::min(levels, kDepthLimit)) + 2u;
+  uint32_t *level_counts = level_counts_.get();
+  uint32_t *level_offsets = level_offsets_.get();
+  unsigned long long *scan_words = level_scan_.get();  // status words, then the ticket
+  hipLaunchKernelGGL(leaf_depths, dim3(ntiles), dim3(kBlock), 0, s, lcp, vals, dg, n, leaf, levels,
+                     ldep, rows, level_counts, scan_words, rows + 1u,
                      reinterpret_cast<uint32_t *>(frame + frame::kUnresolved));
   GTH_LAUNCH_CHECK();
-  exclusive_scan_u32(starts, offs, n, frame + frame::kNodes, scan_tmp_.get(), s);
 
-  // 5. nodes in preorder, then stably by depth (level order)
-  uint64_t *ndep = ndep_.get();
-  uint32_t *nbeg = nbeg_.get();
-  hipLaunchKernelGGL(emit_nodes, dim3(blocks_for(n)), dim3(kBlock), 0, s, lcp, starts, offs, n,
-                     frame, cap, ndep, nbeg);
-  GTH_LAUNCH_CHECK();
-  radix_sort_u64_dev(ndep, nbeg, ndep_alt_.get(), nbeg_alt_.get(), frame + frame::kNodes, cap, 6,
-                     sort_tmp_.get(), s, frame + frame::kSortCtl);
+  // 5-6. nodes in level order, with parents and first children; then per
+  // node its geometry and child count, and a leaf's end (inner nodes' ends
+  // come with their moments)
+  launch_row_scan(rows, ntiles, 1u, reinterpret_cast<uint32_t *>(scan_words + rows), scan_words,
+                  frame + frame::kSortCtl, level_counts, level_offsets, s);
   uint32_t *lstart = lstart_.get();
-  launch_fill_u32(lstart, kDepthLimit + 2, 0u, s);
-  hipLaunchKernelGGL(level_starts, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, frame, cap,
-                     lstart);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(level_counts, dim3(1), dim3(64), 0, s, lstart, frame);
-  GTH_LAUNCH_CHECK();
-
-  // 6. links and geometry
   double *geom = geom_.get();
   int4 *lnk = lnk_.get();
   uint32_t *parent = parent_.get();
-  uint32_t *last_child = last_child_.get();
-  hipLaunchKernelGGL(link_nodes, dim3(blocks_for(cap)), dim3(kBlock), 0, s, ndep, nbeg, lstart,
-                     w[0], w[1], w[2], vals,
+  uint32_t *nbeg = nbeg_.get();
+  uint8_t *ndep = ndep_.get();
+  hipLaunchKernelGGL(emit_levels, dim3(ntiles), dim3(kBlock), 0, s, lcp, ldep, n, rows,
+                     level_offsets, frame, cap, lstart, lnk, parent, nbeg, ndep);
+  GTH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(node_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
+                     lcp, frame, cap, n, w[0], w[1], w[2], vals,
                      p.words == 1 && !batch ? keys : static_cast<const uint64_t *>(nullptr),
-                     p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, frame, cap,
-                     geom, lnk, parent, root_node);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finish_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, frame,
-                     cap, lnk, last_child);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(node_ends, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
-                     last_child, frame, cap, n, lnk);
+                     p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, geom, lnk,
+                     root_node);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, orig,
                      vals);

@@ -237,6 +237,13 @@ size_t radix_tmp_words(size_t n);
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                     size_t n, int bits, unsigned long long *tmp, hipStream_t s,
                     unsigned long long *ctl = nullptr);
+// The exclusive scan of a row-major table of `rows` (at most 256) rows of
+// ntiles counts, the radix sort's digit scan: offsets[r * ntiles + t] = all
+// counts of rows before r plus row r's counts of tiles before t.  ticket (one
+// word) and status (rows words) must be zero and epoch nonzero; ctl as above.
+void launch_row_scan(uint32_t rows, uint32_t ntiles, uint32_t epoch, uint32_t *ticket,
+                     unsigned long long *status, unsigned long long *ctl, const uint32_t *counts,
+                     uint32_t *offsets, hipStream_t s);
 // The same for the first bounded_count(n_dev, cap) pairs of arrays of cap entries.
 void radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                         const unsigned long long *n_dev, size_t cap, int bits,

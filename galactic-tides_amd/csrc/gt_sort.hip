@@ -323,6 +323,16 @@ size_t radix_tmp_words(size_t n) {
   return kStatusWords + 2 * histogram_words(ntiles);
 }
 
+void launch_row_scan(uint32_t rows, uint32_t ntiles, uint32_t epoch, uint32_t *ticket,
+                     unsigned long long *status, unsigned long long *ctl, const uint32_t *counts,
+                     uint32_t *offsets, hipStream_t s) {
+  if (rows == 0 || ntiles == 0) return;
+  if (rows > static_cast<uint32_t>(kDigits)) fail(GT_HIP_ERR_ARGUMENT, "row scan: too many rows");
+  hipLaunchKernelGGL(digit_row_scan, dim3(rows), dim3(kBlock), 0, s, ntiles, epoch, ticket, status,
+                     ctl, counts, offsets);
+  GTH_LAUNCH_CHECK();
+}
+
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                     size_t n, int bits, unsigned long long *tmp, hipStream_t s,
                     unsigned long long *ctl) {
