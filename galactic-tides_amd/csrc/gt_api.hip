@@ -127,6 +127,9 @@ struct gt_hip_context {
   DBuf<float4> bodies, out;
   DBuf<uint32_t> parent, skip, depth, flags, offs, starts, ccount, bcount, coff, boff, cell_list,
       body_list, group_root, flag_node, group_node;
+  // the node depths the last lists were built with (depth above, or the
+  // tree builder's own)
+  const uint32_t *lists_depth = nullptr;
   // systems of the current evaluation (a batch when more than one): count and
   // particle offsets in device memory
   uint32_t systems = 1;
@@ -605,6 +608,8 @@ void clear_counts(unsigned long long *frame, hipStream_t s) {
 // for a tree in device memory whose node count is frame[kNodes].
 // A tree the lists are built for (tsid/roots/root_node: a batch's systems,
 // null for one system).
+// parent/skip/depth: the tree's per-node links as TreeBuilder keeps them
+// (null for a caller's tree: then they are derived from lnk here).
 struct ListsInput {
   const double *geom = nullptr;
   const int4 *lnk = nullptr;
@@ -612,6 +617,7 @@ struct ListsInput {
   const uint32_t *tsid = nullptr;
   const double *roots = nullptr;
   const uint32_t *root_node = nullptr;
+  const uint32_t *parent = nullptr, *skip = nullptr, *depth = nullptr;
 };
 
 // Everything of the lists before the walk: links, records, groups, and the
@@ -622,14 +628,21 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
   reserve_all(c, p);
   const bool hilo = p.hilo;
   const uint32_t n = p.n, cap = p.node_cap;
-  launch_link_up(in.lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
-  launch_prep_lists(in.geom, in.lnk, c->skip.get(), c->depth.get(), in.tpos, in.tmass, n, frame,
+  const uint32_t *parent = in.parent, *skip = in.skip, *depth = in.depth;
+  if (!parent || !skip || !depth) {
+    launch_link_up(in.lnk, frame, cap, c->parent.get(), c->skip.get(), c->depth.get(), s);
+    parent = c->parent.get();
+    skip = c->skip.get();
+    depth = c->depth.get();
+  }
+  c->lists_depth = depth;
+  launch_prep_lists(in.geom, in.lnk, skip, depth, in.tpos, in.tmass, n, frame,
                     cap, cfg->theta, in.tsid, in.roots, c->recs.get(), c->images.get(),
                     c->span.get(),
                     c->cells.get(), c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
                     hilo ? c->cells_lo.get() : nullptr, s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  launch_group_starts(in.lnk, c->parent.get(), frame, cap, n,
+  launch_group_starts(in.lnk, parent, frame, cap, n,
                       static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s,
                       c->flag_node.get());
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
@@ -645,7 +658,7 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
   if (p.predicted) {
     // slots from the learned lengths, packed at scanned offsets
     launch_group_features(c->groups.get(), frame, p.group_cap, cap, in.tpos, in.lnk, in.geom,
-                          groot, c->group_node.get(), c->depth.get(), c->feat.get(), s);
+                          groot, c->group_node.get(), depth, c->feat.get(), s);
     launch_predict_slots(c->feat.get(), frame, p.group_cap, c->pred_params.get(),
                          c->pred_stats.get(), c->slot_c.get(), c->slot_b.get(), c->pred_out.get(),
                          s);
@@ -715,6 +728,9 @@ ListsInput tree_input(const gt_hip_context *c) {
   in.tsid = t.tree_sid();
   in.roots = t.root();
   in.root_node = t.root_nodes();
+  in.parent = t.parents();
+  in.skip = t.skips();
+  in.depth = t.depths();
   return in;
 }
 
@@ -979,7 +995,7 @@ void predictor_after_eval(gt_hip_context *c, const Plan &p, bool overflowed, con
   float *samples = c->pred_samples.ensure(static_cast<size_t>(kPredSampleFloats) * kSampleCap);
   const uint32_t *groot = p.systems > 1 ? c->group_root.get() : nullptr;
   launch_group_features(c->groups.get(), c->frame.get(), p.group_cap, p.node_cap, tpos, lnk, geom,
-                        groot, c->group_node.get(), c->depth.get(), feat, s);
+                        groot, c->group_node.get(), c->lists_depth, feat, s);
   const uint32_t stride = static_cast<uint32_t>(std::max<uint64_t>(1, ng / kSamplesPerEval));
   const uint32_t count = static_cast<uint32_t>((ng + stride - 1) / stride);
   launch_collect_samples(c->frame.get(), p.group_cap, stride, feat, c->ccount.get(),
@@ -1493,8 +1509,6 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
   }
   learn_tree(c, p);
   const uint32_t cap = p.node_cap;
-  launch_link_up(c->tree.links(), frame, cap, c->parent.ensure(cap), c->skip.ensure(cap),
-                 c->depth.ensure(cap), s);
   c->tree.set_counts(static_cast<uint32_t>(c->fw[frame::kNodes]),
                      static_cast<int>(c->fw[frame::kDepth]));
   c->lists_n = 0;  // (the last lists referred to the tree just replaced)
@@ -1745,7 +1759,7 @@ void tree_solve(gt_hip_context *c, const gt_group_config *cfg, uint64_t n, const
   const uint32_t n32 = static_cast<uint32_t>(n);
   unsigned long long *frame = c->frame.get();
   const Plan p = build_single_tree(c, cfg, n32, dp, dm);
-  launch_tree_gravity(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(),
+  launch_tree_gravity(c->tree.geom(), c->tree.links(), c->tree.skips(), c->tree.depths(),
                       c->tree.tree_pos(), c->tree.tree_mass(), c->tree.order(), n32, frame,
                       p.node_cap, cfg->G, cfg->eps, cfg->theta, cfg->quadrupole != 0, da, dq,
                       st_dev, s, dt, static_cast<uint32_t>(np));
@@ -1843,7 +1857,7 @@ int gt_hip_neighbours(gt_hip_context *c, int leaf_size, uint64_t n, const double
                    kQueryKeyBits, c->nb_sort_tmp.ensure(radix_tmp_words(nq)), s,
                    frame + frame::kSortCtl);
     void *recs = c->nb_nodes.ensure((nb_record_bytes(p.node_cap) + 15) / 16);
-    launch_nb_records(c->tree.geom(), c->tree.links(), c->skip.get(), c->depth.get(), frame,
+    launch_nb_records(c->tree.geom(), c->tree.links(), c->tree.skips(), c->tree.depths(), frame,
                       p.node_cap, recs, s);
     launch_fill_u64(dev, 3, 0ull, s);  // [2]: queries whose lists outgrew their slots
     // A call that wants the lists walks once, writing each query's first

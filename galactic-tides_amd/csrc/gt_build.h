@@ -42,6 +42,11 @@ class TreeBuilder {
   const double *tree_pos() const { return tpos_.get(); }
   const double *tree_mass() const { return tmass_.get(); }
   const double *root() const { return root_.get(); }
+  // Per node: its parent (kNone for a root), its depth, and its skip (the
+  // node a LIFO depth-first walk takes after the node's subtree; kNone: none).
+  const uint32_t *parents() const { return parent_.get(); }
+  const uint32_t *depths() const { return node_depth_.get(); }
+  const uint32_t *skips() const { return skip_.get(); }
   // Batches only (null for one system): system of each tree position, and the
   // root node of each system.
   const uint32_t *tree_sid() const { return systems_ > 1 ? tsid_.get() : nullptr; }
@@ -52,9 +57,9 @@ class TreeBuilder {
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, word0_, word1_, word2_;
-  DBuf<uint32_t> vals_, vals_alt_, level_counts_, level_offsets_, nbeg_, lstart_, parent_, sid_,
-      tsid_, root_node_;
-  DBuf<uint8_t> dg_, lcp_, ldep_, ndep_;
+  DBuf<uint32_t> vals_, vals_alt_, level_counts_, level_offsets_, nbeg_, lstart_, parent_, node_depth_,
+      skip_, sid_, tsid_, root_node_;
+  DBuf<uint8_t> dg_, lcp_, ldep_;
   DBuf<unsigned long long> sort_tmp_, level_scan_;
   DBuf<int4> lnk_;
 };

@@ -397,8 +397,10 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
 // that depth comes from wave ballots over the depths the wave's nodes span.
 // Per node it writes its depth, its begin, its parent (p's node one level up,
 // or for the first node at p the last node one level up that begins before
-// p: the one containing p) and its first child (p's node one level down; a
-// node at p's leaf depth is a leaf).  Workgroup 0 writes the level starts
+// p: the one containing p), its first child (p's node one level down; a node
+// at p's leaf depth is a leaf) and its skip, the node a LIFO depth-first walk
+// takes after finishing the node's subtree (its previous sibling, else its
+// parent's skip).  Workgroup 0 writes the level starts
 // (lstart[d]; levels past the deepest 0, lstart[deepest + 1] = the total), the
 // level sizes, the node total and the depth into the frame.  A tree with more
 // nodes than node_cap is not written.
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
     uint32_t rows, const uint32_t *__restrict__ level_offsets,
     unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t *__restrict__ lstart,
     int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ node_begin,
-    uint8_t *__restrict__ node_depth) {
+    uint32_t *__restrict__ node_depth, uint32_t *__restrict__ skip) {
   __shared__ uint32_t rank0[kRankSlots][kDepthLimit + 1];  // counts, then their prefix
   const uint32_t ntiles = gridDim.x, tile = blockIdx.x;
   const int tmax = static_cast<int>(rows) - 2;  // deepest possible level
@@ -491,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
     const uint32_t p = base + i * kBlock;
     uint32_t above = kNone;  // the last node of depth t - 1 that begins at or before p
     uint32_t prev = kNone;   // p's node of depth t - 1
+    uint32_t after = kNone;  // what a LIFO walk takes after the subtree of p's nodes
     for (int t = t0; t <= hi; ++t) {
       const bool in = first[i] <= t && t <= d[i];
       const uint64_t m = __ballot(in);
@@ -498,9 +501,14 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
                           rank0[i * kBuildWaves + wave][t] +
                           static_cast<uint32_t>(__popcll(m & below));
       if (in) {  // p's node of depth t is node `at`
+        // p's deeper nodes are first children (they begin where their parents
+        // do), so the walk leaves all of them for the same node: the previous
+        // sibling of p's first node (node at - 1), none for a root
+        if (t == first[i]) after = t == 0 ? kNone : at - 1u;
         parent[at] = t == 0 ? kNone : above;
         node_begin[at] = p;
-        node_depth[at] = static_cast<uint8_t>(t);
+        node_depth[at] = static_cast<uint32_t>(t);
+        skip[at] = after;
         if (t > first[i]) lnk[prev] = make_int4(static_cast<int>(at), 0, static_cast<int>(p), 0);
         if (t == d[i]) lnk[at] = make_int4(-1, 0, static_cast<int>(p), 0);
         prev = at;
@@ -524,7 +532,7 @@ constexpr uint32_t kEndChunk = 16;
 
 __global__ __launch_bounds__(kBlock) void node_links(
     const uint32_t *__restrict__ parent, const uint32_t *__restrict__ node_begin,
-    const uint8_t *__restrict__ node_depth, const uint8_t *__restrict__ lcp,
+    const uint32_t *__restrict__ node_depth, const uint8_t *__restrict__ lcp,
     const unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t n,
     const uint64_t *__restrict__ w0, const uint64_t *__restrict__ w1,
     const uint64_t *__restrict__ w2, const uint32_t *__restrict__ vals,
@@ -890,7 +898,8 @@ void TreeBuilder::reserve(const Plan &p) {
   level_offsets_.ensure(level_cells);
   level_scan_.ensure(kDepthLimit + 3);
   nbeg_.ensure(nn);
-  ndep_.ensure(nn);
+  node_depth_.ensure(nn);
+  skip_.ensure(nn);
   lstart_.ensure(kDepthLimit + 2);
   geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
   lnk_.ensure(nn);
@@ -1003,9 +1012,9 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   int4 *lnk = lnk_.get();
   uint32_t *parent = parent_.get();
   uint32_t *nbeg = nbeg_.get();
-  uint8_t *ndep = ndep_.get();
+  uint32_t *ndep = node_depth_.get();
   hipLaunchKernelGGL(emit_levels, dim3(ntiles), dim3(kBlock), 0, s, lcp, ldep, n, rows,
-                     level_offsets, frame, cap, lstart, lnk, parent, nbeg, ndep);
+                     level_offsets, frame, cap, lstart, lnk, parent, nbeg, ndep, skip_.get());
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(node_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
                      lcp, frame, cap, n, w[0], w[1], w[2], vals,
