@@ -618,7 +618,22 @@ struct ListsInput {
   const double *roots = nullptr;
   const uint32_t *root_node = nullptr;
   const uint32_t *parent = nullptr, *skip = nullptr, *depth = nullptr;
+  bool records = false;  // the walk's and evaluation's records are written already
 };
+
+// The context's record arrays for evaluations with bounds p (reserved).
+ListRecords list_records(gt_hip_context *c, const gt_group_config *cfg, const Plan &p) {
+  ListRecords r;
+  r.inv_theta = 1.0 / cfg->theta;
+  r.recs = c->recs.get();
+  r.images = c->images.get();
+  r.span = c->span.get();
+  r.cells = c->cells.get();
+  r.bodies = c->bodies.get();
+  r.cells_lo = p.hilo ? c->cells_lo.get() : nullptr;
+  r.bodies_lo = p.hilo ? c->bodies_lo.get() : nullptr;
+  return r;
+}
 
 // Everything of the lists before the walk: links, records, groups, and the
 // predicted slots.
@@ -636,11 +651,9 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
     depth = c->depth.get();
   }
   c->lists_depth = depth;
-  launch_prep_lists(in.geom, in.lnk, skip, depth, in.tpos, in.tmass, n, frame,
-                    cap, cfg->theta, in.tsid, in.roots, c->recs.get(), c->images.get(),
-                    c->span.get(),
-                    c->cells.get(), c->bodies.get(), hilo ? c->bodies_lo.get() : nullptr,
-                    hilo ? c->cells_lo.get() : nullptr, s);
+  if (!in.records)
+    launch_prep_lists(in.geom, in.lnk, skip, depth, in.tpos, in.tmass, n, frame, cap, in.tsid,
+                      in.roots, list_records(c, cfg, p), s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
   launch_group_starts(in.lnk, parent, frame, cap, n,
                       static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s,
@@ -731,6 +744,7 @@ ListsInput tree_input(const gt_hip_context *c) {
   in.parent = t.parents();
   in.skip = t.skips();
   in.depth = t.depths();
+  in.records = t.wrote_records();
   return in;
 }
 
@@ -1024,7 +1038,9 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     launch_fill_u64(frame + frame::kStatus, 1, 0ull, s);
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
-    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig);
+    reserve_all(c, p);
+    const ListRecords rec = list_records(c, cfg, p);
+    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig, &rec);
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     enqueue_tree_lists(c, cfg, p);
@@ -1211,8 +1227,12 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       launch_kick_drift(c->pos.get(), c->vel.get(), c->acc.get(), n, h, dt,
                         frame + frame::kStatus, s);
       clear_counts(frame, s);
-      c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s,
-                      c->orig.get());
+      {
+        reserve_all(c, p);
+        const ListRecords rec = list_records(c, cfg, p);
+        c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s,
+                        c->orig.get(), &rec);
+      }
       break;
     case 1:
       // (overlapped steps launch this phase and the next from graphs per range)

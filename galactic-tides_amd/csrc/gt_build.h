@@ -24,9 +24,13 @@ class TreeBuilder {
   // Batches (p.systems > 1) take the system offsets (p.systems + 1 entries,
   // device memory).  orig (device, may be null: the identity) gives the
   // caller's index of each particle in pos; bodies of a leaf are ordered by it.
+  // rec (may be null): the walk's and the evaluation's records, written by
+  // the build (with each node's moments, each body with its gather into
+  // tree order) instead of by prep_lists.
   void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s,
-               const uint32_t *orig = nullptr);
+               const uint32_t *orig = nullptr, const ListRecords *rec = nullptr);
+  bool wrote_records() const { return wrote_records_; }
   void set_counts(uint32_t n_nodes, int depth) {
     n_nodes_ = n_nodes;
     depth_ = depth;
@@ -54,6 +58,7 @@ class TreeBuilder {
 
  private:
   uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0, systems_ = 1;
+  bool wrote_records_ = false;
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, word0_, word1_, word2_;

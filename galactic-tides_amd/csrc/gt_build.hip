@@ -662,13 +662,24 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
                                                             const uint32_t *__restrict__ sid,
                                                             uint32_t n, double *__restrict__ tpos,
                                                             double *__restrict__ tmass,
-                                                            uint32_t *__restrict__ tsid) {
+                                                            uint32_t *__restrict__ tsid,
+                                                            const double *__restrict__ roots,
+                                                            ListRecords R) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
   const uint32_t i = vals[k];
-  if (sid) tsid[k] = sid[i];
-  for (int a = 0; a < 3; ++a) tpos[3 * static_cast<size_t>(k) + a] = pos[3 * static_cast<size_t>(i) + a];
-  tmass[k] = mass[i];
+  const uint32_t system = sid ? sid[i] : 0u;
+  if (sid) tsid[k] = system;
+  const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
+               z = pos[3 * static_cast<size_t>(i) + 2], m = mass[i];
+  tpos[3 * static_cast<size_t>(k)] = x;
+  tpos[3 * static_cast<size_t>(k) + 1] = y;
+  tpos[3 * static_cast<size_t>(k) + 2] = z;
+  tmass[k] = m;
+  if (R.bodies) {
+    const double *o = roots + 4 * static_cast<size_t>(system);
+    write_body_record(R, k, x, y, z, m, o[0], o[1], o[2]);
+  }
 }
 
 // ---- 7. moments, one level at a time -------------------------------------------------
@@ -680,11 +691,20 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
 // in order, so every sum is the reference's sequence.
 constexpr int kChunk = 8;
 
+// With records (R.cells set: a build for the lists) the node's walk and cell
+// records are written too (Ext: what they need besides the moments).
+struct RecordExt {
+  ListRecords R;
+  const uint32_t *skip, *depth, *tsid;
+  const double *roots;
+  unsigned long long *walk_error;
+};
+
 __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, uint32_t n_nodes, int4 *__restrict__ lnk,
                                     const double *__restrict__ tpos,
                                     const double *__restrict__ tmass,
-                                    double *__restrict__ geom) {
-  const int4 L = lnk[x];
+                                    double *__restrict__ geom, const RecordExt &E) {
+  int4 L = lnk[x];
   double *g = geom + static_cast<size_t>(kGeomStride) * x;
   const double ccx = g[0], ccy = g[1], ccz = g[2];
   double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
@@ -751,7 +771,8 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   } else {
     const uint32_t c0 = static_cast<uint32_t>(L.x), nc = static_cast<uint32_t>(L.y);
     if (L.x < 0 || nc > kChunk || c0 + nc > n_nodes) return;  // malformed links: leave the row unset
-    lnk[x].w = lnk[c0 + nc - 1].w;  // the node ends where its last child does
+    L.w = lnk[c0 + nc - 1].w;  // the node ends where its last child does
+    lnk[x].w = L.w;
     double hm[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], hq[kChunk][6];
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
@@ -804,7 +825,15 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   g[7] = m;
   for (int k = 0; k < 6; ++k) g[8 + k] = q[k];
   const double ex = cx - ccx, ey = cy - ccy, ez = cz - ccz;
-  g[14] = __dsqrt_rn(ex * ex + ey * ey + ez * ez);
+  const double delta = __dsqrt_rn(ex * ex + ey * ey + ez * ez);
+  g[14] = delta;
+  if (E.R.cells) {
+    const double row[15] = {ccx, ccy, ccz, g[3], cx,   cy,   cz,   m,
+                            q[0], q[1], q[2], q[3], q[4], q[5], delta};
+    const double *o = E.roots + 4 * static_cast<size_t>(E.tsid ? E.tsid[L.z] : 0u);
+    write_node_records(E.R, x, row, L, E.skip[x], E.depth[x], o[0], o[1], o[2], o[3],
+                       E.walk_error);
+  }
 }
 
 // Every node of level d: lstart[d] .. lstart[d + 1] (empty for levels past the
@@ -815,11 +844,11 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
                                                         int4 *__restrict__ lnk,
                                                         const double *__restrict__ tpos,
                                                         const double *__restrict__ tmass,
-                                                        double *__restrict__ geom) {
+                                                        double *__restrict__ geom, RecordExt E) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
   for (uint32_t x = first + blockIdx.x * kBlock + threadIdx.x; x < last; x += gridDim.x * kBlock)
-    node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+    node_moments(x, n_nodes, lnk, tpos, tmass, geom, E);
 }
 
 // Levels d and d - 1 in one launch: eight lanes per node of level d - 1, lane
@@ -829,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void level_moments(int d, const uint32_t *_
 __global__ __launch_bounds__(kBlock) void level_pair_moments(
     int d, const uint32_t *__restrict__ lstart, const unsigned long long *__restrict__ frame,
     uint32_t node_cap, int4 *__restrict__ lnk, const double *__restrict__ tpos,
-    const double *__restrict__ tmass, double *__restrict__ geom) {
+    const double *__restrict__ tmass, double *__restrict__ geom, RecordExt E) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t first = lstart[d - 1], last = min(lstart[d], n_nodes);
   const uint32_t lanes = last > first ? 8 * (last - first) : 0u;
@@ -840,10 +869,10 @@ __global__ __launch_bounds__(kBlock) void level_pair_moments(
     if (t < lanes) {
       const int4 L = lnk[x];
       if (L.x >= 0 && j < static_cast<uint32_t>(L.y) && static_cast<uint32_t>(L.x) + j < n_nodes)
-        node_moments(static_cast<uint32_t>(L.x) + j, n_nodes, lnk, tpos, tmass, geom);
+        node_moments(static_cast<uint32_t>(L.x) + j, n_nodes, lnk, tpos, tmass, geom, E);
     }
     __syncthreads();
-    if (t < lanes && j == 0) node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+    if (t < lanes && j == 0) node_moments(x, n_nodes, lnk, tpos, tmass, geom, E);
   }
 }
 
@@ -856,12 +885,13 @@ constexpr int kSerialBlock = 512;
 __global__ __launch_bounds__(kSerialBlock) void serial_level_moments(
     int d_hi, int d_lo, const uint32_t *__restrict__ lstart,
     const unsigned long long *__restrict__ frame, uint32_t node_cap, int4 *__restrict__ lnk,
-    const double *__restrict__ tpos, const double *__restrict__ tmass, double *__restrict__ geom) {
+    const double *__restrict__ tpos, const double *__restrict__ tmass, double *__restrict__ geom,
+    RecordExt E) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   for (int d = d_hi; d >= d_lo; --d) {
     const uint32_t first = lstart[d], last = min(lstart[d + 1], n_nodes);
     for (uint32_t x = first + threadIdx.x; x < last; x += kSerialBlock)
-      node_moments(x, n_nodes, lnk, tpos, tmass, geom);
+      node_moments(x, n_nodes, lnk, tpos, tmass, geom, E);
     __syncthreads();
   }
 }
@@ -910,7 +940,8 @@ void TreeBuilder::reserve(const Plan &p) {
 
 void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                           const unsigned long long *sys_off, unsigned long long *frame,
-                          hipStream_t s, const uint32_t *orig) {
+                          hipStream_t s, const uint32_t *orig, const ListRecords *rec) {
+  wrote_records_ = false;
   const uint32_t n = p.n;
   n_ = n;
   node_cap_ = p.node_cap;
@@ -1027,8 +1058,20 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   GTH_LAUNCH_CHECK();
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();
+  // the walk's and evaluation's records with the moments and the gather
+  RecordExt E{};
+  if (rec && rec->cells) {
+    E.R = *rec;
+    E.skip = skip_.get();
+    E.depth = node_depth_.get();
+    E.tsid = batch ? tsid_.get() : nullptr;
+    E.roots = root_.get();
+    E.walk_error = frame + frame::kWalkError;
+    wrote_records_ = true;
+  }
   hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
-                     sid, n, tpos, tmass, batch ? tsid_.get() : static_cast<uint32_t *>(nullptr));
+                     sid, n, tpos, tmass, batch ? tsid_.get() : static_cast<uint32_t *>(nullptr),
+                     static_cast<const double *>(root_.get()), E.R);
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan
@@ -1049,7 +1092,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   while (lo >= 0 && small(lo)) --lo;
   if (lo < p.depth_cap) {
     hipLaunchKernelGGL(serial_level_moments, dim3(1), dim3(kSerialBlock), 0, s, p.depth_cap,
-                       lo + 1, lstart, frame, cap, lnk, tpos, tmass, geom);
+                       lo + 1, lstart, frame, cap, lnk, tpos, tmass, geom, E);
     GTH_LAUNCH_CHECK();
   }
   auto paired = [&p](int d) {
@@ -1059,7 +1102,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
     if (d >= 1 && paired(d) && paired(d - 1)) {  // levels d and d - 1 together
       const uint32_t blocks = std::min<uint32_t>(8u * p.level_blocks[d - 1], 8u * mgrid);
       hipLaunchKernelGGL(level_pair_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0,
-                         s, d, lstart, frame, cap, lnk, tpos, tmass, geom);
+                         s, d, lstart, frame, cap, lnk, tpos, tmass, geom, E);
       GTH_LAUNCH_CHECK();
       d -= 2;
       continue;
@@ -1067,7 +1110,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
     const uint32_t blocks =
         p.levels_sized && d < frame::kLevels ? std::min<uint32_t>(p.level_blocks[d], mgrid) : mgrid;
     hipLaunchKernelGGL(level_moments, dim3(std::max<uint32_t>(blocks, 1u)), dim3(kBlock), 0, s, d,
-                       lstart, frame, cap, lnk, tpos, tmass, geom);
+                       lstart, frame, cap, lnk, tpos, tmass, geom, E);
     GTH_LAUNCH_CHECK();
     --d;
   }

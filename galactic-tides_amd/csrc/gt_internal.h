@@ -106,6 +106,89 @@ constexpr int kGeomStride = 16;  // doubles per node in the host-compatible geom
 constexpr int kMaxGroup = 64;    // targets per group: one wave64
 constexpr uint32_t kBodiesIfAtMost = 2;  // accepted cells this small are listed as bodies
 
+__device__ inline uint64_t pack_walk_bits(uint32_t first_child, uint32_t skip, uint32_t nch,
+                                          uint32_t depth, bool small) {
+  return static_cast<uint64_t>(first_child & kRecIndexMask) |
+         (static_cast<uint64_t>(skip & kRecIndexMask) << 26) |
+         (static_cast<uint64_t>(nch & 15u) << 52) | (static_cast<uint64_t>(depth & 63u) << 56) |
+         (static_cast<uint64_t>(small ? 1u : 0u) << 62);
+}
+
+// The walk's and the evaluation's records of a tree (reference: the FP32
+// packing of build_group_lists, src/group_lists.cpp:40-63, and its walk
+// nodes, :59-61): per node the FP64 walk record, its FP32 image, its body
+// span and its FP32 cell (with the low part of its centre of mass for hi/lo
+// positions); per body its FP32 position and mass (and low part).  Positions
+// are relative to the system's root centre.  Written by prep_lists, or by the
+// tree build itself for a tree it builds (cells null: none).
+struct ListRecords {
+  double inv_theta = 0.0;
+  WalkRec *recs = nullptr;
+  WalkRec32 *images = nullptr;
+  uint2 *span = nullptr;
+  Cell *cells = nullptr;
+  float4 *cells_lo = nullptr;    // (null: no hi/lo parts)
+  float4 *bodies = nullptr;
+  float4 *bodies_lo = nullptr;   // (null: no hi/lo parts)
+};
+
+#pragma clang fp contract(off)
+
+// Node k's records from its geometry row g (centre, half, centre of mass,
+// mass, quadrupole, delta: the first 15 doubles of a geometry row), links L,
+// skip and depth; o: its system's root centre and half size.  A half size
+// that is not the root's halved depth times is reported in walk_error (bit 2).
+__device__ inline void write_node_records(const ListRecords &R, uint32_t k, const double *g,
+                                          int4 L, uint32_t skip, uint32_t depth, double ox,
+                                          double oy, double oz, double root_half,
+                                          unsigned long long *walk_error) {
+  Cell c;
+  const double mx = g[4] - ox, my = g[5] - oy, mz = g[6] - oz;
+  c.cm = make_float4(static_cast<float>(mx), static_cast<float>(my), static_cast<float>(mz),
+                     static_cast<float>(g[7]));
+  if (R.cells_lo)
+    R.cells_lo[k] = make_float4(static_cast<float>(mx - c.cm.x), static_cast<float>(my - c.cm.y),
+                                static_cast<float>(mz - c.cm.z), 0.0f);
+  c.q0 = make_float4(static_cast<float>(g[8]), static_cast<float>(g[9]),
+                     static_cast<float>(g[10]), static_cast<float>(g[11]));
+  c.q1 = make_float4(static_cast<float>(g[12]), static_cast<float>(g[13]), 0.0f, 0.0f);
+  R.cells[k] = c;
+  // The packed record keeps half as root_half / 2^depth: check that it is.
+  if (depth > 63 || ldexp(root_half, -static_cast<int>(depth)) != g[3])
+    atomicOr(walk_error, 2ull);
+  WalkRec w;
+  w.cx = g[0];
+  w.cy = g[1];
+  w.cz = g[2];
+  w.mx = g[4];
+  w.my = g[5];
+  w.mz = g[6];
+  w.crit = 2.0 * g[3] * R.inv_theta + g[14];
+  const bool small = static_cast<uint32_t>(L.w - L.z) <= kBodiesIfAtMost;
+  w.bits = pack_walk_bits(static_cast<uint32_t>(L.x), skip, static_cast<uint32_t>(L.y), depth,
+                          small);
+  R.recs[k] = w;
+  R.images[k] = make_rec32(w.cx, w.cy, w.cz, w.mx, w.my, w.mz, w.crit, ox, oy, oz, root_half,
+                           image_meta(static_cast<uint32_t>(L.x), static_cast<uint32_t>(L.y)),
+                           small);
+  R.span[k] = make_uint2(static_cast<uint32_t>(L.z), static_cast<uint32_t>(L.w));
+}
+
+// Body k's FP32 position and mass relative to the root centre o (p: its FP64
+// position, m its mass).
+__device__ inline void write_body_record(const ListRecords &R, uint32_t k, double x, double y,
+                                         double z, double m, double ox, double oy, double oz) {
+  const double px = x - ox, py = y - oy, pz = z - oz;
+  const float hx = static_cast<float>(px), hy = static_cast<float>(py),
+              hz = static_cast<float>(pz);
+  R.bodies[k] = make_float4(hx, hy, hz, static_cast<float>(m));
+  if (R.bodies_lo)
+    R.bodies_lo[k] = make_float4(static_cast<float>(px - hx), static_cast<float>(py - hy),
+                                 static_cast<float>(pz - hz), 0.0f);
+}
+
+#pragma clang fp contract(fast)
+
 // ---- XCD-aware block order ---------------------------------------------------------
 
 // Workgroups of a launch are dealt to the 8 XCDs in turn.  Launching
@@ -337,13 +420,12 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 // Batches of independent systems pass tsid (system of each tree position),
 // roots (4 doubles per system: root centre, half) and root_node (root node
 // of each system); for one system they are null (the root is node 0).
-// bodies_lo / cells_lo (may be null): the lo parts of the FP32 positions.
+// The records go to rec's arrays (bodies_lo / cells_lo may be null: no lo
+// parts of the FP32 positions).
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       unsigned long long *frame, uint32_t node_cap, double theta,
-                       const uint32_t *tsid, const double *roots, WalkRec *recs,
-                       WalkRec32 *images, uint2 *span, Cell *cells, float4 *bodies,
-                       float4 *bodies_lo, float4 *cells_lo, hipStream_t s);
+                       unsigned long long *frame, uint32_t node_cap, const uint32_t *tsid,
+                       const double *roots, const ListRecords &rec, hipStream_t s);
 // node_of (may be null): at each flagged start, the node the group is taken
 // from (the chunked node, a lone packed child, or the parent of packed
 // siblings); make_groups copies it per group into group_node (may be null).

@@ -40,14 +40,6 @@ inline uint32_t blocks_for(size_t n, int b = kBlock) {
   return static_cast<uint32_t>((n + b - 1) / b);
 }
 
-__device__ inline uint64_t pack_walk_bits(uint32_t first_child, uint32_t skip, uint32_t nch,
-                                          uint32_t depth, bool small) {
-  return static_cast<uint64_t>(first_child & kRecIndexMask) |
-         (static_cast<uint64_t>(skip & kRecIndexMask) << 26) |
-         (static_cast<uint64_t>(nch & 15u) << 52) | (static_cast<uint64_t>(depth & 63u) << 56) |
-         (static_cast<uint64_t>(small ? 1u : 0u) << 62);
-}
-
 __global__ __launch_bounds__(kBlock) void set_parents(const int4 *__restrict__ lnk,
                                                       const unsigned long long *__restrict__ frame,
                                                       uint32_t node_cap,
@@ -93,63 +85,25 @@ __global__ __launch_bounds__(kBlock) void prep_lists(
     const double *__restrict__ geom, const int4 *__restrict__ lnk,
     const uint32_t *__restrict__ skip, const uint32_t *__restrict__ depth,
     const double *__restrict__ tpos, const double *__restrict__ tmass, uint32_t n,
-    unsigned long long *__restrict__ frame, uint32_t node_cap, double inv_theta,
-    const uint32_t *__restrict__ tsid, const double *__restrict__ roots,
-    WalkRec *__restrict__ recs, WalkRec32 *__restrict__ images, uint2 *__restrict__ span,
-    Cell *__restrict__ cells, float4 *__restrict__ bodies, float4 *__restrict__ bodies_lo,
-    float4 *__restrict__ cells_lo) {
+    unsigned long long *__restrict__ frame, uint32_t node_cap,
+    const uint32_t *__restrict__ tsid, const double *__restrict__ roots, ListRecords R) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   // origin: the root centre of the body's (node's) system (tsid null: one
   // system, whose root is node 0)
   if (k < n) {
     const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[k]) : geom;
-    const double ox = o[0], oy = o[1], oz = o[2];
-    const double px = tpos[3 * k] - ox, py = tpos[3 * k + 1] - oy, pz = tpos[3 * k + 2] - oz;
-    const float hx = static_cast<float>(px), hy = static_cast<float>(py),
-                hz = static_cast<float>(pz);
-    bodies[k] = make_float4(hx, hy, hz, static_cast<float>(tmass[k]));
-    if (bodies_lo)
-      bodies_lo[k] = make_float4(static_cast<float>(px - hx), static_cast<float>(py - hy),
-                                 static_cast<float>(pz - hz), 0.0f);
+    write_body_record(R, k, tpos[3 * k], tpos[3 * k + 1], tpos[3 * k + 2], tmass[k], o[0], o[1],
+                      o[2]);
   }
   if (k < n_nodes) {
     const double *g = geom + static_cast<size_t>(kGeomStride) * k;
     const int4 L = lnk[k];
     const double *o = tsid ? roots + 4 * static_cast<size_t>(tsid[L.z]) : geom;
-    const double ox = o[0], oy = o[1], oz = o[2];
-    Cell c;
-    const double mx = g[4] - ox, my = g[5] - oy, mz = g[6] - oz;
-    c.cm = make_float4(static_cast<float>(mx), static_cast<float>(my), static_cast<float>(mz),
-                       static_cast<float>(g[7]));
-    if (cells_lo)
-      cells_lo[k] = make_float4(static_cast<float>(mx - c.cm.x), static_cast<float>(my - c.cm.y),
-                                static_cast<float>(mz - c.cm.z), 0.0f);
-    c.q0 = make_float4(static_cast<float>(g[8]), static_cast<float>(g[9]),
-                       static_cast<float>(g[10]), static_cast<float>(g[11]));
-    c.q1 = make_float4(static_cast<float>(g[12]), static_cast<float>(g[13]), 0.0f, 0.0f);
-    cells[k] = c;
-    const uint32_t d = depth[k];
-    // The packed record keeps half as root_half / 2^depth: check that it is.
-    const double root_half = tsid ? roots[4 * static_cast<size_t>(tsid[L.z]) + 3] : geom[3];
-    if (d > 63 || ldexp(root_half, -static_cast<int>(d)) != g[3])
-      atomicOr(frame + frame::kWalkError, 2ull);
-    WalkRec w;
-    w.cx = g[0];
-    w.cy = g[1];
-    w.cz = g[2];
-    w.mx = g[4];
-    w.my = g[5];
-    w.mz = g[6];
-    w.crit = 2.0 * g[3] * inv_theta + g[14];
-    const bool small = static_cast<uint32_t>(L.w - L.z) <= kBodiesIfAtMost;
-    w.bits = pack_walk_bits(static_cast<uint32_t>(L.x), skip[k], static_cast<uint32_t>(L.y), d,
-                            small);
-    recs[k] = w;
-    images[k] = make_rec32(w.cx, w.cy, w.cz, w.mx, w.my, w.mz, w.crit, ox, oy, oz, root_half,
-                           image_meta(static_cast<uint32_t>(L.x), static_cast<uint32_t>(L.y)),
-                           small);
-    span[k] = make_uint2(static_cast<uint32_t>(L.z), static_cast<uint32_t>(L.w));
+    double row[15];
+    for (int j = 0; j < 15; ++j) row[j] = g[j];
+    write_node_records(R, k, row, L, skip[k], depth[k], o[0], o[1], o[2], o[3],
+                       frame + frame::kWalkError);
   }
 }
 
@@ -933,15 +887,12 @@ void launch_link_up(const int4 *lnk, const unsigned long long *frame, uint32_t n
 
 void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip,
                        const uint32_t *depth, const double *tpos, const double *tmass, uint32_t n,
-                       unsigned long long *frame, uint32_t node_cap, double theta,
-                       const uint32_t *tsid, const double *roots, WalkRec *recs,
-                       WalkRec32 *images, uint2 *span, Cell *cells, float4 *bodies,
-                       float4 *bodies_lo, float4 *cells_lo, hipStream_t s) {
+                       unsigned long long *frame, uint32_t node_cap, const uint32_t *tsid,
+                       const double *roots, const ListRecords &rec, hipStream_t s) {
   const uint32_t m = n > node_cap ? n : node_cap;
   if (!m) return;
   hipLaunchKernelGGL(prep_lists, dim3(blocks_for(m)), dim3(kBlock), 0, s, geom, lnk, skip, depth,
-                     tpos, tmass, n, frame, node_cap, 1.0 / theta, tsid, roots, recs, images,
-                     span, cells, bodies, bodies_lo, cells_lo);
+                     tpos, tmass, n, frame, node_cap, tsid, roots, rec);
   GTH_LAUNCH_CHECK();
 }
 
