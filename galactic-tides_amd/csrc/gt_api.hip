@@ -1053,7 +1053,8 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
       enqueue_overlapped_eval(c, cfg, p);
       c->overlapped = p.chunks;
       GTH_CHECK(hipEventRecord(c->ev[3], s));
-      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      launch_scatter(c->out.get(), c->tree.inverse_order(), n, acc, pot, frame + frame::kStatus,
+                     s);
       GTH_CHECK(hipEventRecord(c->ev[4], s));
     } else if (slots) {
       enqueue_items(c, cfg, p);
@@ -1061,7 +1062,8 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
       enqueue_eval(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[3], s));
       enqueue_combine(c, cfg, p);
-      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      launch_scatter(c->out.get(), c->tree.inverse_order(), n, acc, pot, frame + frame::kStatus,
+                     s);
       GTH_CHECK(hipEventRecord(c->ev[4], s));
     }
     read_frame(c);
@@ -1079,7 +1081,8 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
       enqueue_eval(c, cfg, p);
       GTH_CHECK(hipEventRecord(c->ev[3], s));
       enqueue_combine(c, cfg, p);
-      launch_scatter(c->out.get(), c->tree.order(), n, acc, pot, frame + frame::kStatus, s);
+      launch_scatter(c->out.get(), c->tree.inverse_order(), n, acc, pot, frame + frame::kStatus,
+                     s);
       GTH_CHECK(hipEventRecord(c->ev[4], s));
       read_frame(c);
       st = c->fw[frame::kStatus];
@@ -1130,6 +1133,7 @@ void permute_state(gt_hip_context *c) {
   launch_compose_u32(c->orig.get(), order, n, ot, s);
   GTH_CHECK(hipMemcpyAsync(c->orig.get(), ot, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   launch_compose_u32(nullptr, nullptr, n, c->tree.order_mutable(), s);
+  launch_compose_u32(nullptr, nullptr, n, c->tree.inverse_order_mutable(), s);
   GTH_CHECK(hipStreamSynchronize(s));
   c->permuted = true;
   c->steps_in_order = 0;
@@ -1245,7 +1249,8 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
     default:
       if (p.chunks <= 1) enqueue_combine(c, cfg, p);
       // the scatter and the closing kick in one pass over the rows
-      launch_scatter_kick(c->out.get(), c->tree.order(), static_cast<uint32_t>(n), c->acc.get(),
+      launch_scatter_kick(c->out.get(), c->tree.inverse_order(), static_cast<uint32_t>(n),
+                          c->acc.get(),
                           c->pot.get(), c->vel.get(), h, frame + frame::kStatus, s);
       launch_count_step(frame, s);
       break;

@@ -43,6 +43,9 @@ class TreeBuilder {
   const int4 *links() const { return lnk_.get(); }
   const uint32_t *order() const { return vals_.get(); }
   uint32_t *order_mutable() { return vals_.get(); }
+  // Each particle's tree position (the inverse of order).
+  const uint32_t *inverse_order() const { return inv_.get(); }
+  uint32_t *inverse_order_mutable() { return inv_.get(); }
   const double *tree_pos() const { return tpos_.get(); }
   const double *tree_mass() const { return tmass_.get(); }
   const double *root() const { return root_.get(); }
@@ -63,7 +66,7 @@ class TreeBuilder {
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, word0_, word1_, word2_;
   DBuf<uint32_t> vals_, vals_alt_, level_counts_, level_offsets_, nbeg_, lstart_, parent_, node_depth_,
-      skip_, sid_, tsid_, root_node_;
+      skip_, inv_, sid_, tsid_, root_node_;
   DBuf<uint8_t> dg_, lcp_, ldep_;
   DBuf<unsigned long long> sort_tmp_, level_scan_;
   DBuf<int4> lnk_;

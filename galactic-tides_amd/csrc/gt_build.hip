@@ -663,11 +663,13 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
                                                             uint32_t n, double *__restrict__ tpos,
                                                             double *__restrict__ tmass,
                                                             uint32_t *__restrict__ tsid,
+                                                            uint32_t *__restrict__ inv,
                                                             const double *__restrict__ roots,
                                                             ListRecords R) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
   const uint32_t i = vals[k];
+  inv[i] = k;
   const uint32_t system = sid ? sid[i] : 0u;
   if (sid) tsid[k] = system;
   const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
@@ -913,6 +915,7 @@ void TreeBuilder::reserve(const Plan &p) {
   }
   keys_.ensure(n);
   vals_.ensure(n);
+  inv_.ensure(n);
   keys_alt_.ensure(n);
   vals_alt_.ensure(n);
   dg_.ensure(n);
@@ -1071,7 +1074,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   }
   hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
                      sid, n, tpos, tmass, batch ? tsid_.get() : static_cast<uint32_t *>(nullptr),
-                     static_cast<const double *>(root_.get()), E.R);
+                     inv_.get(), static_cast<const double *>(root_.get()), E.R);
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan

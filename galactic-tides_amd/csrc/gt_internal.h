@@ -512,10 +512,12 @@ void launch_query_keys(const double *geom, const unsigned long long *frame,
 constexpr int kQueryKeyBits = 30;
 
 // gt_step.hip: each does nothing once status (a frame's kStatus word) is set.
-void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+// acc/pot rows i from the tree-ordered results out[inv[i]] (inv: each
+// particle's tree position).
+void launch_scatter(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                     double *pot, const unsigned long long *status, hipStream_t s);
 // launch_scatter then launch_kick(vel, acc, n, h) as one kernel.
-void launch_scatter_kick(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+void launch_scatter_kick(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                          double *pot, double *vel, double h, const unsigned long long *status,
                          hipStream_t s);
 void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,

@@ -14,19 +14,20 @@ inline uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + kBlock 
 
 #pragma clang fp contract(off)
 
-// acc[order[k]] = out[k] widened to double (the FP32 results are in tree order).
+// acc[i] = out[inv[i]] widened to double (the FP32 results are in tree order;
+// inv[i] is particle i's tree position): one pass over the rows in their own
+// order, the results gathered.
 __global__ __launch_bounds__(kBlock) void scatter_forces(const float4 *__restrict__ out,
-                                                         const uint32_t *__restrict__ order,
+                                                         const uint32_t *__restrict__ inv,
                                                          uint32_t n, double *__restrict__ acc,
                                                          double *__restrict__ pot,
                                                          const unsigned long long *status) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n || *status) return;
-  const float4 a = out[k];
-  const size_t i = order[k];
-  acc[3 * i] = a.x;
-  acc[3 * i + 1] = a.y;
-  acc[3 * i + 2] = a.z;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || *status) return;
+  const float4 a = out[inv[i]];
+  acc[3 * static_cast<size_t>(i)] = a.x;
+  acc[3 * static_cast<size_t>(i) + 1] = a.y;
+  acc[3 * static_cast<size_t>(i) + 2] = a.z;
   pot[i] = a.w;
 }
 
@@ -34,23 +35,23 @@ __global__ __launch_bounds__(kBlock) void scatter_forces(const float4 *__restric
 // scatter_forces writes them, then v += h a for the same row, the same
 // operations as kick (so the result is bit for bit the two kernels').
 __global__ __launch_bounds__(kBlock) void scatter_kick(const float4 *__restrict__ out,
-                                                       const uint32_t *__restrict__ order,
+                                                       const uint32_t *__restrict__ inv,
                                                        uint32_t n, double *__restrict__ acc,
                                                        double *__restrict__ pot,
                                                        double *__restrict__ vel, double h,
                                                        const unsigned long long *status) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n || *status) return;
-  const float4 a = out[k];
-  const size_t i = order[k];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || *status) return;
+  const float4 a = out[inv[i]];
   const double ax = a.x, ay = a.y, az = a.z;
-  acc[3 * i] = ax;
-  acc[3 * i + 1] = ay;
-  acc[3 * i + 2] = az;
+  const size_t r = 3 * static_cast<size_t>(i);
+  acc[r] = ax;
+  acc[r + 1] = ay;
+  acc[r + 2] = az;
   pot[i] = a.w;
-  vel[3 * i] = vel[3 * i] + h * ax;
-  vel[3 * i + 1] = vel[3 * i + 1] + h * ay;
-  vel[3 * i + 2] = vel[3 * i + 2] + h * az;
+  vel[r] = vel[r] + h * ax;
+  vel[r + 1] = vel[r + 1] + h * ay;
+  vel[r + 2] = vel[r + 2] + h * az;
 }
 
 // v += h a; x += dt v   (first half of the step)
@@ -180,19 +181,19 @@ __global__ __launch_bounds__(kBlock) void compose_u32(const uint32_t *__restrict
 
 }  // namespace
 
-void launch_scatter(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+void launch_scatter(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                     double *pot, const unsigned long long *status, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(scatter_forces, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, order, n, acc,
+  hipLaunchKernelGGL(scatter_forces, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, inv, n, acc,
                      pot, status);
   GTH_LAUNCH_CHECK();
 }
 
-void launch_scatter_kick(const float4 *out, const uint32_t *order, uint32_t n, double *acc,
+void launch_scatter_kick(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                          double *pot, double *vel, double h, const unsigned long long *status,
                          hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(scatter_kick, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, order, n, acc,
+  hipLaunchKernelGGL(scatter_kick, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, inv, n, acc,
                      pot, vel, h, status);
   GTH_LAUNCH_CHECK();
 }
