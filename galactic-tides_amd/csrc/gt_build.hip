@@ -526,7 +526,9 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
 // level below, from its first child on while their parent is the node: at
 // most eight, read together), each system's root node, and a leaf's end: the
 // first position q + 1 after its begin whose predecessor q shares fewer than
-// the leaf's depth levels with it (lcp, read sixteen at a time).  An inner
+// the leaf's depth levels with it (lcp, read sixteen at a time).  (Both reads
+// are whole aligned 16-byte words: one address per lane instead of one per
+// value.)  An inner
 // node's end is its last child's, set with its moments.
 constexpr uint32_t kEndChunk = 16;
 
@@ -574,28 +576,45 @@ __global__ __launch_bounds__(kBlock) void node_links(
   g[3] = h;
   g[15] = 0.0;
   if (d == 0 && root_node) root_node[system] = x;
-  if (first >= 0) {  // children
-    const uint32_t c0 = static_cast<uint32_t>(first);
-    uint32_t up[7];
-#pragma unroll
-    for (uint32_t k = 0; k < 7; ++k) up[k] = c0 + 1 + k < n_nodes ? parent[c0 + 1 + k] : kNone;
+  if (first >= 0) {  // children: parents of nodes c0 + 1 .. c0 + 7, read as three
+                     // aligned 16-byte words (parent is padded past node_cap)
+    const uint32_t c0 = static_cast<uint32_t>(first), a = (c0 + 1u) & ~3u;
+    const uint4 *pw = reinterpret_cast<const uint4 *>(parent + a);
+    const uint4 w0 = pw[0], w1 = pw[1], w2 = pw[2];
+    const uint32_t up[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y,
+                             w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
     int children = 1;
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
-      if (children == k + 1 && up[k] == x) ++children;
+    for (uint32_t k = 0; k < 7; ++k) {
+      const uint32_t c = c0 + 1u + k;
+      uint32_t pc = up[0];
+#pragma unroll
+      for (uint32_t j = 1; j < 12; ++j) pc = c - a == j ? up[j] : pc;
+      if (children == static_cast<int>(k) + 1 && c < n_nodes && pc == x) ++children;
+    }
     lnk[x].y = children;
     return;
   }
-  // a leaf (lnk[x].y stays 0): its end
+  // a leaf (lnk[x].y stays 0): its end, from lcp read as aligned 16-byte
+  // words (lcp is padded past n; positions from n - 1 on end every leaf)
   uint32_t q = b;
   for (;;) {
-    uint8_t c[kEndChunk];
+    const uint32_t base = q & ~(kEndChunk - 1u);
+    const uint4 w = *reinterpret_cast<const uint4 *>(lcp + base);
+    const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+    uint32_t stop = kEndChunk;
 #pragma unroll
-    for (uint32_t k = 0; k < kEndChunk; ++k) c[k] = q + k + 1 < n ? lcp[q + k] : kNoShare;
-    uint32_t k = 0;
-    while (k < kEndChunk && shared(c[k]) >= static_cast<int>(d)) ++k;
-    q += k;
-    if (k < kEndChunk) break;
+    for (int j = static_cast<int>(kEndChunk) - 1; j >= 0; --j) {
+      const uint32_t t = base + static_cast<uint32_t>(j);
+      const uint8_t v = static_cast<uint8_t>(words[j >> 2] >> (8 * (j & 3)));
+      if (t >= q && (t + 1u >= n || shared(v) < static_cast<int>(d)))
+        stop = static_cast<uint32_t>(j);
+    }
+    if (stop < kEndChunk) {
+      q = base + stop;
+      break;
+    }
+    q = base + kEndChunk;
   }
   lnk[x].w = static_cast<int>(q + 1);
 }
@@ -656,6 +675,11 @@ __global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ ln
   }
 }
 
+// Positions and masses into tree order (tpos/tmass, and the system of each
+// tree position for a batch), each particle's tree position (inv), and with
+// records the FP32 bodies.  The positions go out through LDS: a block's 256
+// rows of three doubles are written as 768 consecutive doubles, one per lane
+// per store.
 __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__restrict__ vals,
                                                             const double *__restrict__ pos,
                                                             const double *__restrict__ mass,
@@ -666,22 +690,28 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
                                                             uint32_t *__restrict__ inv,
                                                             const double *__restrict__ roots,
                                                             ListRecords R) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t i = vals[k];
-  inv[i] = k;
-  const uint32_t system = sid ? sid[i] : 0u;
-  if (sid) tsid[k] = system;
-  const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
-               z = pos[3 * static_cast<size_t>(i) + 2], m = mass[i];
-  tpos[3 * static_cast<size_t>(k)] = x;
-  tpos[3 * static_cast<size_t>(k) + 1] = y;
-  tpos[3 * static_cast<size_t>(k) + 2] = z;
-  tmass[k] = m;
-  if (R.bodies) {
-    const double *o = roots + 4 * static_cast<size_t>(system);
-    write_body_record(R, k, x, y, z, m, o[0], o[1], o[2]);
+  __shared__ double rows[3 * kBlock];
+  const uint32_t k0 = blockIdx.x * kBlock, k = k0 + threadIdx.x;
+  if (k < n) {
+    const uint32_t i = vals[k];
+    inv[i] = k;
+    const uint32_t system = sid ? sid[i] : 0u;
+    if (sid) tsid[k] = system;
+    const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
+                 z = pos[3 * static_cast<size_t>(i) + 2], m = mass[i];
+    rows[3 * threadIdx.x] = x;
+    rows[3 * threadIdx.x + 1] = y;
+    rows[3 * threadIdx.x + 2] = z;
+    tmass[k] = m;
+    if (R.bodies) {
+      const double *o = roots + 4 * static_cast<size_t>(system);
+      write_body_record(R, k, x, y, z, m, o[0], o[1], o[2]);
+    }
   }
+  __syncthreads();
+  const uint32_t in_block = n - k0 < kBlock ? n - k0 : kBlock;
+  for (uint32_t j = threadIdx.x; j < 3 * in_block; j += kBlock)
+    tpos[3 * static_cast<size_t>(k0) + j] = rows[j];
 }
 
 // ---- 7. moments, one level at a time -------------------------------------------------
@@ -923,7 +953,7 @@ void TreeBuilder::reserve(const Plan &p) {
   if (p.words > 1) word1_.ensure(n);
   if (p.words > 2) word2_.ensure(n);
   sort_tmp_.ensure(radix_tmp_words(n));
-  lcp_.ensure(n);
+  lcp_.ensure(n + kEndChunk);  // (node_links reads whole aligned 16-byte words)
   ldep_.ensure(n);
   const size_t level_cells =
       static_cast<size_t>(kDepthLimit + 2) * ((n + kLevelTile - 1) / kLevelTile);
@@ -936,7 +966,7 @@ void TreeBuilder::reserve(const Plan &p) {
   lstart_.ensure(kDepthLimit + 2);
   geom_.ensure(static_cast<size_t>(kGeomStride) * nn);
   lnk_.ensure(nn);
-  parent_.ensure(nn);
+  parent_.ensure(nn + 16);  // (node_links reads whole aligned 16-byte words)
   tpos_.ensure(3 * n);
   tmass_.ensure(n);
 }
