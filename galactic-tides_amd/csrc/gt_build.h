@@ -26,7 +26,8 @@ class TreeBuilder {
   // caller's index of each particle in pos; bodies of a leaf are ordered by it.
   // rec (may be null): the walk's and the evaluation's records, written by
   // the build (with each node's moments, each body with its gather into
-  // tree order) instead of by prep_lists.
+  // tree order) instead of by prep_lists; a build for the lists also keeps
+  // the inverse of its order (inverse_order) for the force scatter.
   void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s,
                const uint32_t *orig = nullptr, const ListRecords *rec = nullptr);

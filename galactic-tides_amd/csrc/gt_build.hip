@@ -676,8 +676,8 @@ __global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ ln
 }
 
 // Positions and masses into tree order (tpos/tmass, and the system of each
-// tree position for a batch), each particle's tree position (inv), and with
-// records the FP32 bodies.  The positions go out through LDS: a block's 256
+// tree position for a batch), each particle's tree position (inv; null: not
+// needed), and with records the FP32 bodies.  The positions go out through LDS: a block's 256
 // rows of three doubles are written as 768 consecutive doubles, one per lane
 // per store.
 __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__restrict__ vals,
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void gather_tree_order(const uint32_t *__re
   const uint32_t k0 = blockIdx.x * kBlock, k = k0 + threadIdx.x;
   if (k < n) {
     const uint32_t i = vals[k];
-    inv[i] = k;
+    if (inv) inv[i] = k;
     const uint32_t system = sid ? sid[i] : 0u;
     if (sid) tsid[k] = system;
     const double x = pos[3 * static_cast<size_t>(i)], y = pos[3 * static_cast<size_t>(i) + 1],
@@ -1104,7 +1104,8 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   }
   hipLaunchKernelGGL(gather_tree_order, dim3(blocks_for(n)), dim3(kBlock), 0, s, vals, pos, mass,
                      sid, n, tpos, tmass, batch ? tsid_.get() : static_cast<uint32_t *>(nullptr),
-                     inv_.get(), static_cast<const double *>(root_.get()), E.R);
+                     rec ? inv_.get() : static_cast<uint32_t *>(nullptr),
+                     static_cast<const double *>(root_.get()), E.R);
   GTH_LAUNCH_CHECK();
 
   // 7. moments, deepest level first, down from the deepest level the plan
