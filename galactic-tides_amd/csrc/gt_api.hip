@@ -655,9 +655,8 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
     launch_prep_lists(in.geom, in.lnk, skip, depth, in.tpos, in.tmass, n, frame, cap, in.tsid,
                       in.roots, list_records(c, cfg, p), s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  launch_group_starts(in.lnk, parent, frame, cap, n,
-                      static_cast<uint32_t>(cfg->group_size), pack, c->flags.get(), s,
-                      c->flag_node.get());
+  launch_group_starts(in.lnk, parent, frame, cap, n, static_cast<uint32_t>(cfg->group_size), pack,
+                      c->flags.get(), s, c->flag_node.get());
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
                      s);
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;

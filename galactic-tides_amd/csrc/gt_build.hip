@@ -518,6 +518,53 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
   }
 }
 
+// Bodies of a leaf back into particle order (insertion sort; leaves are small).
+constexpr uint32_t kLeafRegs = 16;  // leaves up to this size are sorted in registers
+// orig (may be null: the identity) maps the positions in pos to the caller's
+// particle indices, which decide the order.
+__device__ inline void sort_leaf(uint32_t *__restrict__ vals, const uint32_t *__restrict__ orig,
+                                 uint32_t b, uint32_t e) {
+  if (e - b <= kLeafRegs) {  // the leaf in registers: its loads issue together
+    uint32_t v[kLeafRegs], key[kLeafRegs];
+#pragma unroll
+    for (uint32_t i = 0; i < kLeafRegs; ++i)
+      if (b + i < e) v[i] = vals[b + i];
+#pragma unroll
+    for (uint32_t i = 0; i < kLeafRegs; ++i)
+      if (b + i < e) key[i] = orig ? orig[v[i]] : v[i];
+    bool moved = false;
+#pragma unroll
+    for (uint32_t i = 1; i < kLeafRegs; ++i) {  // insertion sort by key, stable
+#pragma unroll
+      for (uint32_t j = i; j > 0; --j) {
+        if (b + i < e && key[j - 1] > key[j]) {
+          const uint32_t tk = key[j - 1], tv = v[j - 1];
+          key[j - 1] = key[j];
+          v[j - 1] = v[j];
+          key[j] = tk;
+          v[j] = tv;
+          moved = true;
+        }
+      }
+    }
+    if (moved) {
+#pragma unroll
+      for (uint32_t i = 0; i < kLeafRegs; ++i)
+        if (b + i < e) vals[b + i] = v[i];
+    }
+    return;
+  }
+  for (uint32_t k = b + 1; k < e; ++k) {
+    const uint32_t v = vals[k], key = orig ? orig[v] : v;
+    uint32_t j = k;
+    while (j > b && (orig ? orig[vals[j - 1]] : vals[j - 1]) > key) {
+      vals[j] = vals[j - 1];
+      --j;
+    }
+    vals[j] = v;
+  }
+}
+
 // Per node: its centre and half size (the octant path of its first body
 // replayed from its system's root with the reference's FP64 centre updates:
 // with one path word and one system, sorted_keys set, the path is the sorted
@@ -526,9 +573,12 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
 // level below, from its first child on while their parent is the node: at
 // most eight, read together), each system's root node, and a leaf's end: the
 // first position q + 1 after its begin whose predecessor q shares fewer than
-// the leaf's depth levels with it (lcp, read sixteen at a time).  (Both reads
+// the leaf's depth levels with it (lcp, read sixteen at a time); then a
+// leaf's bodies are put back into particle order (sort_leaf).  (Both reads
 // are whole aligned 16-byte words: one address per lane instead of one per
-// value.)  An inner
+// value.)  Reading vals[begin] for an inner node's path while a leaf that
+// begins there is sorted is harmless: every body of the leaf has the node's
+// path.  An inner
 // node's end is its last child's, set with its moments.
 constexpr uint32_t kEndChunk = 16;
 
@@ -537,10 +587,10 @@ __global__ __launch_bounds__(kBlock) void node_links(
     const uint32_t *__restrict__ node_depth, const uint8_t *__restrict__ lcp,
     const unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t n,
     const uint64_t *__restrict__ w0, const uint64_t *__restrict__ w1,
-    const uint64_t *__restrict__ w2, const uint32_t *__restrict__ vals,
+    const uint64_t *__restrict__ w2, uint32_t *__restrict__ vals,
     const uint64_t *__restrict__ sorted_keys, int key_shift, const double *__restrict__ roots,
     const uint32_t *__restrict__ sid, double *__restrict__ geom, int4 *__restrict__ lnk,
-    uint32_t *__restrict__ root_node) {
+    uint32_t *__restrict__ root_node, const uint32_t *__restrict__ orig) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
@@ -617,62 +667,7 @@ __global__ __launch_bounds__(kBlock) void node_links(
     q = base + kEndChunk;
   }
   lnk[x].w = static_cast<int>(q + 1);
-}
-
-// Bodies of a leaf back into particle order (insertion sort; leaves are small).
-constexpr uint32_t kLeafRegs = 16;  // leaves up to this size are sorted in registers
-// orig (may be null: the identity) maps the positions in pos to the caller's
-// particle indices, which decide the order.
-__global__ __launch_bounds__(kBlock) void leaf_order(const int4 *__restrict__ lnk,
-                                                     const unsigned long long *__restrict__ frame,
-                                                     uint32_t node_cap,
-                                                     const uint32_t *__restrict__ orig,
-                                                     uint32_t *__restrict__ vals) {
-  const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  if (x >= n_nodes) return;
-  const int4 L = lnk[x];
-  if (L.y != 0) return;
-  const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-  if (e - b <= kLeafRegs) {  // the leaf in registers: its loads issue together
-    uint32_t v[kLeafRegs], key[kLeafRegs];
-#pragma unroll
-    for (uint32_t i = 0; i < kLeafRegs; ++i)
-      if (b + i < e) v[i] = vals[b + i];
-#pragma unroll
-    for (uint32_t i = 0; i < kLeafRegs; ++i)
-      if (b + i < e) key[i] = orig ? orig[v[i]] : v[i];
-    bool moved = false;
-#pragma unroll
-    for (uint32_t i = 1; i < kLeafRegs; ++i) {  // insertion sort by key, stable
-#pragma unroll
-      for (uint32_t j = i; j > 0; --j) {
-        if (b + i < e && key[j - 1] > key[j]) {
-          const uint32_t tk = key[j - 1], tv = v[j - 1];
-          key[j - 1] = key[j];
-          v[j - 1] = v[j];
-          key[j] = tk;
-          v[j] = tv;
-          moved = true;
-        }
-      }
-    }
-    if (moved) {
-#pragma unroll
-      for (uint32_t i = 0; i < kLeafRegs; ++i)
-        if (b + i < e) vals[b + i] = v[i];
-    }
-    return;
-  }
-  for (uint32_t k = b + 1; k < e; ++k) {
-    const uint32_t v = vals[k], key = orig ? orig[v] : v;
-    uint32_t j = k;
-    while (j > b && (orig ? orig[vals[j - 1]] : vals[j - 1]) > key) {
-      vals[j] = vals[j - 1];
-      --j;
-    }
-    vals[j] = v;
-  }
+  sort_leaf(vals, orig, b, q + 1);
 }
 
 // Positions and masses into tree order (tpos/tmass, and the system of each
@@ -1014,7 +1009,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   // leaf depths report it, and the host builds again with another word.  With
   // one word the sort orders by its first p.path_levels levels only: the tree
   // needs the bodies grouped down to the deepest leaf, and the bodies of a
-  // leaf go back into particle order anyway (leaf_order).  A leaf deeper than
+  // leaf go back into particle order anyway (sort_leaf).  A leaf deeper than
   // that is reported the same way, and the host builds again with all 21.
   const int levels = p.words == 1 ? std::min(std::max(p.path_levels, 1), kLevels) : kLevels * p.words;
   uint64_t *keys = keys_.get();
@@ -1084,10 +1079,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                      lcp, frame, cap, n, w[0], w[1], w[2], vals,
                      p.words == 1 && !batch ? keys : static_cast<const uint64_t *>(nullptr),
                      p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, geom, lnk,
-                     root_node);
-  GTH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(leaf_order, dim3(blocks_for(cap)), dim3(kBlock), 0, s, lnk, frame, cap, orig,
-                     vals);
+                     root_node, orig);
   GTH_LAUNCH_CHECK();
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();

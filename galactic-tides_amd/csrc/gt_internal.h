@@ -132,6 +132,50 @@ struct ListRecords {
   float4 *bodies_lo = nullptr;   // (null: no hi/lo parts)
 };
 
+// The group starts node x contributes (reference: visit/chunk,
+// src/group_lists.cpp:65-113, as flags on tree positions):
+// every node holding more than group_size bodies, and every root, is one the
+// reference's depth-first visit reaches; a small one or a leaf is chunked
+// (starts every group_size bodies), a larger one flags the starts of its runs
+// of small children (packed greedily while they fit) and leaves its large
+// children to flag their own.  L: x's links (end known); child(c) gives child
+// c's links.  flags must start cleared.
+template <typename ChildLinks>
+__device__ inline void flag_group_starts(uint32_t x, int4 L, bool root, uint32_t gs, int pack,
+                                         uint32_t *__restrict__ flags,
+                                         uint32_t *__restrict__ node_of, ChildLinks child) {
+  const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
+  // not reached by visit(): a small child of its parent (roots always are)
+  if (!root && e - b <= gs) return;
+  if (e - b <= gs || L.y == 0) {  // chunk(begin, end)
+    for (uint32_t s = b; s < e; s += gs) {
+      flags[s] = 1u;
+      if (node_of) node_of[s] = x;
+    }
+    return;
+  }
+  bool open = false;
+  uint32_t run = 0, run_start = 0;
+  for (int c = 0; c < L.y; ++c) {
+    const int4 C = child(c);
+    const uint32_t cc = static_cast<uint32_t>(C.w - C.z);
+    if (cc <= gs) {
+      if (pack && open && run + cc <= gs) {
+        run += cc;
+        if (node_of) node_of[run_start] = x;  // siblings packed: their parent holds them
+      } else {
+        flags[C.z] = 1u;
+        if (node_of) node_of[C.z] = static_cast<uint32_t>(L.x + c);
+        run = cc;
+        run_start = static_cast<uint32_t>(C.z);
+        open = true;
+      }
+    } else {
+      open = false;  // visited in its own right: it flags its own starts
+    }
+  }
+}
+
 #pragma clang fp contract(off)
 
 // Node k's records from its geometry row g (centre, half, centre of mass,

@@ -726,36 +726,8 @@ __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ 
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const int4 L = lnk[x];
-  const uint32_t b = static_cast<uint32_t>(L.z), e = static_cast<uint32_t>(L.w);
-  // not reached by visit(): a small child of its parent (roots always are)
-  if (parent[x] != kNone && e - b <= gs) return;
-  if (e - b <= gs || L.y == 0) {      // chunk(begin, end)
-    for (uint32_t s = b; s < e; s += gs) {
-      flags[s] = 1u;
-      if (node_of) node_of[s] = x;
-    }
-    return;
-  }
-  bool open = false;
-  uint32_t run = 0, run_start = 0;
-  for (int c = 0; c < L.y; ++c) {
-    const int4 C = lnk[L.x + c];
-    const uint32_t cc = static_cast<uint32_t>(C.w - C.z);
-    if (cc <= gs) {
-      if (pack && open && run + cc <= gs) {
-        run += cc;
-        if (node_of) node_of[run_start] = x;  // siblings packed: their parent holds them
-      } else {
-        flags[C.z] = 1u;
-        if (node_of) node_of[C.z] = static_cast<uint32_t>(L.x + c);
-        run = cc;
-        run_start = static_cast<uint32_t>(C.z);
-        open = true;
-      }
-    } else {
-      open = false;  // visited in its own right: it flags its own starts
-    }
-  }
+  flag_group_starts(x, L, parent[x] == kNone, gs, pack, flags, node_of,
+                    [&](int c) { return lnk[L.x + c]; });
 }
 
 __global__ __launch_bounds__(kBlock) void compact_starts(const uint32_t *__restrict__ flags,
