@@ -658,11 +658,10 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
   launch_group_starts(in.lnk, parent, frame, cap, n, static_cast<uint32_t>(cfg->group_size), pack,
                       c->flags.get(), s, c->flag_node.get());
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
-                     s);
+                     s, c->starts.get());
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
-  launch_make_groups(c->flags.get(), c->offs.get(), n, frame, p.group_cap, in.tsid,
-                     in.root_node, c->starts.get(), c->groups.get(), groot, s,
-                     c->flag_node.get(), c->group_node.get());
+  launch_make_groups(n, frame, p.group_cap, in.tsid, in.root_node, c->starts.get(),
+                     c->groups.get(), groot, s, c->flag_node.get(), c->group_node.get());
   // (the child-set walks write the hi/lo group centres themselves)
   if (hilo && !walk_writes_centres(cfg, p))
     launch_group_centres(c->groups.get(), frame, p.group_cap, in.tpos, in.tsid, in.roots, in.geom,

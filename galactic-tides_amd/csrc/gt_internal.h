@@ -350,9 +350,11 @@ __device__ inline uint32_t bounded_count(const unsigned long long *c, uint32_t c
 
 // gt_scan.hip: exclusive prefix sum of n uint32 values into out (may alias in),
 // total written to *total (device).  tmp must hold scan_tmp_words(n) words.
+// compact (may be null; in holds 0/1 flags, out must not alias in): also
+// compact[out[p]] = p for every flagged p.
 size_t scan_tmp_words(size_t n);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned long long *total,
-                        unsigned long long *tmp, hipStream_t s);
+                        unsigned long long *tmp, hipStream_t s, uint32_t *compact = nullptr);
 
 // gt_sort.hip: stable LSD radix sort of (key, value) pairs on key bits [0, bits).
 // The sorted pairs end in (keys, vals); keys_alt/vals_alt are scratch of n
@@ -476,11 +478,11 @@ void launch_prep_lists(const double *geom, const int4 *lnk, const uint32_t *skip
 void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned long long *frame,
                          uint32_t node_cap, uint32_t n, uint32_t gs, int pack, uint32_t *flags,
                          hipStream_t s, uint32_t *node_of = nullptr);
-void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
-                        const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
-                        const uint32_t *root_node, uint32_t *starts, gt_group *groups,
-                        uint32_t *group_root, hipStream_t s, const uint32_t *node_of = nullptr,
-                        uint32_t *group_node = nullptr);
+// starts: the flagged positions in order (the compaction of the flags scan).
+void launch_make_groups(uint32_t n, const unsigned long long *frame, uint32_t group_cap,
+                        const uint32_t *tsid, const uint32_t *root_node, const uint32_t *starts,
+                        gt_group *groups, uint32_t *group_root, hipStream_t s,
+                        const uint32_t *node_of = nullptr, uint32_t *group_node = nullptr);
 // group_root: the node each group's walk starts from (null: node 0); geom
 // gives the root half sizes.  Coherent walks (lanes > 1) need a single tree.
 // Capacity mode: per-group slots of cap_c / cap_b entries at g * cap, or (with

@@ -68,10 +68,13 @@ __global__ __launch_bounds__(kBlock) void scan_totals(unsigned long long *__rest
 
 // kSum: tot holds the tiles' totals and the workgroup sums those before it
 // (the last tile writes the grand total); otherwise tot holds the offsets.
+// compact (may be null): for every position p whose value is 1 (flags),
+// compact[out[p]] = p as well.
 template <bool kSum>
 __global__ __launch_bounds__(kBlock) void tile_scan(const uint32_t *in, uint32_t *out, size_t n,
                                                     const unsigned long long *__restrict__ tot,
-                                                    unsigned long long *__restrict__ total) {
+                                                    unsigned long long *__restrict__ total,
+                                                    uint32_t *__restrict__ compact) {
   __shared__ unsigned long long wt[kBlock / kWave];
   __shared__ unsigned long long offset_s;
   unsigned long long offset;
@@ -97,7 +100,10 @@ __global__ __launch_bounds__(kBlock) void tile_scan(const uint32_t *in, uint32_t
   unsigned long long run = offset + inc - s;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    if (base + k < n) out[base + k] = static_cast<uint32_t>(run);
+    if (base + k < n) {
+      out[base + k] = static_cast<uint32_t>(run);
+      if (compact && v[k]) compact[run] = static_cast<uint32_t>(base + k);
+    }
     run += v[k];
   }
   if (kSum && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total = offset + inc;
@@ -110,7 +116,7 @@ __global__ void zero_total(unsigned long long *total) { *total = 0ull; }
 size_t scan_tmp_words(size_t n) { return (n + kTile - 1) / kTile + 1; }
 
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned long long *total,
-                        unsigned long long *tmp, hipStream_t s) {
+                        unsigned long long *tmp, hipStream_t s, uint32_t *compact) {
   const size_t ntiles = (n + kTile - 1) / kTile;
   if (ntiles == 0) {
     hipLaunchKernelGGL(zero_total, dim3(1), dim3(1), 0, s, total);
@@ -124,14 +130,14 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, unsigned lo
   }
   if (ntiles <= kSummedTiles) {
     hipLaunchKernelGGL(tile_scan<true>, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s,
-                       in, out, n, tmp, total);
+                       in, out, n, tmp, total, compact);
     GTH_LAUNCH_CHECK();
     return;
   }
   hipLaunchKernelGGL(scan_totals, dim3(1), dim3(kBlock), 0, s, tmp, ntiles, total);
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(tile_scan<false>, dim3(static_cast<uint32_t>(ntiles)), dim3(kBlock), 0, s, in,
-                     out, n, tmp, total);
+                     out, n, tmp, total, compact);
   GTH_LAUNCH_CHECK();
 }
 

@@ -730,13 +730,6 @@ __global__ __launch_bounds__(kBlock) void group_starts(const int4 *__restrict__ 
                     [&](int c) { return lnk[L.x + c]; });
 }
 
-__global__ __launch_bounds__(kBlock) void compact_starts(const uint32_t *__restrict__ flags,
-                                                         const uint32_t *__restrict__ offs,
-                                                         uint32_t n, uint32_t *__restrict__ starts) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p < n && flags[p]) starts[offs[p]] = p;
-}
-
 __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict__ starts,
                                                       const unsigned long long *__restrict__ frame,
                                                       uint32_t group_cap, uint32_t n,
@@ -878,16 +871,11 @@ void launch_group_starts(const int4 *lnk, const uint32_t *parent, const unsigned
   GTH_LAUNCH_CHECK();
 }
 
-void launch_make_groups(const uint32_t *flags, const uint32_t *offs, uint32_t n,
-                        const unsigned long long *frame, uint32_t group_cap, const uint32_t *tsid,
-                        const uint32_t *root_node, uint32_t *starts, gt_group *groups,
-                        uint32_t *group_root, hipStream_t s, const uint32_t *node_of,
-                        uint32_t *group_node) {
-  if (!n) return;
-  hipLaunchKernelGGL(compact_starts, dim3(blocks_for(n)), dim3(kBlock), 0, s, flags, offs, n,
-                     starts);
-  GTH_LAUNCH_CHECK();
-  if (!group_cap) return;
+void launch_make_groups(uint32_t n, const unsigned long long *frame, uint32_t group_cap,
+                        const uint32_t *tsid, const uint32_t *root_node, const uint32_t *starts,
+                        gt_group *groups, uint32_t *group_root, hipStream_t s,
+                        const uint32_t *node_of, uint32_t *group_node) {
+  if (!n || !group_cap) return;
   hipLaunchKernelGGL(make_groups, dim3(blocks_for(group_cap)), dim3(kBlock), 0, s, starts, frame,
                      group_cap, n, tsid, root_node, groups, group_root, node_of, group_node);
   GTH_LAUNCH_CHECK();
