@@ -734,6 +734,17 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   int4 L = lnk[x];
   double *g = geom + static_cast<size_t>(kGeomStride) * x;
   const double ccx = g[0], ccy = g[1], ccz = g[2];
+  // the records' other inputs, loaded before this node's stores (which the
+  // compiler may not move them past)
+  uint32_t skip_x = 0, depth_x = 0;
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  if (E.R.cells) {
+    skip_x = E.skip[x];
+    depth_x = E.depth[x];
+    const double *r = E.roots;  // one system: its root, not waiting for the links
+    if (E.tsid) r += 4 * static_cast<size_t>(E.tsid[L.z]);
+    for (int j = 0; j < 4; ++j) o[j] = r[j];
+  }
   double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
   double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (L.y == 0) {
@@ -798,8 +809,9 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   } else {
     const uint32_t c0 = static_cast<uint32_t>(L.x), nc = static_cast<uint32_t>(L.y);
     if (L.x < 0 || nc > kChunk || c0 + nc > n_nodes) return;  // malformed links: leave the row unset
-    L.w = lnk[c0 + nc - 1].w;  // the node ends where its last child does
-    lnk[x].w = L.w;
+    // the node ends where its last child does (stored after the children's
+    // loads, which the store would otherwise hold back)
+    const int end = lnk[c0 + nc - 1].w;
     double hm[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], hq[kChunk][6];
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
@@ -845,6 +857,8 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
         q[5] += hq[i][5] + mc * 3 * dy * dz;
       }
     }
+    L.w = end;
+    lnk[x].w = end;
   }
   g[4] = cx;
   g[5] = cy;
@@ -857,9 +871,7 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   if (E.R.cells) {
     const double row[15] = {ccx, ccy, ccz, g[3], cx,   cy,   cz,   m,
                             q[0], q[1], q[2], q[3], q[4], q[5], delta};
-    const double *o = E.roots + 4 * static_cast<size_t>(E.tsid ? E.tsid[L.z] : 0u);
-    write_node_records(E.R, x, row, L, E.skip[x], E.depth[x], o[0], o[1], o[2], o[3],
-                       E.walk_error);
+    write_node_records(E.R, x, row, L, skip_x, depth_x, o[0], o[1], o[2], o[3], E.walk_error);
   }
 }
 
