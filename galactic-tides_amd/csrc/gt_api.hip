@@ -100,6 +100,7 @@ struct gt_hip_context {
   hipStream_t stream2 = nullptr;  // evaluation of walked chunks, alongside the walk
   hipEvent_t ev[6] = {};
   hipEvent_t walk_done[kMaxChunks] = {}, eval_done = nullptr;
+  bool groups_with_tree = false;  // the last tree build flagged the group starts
   hipEvent_t eval_t0[kMaxChunks] = {}, eval_t1[kMaxChunks] = {};
   uint32_t overlapped = 0;  // chunks of the last evaluation run alongside its walk (0: none)
   std::string err;
@@ -619,6 +620,7 @@ struct ListsInput {
   const uint32_t *root_node = nullptr;
   const uint32_t *parent = nullptr, *skip = nullptr, *depth = nullptr;
   bool records = false;  // the walk's and evaluation's records are written already
+  bool groups = false;   // the group starts are flagged already (by the tree build)
 };
 
 // The context's record arrays for evaluations with bounds p (reserved).
@@ -637,6 +639,17 @@ ListRecords list_records(gt_hip_context *c, const gt_group_config *cfg, const Pl
 
 // Everything of the lists before the walk: links, records, groups, and the
 // predicted slots.
+// The flags of the group starts, for the tree build to write (node_links),
+// or none for a batch's build (systems: lists of a single tree only).
+GroupFlags tree_group_flags(gt_hip_context *c, const gt_group_config *cfg) {
+  GroupFlags f;
+  f.gs = static_cast<uint32_t>(cfg->group_size);
+  f.pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
+  f.flags = c->flags.get();
+  f.node_of = c->flag_node.get();
+  return f;
+}
+
 void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Plan &p,
                         const ListsInput &in, hipStream_t s) {
   unsigned long long *frame = c->frame.get();
@@ -655,8 +668,9 @@ void enqueue_lists_prep(gt_hip_context *c, const gt_group_config *cfg, const Pla
     launch_prep_lists(in.geom, in.lnk, skip, depth, in.tpos, in.tmass, n, frame, cap, in.tsid,
                       in.roots, list_records(c, cfg, p), s);
   const int pack = cfg->pack < 0 ? 1 : (cfg->pack != 0);
-  launch_group_starts(in.lnk, parent, frame, cap, n, static_cast<uint32_t>(cfg->group_size), pack,
-                      c->flags.get(), s, c->flag_node.get());
+  if (!in.groups)
+    launch_group_starts(in.lnk, parent, frame, cap, n, static_cast<uint32_t>(cfg->group_size),
+                        pack, c->flags.get(), s, c->flag_node.get());
   exclusive_scan_u32(c->flags.get(), c->offs.get(), n, frame + frame::kGroups, c->scan_tmp.get(),
                      s, c->starts.get());
   uint32_t *groot = in.tsid ? c->group_root.get() : nullptr;
@@ -743,6 +757,7 @@ ListsInput tree_input(const gt_hip_context *c) {
   in.skip = t.skips();
   in.depth = t.depths();
   in.records = t.wrote_records();
+  in.groups = c->groups_with_tree;
   return in;
 }
 
@@ -1038,7 +1053,9 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     GTH_CHECK(hipEventRecord(c->ev[0], s));
     reserve_all(c, p);
     const ListRecords rec = list_records(c, cfg, p);
-    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig, &rec);
+    const GroupFlags gf = tree_group_flags(c, cfg);
+    c->groups_with_tree = true;
+    c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig, &rec, &gf);
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     enqueue_tree_lists(c, cfg, p);
@@ -1232,8 +1249,10 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       {
         reserve_all(c, p);
         const ListRecords rec = list_records(c, cfg, p);
+        const GroupFlags gf = tree_group_flags(c, cfg);
+        c->groups_with_tree = true;
         c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s,
-                        c->orig.get(), &rec);
+                        c->orig.get(), &rec, &gf);
       }
       break;
     case 1:
@@ -1518,6 +1537,7 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
     c->tree_on_state = false;
+    c->groups_with_tree = false;
     c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     read_frame(c);

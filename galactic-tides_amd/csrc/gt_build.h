@@ -28,9 +28,12 @@ class TreeBuilder {
   // the build (with each node's moments, each body with its gather into
   // tree order) instead of by prep_lists; a build for the lists also keeps
   // the inverse of its order (inverse_order) for the force scatter.
+  // rec: the lists' records to write with the moments and the gather (null:
+  // none); groups: the group starts to flag from the nodes' links (null: none).
   void enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                const unsigned long long *sys_off, unsigned long long *frame, hipStream_t s,
-               const uint32_t *orig = nullptr, const ListRecords *rec = nullptr);
+               const uint32_t *orig = nullptr, const ListRecords *rec = nullptr,
+               const GroupFlags *groups = nullptr);
   bool wrote_records() const { return wrote_records_; }
   void set_counts(uint32_t n_nodes, int depth) {
     n_nodes_ = n_nodes;

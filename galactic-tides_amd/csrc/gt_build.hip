@@ -400,7 +400,10 @@ __global__ __launch_bounds__(kBlock) void leaf_depths(const uint8_t *__restrict_
 // p: the one containing p), its first child (p's node one level down; a node
 // at p's leaf depth is a leaf) and its skip, the node a LIFO depth-first walk
 // takes after finishing the node's subtree (its previous sibling, else its
-// parent's skip).  Workgroup 0 writes the level starts
+// parent's skip).  Position p also ends its nodes deeper than the levels it
+// shares with p + 1 (the nodes of those depths that contain p: the one that
+// begins at p, or else the last one before it), and writes their end, p + 1.
+// Workgroup 0 writes the level starts
 // (lstart[d]; levels past the deepest 0, lstart[deepest + 1] = the total), the
 // level sizes, the node total and the depth into the frame.  A tree with more
 // nodes than node_cap is not written.
@@ -416,12 +419,21 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
+// A node's first child, child count (0 here) and begin, leaving its end.
+__device__ inline void put_link(int4 *l, int first_child, int begin) {
+  int *w = reinterpret_cast<int *>(l);
+  w[0] = first_child;
+  w[1] = 0;
+  w[2] = begin;
+}
+
 __global__ __launch_bounds__(kBlock) void emit_levels(
     const uint8_t *__restrict__ lcp, const uint8_t *__restrict__ ldep, uint32_t n,
     uint32_t rows, const uint32_t *__restrict__ level_offsets,
     unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t *__restrict__ lstart,
     int4 *__restrict__ lnk, uint32_t *__restrict__ parent, uint32_t *__restrict__ node_begin,
-    uint32_t *__restrict__ node_depth, uint32_t *__restrict__ skip) {
+    uint32_t *__restrict__ node_depth, uint32_t *__restrict__ skip,
+    uint32_t *__restrict__ group_flags) {
   __shared__ uint32_t rank0[kRankSlots][kDepthLimit + 1];  // counts, then their prefix
   const uint32_t ntiles = gridDim.x, tile = blockIdx.x;
   const int tmax = static_cast<int>(rows) - 2;  // deepest possible level
@@ -445,29 +457,45 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
       frame[frame::kDepth] = static_cast<unsigned long long>(deepest);
     }
   }
+  if (group_flags)  // (cleared for node_links to flag the group starts)
+    for (uint32_t i = 0; i < kLevelItems; ++i) {
+      const uint32_t p = tile * kLevelTile + i * kBlock + threadIdx.x;
+      if (p < n) group_flags[p] = 0u;
+    }
   if (total > node_cap) return;  // over capacity: no tree (the whole grid)
   for (uint32_t k = threadIdx.x; k < kRankSlots * (kDepthLimit + 1); k += kBlock)
     (&rank0[0][0])[k] = 0u;
   const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const uint32_t base = tile * kLevelTile + threadIdx.x;
-  int first[kLevelItems], d[kLevelItems];
-  int lo = kDepthLimit + 1, hi = -1;  // the depths of the wave's nodes
+  int first[kLevelItems], d[kLevelItems], last[kLevelItems];
+  int lo = kDepthLimit + 1, hi = -1;  // the depths of the nodes that begin in the wave
+  int end_lo = kDepthLimit + 1, end_hi = -1;  // and of those that end in it
 #pragma unroll
   for (uint32_t i = 0; i < kLevelItems; ++i) {
     const uint32_t p = base + i * kBlock;
     first[i] = 1;
     d[i] = 0;  // (no nodes past n)
+    last[i] = 1;
     if (p < n) {
       first[i] = first_level(lcp, p);
       d[i] = ldep[p];
+      // p is the last body of its nodes deeper than the levels it shares
+      // with p + 1 (all of them at the end, or at a system's end)
+      last[i] = p + 1 < n ? shared(lcp[p]) + 1 : 0;
     }
     if (first[i] <= d[i]) {
       lo = min(lo, first[i]);
       hi = max(hi, d[i]);
     }
+    if (last[i] <= d[i]) {
+      end_lo = min(end_lo, last[i]);
+      end_hi = max(end_hi, d[i]);
+    }
   }
   lo = wave_min(lo);
   hi = wave_max(hi);
+  end_lo = wave_min(end_lo);
+  end_hi = wave_max(end_hi);
   __syncthreads();  // (counts cleared)
 #pragma unroll
   for (uint32_t i = 0; i < kLevelItems; ++i)
@@ -485,16 +513,18 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
     }
   }
   __syncthreads();
-  if (hi < 0) return;  // no nodes begin in this wave
+  if (hi < 0 && end_hi < 0) return;  // no nodes begin or end in this wave
   const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
-  const int t0 = lo > 0 ? lo - 1 : 0;  // from the level above (the first nodes' parents)
+  // from the level above the first nodes that begin (their parents), or the
+  // first level of a node that ends
+  const int t0 = min(lo > 0 ? lo - 1 : 0, end_lo), t1 = max(hi, end_hi);
 #pragma unroll
   for (uint32_t i = 0; i < kLevelItems; ++i) {
     const uint32_t p = base + i * kBlock;
     uint32_t above = kNone;  // the last node of depth t - 1 that begins at or before p
     uint32_t prev = kNone;   // p's node of depth t - 1
     uint32_t after = kNone;  // what a LIFO walk takes after the subtree of p's nodes
-    for (int t = t0; t <= hi; ++t) {
+    for (int t = t0; t <= t1; ++t) {
       const bool in = first[i] <= t && t <= d[i];
       const uint64_t m = __ballot(in);
       const uint32_t at = level_offsets[static_cast<size_t>(t) * ntiles + tile] +
@@ -509,10 +539,15 @@ __global__ __launch_bounds__(kBlock) void emit_levels(
         node_begin[at] = p;
         node_depth[at] = static_cast<uint32_t>(t);
         skip[at] = after;
-        if (t > first[i]) lnk[prev] = make_int4(static_cast<int>(at), 0, static_cast<int>(p), 0);
-        if (t == d[i]) lnk[at] = make_int4(-1, 0, static_cast<int>(p), 0);
+        // (first child, child count, begin: the end is another position's)
+        if (t > first[i]) put_link(lnk + prev, static_cast<int>(at), static_cast<int>(p));
+        if (t == d[i]) put_link(lnk + at, -1, static_cast<int>(p));
         prev = at;
       }
+      // p's node of depth t ends after p: the one that begins at p, or else
+      // the last that began before it
+      if (last[i] <= t && t <= d[i])
+        reinterpret_cast<int *>(lnk + (in ? at : at - 1u))[3] = static_cast<int>(p + 1);
       above = in ? at : at - 1u;  // (at - 1: this tile's or an earlier tile's last before p)
     }
   }
@@ -571,31 +606,26 @@ __device__ inline void sort_leaf(uint32_t *__restrict__ vals, const uint32_t *__
 // key at the node's begin shifted back into place, else the path words of
 // particle vals[begin]), its child count (its children are consecutive on the
 // level below, from its first child on while their parent is the node: at
-// most eight, read together), each system's root node, and a leaf's end: the
-// first position q + 1 after its begin whose predecessor q shares fewer than
-// the leaf's depth levels with it (lcp, read sixteen at a time); then a
-// leaf's bodies are put back into particle order (sort_leaf).  (Both reads
-// are whole aligned 16-byte words: one address per lane instead of one per
-// value.)  Reading vals[begin] for an inner node's path while a leaf that
-// begins there is sorted is harmless: every body of the leaf has the node's
-// path.  An inner
-// node's end is its last child's, set with its moments.
-constexpr uint32_t kEndChunk = 16;
-
+// most eight, read together as whole aligned 16-byte words), and each
+// system's root node; then a leaf's bodies are put back into particle order
+// (sort_leaf; the leaf's end is in its link already).  Reading vals[begin]
+// for an inner node's path while a leaf that begins there is sorted is
+// harmless: every body of the leaf has the node's path.
 __global__ __launch_bounds__(kBlock) void node_links(
     const uint32_t *__restrict__ parent, const uint32_t *__restrict__ node_begin,
-    const uint32_t *__restrict__ node_depth, const uint8_t *__restrict__ lcp,
-    const unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t n,
+    const uint32_t *__restrict__ node_depth, const unsigned long long *__restrict__ frame, uint32_t node_cap, uint32_t n,
     const uint64_t *__restrict__ w0, const uint64_t *__restrict__ w1,
     const uint64_t *__restrict__ w2, uint32_t *__restrict__ vals,
     const uint64_t *__restrict__ sorted_keys, int key_shift, const double *__restrict__ roots,
     const uint32_t *__restrict__ sid, double *__restrict__ geom, int4 *__restrict__ lnk,
-    uint32_t *__restrict__ root_node, const uint32_t *__restrict__ orig) {
+    uint32_t *__restrict__ root_node, const uint32_t *__restrict__ orig, GroupFlags groups) {
   const uint32_t n_nodes = bounded_count(frame + frame::kNodes, node_cap);
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   if (x >= n_nodes) return;
   const uint32_t d = node_depth[x], b = node_begin[x];
-  const int first = lnk[x].x;
+  int4 L = lnk[x];  // (first child, -, begin, end)
+  const int first = L.x;
+  const bool is_root = parent[x] == kNone;
   // centre and half size
   uint64_t kw[3] = {0ull, 0ull, 0ull};
   uint32_t system = 0;
@@ -643,31 +673,20 @@ __global__ __launch_bounds__(kBlock) void node_links(
       if (children == static_cast<int>(k) + 1 && c < n_nodes && pc == x) ++children;
     }
     lnk[x].y = children;
+    L.y = children;
+    // the group starts it flags (its children's begins and ends are theirs
+    // from emit_levels already)
+    if (groups.flags)
+      flag_group_starts(x, L, is_root, groups.gs, groups.pack, groups.flags, groups.node_of,
+                        [&](int c) { return lnk[first + c]; });
     return;
   }
-  // a leaf (lnk[x].y stays 0): its end, from lcp read as aligned 16-byte
-  // words (lcp is padded past n; positions from n - 1 on end every leaf)
-  uint32_t q = b;
-  for (;;) {
-    const uint32_t base = q & ~(kEndChunk - 1u);
-    const uint4 w = *reinterpret_cast<const uint4 *>(lcp + base);
-    const uint32_t words[4] = {w.x, w.y, w.z, w.w};
-    uint32_t stop = kEndChunk;
-#pragma unroll
-    for (int j = static_cast<int>(kEndChunk) - 1; j >= 0; --j) {
-      const uint32_t t = base + static_cast<uint32_t>(j);
-      const uint8_t v = static_cast<uint8_t>(words[j >> 2] >> (8 * (j & 3)));
-      if (t >= q && (t + 1u >= n || shared(v) < static_cast<int>(d)))
-        stop = static_cast<uint32_t>(j);
-    }
-    if (stop < kEndChunk) {
-      q = base + stop;
-      break;
-    }
-    q = base + kEndChunk;
-  }
-  lnk[x].w = static_cast<int>(q + 1);
-  sort_leaf(vals, orig, b, q + 1);
+  // a leaf (lnk[x].y stays 0): its group starts, then its bodies back into
+  // particle order
+  if (groups.flags)
+    flag_group_starts(x, L, is_root, groups.gs, groups.pack, groups.flags, groups.node_of,
+                      [&](int c) { return lnk[first + c]; });
+  sort_leaf(vals, orig, b, static_cast<uint32_t>(L.w));
 }
 
 // Positions and masses into tree order (tpos/tmass, and the system of each
@@ -809,9 +828,6 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
   } else {
     const uint32_t c0 = static_cast<uint32_t>(L.x), nc = static_cast<uint32_t>(L.y);
     if (L.x < 0 || nc > kChunk || c0 + nc > n_nodes) return;  // malformed links: leave the row unset
-    // the node ends where its last child does (stored after the children's
-    // loads, which the store would otherwise hold back)
-    const int end = lnk[c0 + nc - 1].w;
     double hm[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], hq[kChunk][6];
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
@@ -857,8 +873,6 @@ __device__ __attribute__((always_inline)) inline void node_moments(uint32_t x, u
         q[5] += hq[i][5] + mc * 3 * dy * dz;
       }
     }
-    L.w = end;
-    lnk[x].w = end;
   }
   g[4] = cx;
   g[5] = cy;
@@ -960,7 +974,7 @@ void TreeBuilder::reserve(const Plan &p) {
   if (p.words > 1) word1_.ensure(n);
   if (p.words > 2) word2_.ensure(n);
   sort_tmp_.ensure(radix_tmp_words(n));
-  lcp_.ensure(n + kEndChunk);  // (node_links reads whole aligned 16-byte words)
+  lcp_.ensure(n);
   ldep_.ensure(n);
   const size_t level_cells =
       static_cast<size_t>(kDepthLimit + 2) * ((n + kLevelTile - 1) / kLevelTile);
@@ -980,7 +994,8 @@ void TreeBuilder::reserve(const Plan &p) {
 
 void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, int leaf_size,
                           const unsigned long long *sys_off, unsigned long long *frame,
-                          hipStream_t s, const uint32_t *orig, const ListRecords *rec) {
+                          hipStream_t s, const uint32_t *orig, const ListRecords *rec,
+                          const GroupFlags *groups) {
   wrote_records_ = false;
   const uint32_t n = p.n;
   n_ = n;
@@ -1073,9 +1088,9 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                      reinterpret_cast<uint32_t *>(frame + frame::kUnresolved));
   GTH_LAUNCH_CHECK();
 
-  // 5-6. nodes in level order, with parents and first children; then per
-  // node its geometry and child count, and a leaf's end (inner nodes' ends
-  // come with their moments)
+  // 5-6. nodes in level order, with parents, first children, begins and
+  // ends; then per node its geometry and child count, and a leaf's bodies in
+  // particle order
   launch_row_scan(rows, ntiles, 1u, reinterpret_cast<uint32_t *>(scan_words + rows), scan_words,
                   frame + frame::kSortCtl, level_counts, level_offsets, s);
   uint32_t *lstart = lstart_.get();
@@ -1085,13 +1100,14 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint32_t *nbeg = nbeg_.get();
   uint32_t *ndep = node_depth_.get();
   hipLaunchKernelGGL(emit_levels, dim3(ntiles), dim3(kBlock), 0, s, lcp, ldep, n, rows,
-                     level_offsets, frame, cap, lstart, lnk, parent, nbeg, ndep, skip_.get());
+                     level_offsets, frame, cap, lstart, lnk, parent, nbeg, ndep, skip_.get(),
+                     groups ? groups->flags : static_cast<uint32_t *>(nullptr));
   GTH_LAUNCH_CHECK();
   hipLaunchKernelGGL(node_links, dim3(blocks_for(cap)), dim3(kBlock), 0, s, parent, nbeg, ndep,
-                     lcp, frame, cap, n, w[0], w[1], w[2], vals,
+                     frame, cap, n, w[0], w[1], w[2], vals,
                      p.words == 1 && !batch ? keys : static_cast<const uint64_t *>(nullptr),
                      p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, geom, lnk,
-                     root_node, orig);
+                     root_node, orig, groups ? *groups : GroupFlags{});
   GTH_LAUNCH_CHECK();
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();

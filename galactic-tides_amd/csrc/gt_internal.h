@@ -132,6 +132,16 @@ struct ListRecords {
   float4 *bodies_lo = nullptr;   // (null: no hi/lo parts)
 };
 
+// Group starts flagged on tree positions by the tree build (emit_levels
+// clears the n flags, node_links flags each node's starts): group size,
+// packing, flags, and (may be null) the node each start is taken from.
+struct GroupFlags {
+  uint32_t gs = 0;
+  int pack = 1;
+  uint32_t *flags = nullptr;
+  uint32_t *node_of = nullptr;
+};
+
 // The group starts node x contributes (reference: visit/chunk,
 // src/group_lists.cpp:65-113, as flags on tree positions):
 // every node holding more than group_size bodies, and every root, is one the
