@@ -261,15 +261,25 @@ __global__ __launch_bounds__(kBlock) void gather_sid_keys(const uint32_t *__rest
 // ---- 4. shared prefixes and leaf depths ---------------------------------------------
 
 // Levels shared by the paths of sorted neighbours p, p+1 (at most 21 words),
-// at most cap (the levels the sort ordered by).
+// at most cap (the levels the sort ordered by).  With one path word and one
+// system (sorted_keys set) the sorted keys are those paths' first cap levels,
+// in order: read consecutively instead of through vals.
 __global__ __launch_bounds__(kBlock) void shared_levels(const uint64_t *__restrict__ w0,
                                                         const uint64_t *__restrict__ w1,
                                                         const uint64_t *__restrict__ w2, int words,
                                                         int cap, const uint32_t *__restrict__ vals,
                                                         const uint32_t *__restrict__ sid,
-                                                        uint32_t n, uint8_t *__restrict__ lcp) {
+                                                        const uint64_t *__restrict__ sorted_keys,
+                                                        int key_shift, uint32_t n,
+                                                        uint8_t *__restrict__ lcp) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p + 1 >= n) return;
+  if (sorted_keys) {
+    const uint64_t d = (sorted_keys[p] ^ sorted_keys[p + 1]) << key_shift;
+    const int common = d ? (__clzll(d) - 1) / 3 : kLevels;
+    lcp[p] = static_cast<uint8_t>(min(common, cap));
+    return;
+  }
   const uint32_t a = vals[p], b = vals[p + 1];
   if (sid && sid[a] != sid[b]) {
     lcp[p] = kNoShare;
@@ -1076,7 +1086,9 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
   uint8_t *lcp = lcp_.get();
   uint8_t *ldep = ldep_.get();
   hipLaunchKernelGGL(shared_levels, dim3(blocks_for(n)), dim3(kBlock), 0, s, w[0], w[1], w[2],
-                     p.words, levels, vals, sid, n, lcp);
+                     p.words, levels, vals, sid,
+                     p.words == 1 && !batch ? keys : static_cast<const uint64_t *>(nullptr),
+                     p.words == 1 ? 3 * (kLevels - levels) : 0, n, lcp);
   GTH_LAUNCH_CHECK();
   const uint32_t ntiles = static_cast<uint32_t>((n + kLevelTile - 1) / kLevelTile);
   const uint32_t rows = static_cast<uint32_t>(std::min(levels, kDepthLimit)) + 2u;
