@@ -478,21 +478,30 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   const uint32_t used = (n_items + waves_per_block - 1) / waves_per_block;
   // queue: a persistent grid; the waves on each XCD take that XCD's share of
   // the items (the same shares as the static mapping) one at a time from the
-  // XCD's counter, in order.
-  uint32_t wi_lo = 0, wi_hi = 0, xcd = 0;
-  if (queue) {
-    xcd = blockIdx.x % kXcds;
-    const uint32_t q = (used + kXcds - 1) / kXcds;
-    wi_lo = min(xcd * q * waves_per_block, n_items);
-    wi_hi = min((xcd + 1) * q * waves_per_block, n_items);
-  }
+  // XCD's counter, in order.  A wave whose XCD's share is used up takes the
+  // next items of the other XCDs' shares, nearest XCD number first, so no XCD
+  // idles while another still has a tail of items.
+  const uint32_t q_items =
+      queue ? (used + kXcds - 1) / kXcds * waves_per_block : 0u;  // items per share
+  const uint32_t home = blockIdx.x % kXcds;
+  uint32_t share = 0;  // home + share (mod 8): the share being taken from
   for (uint32_t step = 0;; ++step) {
   uint32_t wi;
   if (queue) {
-    uint32_t t = 0;
-    if (threadIdx.x % kWave == 0) t = atomicAdd(queue + xcd * kQueueStride, 1u);
-    wi = __builtin_amdgcn_readfirstlane(wi_lo + __builtin_amdgcn_readfirstlane(t));
-    if (wi >= wi_hi) break;
+    wi = n_items;
+    while (share < kXcds) {
+      const uint32_t x = (home + share) % kXcds;
+      uint32_t t = 0;
+      if (threadIdx.x % kWave == 0) t = atomicAdd(queue + x * kQueueStride, 1u);
+      t = __builtin_amdgcn_readfirstlane(t);
+      const uint32_t lo = min(x * q_items, n_items), hi = min(lo + q_items, n_items);
+      if (t < hi - lo) {
+        wi = lo + t;
+        break;
+      }
+      ++share;  // (this share is used up: its counter only grows from here)
+    }
+    if (wi >= n_items) break;
   } else {
     if (step) break;
     const uint32_t blk = xcd_block_used(blockIdx.x, used);
