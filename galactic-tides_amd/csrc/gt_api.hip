@@ -100,7 +100,6 @@ struct gt_hip_context {
   hipStream_t stream2 = nullptr;  // evaluation of walked chunks, alongside the walk
   hipEvent_t ev[6] = {};
   hipEvent_t walk_done[kMaxChunks] = {}, eval_done = nullptr;
-  bool groups_with_tree = false;  // the last tree build flagged the group starts
   hipEvent_t eval_t0[kMaxChunks] = {}, eval_t1[kMaxChunks] = {};
   uint32_t overlapped = 0;  // chunks of the last evaluation run alongside its walk (0: none)
   std::string err;
@@ -639,8 +638,8 @@ ListRecords list_records(gt_hip_context *c, const gt_group_config *cfg, const Pl
 
 // Everything of the lists before the walk: links, records, groups, and the
 // predicted slots.
-// The flags of the group starts, for the tree build to write (node_links),
-// or none for a batch's build (systems: lists of a single tree only).
+// The flags of the group starts, for the tree build to write (node_links;
+// the lists then take them instead of flagging them again).
 GroupFlags tree_group_flags(gt_hip_context *c, const gt_group_config *cfg) {
   GroupFlags f;
   f.gs = static_cast<uint32_t>(cfg->group_size);
@@ -757,7 +756,7 @@ ListsInput tree_input(const gt_hip_context *c) {
   in.skip = t.skips();
   in.depth = t.depths();
   in.records = t.wrote_records();
-  in.groups = c->groups_with_tree;
+  in.groups = t.flagged_groups();
   return in;
 }
 
@@ -1054,7 +1053,6 @@ void force_eval(gt_hip_context *c, const gt_group_config *cfg, const double *pos
     reserve_all(c, p);
     const ListRecords rec = list_records(c, cfg, p);
     const GroupFlags gf = tree_group_flags(c, cfg);
-    c->groups_with_tree = true;
     c->tree.enqueue(pos, mass, p, cfg->leaf_size, c->sys_off, frame, s, orig, &rec, &gf);
     geom = c->tree.geom();
     GTH_CHECK(hipEventRecord(c->ev[1], s));
@@ -1250,7 +1248,6 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
         reserve_all(c, p);
         const ListRecords rec = list_records(c, cfg, p);
         const GroupFlags gf = tree_group_flags(c, cfg);
-        c->groups_with_tree = true;
         c->tree.enqueue(c->pos.get(), c->mass.get(), p, cfg->leaf_size, c->sys_off, frame, s,
                         c->orig.get(), &rec, &gf);
       }
@@ -1537,7 +1534,6 @@ Plan build_single_tree(gt_hip_context *c, const gt_group_config *cfg, uint32_t n
     clear_counts(frame, s);
     GTH_CHECK(hipEventRecord(c->ev[0], s));
     c->tree_on_state = false;
-    c->groups_with_tree = false;
     c->tree.enqueue(dp, dm, p, cfg->leaf_size, nullptr, frame, s);
     GTH_CHECK(hipEventRecord(c->ev[1], s));
     read_frame(c);

@@ -35,6 +35,7 @@ class TreeBuilder {
                const uint32_t *orig = nullptr, const ListRecords *rec = nullptr,
                const GroupFlags *groups = nullptr);
   bool wrote_records() const { return wrote_records_; }
+  bool flagged_groups() const { return flagged_groups_; }  // the last build flagged group starts
   void set_counts(uint32_t n_nodes, int depth) {
     n_nodes_ = n_nodes;
     depth_ = depth;
@@ -66,6 +67,7 @@ class TreeBuilder {
  private:
   uint32_t n_ = 0, n_nodes_ = 0, node_cap_ = 0, systems_ = 1;
   bool wrote_records_ = false;
+  bool flagged_groups_ = false;
   int depth_ = 0;
   DBuf<double> part_, root_, geom_, tpos_, tmass_;
   DBuf<uint64_t> keys_, keys_alt_, word0_, word1_, word2_;

@@ -1007,6 +1007,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                           hipStream_t s, const uint32_t *orig, const ListRecords *rec,
                           const GroupFlags *groups) {
   wrote_records_ = false;
+  flagged_groups_ = false;
   const uint32_t n = p.n;
   n_ = n;
   node_cap_ = p.node_cap;
@@ -1121,6 +1122,7 @@ void TreeBuilder::enqueue(const double *pos, const double *mass, const Plan &p, 
                      p.words == 1 ? 3 * (kLevels - levels) : 0, root_.get(), sid, geom, lnk,
                      root_node, orig, groups ? *groups : GroupFlags{});
   GTH_LAUNCH_CHECK();
+  flagged_groups_ = groups && groups->flags;
   double *tpos = tpos_.get();
   double *tmass = tmass_.get();
   // the walk's and evaluation's records with the moments and the gather
