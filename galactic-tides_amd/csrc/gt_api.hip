@@ -201,7 +201,8 @@ struct gt_hip_context {
   DBuf<double> nb_stage_dist2;
   DBuf<uint32_t> nb_over;  // queries whose lists outgrew their slots
   uint32_t nb_stage = 64;  // slot entries per query (grows when many overflow)
-  DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation (kQueueStride apart)
+  DBuf<uint32_t> eval_queue;  // per-XCD item counters of the persistent evaluation and its
+                              // count of waves out (kQueueStride apart; zero between launches)
   void *noalloc_scratch = nullptr;  // this device's gt_hip_tree_noalloc scratch
   DBuf<double> pt_pos;  // gt_hip_tree_gravity_at's points
   // predictor kernels on caller data (test hooks)
@@ -1587,7 +1588,8 @@ int gt_hip_create(gt_hip_context **out, int device) {
     for (auto &e : c->walk_done) GTH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     GTH_CHECK(hipEventCreateWithFlags(&c->eval_done, hipEventDisableTiming));
     launch_fill_u64(c->frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
-    c->eval_queue.ensure(kXcds * kQueueStride);
+    launch_fill_u32(c->eval_queue.ensure((kXcds + 1) * kQueueStride), (kXcds + 1) * kQueueStride,
+                    0u, c->stream);
     launch_fill_u64(c->raw_frame.ensure(frame::kWords), frame::kWords, 0ull, c->stream);
     // the no-allocation entry point's device scratch on this device (current)
     c->noalloc_scratch = noalloc_scratch(device);

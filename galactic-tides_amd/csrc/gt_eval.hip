@@ -565,6 +565,19 @@ __global__ __launch_bounds__(kBlock, WAVES) void eval_fast(
   }
   }
   }
+  // The last wave out of the persistent grid clears the counters for the next
+  // launch (every wave has taken its last item by then: its taking waited for
+  // the counter's value), so no fill launch is needed between evaluations.
+  // queue[kXcds * kQueueStride] counts the waves out.
+  if (queue && threadIdx.x % kWave == 0) {
+    uint32_t *const out_count = queue + kXcds * kQueueStride;
+    if (atomicAdd(out_count, 1u) == gridDim.x * waves_per_block - 1u) {
+      for (uint32_t x = 0; x < kXcds; ++x)
+        __hip_atomic_store(queue + x * kQueueStride, 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(out_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // ---- long lists in parts -----------------------------------------------------------
@@ -733,7 +746,8 @@ void launch_eval(const float4 *bodies, const Cell *cells, const uint32_t *cell_l
   const uint32_t item_cap = parts ? split->item_cap : g_span;
   const uint32_t blocks = (item_cap + waves_per_block - 1) / waves_per_block;
   if (exact || !persistent_blocks) queue = nullptr;
-  if (queue) launch_fill_u32(queue, kXcds * kQueueStride, 0u, s);
+  // (queue: kXcds + 1 counters kQueueStride apart, zero at the first launch;
+  // each launch leaves them zero)
   const dim3 grid(exact ? blocks : queue ? xcd_grid(persistent_blocks) : xcd_grid(blocks)),
       block(kBlock);
   const uint2 *items = parts ? split->items : nullptr;
