@@ -724,8 +724,8 @@ void enqueue_lists_finish(gt_hip_context *c, const Plan &p, hipStream_t s) {
   if (p.predicted)
     launch_prediction_error(frame, p.group_cap, c->pred_out.get(), c->ccount.get(),
                             c->bcount.get(), frame + frame::kPredError, s);
-  launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s);
-  launch_check_frame(frame, p, true, s);
+  launch_list_stats(c->ccount.get(), c->bcount.get(), c->groups.get(), frame, p.group_cap, s,
+                    &p);
 }
 
 // The whole lists phase on the main stream; with p.chunks > 1 the walk runs

@@ -514,9 +514,11 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
                  uint32_t *body_list, hipStream_t s, const uint32_t *slot_c = nullptr,
                  const uint32_t *slot_b = nullptr, uint32_t g_begin = 0, uint32_t g_span = 0,
                  float4 *group_hilo = nullptr);
+// check (may be null): also set the status as launch_check_frame(frame,
+// *check, true) does, in the same launch.
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                       hipStream_t s);
+                       hipStream_t s, const Plan *check = nullptr);
 // Sets frame[kStatus] from the counts against the plan's bounds (list slots
 // are checked only when check_slots).
 void launch_check_frame(unsigned long long *frame, const Plan &plan, bool check_slots,

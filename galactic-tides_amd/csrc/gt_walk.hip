@@ -752,17 +752,36 @@ __global__ __launch_bounds__(kBlock) void make_groups(const uint32_t *__restrict
   if (group_node) group_node[g] = node_of[G.begin];
 }
 
+// The status of an evaluation from its counts and the plan's bounds (one thread).
+__device__ inline void check_counts(unsigned long long *frame, uint32_t node_cap,
+                                    uint32_t depth_cap, uint32_t group_cap, int check_slots) {
+  unsigned long long st = 0;
+  if (frame[frame::kNodes] > node_cap) st |= kAbortNodes;
+  if (frame[frame::kUnresolved] & 1ull) st |= kAbortPaths;
+  if (frame[frame::kDepth] > depth_cap) st |= kAbortDepth;
+  if (frame[frame::kGroups] > group_cap) st |= kAbortGroups;
+  if (check_slots && frame[frame::kSlotOverflow]) st |= kAbortLists;
+  if (frame[frame::kWalkError] & 1ull) st |= kAbortWalk;
+  if (st) atomicOr(frame + frame::kStatus, st);  // (another stream may set bits too)
+}
+
 // stats: [0] cell interactions, [1] body interactions, [2] max list, [3] max group,
 // [4] most cells of a group, [5] most bodies of a group, [6] all cells, [7] all bodies.
 // A few hundred blocks stride over the groups and each adds its totals with one
 // set of atomics (one set per wave made the atomics the kernel's cost).
 constexpr uint32_t kStatsBlocks = 256;
 
+// With check set, thread 0 of workgroup 0 also sets the status from the
+// counts and the plan's bounds (check_frame's test with the slots checked:
+// the counts it reads are final once the walk has ended).
 __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict__ cell_count,
                                                      const uint32_t *__restrict__ body_count,
                                                      const gt_group *__restrict__ groups,
                                                      unsigned long long *__restrict__ frame,
-                                                     uint32_t group_cap) {
+                                                     uint32_t group_cap, int check,
+                                                     uint32_t node_cap, uint32_t depth_cap) {
+  if (check && blockIdx.x == 0 && threadIdx.x == 0)
+    check_counts(frame, node_cap, depth_cap, group_cap, 1);
   __shared__ unsigned long long part[kBlock / 64][8];
   const uint32_t ng = bounded_count(frame + frame::kGroups, group_cap);
   unsigned long long *stats = frame + frame::kStats;
@@ -801,17 +820,9 @@ __global__ __launch_bounds__(kBlock) void list_stats(const uint32_t *__restrict_
   }
 }
 
-// One thread: the status of an evaluation from its counts and the plan's bounds.
 __global__ void check_frame(unsigned long long *frame, uint32_t node_cap, uint32_t depth_cap,
                             uint32_t group_cap, int check_slots) {
-  unsigned long long st = 0;
-  if (frame[frame::kNodes] > node_cap) st |= kAbortNodes;
-  if (frame[frame::kUnresolved] & 1ull) st |= kAbortPaths;
-  if (frame[frame::kDepth] > depth_cap) st |= kAbortDepth;
-  if (frame[frame::kGroups] > group_cap) st |= kAbortGroups;
-  if (check_slots && frame[frame::kSlotOverflow]) st |= kAbortLists;
-  if (frame[frame::kWalkError] & 1ull) st |= kAbortWalk;
-  if (st) atomicOr(frame + frame::kStatus, st);  // (another stream may set bits too)
+  check_counts(frame, node_cap, depth_cap, group_cap, check_slots);
 }
 
 __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
@@ -960,12 +971,16 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
 
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
-                       hipStream_t s) {
+                       hipStream_t s, const Plan *check) {
   launch_fill_u64(frame + frame::kStats, 8, 0ull, s);
-  if (!group_cap) return;
+  if (!group_cap) {
+    if (check) launch_check_frame(frame, *check, true, s);
+    return;
+  }
   const uint32_t blocks = std::min<uint32_t>(blocks_for(group_cap), kStatsBlocks);
   hipLaunchKernelGGL(list_stats, dim3(blocks), dim3(kBlock), 0, s, cell_count, body_count, groups,
-                     frame, group_cap);
+                     frame, group_cap, check ? 1 : 0, check ? check->node_cap : 0u,
+                     check ? static_cast<uint32_t>(check->depth_cap) : 0u);
   GTH_LAUNCH_CHECK();
 }
 
