@@ -514,8 +514,9 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
                  uint32_t *body_list, hipStream_t s, const uint32_t *slot_c = nullptr,
                  const uint32_t *slot_b = nullptr, uint32_t g_begin = 0, uint32_t g_span = 0,
                  float4 *group_hilo = nullptr);
-// check (may be null): also set the status as launch_check_frame(frame,
-// *check, true) does, in the same launch.
+// The stats words must start cleared (they are among the counts every
+// evaluation clears first).  check (may be null): also set the status as
+// launch_check_frame(frame, *check, true) does, in the same launch.
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s, const Plan *check = nullptr);

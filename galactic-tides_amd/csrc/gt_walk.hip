@@ -972,7 +972,6 @@ void launch_walk(int mode, int lanes, const WalkRec *recs, const WalkRec32 *imag
 void launch_list_stats(const uint32_t *cell_count, const uint32_t *body_count,
                        const gt_group *groups, unsigned long long *frame, uint32_t group_cap,
                        hipStream_t s, const Plan *check) {
-  launch_fill_u64(frame + frame::kStats, 8, 0ull, s);
   if (!group_cap) {
     if (check) launch_check_frame(frame, *check, true, s);
     return;
