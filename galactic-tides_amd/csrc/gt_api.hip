@@ -1263,11 +1263,11 @@ void enqueue_step_phase(gt_hip_context *c, const gt_group_config *cfg, const Pla
       break;
     default:
       if (p.chunks <= 1) enqueue_combine(c, cfg, p);
-      // the scatter and the closing kick in one pass over the rows
+      // the scatter and the closing kick in one pass over the rows, which
+      // also counts the step
       launch_scatter_kick(c->out.get(), c->tree.inverse_order(), static_cast<uint32_t>(n),
-                          c->acc.get(),
-                          c->pot.get(), c->vel.get(), h, frame + frame::kStatus, s);
-      launch_count_step(frame, s);
+                          c->acc.get(), c->pot.get(), c->vel.get(), h, frame + frame::kStatus, s,
+                          frame);
       break;
   }
 }

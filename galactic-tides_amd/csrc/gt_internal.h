@@ -575,10 +575,11 @@ constexpr int kQueryKeyBits = 30;
 // particle's tree position).
 void launch_scatter(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                     double *pot, const unsigned long long *status, hipStream_t s);
-// launch_scatter then launch_kick(vel, acc, n, h) as one kernel.
+// launch_scatter then launch_kick(vel, acc, n, h) as one kernel; with
+// step_frame (the frame) also launch_count_step's count, in the same launch.
 void launch_scatter_kick(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                          double *pot, double *vel, double h, const unsigned long long *status,
-                         hipStream_t s);
+                         hipStream_t s, unsigned long long *step_frame = nullptr);
 void launch_kick_drift(double *pos, double *vel, const double *acc, size_t n, double h,
                        double dt, const unsigned long long *status, hipStream_t s);
 // Permutations of the resident state: out[k] = in[order[k]] (rows of w

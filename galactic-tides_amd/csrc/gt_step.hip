@@ -39,9 +39,15 @@ __global__ __launch_bounds__(kBlock) void scatter_kick(const float4 *__restrict_
                                                        uint32_t n, double *__restrict__ acc,
                                                        double *__restrict__ pot,
                                                        double *__restrict__ vel, double h,
-                                                       const unsigned long long *status) {
+                                                       const unsigned long long *status,
+                                                       unsigned long long *step_frame) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || *status) return;
+  if (step_frame && i == 0) {  // the step is done: count it and its interactions
+    step_frame[frame::kStepsDone] += 1;
+    step_frame[frame::kStepInteractions] +=
+        step_frame[frame::kStats] + step_frame[frame::kStats + 1];
+  }
   const float4 a = out[inv[i]];
   const double ax = a.x, ay = a.y, az = a.z;
   const size_t r = 3 * static_cast<size_t>(i);
@@ -191,10 +197,13 @@ void launch_scatter(const float4 *out, const uint32_t *inv, uint32_t n, double *
 
 void launch_scatter_kick(const float4 *out, const uint32_t *inv, uint32_t n, double *acc,
                          double *pot, double *vel, double h, const unsigned long long *status,
-                         hipStream_t s) {
-  if (!n) return;
+                         hipStream_t s, unsigned long long *step_frame) {
+  if (!n) {
+    if (step_frame) launch_count_step(step_frame, s);
+    return;
+  }
   hipLaunchKernelGGL(scatter_kick, dim3(blocks_for(n)), dim3(kBlock), 0, s, out, inv, n, acc,
-                     pot, vel, h, status);
+                     pot, vel, h, status, step_frame);
   GTH_LAUNCH_CHECK();
 }
 
